@@ -1,0 +1,122 @@
+/*
+ * wam_gpu.h -- C-ABI of the MI355X implementation of WAM Cycle 7's per-step
+ * hot path: IMPLSCH (implicit source-term integration) and PROPAGS
+ * (first-order upwind propagation over the sea-point neighbour tables).
+ *
+ * Reference interfaces replaced (myWAveModel/WAM):
+ *   IMPLSCH  src/mod/wam_source_module.f90:167-170 (generic), :374,422-431
+ *            IMPLSCH(FL3, U10, UDIR, TAUW, USTAR, Z0, ROAIRN, WSTAR, DEPTH, INDEP)
+ *            called from src/chief/wamodel.f90:354
+ *   PROPAGS  src/mod/wam_propagation_module.f90:151-154, :635,687
+ *            PROPAGS(F3(NIJS:NIJL,KL,ML)), called from src/chief/wamodel.f90:332
+ *   PREPARE_SOURCE       wam_source_module.f90:704-794   (initmdl.f90:212)
+ *   PREPARE_PROPAGATION  wam_propagation_module.f90:279-598 (initmdl.f90:204, wamodel.f90:326)
+ *
+ * Layout: spectra are the Fortran array FL3(IJ,K,M) seen from C, i.e.
+ * fl3[(m*KL + k)*NSEA + ij] with the sea point fastest (bin-major SoA).
+ * Per-point fields are float[NSEA].  All entry points return 0 on success and
+ * a positive WAM_E_* code otherwise (the reference aborts via ABORT1); the
+ * Fortran shim turns a non-zero code into the host's abort.
+ * One context is used from one host thread; calls are ordered on the
+ * context's HIP stream.  Host-pointer entry points are synchronous.
+ */
+#ifndef WAM_GPU_H
+#define WAM_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    WAM_OK = 0,
+    WAM_E_ARG = 1,        /* bad argument, size mismatch or unsupported option */
+    WAM_E_HIP = 2,        /* HIP runtime error (allocation, launch) */
+    WAM_E_CFL = 3,        /* negative propagation weight (CHECK_CFL abort) */
+    WAM_E_STATE = 4,      /* call order violated (e.g. IMPLSCH before PREPARE_PROPAGATION) */
+    WAM_E_GRID = 5        /* empty or invalid sea mask */
+};
+
+/* Model options and grid definition.  Mirrors the module variables the
+   reference reads through USE statements: wam_timopt_module.f90:47-68
+   (IPHYS, SHALLOW_RUN, SPHERICAL_RUN, REFRACTION_D/C_RUN, ISNONLIN, IDELPRO,
+   IDELT), wam_fre_dir_module.f90 (KL, ML, FR1, IREF), wam_tables_module.f90
+   (NDEPTH, DEPTHA, DEPTHD), wam_general_module.f90:60-63 (BETAMAX, CDFAC,
+   ZALP, ALPHA) and the regular grid of preproc_module.f90:2043-2381 with
+   coordinates in the reference's M_SEC units (1 degree = 360000). */
+typedef struct wam_config {
+    int32_t kl, ml;
+    double fr1;
+    int32_t iref;
+    int32_t iphys;                 /* 0: Janssen/Komen (default), 2: ST6 */
+    int32_t shallow, spherical, refr_d, refr_c, iper;
+    int32_t isnonlin;              /* only 0 (the default) is supported */
+    int32_t idelpro, idelt;        /* propagation / source time step [s] */
+    int32_t ndepth;
+    double deptha, depthd;
+    double betamax, zalp, alpha, cdfac;
+    int32_t nx, ny;
+    int64_t amosop, amonop, amowep, amoeap, xdella, xdello;
+} wam_config;
+
+typedef struct wam_ctx wam_ctx;
+
+/* INITMDL's grid and table part: PREPARE_GRID neighbour tables from the sea
+   mask (nx*ny bytes, row-major from the south-west corner, non-zero = sea),
+   MAKE_FRE_DIR, MIN_ENERGY, MAKE_SHALLOW_TABLES and PREPARE_SOURCE (NLWEIGT,
+   INIT_SNONLIN, INIT_X0TAUHF); uploads the tables to the device. */
+int wam_gpu_init(wam_ctx **ctx, const wam_config *cfg, const uint8_t *sea_mask);
+
+/* Host-only form of wam_gpu_init: builds the same tables without touching a
+   device (for hosts that only need the tables, and for CPU-side checks).
+   Such a context serves wam_gpu_get_table / wam_gpu_get_scalar /
+   wam_gpu_finalize only. */
+int wam_gpu_init_host(wam_ctx **ctx, const wam_config *cfg, const uint8_t *sea_mask);
+
+/* PREPARE_PROPAGATION (+ PREPARE_PROPAGATION_COEF and CHECK_CFL): depth and
+   currents per sea point (host arrays of NSEA; u, v may be NULL = no
+   current).  Computes INDEP, the propagation weights on the device, the CFL
+   sub-step count, and the per-point wave numbers for ST6 (WAVNU2 depends on
+   depth only).  Must precede IMPLSCH and PROPAGS. */
+int wam_gpu_prepare_propagation(wam_ctx *ctx, const float *depth, const float *u, const float *v);
+
+/* Drop-in PROPAGS: F3(NIJS:NIJL,KL,ML) in host memory, updated in place. */
+int wam_gpu_propags(wam_ctx *ctx, int32_t nijs, int32_t nijl, int32_t kl, int32_t ml, float *f3);
+
+/* Drop-in IMPLSCH with the reference's argument list (host memory, NSEA
+   points).  lcflx != 0 also computes the WNFLUXES side outputs.  phiaw (may be
+   NULL) receives PHIAW; fluxes (may be NULL) receives [6][NSEA] = PHIOC,
+   TAUOC_X, TAUOC_Y, PHIBOT, TAUBOT_X, TAUBOT_Y (wam_flux_module.f90:15-21). */
+int wam_gpu_implsch(wam_ctx *ctx, int32_t nsea, int32_t kl, int32_t ml, float *fl3,
+                    const float *u10, const float *udir, float *tauw, float *ustar, float *z0,
+                    const float *roairn, const float *wstar, const float *depth,
+                    const int32_t *indep, int32_t lcflx, float *phiaw, float *fluxes);
+
+/* Device-resident forms (pointers into HBM, ordered on the context stream,
+   asynchronous).  propags_dev reads d_in and writes d_out (distinct buffers of
+   NSEA*KL*ML floats).  implsch_dev updates d_fl3 in place; d_phiaw, d_fluxes
+   and d_mij may be NULL. */
+int wam_gpu_propags_dev(wam_ctx *ctx, const float *d_in, float *d_out);
+int wam_gpu_implsch_dev(wam_ctx *ctx, float *d_fl3, const float *d_u10, const float *d_udir,
+                        float *d_tauw, float *d_ustar, float *d_z0, const float *d_roairn,
+                        const float *d_wstar, int32_t lcflx, float *d_phiaw, float *d_fluxes,
+                        int32_t *d_mij);
+
+/* Stream used by the context (a hipStream_t; NULL = the context's own). */
+int wam_gpu_set_stream(wam_ctx *ctx, void *stream);
+int wam_gpu_sync(wam_ctx *ctx);
+
+/* Introspection for parity checks: copies a named table into out (n elements
+   of the table's type: int32 for index tables, float otherwise); returns the
+   table length, or -1 for an unknown name.  out may be NULL to query length. */
+int64_t wam_gpu_get_table(wam_ctx *ctx, const char *name, void *out, int64_t n);
+double wam_gpu_get_scalar(wam_ctx *ctx, const char *name);
+const char *wam_gpu_last_error(wam_ctx *ctx);
+
+void wam_gpu_finalize(wam_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,154 @@
+"""ORACLE -- test infrastructure only.
+
+ctypes wrapper of the CPU restatement in this directory (liborc32.so for
+REAL*4, liborc64.so for REAL*8).  Only tests/, the smoke check and bench.py's
+cpu_baseline leg may import this module; the product package never does.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+class OrcConfig(ctypes.Structure):
+    _fields_ = [
+        ("kl", ctypes.c_int), ("ml", ctypes.c_int), ("fr1", ctypes.c_double), ("iref", ctypes.c_int),
+        ("iphys", ctypes.c_int), ("shallow", ctypes.c_int), ("spherical", ctypes.c_int),
+        ("refr_d", ctypes.c_int), ("refr_c", ctypes.c_int), ("iper", ctypes.c_int),
+        ("isnonlin", ctypes.c_int), ("idelpro", ctypes.c_int), ("idelt", ctypes.c_int),
+        ("ndepth", ctypes.c_int), ("deptha", ctypes.c_double), ("depthd", ctypes.c_double),
+        ("betamax", ctypes.c_double), ("zalp", ctypes.c_double), ("alpha", ctypes.c_double),
+        ("cdfac", ctypes.c_double),
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+        ("amosop", ctypes.c_int64), ("amonop", ctypes.c_int64), ("amowep", ctypes.c_int64),
+        ("amoeap", ctypes.c_int64), ("xdella", ctypes.c_int64), ("xdello", ctypes.c_int64),
+    ]
+
+
+def make_config(cfg):
+    """cfg: dict with the model options (see wam_amd.config.ModelConfig.as_dict)."""
+    c = OrcConfig()
+    for name, _ in OrcConfig._fields_:
+        setattr(c, name, cfg[name])
+    return c
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+_LIBS = {}
+
+
+def _lib(fp64):
+    key = "liborc64.so" if fp64 else "liborc32.so"
+    if key not in _LIBS:
+        path = os.path.join(_HERE, key)
+        if not os.path.exists(path):
+            build()
+        lib = ctypes.CDLL(path)
+        lib.orc_api_create.restype = ctypes.c_void_p
+        lib.orc_api_create.argtypes = [ctypes.POINTER(OrcConfig), ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+        lib.orc_api_free.argtypes = [ctypes.c_void_p]
+        lib.orc_api_get.restype = ctypes.c_void_p
+        lib.orc_api_get.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_long),
+                                    ctypes.POINTER(ctypes.c_int)]
+        lib.orc_api_scalar.restype = ctypes.c_double
+        lib.orc_api_scalar.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+        lib.orc_api_prepare_propagation.restype = ctypes.c_int
+        lib.orc_api_prepare_propagation.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 3
+        lib.orc_api_propags.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        lib.orc_api_implsch.argtypes = ([ctypes.c_void_p] + [ctypes.c_void_p] * 9 +
+                                        [ctypes.c_int] + [ctypes.c_void_p] * 3 + [ctypes.c_int])
+        lib.orc_api_wavnu2.argtypes = [ctypes.c_void_p] * 4
+        lib.orc_api_aki.restype = ctypes.c_double
+        lib.orc_api_aki.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        _LIBS[key] = lib
+    return _LIBS[key]
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Oracle:
+    """One model configuration on one regular grid (sea mask nx*ny, row-major from the south-west)."""
+
+    def __init__(self, cfg, mask, fp64=False):
+        self.cfg = dict(cfg)
+        self.fp64 = fp64
+        self.dtype = np.float64 if fp64 else np.float32
+        self.lib = _lib(fp64)
+        self._c = make_config(cfg)
+        m = np.ascontiguousarray(mask, dtype=np.uint8).ravel()
+        st = ctypes.c_int(0)
+        self.ctx = self.lib.orc_api_create(ctypes.byref(self._c), _p(m), ctypes.byref(st))
+        if st.value != 0:
+            raise ValueError(f"oracle create failed with status {st.value}")
+        self.kl, self.ml = cfg["kl"], cfg["ml"]
+        self.nsea = int(self.scalar("NSEA"))
+
+    def __del__(self):
+        if getattr(self, "ctx", None):
+            self.lib.orc_api_free(self.ctx)
+            self.ctx = None
+
+    def get(self, name):
+        n = ctypes.c_long(0)
+        t = ctypes.c_int(0)
+        p = self.lib.orc_api_get(self.ctx, name.encode(), ctypes.byref(n), ctypes.byref(t))
+        if not p or n.value == 0:
+            raise KeyError(name)
+        ct = ctypes.c_int if t.value == 1 else (ctypes.c_double if self.fp64 else ctypes.c_float)
+        arr = np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ct)), shape=(n.value,))
+        return arr.copy()
+
+    def scalar(self, name):
+        return self.lib.orc_api_scalar(self.ctx, name.encode())
+
+    def prepare_propagation(self, depth, u=None, v=None):
+        d = np.ascontiguousarray(depth, dtype=self.dtype)
+        uu = None if u is None else np.ascontiguousarray(u, dtype=self.dtype)
+        vv = None if v is None else np.ascontiguousarray(v, dtype=self.dtype)
+        st = self.lib.orc_api_prepare_propagation(self.ctx, _p(d), _p(uu), _p(vv))
+        if st != 0:
+            raise ValueError(f"oracle prepare_propagation status {st}")
+        return int(self.scalar("COUNTER")), float(self.scalar("CFL_MAX"))
+
+    def propags(self, fl3):
+        f = np.ascontiguousarray(fl3, dtype=self.dtype)
+        self.lib.orc_api_propags(self.ctx, _p(f))
+        return f
+
+    def implsch(self, fl3, u10, udir, tauw, z0, depth, roairn=None, wstar=None, lcflx=False, nthreads=0):
+        n = self.nsea
+        dt = self.dtype
+        f = np.ascontiguousarray(fl3, dtype=dt).copy()
+        u10 = np.ascontiguousarray(u10, dtype=dt)
+        udir = np.ascontiguousarray(udir, dtype=dt)
+        tauw = np.ascontiguousarray(tauw, dtype=dt).copy()
+        z0 = np.ascontiguousarray(z0, dtype=dt).copy()
+        depth = np.ascontiguousarray(depth, dtype=dt)
+        roairn = np.full(n, 1.225, dt) if roairn is None else np.ascontiguousarray(roairn, dtype=dt)
+        wstar = np.zeros(n, dt) if wstar is None else np.ascontiguousarray(wstar, dtype=dt)
+        ustar = np.zeros(n, dt)
+        phiaw = np.zeros(n, dt)
+        flux = np.zeros((6, n), dt)
+        mij = np.zeros(n, np.int32)
+        self.lib.orc_api_implsch(self.ctx, _p(f), _p(u10), _p(udir), _p(tauw), _p(ustar), _p(z0),
+                                 _p(roairn), _p(wstar), _p(depth), int(bool(lcflx)), _p(phiaw),
+                                 _p(flux), _p(mij), int(nthreads))
+        return dict(fl3=f, tauw=tauw, ustar=ustar, z0=z0, phiaw=phiaw, flux=flux, mij=mij)
+
+    def wavnu2(self, depth):
+        d = np.ascontiguousarray(depth, dtype=self.dtype)
+        wn = np.zeros(self.ml * self.nsea, self.dtype)
+        cg = np.zeros(self.ml * self.nsea, self.dtype)
+        self.lib.orc_api_wavnu2(self.ctx, _p(d), _p(wn), _p(cg))
+        return wn.reshape(self.ml, self.nsea), cg.reshape(self.ml, self.nsea)
+
+    def aki(self, om, depth):
+        return self.lib.orc_api_aki(self.ctx, float(om), float(depth))

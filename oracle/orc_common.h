@@ -1,0 +1,144 @@
+/*
+ * ORACLE -- test infrastructure only.  Nothing in the product (wam_amd/) may
+ * include, link or call anything in this directory; tests/, the smoke check
+ * and bench.py's cpu_baseline leg use it as the checker.
+ *
+ * orc_common.h: precision switch and the data structures of the CPU
+ * restatement of WAM Cycle 7's IMPLSCH + PROPAGS path and its set-up.
+ * Arrays follow the Fortran layout of the reference: FL3(IJ,K,M) is
+ * fl3[(m*KL + k)*NIJ + ij] (0-based), sea point fastest.
+ *
+ * Build FP32 (default, the reference's REAL) or FP64 (-DORC_FP64: the
+ * reference built with 8-byte reals).  The FP32 build evaluates the Fortran
+ * intrinsics with wam_math.h (deterministic single precision); the FP64 build
+ * uses libm double.
+ */
+#ifndef ORC_COMMON_H
+#define ORC_COMMON_H
+
+#include <stdint.h>
+#include <stddef.h>
+#include <math.h>
+
+#ifdef ORC_FP64
+typedef double real;
+#define R(x) (x)
+#define EXP(x) exp(x)
+#define LOG(x) log(x)
+#define LOG10(x) log10(x)
+#define TANH(x) tanh(x)
+#define COSH(x) cosh(x)
+#define SINH(x) sinh(x)
+#define COS(x) cos(x)
+#define SIN(x) sin(x)
+#define SQRT(x) sqrt(x)
+#define ACOS(x) acos(x)
+#define POW(x, y) pow(x, y)
+#define ABS(x) fabs(x)
+static inline real ipow_r(real x, int n)
+{
+    real r = 1.0, b = x; int first = 1;
+    while (n > 0) { if (n & 1) { r = first ? b : r * b; first = 0; } n >>= 1; if (n) b = b * b; }
+    return r;
+}
+#define IPOW(x, n) ipow_r(x, n)
+static inline int NINT(real x) { return (int)lround(x); }
+static inline int INTR(real x) { return (int)trunc(x); }
+static inline real FLOORR(real x) { return floor(x); }
+static inline real CEILR(real x) { return ceil(x); }
+static inline real SIGNR(real a, real b) { return b >= 0 ? fabs(a) : -fabs(a); }
+#else
+#include "wam_math.h"
+typedef float real;
+#define R(x) (x##f)
+#define EXP(x) wam_expf(x)
+#define LOG(x) wam_logf(x)
+#define LOG10(x) wam_log10f(x)
+#define TANH(x) wam_tanhf(x)
+#define COSH(x) wam_coshf(x)
+#define SINH(x) wam_sinhf(x)
+#define COS(x) wam_cosf(x)
+#define SIN(x) wam_sinf(x)
+#define SQRT(x) sqrtf(x)
+#define ACOS(x) ((float)acos((double)(x)))
+#define POW(x, y) wam_powf(x, y)
+#define ABS(x) wam_fabsf(x)
+#define IPOW(x, n) wam_ipowf(x, n)
+static inline int NINT(real x) { return wam_nint(x); }
+static inline int INTR(real x) { return wam_int(x); }
+static inline real FLOORR(real x) { return wam_floorf(x); }
+static inline real CEILR(real x) { return wam_ceilf(x); }
+static inline real SIGNR(real a, real b) { return wam_signf(a, b); }
+#endif
+
+static inline real MAXR(real a, real b) { return a > b ? a : b; }
+static inline real MINR(real a, real b) { return a < b ? a : b; }
+static inline int MAXI(int a, int b) { return a > b ? a : b; }
+static inline int MINI(int a, int b) { return a < b ? a : b; }
+
+/* Options and dimensions (wam_timopt_module.f90:47-68, wam_fre_dir_module.f90). */
+typedef struct {
+    int kl, ml;             /* directions, frequencies */
+    double fr1;             /* reference frequency [Hz] */
+    int iref;               /* bin of the reference frequency */
+    int iphys;              /* 0 Janssen, 2 ST6 */
+    int shallow, spherical, refr_d, refr_c, iper;
+    int isnonlin;
+    int idelpro, idelt;     /* propagation / source time step [s] */
+    int ndepth;             /* shallow-water table length */
+    double deptha, depthd;  /* first depth, depth ratio */
+    double betamax, zalp, alpha, cdfac;
+    /* regular lat-lon grid in M_SEC units (1 degree = 360000) */
+    int nx, ny;
+    int64_t amosop, amonop, amowep, amoeap, xdella, xdello;
+} orc_config;
+
+/* Everything the path reads from module globals in the reference. */
+typedef struct {
+    orc_config cfg;
+    int KL, ML;
+    /* constants (wam_general_module.f90:41-85) */
+    real G, PI, ZPI, DEG, RAD, CIRC, RADIUS, ROAIR, RNUAIR, RNUAIRM, ROWATER,
+         XKAPPA, XNLEV, DKMAX, BETAMAX, ZALP, ALPHA, CDFAC, EMIN, EPS1;
+    /* fre_dir (wam_fre_dir_module.f90:297-461) */
+    real CO, INV_LOG_CO, DELTH, DELTR, MO_TAIL, MM1_TAIL, MP1_TAIL, MP2_TAIL, FMIN;
+    real *FR, *DF, *DF_FR, *DF_FR2, *DFIM, *DFIMOFR, *DFIM_FR, *DFIM_FR2, *FR5, *FRM5,
+         *RHOWG_DFIM, *GOM, *C, *TH, *COSTH, *SINTH;
+    int *MPM, *KPM, *JXO, *JYO;            /* [ML][3], [KL][3], [KL][2], [KL][2] */
+    /* tables (wam_tables_module.f90) */
+    int JUMAX; real UMAX, DELU, FLMIN;
+    real *FLMINFR;                          /* [ML][JUMAX]: FLMINFR(JU,M) */
+    real DEPTHA, DEPTHD;
+    real *TFAK, *TCGOND, *TSIHKD, *T_TAIL;  /* [ML][NDEPTH] */
+    /* DIA (wam_source_module.f90:1056-1523) */
+    int ISP, ISM, MFRSTLW, MLSTHG, KFRH;
+    int *IKP, *IKP1, *IKM, *IKM1;           /* index MFRSTLW..MLSTHG -> [M-MFRSTLW] */
+    int *K1W, *K2W, *K11W, *K21W;           /* [2][KL] (KH, K) */
+    real *AF11, *FKLAP, *FKLAP1, *FKLAM, *FKLAM1, *FRH, *FTRF;
+    real ACL1, ACL2, CL11, CL21, DAL1, DAL2;
+    int *INLCOEF;                           /* [MLSTHG][5] */
+    real *RNLCOEF;                          /* [MLSTHG][25] */
+    /* TAU_PHI_HF (wam_source_module.f90:3784-3849) */
+    real X0TAUHF, WTAUHF[19];
+    /* grid (preproc_module.f90:2043-2381) */
+    int NSEA;
+    real DELPHI;
+    real *SINPH, *COSPH, *DELLAM;           /* [NY] */
+    int *ZDELLO, *IXLG, *KXLT;              /* 1-based grid indices per sea point */
+    int *KLAT;                              /* [2 (ICL)][2 (IC)][NSEA] : KLAT(IJ,IC,ICL) 1-based, 0 = land */
+    int *KLON;                              /* [2][NSEA] */
+    real *WLAT;                             /* [2][NSEA] */
+    real *DCO;                              /* [NSEA+1], index 0 = land slot */
+    real *DPSN;                             /* [2][NSEA] */
+    /* propagation */
+    int counter; real cfl_max, cfl_min;
+    int ncoef;                              /* coefficient planes present */
+    real *COEF;                             /* [ncoef][ML][KL][NSEA] */
+    int *INDEP;                             /* [NSEA] */
+    real *WN, *CGG;                         /* WAVNU2 results [ML][NSEA] (IPHYS=2) */
+} orc_ctx;
+
+/* coefficient plane order in COEF: */
+enum { C_SUM = 0, C_LAT11, C_LAT21, C_LON1, C_LON2, C_TH1, C_TH2, C_SIG1, C_SIG2, C_NPLANES };
+
+#endif

@@ -1,0 +1,8 @@
+"""MI355X implementation of WAM Cycle 7's per-step hot path (IMPLSCH + PROPAGS).
+
+The compute path is libwamgpu.so (hand-written HIP for gfx950 behind the
+C-ABI in include/wam_gpu.h); this package holds its ctypes binding, the host
+mirror of the reference's call sequence, and seeded synthetic workloads.
+"""
+from .model import WamError, WamGpu  # noqa: F401
+from . import workloads  # noqa: F401

@@ -1,0 +1,63 @@
+"""ctypes binding of libwamgpu.so (include/wam_gpu.h).
+
+The library is built in-tree by wam_amd/build.py.  Loading fails loudly if it
+is missing: there is no CPU fallback for the product path.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libwamgpu.so")
+
+
+class WamConfig(ctypes.Structure):
+    """struct wam_config of include/wam_gpu.h."""
+    _fields_ = [
+        ("kl", ctypes.c_int32), ("ml", ctypes.c_int32), ("fr1", ctypes.c_double), ("iref", ctypes.c_int32),
+        ("iphys", ctypes.c_int32), ("shallow", ctypes.c_int32), ("spherical", ctypes.c_int32),
+        ("refr_d", ctypes.c_int32), ("refr_c", ctypes.c_int32), ("iper", ctypes.c_int32),
+        ("isnonlin", ctypes.c_int32), ("idelpro", ctypes.c_int32), ("idelt", ctypes.c_int32),
+        ("ndepth", ctypes.c_int32), ("deptha", ctypes.c_double), ("depthd", ctypes.c_double),
+        ("betamax", ctypes.c_double), ("zalp", ctypes.c_double), ("alpha", ctypes.c_double),
+        ("cdfac", ctypes.c_double),
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
+        ("amosop", ctypes.c_int64), ("amonop", ctypes.c_int64), ("amowep", ctypes.c_int64),
+        ("amoeap", ctypes.c_int64), ("xdella", ctypes.c_int64), ("xdello", ctypes.c_int64),
+    ]
+
+
+# Entry points declared in include/wam_gpu.h, with their ctypes signatures.
+_P = ctypes.c_void_p
+_I = ctypes.c_int32
+SIGNATURES = {
+    "wam_gpu_init": (_I, [ctypes.POINTER(_P), ctypes.POINTER(WamConfig), _P]),
+    "wam_gpu_init_host": (_I, [ctypes.POINTER(_P), ctypes.POINTER(WamConfig), _P]),
+    "wam_gpu_prepare_propagation": (_I, [_P, _P, _P, _P]),
+    "wam_gpu_propags": (_I, [_P, _I, _I, _I, _I, _P]),
+    "wam_gpu_implsch": (_I, [_P, _I, _I, _I] + [_P] * 11 + [_I, _P, _P]),
+    "wam_gpu_propags_dev": (_I, [_P, _P, _P]),
+    "wam_gpu_implsch_dev": (_I, [_P] + [_P] * 8 + [_I, _P, _P, _P]),
+    "wam_gpu_set_stream": (_I, [_P, _P]),
+    "wam_gpu_sync": (_I, [_P]),
+    "wam_gpu_get_table": (ctypes.c_int64, [_P, ctypes.c_char_p, _P, ctypes.c_int64]),
+    "wam_gpu_get_scalar": (ctypes.c_double, [_P, ctypes.c_char_p]),
+    "wam_gpu_last_error": (ctypes.c_char_p, [_P]),
+    "wam_gpu_finalize": (None, [_P]),
+}
+
+_lib = None
+
+
+def load():
+    """Returns the loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python wam_amd/build.py` (no CPU fallback exists)")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib

@@ -1,0 +1,489 @@
+// wam_capi.hip -- context management and the extern "C" entry points of
+// include/wam_gpu.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/wam_gpu.h"
+#include "wam_dev.hpp"
+#include "wam_setup.hpp"
+
+namespace wam {
+__global__ void prep_coef_kernel(DevTables t, float *coef);
+__global__ void minmax_kernel(const float *x, size_t n, float *out);
+__global__ void rescale_kernel(float *coef, size_t nb, int nplanes, float rc);
+__global__ void propags_nocurr_kernel(DevTables t, const float *coef, const float *fin, float *fout);
+__global__ void propags_curr_kernel(DevTables t, const float *coef, const float *fin, float *fout);
+__global__ void wavnu2_kernel(DevTables t, float *wn, float *cgg);
+hipError_t launch_implsch(const DevTables &t, const ImplArgs &a, int nblocks, hipStream_t st);
+}  // namespace wam
+
+using namespace wam;
+
+struct wam_ctx {
+    Tables tab;
+    DevTables dt{};
+    hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;
+    std::vector<void *> dev_allocs;       // static tables (freed at finalize)
+    std::vector<void *> point_allocs;     // per depth field
+    float *coef = nullptr;
+    int ncoef = 0;
+    int counter = 1;
+    float cfl_max = 0.f, cfl_min = 0.f;
+    float *wn = nullptr, *cgg = nullptr;
+    float *fscratch = nullptr;            // COUNTER > 1 ping-pong
+    float *stage = nullptr;               // host drop-in staging: FL3 + per-point fields
+    size_t stage_n = 0;
+    bool prepared = false;
+    int nblocks_implsch = 0;
+    std::string err;
+};
+
+#define HIPCHK(x)                                                                  \
+    do {                                                                           \
+        hipError_t e_ = (x);                                                       \
+        if (e_ != hipSuccess) {                                                    \
+            ctx->err = std::string(#x) + ": " + hipGetErrorString(e_);             \
+            return WAM_E_HIP;                                                      \
+        }                                                                          \
+    } while (0)
+
+template <class T>
+static int upload(wam_ctx *ctx, std::vector<void *> &owner, const T *host, size_t n, const T **dev)
+{
+    void *p = nullptr;
+    HIPCHK(hipMalloc(&p, (n ? n : 1) * sizeof(T)));
+    owner.push_back(p);
+    if (n) HIPCHK(hipMemcpy(p, host, n * sizeof(T), hipMemcpyHostToDevice));
+    *dev = (const T *)p;
+    return WAM_OK;
+}
+
+static void free_list(std::vector<void *> &v)
+{
+    for (void *p : v) (void)hipFree(p);
+    v.clear();
+}
+
+#define UP(owner, vec, field)                                                        \
+    do {                                                                           \
+        int r_ = upload(ctx, owner, (vec).data(), (vec).size(), &ctx->dt.field);   \
+        if (r_) return r_;                                                         \
+    } while (0)
+
+static int upload_static(wam_ctx *ctx)
+{
+    Tables &t = ctx->tab;
+    DevTables &d = ctx->dt;
+    d.KL = t.KL; d.ML = t.ML; d.NSEA = t.NSEA; d.NDEPTH = t.cfg.ndepth; d.JUMAX = t.JUMAX;
+    d.MFRSTLW = t.MFRSTLW; d.MLSTHG = t.MLSTHG; d.NQ = t.NQ; d.NCMAX = t.NCMAX; d.NK10 = t.NK10;
+    d.iphys = t.cfg.iphys; d.shallow = t.cfg.shallow; d.refr_d = t.cfg.refr_d; d.refr_c = t.cfg.refr_c;
+    const Consts &k = t.k;
+    d.G = k.G; d.ZPI = k.ZPI; d.ROAIR = k.ROAIR; d.RNUAIRM = k.RNUAIRM; d.ROWATER = k.ROWATER;
+    d.XKAPPA = k.XKAPPA; d.XNLEV = k.XNLEV; d.BETAMAX = k.BETAMAX; d.ZALP = k.ZALP; d.ALPHA = k.ALPHA;
+    d.CDFAC = k.CDFAC; d.EMIN = k.EMIN; d.EPS1 = k.EPS1; d.CO = k.CO;
+    d.DELTH = t.DELTH; d.INV_LOG_CO = t.INV_LOG_CO; d.MO_TAIL = t.MO_TAIL; d.MM1_TAIL = t.MM1_TAIL;
+    d.MP1_TAIL = t.MP1_TAIL; d.DELU = t.DELU; d.DAL1 = t.DAL1; d.DAL2 = t.DAL2; d.X0TAUHF = t.X0TAUHF;
+    d.DELT = (float)t.cfg.idelt;
+    // constant expressions the Fortran compiler folds: one correct rounding each
+    d.ZPI4 = (float)std::pow((double)k.ZPI, 4.0);
+    const float g4 = (float)std::pow((double)k.G, 4.0);
+    d.CONSD = ((-1.33f) * (float)std::pow((double)k.ZPI, 9.0)) / g4;
+    d.CONSS = (-1.33f) * k.ZPI;
+    d.C5G = 5.0E-07f * k.G;
+    for (int j = 0; j < 19; j++) d.WTAUHF[j] = t.WTAUHF[j];
+    const float delpro = (float)t.cfg.idelpro;
+    d.DELTH0 = (0.5f * delpro) / t.DELTR;
+    d.DELPH0 = (0.5f * delpro) / t.DELPHI;
+    d.DELTHR = (0.5f * delpro) / t.DELTH;
+    d.DELFR0 = t.cfg.shallow ? (0.5f * delpro) / (0.1f * k.ZPI) : ((0.5f * delpro) * 2.1f) / 0.2f;
+
+    auto &o = ctx->dev_allocs;
+    UP(o, t.FR, FR); UP(o, t.DF, DF); UP(o, t.DFIM, DFIM); UP(o, t.DFIMOFR, DFIMOFR);
+    UP(o, t.DFIM_FR, DFIM_FR); UP(o, t.FR5, FR5); UP(o, t.FRM5, FRM5); UP(o, t.RHOWG_DFIM, RHOWG_DFIM);
+    UP(o, t.GOM, GOM); UP(o, t.TH, TH); UP(o, t.COSTH, COSTH); UP(o, t.SINTH, SINTH);
+    UP(o, t.FLMINFR, FLMINFR); UP(o, t.TFAK, TFAK); UP(o, t.TCGOND, TCGOND); UP(o, t.TSIHKD, TSIHKD);
+    UP(o, t.SIG10, SIG10); UP(o, t.DSII10, DSII10);
+    UP(o, t.INLCOEF, INLCOEF); UP(o, t.RNLCOEF, RNLCOEF);
+    std::vector<float> af(t.MLSTHG);
+    for (int mc = 1; mc <= t.MLSTHG; mc++) af[mc - 1] = t.AF11[mc - t.MFRSTLW];
+    UP(o, af, AF11);
+    auto zero_based = [](const std::vector<int> &v) { std::vector<int> r(v); for (auto &x : r) x -= 1; return r; };
+    std::vector<int> k1 = zero_based(t.K1W), k2 = zero_based(t.K2W), k11 = zero_based(t.K11W), k21 = zero_based(t.K21W);
+    UP(o, k1, K1W); UP(o, k2, K2W); UP(o, k11, K11W); UP(o, k21, K21W);
+    UP(o, t.CONTRIB, CONTRIB); UP(o, t.NCONTRIB, NCONTRIB);
+    std::vector<int> ks = zero_based(t.KLAT_S), kn = zero_based(t.KLAT_N), kw = zero_based(t.KLON_W), ke = zero_based(t.KLON_E);
+    UP(o, ks, KLAT_S); UP(o, kn, KLAT_N); UP(o, kw, KLON_W); UP(o, ke, KLON_E);
+    UP(o, t.DPSN_S, DPSN_S); UP(o, t.DPSN_N, DPSN_N);
+    UP(o, t.JXO1, JXO1); UP(o, t.JXO2, JXO2); UP(o, t.JYO1, JYO1); UP(o, t.JYO2, JYO2);
+    UP(o, t.KPM_p, KPM_P); UP(o, t.KPM_m, KPM_M);
+    return WAM_OK;
+}
+
+extern "C" {
+
+int wam_gpu_init_host(wam_ctx **out, const wam_config *cfg, const uint8_t *sea_mask)
+{
+    if (!out || !cfg || !sea_mask) return WAM_E_ARG;
+    wam_ctx *ctx = new wam_ctx();
+    *out = ctx;
+    return build_tables(ctx->tab, *cfg, sea_mask, ctx->err);
+}
+
+int wam_gpu_init(wam_ctx **out, const wam_config *cfg, const uint8_t *sea_mask)
+{
+    if (!out || !cfg || !sea_mask) return WAM_E_ARG;
+    *out = nullptr;
+    int rc = wam_gpu_init_host(out, cfg, sea_mask);
+    wam_ctx *ctx = *out;
+    if (rc) return rc;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) { ctx->err = "no HIP device"; *out = ctx; return WAM_E_HIP; }
+    if (hipStreamCreate(&ctx->own_stream) != hipSuccess) { ctx->err = "hipStreamCreate failed"; *out = ctx; return WAM_E_HIP; }
+    ctx->stream = ctx->own_stream;
+    rc = upload_static(ctx);
+    if (!rc) {
+        int ncu = 256;
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
+        const int want = ncu * 2;
+        ctx->nblocks_implsch = want < ctx->tab.NSEA ? want : ctx->tab.NSEA;
+    }
+    *out = ctx;
+    return rc;
+}
+
+int wam_gpu_prepare_propagation(wam_ctx *ctx, const float *depth, const float *u, const float *v)
+{
+    if (!ctx || !depth) return WAM_E_ARG;
+    Tables &t = ctx->tab;
+    int rc = prepare_points(t, depth, u, v, ctx->err);
+    if (rc) return rc;
+    free_list(ctx->point_allocs);
+    ctx->coef = nullptr; ctx->wn = nullptr; ctx->cgg = nullptr;
+    const int ns = t.NSEA;
+    auto &o = ctx->point_allocs;
+    std::vector<float> d(depth, depth + ns), uu(ns, 0.f), vv(ns, 0.f);
+    if (u) uu.assign(u, u + ns);
+    if (v) vv.assign(v, v + ns);
+    UP(o, d, DEPTH); UP(o, uu, U); UP(o, vv, V);
+    UP(o, t.INDEP, INDEP); UP(o, t.INDEP_G, INDEP_G); UP(o, t.DELLA0, DELLA0); UP(o, t.DRGM, DRGM);
+    UP(o, t.DDPHI, DDPHI); UP(o, t.DDLAM, DDLAM); UP(o, t.DUPHI, DUPHI); UP(o, t.DULAM, DULAM);
+    UP(o, t.DVPHI, DVPHI); UP(o, t.DVLAM, DVLAM);
+
+    const size_t nb = (size_t)t.KL * t.ML * ns;
+    ctx->ncoef = t.cfg.refr_c ? 9 : 5;
+    void *p = nullptr;
+    HIPCHK(hipMalloc(&p, nb * ctx->ncoef * sizeof(float)));
+    o.push_back(p);
+    ctx->coef = (float *)p;
+    hipStream_t st = ctx->stream;
+    hipLaunchKernelGGL(prep_coef_kernel, dim3((ns + 255) / 256, t.KL * t.ML), dim3(256), 0, st, ctx->dt, ctx->coef);
+    HIPCHK(hipGetLastError());
+    // CHECK_CFL
+    const int nblk = 1024;
+    float *mm = nullptr;
+    HIPCHK(hipMalloc(&mm, 2 * nblk * sizeof(float) * ctx->ncoef));
+    for (int pl = 0; pl < ctx->ncoef; pl++)
+        hipLaunchKernelGGL(minmax_kernel, dim3(nblk), dim3(256), 0, st, ctx->coef + pl * nb, nb, mm + 2 * nblk * pl);
+    std::vector<float> h(2 * nblk * ctx->ncoef);
+    HIPCHK(hipMemcpyAsync(h.data(), mm, h.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    (void)hipFree(mm);
+    std::vector<float> lo(ctx->ncoef, 3.4e38f), hi(ctx->ncoef, -3.4e38f);
+    for (int pl = 0; pl < ctx->ncoef; pl++)
+        for (int b = 0; b < nblk; b++) {
+            lo[pl] = fminf(lo[pl], h[2 * nblk * pl + 2 * b]);
+            hi[pl] = fmaxf(hi[pl], h[2 * nblk * pl + 2 * b + 1]);
+        }
+    // plane order: no current SUM LAT1 LON1 TH1 TH2; current SUM LAT1 LAT2 LON1 LON2 TH1 TH2 SIG1 SIG2
+    float lat11mx, lat11mn, lon2mx = 0.1f, lon2mn = 0.1f, s1mx = 0.1f, s1mn = 0.1f, s2mx = 0.1f, s2mn = 0.1f;
+    float lon1mx, th1mx, th1mn, th2mx, th2mn;
+    if (t.cfg.refr_c) {
+        lat11mx = fmaxf(hi[P_LAT1], hi[P_LAT2]); lat11mn = fminf(lo[P_LAT1], lo[P_LAT2]);
+        lon1mx = hi[P_LON1]; lon2mx = hi[P_LON2]; lon2mn = lo[P_LON2];
+        th1mx = hi[P_TH1]; th1mn = lo[P_TH1]; th2mx = hi[P_TH2]; th2mn = lo[P_TH2];
+        s1mx = hi[P_SIG1]; s1mn = lo[P_SIG1]; s2mx = hi[P_SIG2]; s2mn = lo[P_SIG2];
+    } else {
+        lat11mx = hi[1]; lat11mn = lo[1]; lon1mx = hi[2];
+        th1mx = hi[3]; th1mn = lo[3]; th2mx = hi[4]; th2mn = lo[4];
+    }
+    const float summx = 1.f - lo[0], summn = 1.f - hi[0];
+    float cmax = lat11mx;
+    for (float x : {0.1f, lon1mx, lon2mx, th1mx, th2mx, s1mx, s2mx, summx}) cmax = fmaxf(cmax, x);
+    // the reference's minimum omits COEF_LON_1_1_MIN and lists COEF_LON_1_2_MIN twice
+    float cmin = lat11mn;
+    for (float x : {0.1f, lon2mn, lon2mn, th1mn, th2mn, s1mn, s2mn, summn}) cmin = fminf(cmin, x);
+    ctx->cfl_max = cmax; ctx->cfl_min = cmin;
+    if (cmin < 0.f) { ctx->err = "negative propagation weight (CHECK_CFL)"; return WAM_E_CFL; }
+    ctx->counter = 1;
+    if (cmax > 0.999f) {
+        ctx->counter = (int)std::ceil(cmax);
+        hipLaunchKernelGGL(rescale_kernel, dim3(4096), dim3(256), 0, st, ctx->coef, nb, ctx->ncoef, (float)ctx->counter);
+        HIPCHK(hipGetLastError());
+    }
+    if (t.cfg.iphys == 2) {
+        HIPCHK(hipMalloc(&p, 2 * (size_t)t.ML * ns * sizeof(float)));
+        o.push_back(p);
+        ctx->wn = (float *)p; ctx->cgg = ctx->wn + (size_t)t.ML * ns;
+        hipLaunchKernelGGL(wavnu2_kernel, dim3((ns + 255) / 256, t.ML), dim3(256), 0, st, ctx->dt, ctx->wn, ctx->cgg);
+        HIPCHK(hipGetLastError());
+        ctx->dt.WN = ctx->wn; ctx->dt.CGG = ctx->cgg;
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    ctx->prepared = true;
+    return WAM_OK;
+}
+
+int wam_gpu_propags_dev(wam_ctx *ctx, const float *d_in, float *d_out)
+{
+    if (!ctx || !d_in || !d_out || d_in == d_out) return WAM_E_ARG;
+    if (!ctx->prepared) { ctx->err = "PREPARE_PROPAGATION not done"; return WAM_E_STATE; }
+    const Tables &t = ctx->tab;
+    const size_t nb = (size_t)t.KL * t.ML * t.NSEA;
+    if (ctx->counter > 1 && !ctx->fscratch) HIPCHK(hipMalloc(&ctx->fscratch, nb * sizeof(float)));
+    const int nch = (t.NSEA + 255) / 256;
+    const int grid = ((nch + 7) / 8) * 8;
+    auto kern = t.cfg.refr_c ? propags_curr_kernel : propags_nocurr_kernel;
+    // sub-steps alternate between out and scratch so that the last one lands in d_out
+    const float *src = d_in;
+    for (int i = 0; i < ctx->counter; i++) {
+        const int left = ctx->counter - 1 - i;
+        float *dst = (left % 2 == 0) ? d_out : ctx->fscratch;
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, ctx->stream, ctx->dt, ctx->coef, src, dst);
+        HIPCHK(hipGetLastError());
+        src = dst;
+    }
+    return WAM_OK;
+}
+
+int wam_gpu_implsch_dev(wam_ctx *ctx, float *d_fl3, const float *d_u10, const float *d_udir, float *d_tauw,
+                        float *d_ustar, float *d_z0, const float *d_roairn, const float *d_wstar, int32_t lcflx,
+                        float *d_phiaw, float *d_fluxes, int32_t *d_mij)
+{
+    if (!ctx || !d_fl3 || !d_u10 || !d_udir || !d_tauw || !d_ustar || !d_z0 || !d_roairn || !d_wstar) return WAM_E_ARG;
+    if (!ctx->prepared) { ctx->err = "PREPARE_PROPAGATION not done"; return WAM_E_STATE; }
+    ImplArgs a{};
+    a.fl3 = d_fl3; a.u10 = d_u10; a.udir = d_udir; a.tauw = d_tauw; a.ustar = d_ustar; a.z0 = d_z0;
+    a.roairn = d_roairn; a.wstar = d_wstar; a.lcflx = lcflx ? 1 : 0; a.phiaw = d_phiaw; a.fluxes = d_fluxes;
+    a.mij = d_mij;
+    HIPCHK(launch_implsch(ctx->dt, a, ctx->nblocks_implsch, ctx->stream));
+    return WAM_OK;
+}
+
+static int ensure_stage(wam_ctx *ctx, size_t n)
+{
+    if (ctx->stage_n >= n) return WAM_OK;
+    if (ctx->stage) (void)hipFree(ctx->stage);
+    ctx->stage = nullptr; ctx->stage_n = 0;
+    HIPCHK(hipMalloc(&ctx->stage, n * sizeof(float)));
+    ctx->stage_n = n;
+    return WAM_OK;
+}
+
+int wam_gpu_propags(wam_ctx *ctx, int32_t nijs, int32_t nijl, int32_t kl, int32_t ml, float *f3)
+{
+    if (!ctx || !f3) return WAM_E_ARG;
+    const Tables &t = ctx->tab;
+    if (nijl - nijs + 1 != t.NSEA || kl != t.KL || ml != t.ML) { ctx->err = "PROPAGS size mismatch"; return WAM_E_ARG; }
+    const size_t nb = (size_t)t.KL * t.ML * t.NSEA;
+    int rc = ensure_stage(ctx, 2 * nb);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(ctx->stage, f3, nb * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    rc = wam_gpu_propags_dev(ctx, ctx->stage, ctx->stage + nb);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(f3, ctx->stage + nb, nb * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return WAM_OK;
+}
+
+int wam_gpu_implsch(wam_ctx *ctx, int32_t nsea, int32_t kl, int32_t ml, float *fl3, const float *u10,
+                    const float *udir, float *tauw, float *ustar, float *z0, const float *roairn,
+                    const float *wstar, const float *depth, const int32_t *indep, int32_t lcflx, float *phiaw,
+                    float *fluxes)
+{
+    if (!ctx || !fl3 || !u10 || !udir || !tauw || !ustar || !z0 || !roairn || !wstar) return WAM_E_ARG;
+    const Tables &t = ctx->tab;
+    if (nsea != t.NSEA || kl != t.KL || ml != t.ML) { ctx->err = "IMPLSCH size mismatch"; return WAM_E_ARG; }
+    if (!ctx->prepared) { ctx->err = "PREPARE_PROPAGATION not done"; return WAM_E_STATE; }
+    // DEPTH and INDEP are the fields given to wam_gpu_prepare_propagation
+    // (the reference passes DEPTH(NIJS:NIJL), INDEP(NIJS:NIJL) of the same module arrays)
+    (void)depth;
+    (void)indep;
+    const size_t nb = (size_t)t.KL * t.ML * nsea, ns = nsea;
+    int rc = ensure_stage(ctx, nb + 13 * ns);
+    if (rc) return rc;
+    float *d = ctx->stage;
+    float *dfl = d, *du = d + nb, *dd = du + ns, *dt = dd + ns, *dus = dt + ns, *dz = dus + ns, *dr = dz + ns,
+          *dw = dr + ns, *dph = dw + ns, *dfx = dph + ns;
+    hipStream_t st = ctx->stream;
+    HIPCHK(hipMemcpyAsync(dfl, fl3, nb * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(du, u10, ns * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dd, udir, ns * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dt, tauw, ns * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dz, z0, ns * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dr, roairn, ns * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dw, wstar, ns * 4, hipMemcpyHostToDevice, st));
+    rc = wam_gpu_implsch_dev(ctx, dfl, du, dd, dt, dus, dz, dr, dw, lcflx, dph, lcflx ? dfx : nullptr, nullptr);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(fl3, dfl, nb * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(tauw, dt, ns * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(ustar, dus, ns * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(z0, dz, ns * 4, hipMemcpyDeviceToHost, st));
+    if (phiaw) HIPCHK(hipMemcpyAsync(phiaw, dph, ns * 4, hipMemcpyDeviceToHost, st));
+    if (fluxes && lcflx) HIPCHK(hipMemcpyAsync(fluxes, dfx, 6 * ns * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return WAM_OK;
+}
+
+int wam_gpu_set_stream(wam_ctx *ctx, void *stream)
+{
+    if (!ctx) return WAM_E_ARG;
+    ctx->stream = stream ? (hipStream_t)stream : ctx->own_stream;
+    return WAM_OK;
+}
+
+int wam_gpu_sync(wam_ctx *ctx)
+{
+    if (!ctx) return WAM_E_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return WAM_OK;
+}
+
+int64_t wam_gpu_get_table(wam_ctx *ctx, const char *name, void *out, int64_t n)
+{
+    if (!ctx || !name) return -1;
+    const Tables &t = ctx->tab;
+    auto put = [&](const auto &v) -> int64_t {
+        const int64_t len = (int64_t)v.size();
+        if (out && n >= len) std::memcpy(out, v.data(), len * sizeof(v[0]));
+        return len;
+    };
+    const std::string s(name);
+    if (s == "FR") return put(t.FR);
+    if (s == "DF") return put(t.DF);
+    if (s == "DFIM") return put(t.DFIM);
+    if (s == "DFIMOFR") return put(t.DFIMOFR);
+    if (s == "DFIM_FR") return put(t.DFIM_FR);
+    if (s == "FR5") return put(t.FR5);
+    if (s == "FRM5") return put(t.FRM5);
+    if (s == "RHOWG_DFIM") return put(t.RHOWG_DFIM);
+    if (s == "GOM") return put(t.GOM);
+    if (s == "TH") return put(t.TH);
+    if (s == "COSTH") return put(t.COSTH);
+    if (s == "SINTH") return put(t.SINTH);
+    if (s == "FLMINFR") return put(t.FLMINFR);
+    if (s == "TFAK") return put(t.TFAK);
+    if (s == "TCGOND") return put(t.TCGOND);
+    if (s == "TSIHKD") return put(t.TSIHKD);
+    if (s == "IKP") return put(t.IKP);
+    if (s == "IKP1") return put(t.IKP1);
+    if (s == "IKM") return put(t.IKM);
+    if (s == "IKM1") return put(t.IKM1);
+    if (s == "K1W") return put(t.K1W);
+    if (s == "K2W") return put(t.K2W);
+    if (s == "K11W") return put(t.K11W);
+    if (s == "K21W") return put(t.K21W);
+    if (s == "AF11") return put(t.AF11);
+    if (s == "FKLAP") return put(t.FKLAP);
+    if (s == "FKLAP1") return put(t.FKLAP1);
+    if (s == "FKLAM") return put(t.FKLAM);
+    if (s == "FKLAM1") return put(t.FKLAM1);
+    if (s == "FRH") return put(t.FRH);
+    if (s == "FTRF") return put(t.FTRF);
+    if (s == "INLCOEF") return put(t.INLCOEF);
+    if (s == "RNLCOEF") return put(t.RNLCOEF);
+    if (s == "SINPH") return put(t.SINPH);
+    if (s == "COSPH") return put(t.COSPH);
+    if (s == "DELLAM") return put(t.DELLAM);
+    if (s == "IXLG") return put(t.IXLG);
+    if (s == "KXLT") return put(t.KXLT);
+    if (s == "DCO") return put(t.DCO);
+    if (s == "INDEP") return put(t.INDEP);
+    if (s == "NCONTRIB") return put(t.NCONTRIB);
+    if (s == "KLAT" || s == "KLON" || s == "DPSN") {
+        // reference layout: KLAT(IJ,IC,ICL) = [ICL][IC][IJ]; KLON(IJ,IC); DPSN(IJ,IC)
+        std::vector<int> vi; std::vector<float> vf;
+        if (s == "KLAT") {
+            for (int icl = 0; icl < 2; icl++) { vi.insert(vi.end(), t.KLAT_S.begin(), t.KLAT_S.end()); vi.insert(vi.end(), t.KLAT_N.begin(), t.KLAT_N.end()); }
+            return put(vi);
+        }
+        if (s == "KLON") { vi = t.KLON_W; vi.insert(vi.end(), t.KLON_E.begin(), t.KLON_E.end()); return put(vi); }
+        vf = t.DPSN_S; vf.insert(vf.end(), t.DPSN_N.begin(), t.DPSN_N.end());
+        return put(vf);
+    }
+    if ((s == "WN" || s == "CGG") && ctx->wn) {
+        std::vector<float> h((size_t)t.ML * t.NSEA);
+        if (hipMemcpy(h.data(), s == "WN" ? ctx->wn : ctx->cgg, h.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+        return put(h);
+    }
+    if (s == "COEF" && ctx->coef) {
+        // expanded to the nine reference planes (absent planes are zero)
+        const size_t nb = (size_t)t.KL * t.ML * t.NSEA;
+        std::vector<float> h((size_t)P_NPLANES * nb, 0.f), dev((size_t)ctx->ncoef * nb);
+        if (hipMemcpy(dev.data(), ctx->coef, dev.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+        const int map5[5] = {P_SUM, P_LAT1, P_LON1, P_TH1, P_TH2};
+        for (int pl = 0; pl < ctx->ncoef; pl++) {
+            const int dst = ctx->ncoef == 9 ? pl : map5[pl];
+            std::memcpy(h.data() + dst * nb, dev.data() + pl * nb, nb * 4);
+        }
+        return put(h);
+    }
+    return -1;
+}
+
+double wam_gpu_get_scalar(wam_ctx *ctx, const char *name)
+{
+    if (!ctx || !name) return NAN;
+    const Tables &t = ctx->tab;
+    const std::string s(name);
+    if (s == "NSEA") return t.NSEA;
+    if (s == "COUNTER") return ctx->counter;
+    if (s == "CFL_MAX") return ctx->cfl_max;
+    if (s == "CFL_MIN") return ctx->cfl_min;
+    if (s == "ACL1") return t.ACL1;
+    if (s == "ACL2") return t.ACL2;
+    if (s == "CL11") return t.CL11;
+    if (s == "CL21") return t.CL21;
+    if (s == "DAL1") return t.DAL1;
+    if (s == "DAL2") return t.DAL2;
+    if (s == "ISP") return t.ISP;
+    if (s == "ISM") return t.ISM;
+    if (s == "MFRSTLW") return t.MFRSTLW;
+    if (s == "MLSTHG") return t.MLSTHG;
+    if (s == "KFRH") return t.KFRH;
+    if (s == "X0TAUHF") return t.X0TAUHF;
+    if (s == "DELPHI") return t.DELPHI;
+    if (s == "DELTH") return t.DELTH;
+    if (s == "DELU") return t.DELU;
+    if (s == "INV_LOG_CO") return t.INV_LOG_CO;
+    if (s == "MO_TAIL") return t.MO_TAIL;
+    if (s == "MM1_TAIL") return t.MM1_TAIL;
+    if (s == "MP1_TAIL") return t.MP1_TAIL;
+    if (s == "NQ") return t.NQ;
+    if (s == "NCMAX") return t.NCMAX;
+    if (s == "NK10") return t.NK10;
+    if (s == "ALPHA") return t.k.ALPHA;
+    if (s == "NCOEF") return ctx->ncoef;
+    return NAN;
+}
+
+const char *wam_gpu_last_error(wam_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+void wam_gpu_finalize(wam_ctx *ctx)
+{
+    if (!ctx) return;
+    free_list(ctx->point_allocs);
+    free_list(ctx->dev_allocs);
+    if (ctx->fscratch) (void)hipFree(ctx->fscratch);
+    if (ctx->stage) (void)hipFree(ctx->stage);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+}
+
+}  // extern "C"

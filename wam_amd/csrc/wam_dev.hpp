@@ -1,0 +1,53 @@
+// wam_dev.hpp -- device-side views of the WAM tables and kernel entry points.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace wam {
+
+// Coefficient planes of the propagation weights, [plane][M][K][NSEA]:
+// COEF_SUM, COEF_LAT(.,1,1), COEF_LAT(.,2,1), COEF_LON(.,1), COEF_LON(.,2),
+// COEF_THETA(.,1), COEF_THETA(.,2), COEF_SIGD(.,1), COEF_SIGD(.,2)
+// (wam_propagation_module.f90:75-110).  Without current refraction only the
+// first five are stored, in the order SUM, LAT1, LON1, TH1, TH2.
+enum { P_SUM = 0, P_LAT1, P_LAT2, P_LON1, P_LON2, P_TH1, P_TH2, P_SIG1, P_SIG2, P_NPLANES };
+
+// Everything IMPLSCH and PROPAGS read from module globals in the reference,
+// as device pointers and scalars (passed by value to every kernel).
+struct DevTables {
+    int KL, ML, NSEA, NDEPTH, JUMAX, MFRSTLW, MLSTHG, NQ, NCMAX, NK10;
+    int iphys, shallow, refr_d, refr_c;
+    float G, ZPI, ROAIR, RNUAIRM, ROWATER, XKAPPA, XNLEV, BETAMAX, ZALP, ALPHA, CDFAC, EMIN, EPS1, CO;
+    float DELTH, INV_LOG_CO, MO_TAIL, MM1_TAIL, MP1_TAIL, DELU, DAL1, DAL2, X0TAUHF, DELT;
+    float ZPI4, CONSD, CONSS, C5G;        // folded constants of STRESSO, SDISSIP and the limiter
+    float WTAUHF[19];
+    // per frequency / direction
+    const float *FR, *DF, *DFIM, *DFIMOFR, *DFIM_FR, *FR5, *FRM5, *RHOWG_DFIM, *GOM, *TH, *COSTH, *SINTH;
+    const float *FLMINFR;                 // [ML][JUMAX]
+    const float *TFAK, *TCGOND, *TSIHKD;  // [ML][NDEPTH]
+    const float *SIG10, *DSII10;          // [NK10]
+    // SNONLIN
+    const int *INLCOEF;                   // [MLSTHG][5]
+    const float *RNLCOEF;                 // [MLSTHG][25]
+    const float *AF11;                    // [MLSTHG] (index MC-1)
+    const int *K1W, *K2W, *K11W, *K21W;   // [2][KL], 0-based
+    const int *CONTRIB, *NCONTRIB;        // [NB][NCMAX], [NB]
+    // per sea point (PREPARE_PROPAGATION), 0-based neighbour indices, -1 = land
+    const int *KLAT_S, *KLAT_N, *KLON_W, *KLON_E;
+    const int *INDEP, *INDEP_G;
+    const float *DEPTH, *DPSN_S, *DPSN_N, *DELLA0, *DRGM, *U, *V;
+    const float *DDPHI, *DDLAM, *DUPHI, *DULAM, *DVPHI, *DVLAM;
+    const float *WN, *CGG;                // [ML][NSEA] (IPHYS = 2)
+    const int *JXO1, *JYO1, *JXO2, *JYO2, *KPM_P, *KPM_M;   // 1-based
+    float DELTH0, DELPH0, DELTHR, DELFR0;
+};
+
+struct ImplArgs {
+    float *fl3;
+    const float *u10, *udir, *roairn, *wstar;
+    float *tauw, *ustar, *z0, *phiaw, *fluxes;
+    int *mij;
+    int lcflx;
+};
+
+}  // namespace wam

@@ -1,0 +1,72 @@
+// wam_setup.hpp -- host-side set-up tables of the WAM hot path.
+//
+// These are the module tables that IMPLSCH and PROPAGS read through USE
+// statements in the reference; a Fortran host that already holds them can
+// equally pass its own.  Each builder names the reference routine it follows;
+// the floating-point operations are evaluated in the reference's order with
+// separate roundings, so the tables are bit-identical to the Fortran REAL*4
+// path wherever the elementary functions agree.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+#include "../../include/wam_gpu.h"
+
+namespace wam {
+
+struct Consts {           // wam_general_module.f90:41-85
+    float G, PI, ZPI, DEG, RAD, CIRC, RADIUS, ROAIR, RNUAIR, RNUAIRM, ROWATER, XKAPPA, XNLEV,
+        DKMAX, BETAMAX, ZALP, ALPHA, CDFAC, EMIN, EPS1, CO;
+};
+
+struct Tables {
+    wam_config cfg{};
+    int KL = 0, ML = 0;
+    Consts k{};
+    // frequency / direction grid (MAKE_FRE_DIR)
+    float INV_LOG_CO = 0, DELTH = 0, DELTR = 0, MO_TAIL = 0, MM1_TAIL = 0, MP1_TAIL = 0, MP2_TAIL = 0;
+    std::vector<float> FR, DF, DFIM, DFIMOFR, DFIM_FR, FR5, FRM5, RHOWG_DFIM, GOM, TH, COSTH, SINTH;
+    std::vector<int> KPM_m, KPM_p, MPM_m, MPM_p, JXO1, JXO2, JYO1, JYO2;   // 1-based values
+    // MIN_ENERGY / MAKE_SHALLOW_TABLES
+    int JUMAX = 300;
+    float DELU = 0;
+    std::vector<float> FLMINFR;                          // [ML][JUMAX]
+    std::vector<float> TFAK, TCGOND, TSIHKD;             // [ML][NDEPTH]
+    // DIA (NLWEIGT, INIT_SNONLIN)
+    int ISP = 0, ISM = 0, MFRSTLW = 0, MLSTHG = 0, KFRH = 0;
+    float ACL1 = 0, ACL2 = 0, CL11 = 0, CL21 = 0, DAL1 = 0, DAL2 = 0;
+    std::vector<int> IKP, IKP1, IKM, IKM1;               // index MC - MFRSTLW
+    std::vector<int> K1W, K2W, K11W, K21W;               // [2][KL], 1-based
+    std::vector<float> AF11, FKLAP, FKLAP1, FKLAM, FKLAM1, FRH, FTRF;
+    std::vector<int> INLCOEF;                            // [MLSTHG][5]
+    std::vector<float> RNLCOEF;                          // [MLSTHG][25]
+    // TAU_PHI_HF
+    float X0TAUHF = 0;
+    float WTAUHF[19] = {};
+    // LFACTOR 10-Hz extension (depends on FR(1) only)
+    int NK10 = 0;
+    std::vector<float> SIG10, DSII10;
+    // grid (PREPARE_GRID, regular) -- sea points numbered row-major from SW
+    int NSEA = 0;
+    float DELPHI = 0;
+    std::vector<float> SINPH, COSPH, DELLAM;             // [NY]
+    std::vector<int> IXLG, KXLT;                         // 1-based
+    std::vector<int> KLAT_S, KLAT_N, KLON_W, KLON_E;     // 1-based sea point, 0 = land
+    std::vector<float> DCO, DPSN_S, DPSN_N;              // DCO[NSEA]
+    // propagation per-point data (PREPARE_PROPAGATION)
+    std::vector<int> INDEP, INDEP_G;
+    std::vector<float> DDPHI, DDLAM, DUPHI, DULAM, DVPHI, DVLAM, DELLA0, DRGM;
+    // SNONLIN gather lists: per bin, contributions in the reference's order
+    int NQ = 0, NCMAX = 0;
+    std::vector<int32_t> CONTRIB;                        // [NB][NCMAX]
+    std::vector<int32_t> NCONTRIB;                       // [NB]
+};
+
+// Builds constants, fre_dir, tables, DIA, X0TAUHF and the grid.  Returns a
+// WAM_E_* code.
+int build_tables(Tables &t, const wam_config &cfg, const uint8_t *mask, std::string &err);
+// PREPARE_PROPAGATION per-point part (INDEP, gradients, INDEP_G, DELLA0, DRGM).
+int prepare_points(Tables &t, const float *depth, const float *u, const float *v, std::string &err);
+float aki(const Tables &t, float OM, float BETA);
+
+}  // namespace wam

@@ -1,0 +1,131 @@
+"""Host-side mirror of WAM's interface for the hot path.
+
+``WamGpu`` mirrors the reference's call sequence (wamodel.f90:286-451):
+``prepare_propagation`` stands for PREPARE_PROPAGATION (initmdl.f90:204,
+wamodel.f90:326), ``propags`` for PROPAGS(FL3) (wamodel.f90:332) and
+``implsch`` for IMPLSCH(FL3, U10, UDIR, TAUW, USTAR, Z0, ROAIRN, WSTAR, DEPTH,
+INDEP) (wamodel.f90:354).  Host numpy arrays go through the drop-in entry
+points; torch tensors on the GPU go through the device-resident ones.
+Errors raise ``WamError`` carrying the library's code and message, where the
+reference calls ABORT1.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+
+class WamError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"WAM error {code}: {msg}")
+        self.code = code
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data_as(ctypes.c_void_p)
+    return ctypes.c_void_p(a.data_ptr())      # torch tensor
+
+
+class WamGpu:
+    def __init__(self, cfg, mask, host_only=False):
+        self.lib = _lib.load()
+        self.cfg = dict(cfg)
+        c = _lib.WamConfig()
+        for name, _ in _lib.WamConfig._fields_:
+            setattr(c, name, cfg[name])
+        self._c = c
+        m = np.ascontiguousarray(mask, dtype=np.uint8).ravel()
+        if m.size != cfg["nx"] * cfg["ny"]:
+            raise ValueError("mask size must be nx*ny")
+        h = ctypes.c_void_p()
+        init = self.lib.wam_gpu_init_host if host_only else self.lib.wam_gpu_init
+        rc = init(ctypes.byref(h), ctypes.byref(c), _ptr(m))
+        self.ctx = h
+        self._check(rc)
+        self.kl, self.ml = cfg["kl"], cfg["ml"]
+        self.nsea = int(self.scalar("NSEA"))
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = self.lib.wam_gpu_last_error(self.ctx)
+            raise WamError(rc, msg.decode() if msg else "")
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.wam_gpu_finalize(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- set-up -------------------------------------------------------------
+    def prepare_propagation(self, depth, u=None, v=None):
+        d = np.ascontiguousarray(depth, dtype=np.float32)
+        uu = None if u is None else np.ascontiguousarray(u, dtype=np.float32)
+        vv = None if v is None else np.ascontiguousarray(v, dtype=np.float32)
+        for a in (d, uu, vv):
+            if a is not None and a.size != self.nsea:
+                raise ValueError("per-point arrays must have NSEA elements")
+        self._check(self.lib.wam_gpu_prepare_propagation(self.ctx, _ptr(d), _ptr(uu), _ptr(vv)))
+        return int(self.scalar("COUNTER")), float(self.scalar("CFL_MAX"))
+
+    def table(self, name):
+        n = self.lib.wam_gpu_get_table(self.ctx, name.encode(), None, 0)
+        if n < 0:
+            raise KeyError(name)
+        ints = name in ("IKP", "IKP1", "IKM", "IKM1", "K1W", "K2W", "K11W", "K21W", "INLCOEF", "IXLG", "KXLT",
+                        "INDEP", "KLAT", "KLON", "NCONTRIB")
+        out = np.zeros(n, np.int32 if ints else np.float32)
+        self.lib.wam_gpu_get_table(self.ctx, name.encode(), _ptr(out), n)
+        return out
+
+    def scalar(self, name):
+        return self.lib.wam_gpu_get_scalar(self.ctx, name.encode())
+
+    def set_stream(self, stream_handle):
+        self._check(self.lib.wam_gpu_set_stream(self.ctx, ctypes.c_void_p(stream_handle)))
+
+    def sync(self):
+        self._check(self.lib.wam_gpu_sync(self.ctx))
+
+    # -- drop-in (host memory) ---------------------------------------------
+    def propags(self, fl3):
+        f = np.ascontiguousarray(fl3, dtype=np.float32).copy()
+        self._check(self.lib.wam_gpu_propags(self.ctx, 1, self.nsea, self.kl, self.ml, _ptr(f)))
+        return f
+
+    def implsch(self, fl3, u10, udir, tauw, z0, depth=None, indep=None, roairn=None, wstar=None, lcflx=False):
+        n = self.nsea
+        f = np.ascontiguousarray(fl3, dtype=np.float32).copy()
+        tw = np.ascontiguousarray(tauw, dtype=np.float32).copy()
+        z = np.ascontiguousarray(z0, dtype=np.float32).copy()
+        us = np.zeros(n, np.float32)
+        ph = np.zeros(n, np.float32)
+        fx = np.zeros((6, n), np.float32)
+        ro = np.full(n, 1.225, np.float32) if roairn is None else np.ascontiguousarray(roairn, np.float32)
+        ws = np.zeros(n, np.float32) if wstar is None else np.ascontiguousarray(wstar, np.float32)
+        u = np.ascontiguousarray(u10, np.float32)
+        d = np.ascontiguousarray(udir, np.float32)
+        if f.size != n * self.kl * self.ml:
+            raise ValueError("FL3 must have NSEA*KL*ML elements")
+        self._check(self.lib.wam_gpu_implsch(self.ctx, n, self.kl, self.ml, _ptr(f), _ptr(u), _ptr(d), _ptr(tw),
+                                             _ptr(us), _ptr(z), _ptr(ro), _ptr(ws), None, None,
+                                             int(bool(lcflx)), _ptr(ph), _ptr(fx)))
+        return dict(fl3=f, tauw=tw, ustar=us, z0=z, phiaw=ph, flux=fx)
+
+    # -- device resident (torch tensors in HBM) ----------------------------
+    def propags_dev(self, d_in, d_out):
+        self._check(self.lib.wam_gpu_propags_dev(self.ctx, _ptr(d_in), _ptr(d_out)))
+
+    def implsch_dev(self, fl3, u10, udir, tauw, ustar, z0, roairn, wstar, lcflx=False, phiaw=None, fluxes=None,
+                    mij=None):
+        self._check(self.lib.wam_gpu_implsch_dev(self.ctx, _ptr(fl3), _ptr(u10), _ptr(udir), _ptr(tauw),
+                                                 _ptr(ustar), _ptr(z0), _ptr(roairn), _ptr(wstar),
+                                                 int(bool(lcflx)), _ptr(phiaw), _ptr(fluxes), _ptr(mij)))

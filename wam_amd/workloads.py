@@ -1,0 +1,131 @@
+"""Model configurations and seeded synthetic inputs.
+
+The two single-GPU workloads of the hot path:
+  * ``c1``: spherical 60x60 all-sea grid at 0.5 deg (3600 points), deep
+    water, 24 directions x 25 frequencies (f1 = 0.0418 Hz), ST6 physics
+    (IPHYS=2), uniform 20 m/s wind from 270 deg, JONSWAP initial spectra,
+    600 s steps.
+  * ``c2``: 0.1 deg spherical grid 901 x 1600 with a seeded fractal land mask
+    (30 % land, about 1.0 M sea points), 36 x 36 spectra, log-normal fractal
+    bathymetry 5-5000 m with depth and current refraction, Gaussian random
+    wind field (mean 10, sd 5 m/s), divergence-free currents up to 1 m/s,
+    300 s steps, Janssen physics (IPHYS=0).
+Any grid can be cut to a smaller sub-box for parity runs.  Coordinates are in
+the reference's M_SEC units (1 degree = 360000, wam_coordinate_module.f90:41-43).
+"""
+import math
+
+import numpy as np
+
+M_DEGREE = 360000
+
+
+def base_config(**kw):
+    cfg = dict(kl=24, ml=25, fr1=0.0418, iref=1, iphys=0, shallow=0, spherical=1, refr_d=0, refr_c=0,
+               iper=0, isnonlin=0, idelpro=600, idelt=600, ndepth=69, deptha=1.0, depthd=1.1,
+               betamax=1.20, zalp=0.008, alpha=0.006, cdfac=1.13,
+               nx=60, ny=60, amosop=-15 * M_DEGREE, amonop=int(14.5 * M_DEGREE), amowep=0,
+               amoeap=int(29.5 * M_DEGREE), xdella=M_DEGREE // 2, xdello=M_DEGREE // 2)
+    cfg.update(kw)
+    return cfg
+
+
+def grid_config(cfg, nx, ny, south_deg, west_deg, dlat_deg, dlon_deg):
+    c = dict(cfg)
+    c.update(nx=nx, ny=ny, amosop=int(round(south_deg * M_DEGREE)), amowep=int(round(west_deg * M_DEGREE)),
+             xdella=int(round(dlat_deg * M_DEGREE)), xdello=int(round(dlon_deg * M_DEGREE)))
+    c["amonop"] = c["amosop"] + (ny - 1) * c["xdella"]
+    c["amoeap"] = c["amowep"] + (nx - 1) * c["xdello"]
+    return c
+
+
+def _fractal(ny, nx, beta, rng):
+    """Gaussian random field with an isotropic k**-beta power spectrum, unit variance."""
+    ky = np.fft.fftfreq(ny)[:, None]
+    kx = np.fft.rfftfreq(nx)[None, :]
+    k = np.sqrt(kx * kx + ky * ky)
+    k[0, 0] = 1.0
+    amp = k ** (-beta / 2.0)
+    amp[0, 0] = 0.0
+    noise = rng.standard_normal((ny, nx // 2 + 1)) + 1j * rng.standard_normal((ny, nx // 2 + 1))
+    f = np.fft.irfft2(noise * amp, s=(ny, nx))
+    f = (f - f.mean()) / (f.std() + 1e-30)
+    return f
+
+
+def _smooth_field(ny, nx, scale_cells, rng):
+    """Gaussian random field with a Gaussian correlation of the given length (cells)."""
+    ky = np.fft.fftfreq(ny)[:, None]
+    kx = np.fft.rfftfreq(nx)[None, :]
+    k2 = kx * kx + ky * ky
+    amp = np.exp(-0.5 * k2 * (2 * math.pi * scale_cells) ** 2 / 2.0)
+    noise = rng.standard_normal((ny, nx // 2 + 1)) + 1j * rng.standard_normal((ny, nx // 2 + 1))
+    f = np.fft.irfft2(noise * amp, s=(ny, nx))
+    return (f - f.mean()) / (f.std() + 1e-30)
+
+
+def c1_case():
+    """3600-point deep-water ST6 case: (cfg, mask, fields)."""
+    cfg = base_config(iphys=2, kl=24, ml=25, idelpro=600, idelt=600)
+    mask = np.ones((cfg["ny"], cfg["nx"]), np.uint8)
+    n = int(mask.sum())
+    fields = dict(depth=np.full(n, 999.0, np.float32), u=None, v=None,
+                  u10=np.full(n, 20.0, np.float32), udir=np.full(n, math.pi / 2, np.float32))
+    return cfg, mask, fields
+
+
+def c2_case(ny=901, nx=1600, seed=20261203, iphys=0, kl=36, ml=36, dlat=0.1, dlon=0.1, south=-45.0, west=0.0,
+            land_frac=0.30):
+    """Production-shaped shallow/current case; sub-boxes keep the same statistics."""
+    cfg = base_config(iphys=iphys, kl=kl, ml=ml, shallow=1, refr_d=1, refr_c=1, idelpro=300, idelt=300)
+    cfg = grid_config(cfg, nx, ny, south, west, dlat, dlon)
+    rng = np.random.default_rng(seed)
+    land = _fractal(ny, nx, 3.0, rng)
+    thr = np.quantile(land, 1.0 - land_frac)
+    mask = (land < thr).astype(np.uint8)
+    sea = mask.astype(bool)
+    n = int(sea.sum())
+    # log-normal fractal bathymetry, shallower towards the coast
+    bath = _fractal(ny, nx, 3.2, rng)
+    depth2 = np.exp(np.log(800.0) + 1.4 * bath - 1.5 * np.clip(land - thr + 1.0, 0.0, None))
+    depth2 = np.clip(depth2, 5.0, 5000.0)
+    # wind speed: Gaussian random field, mean 10 and sd 5 m/s, ~500 km scale (0.1 deg ~ 11 km)
+    cells = 500e3 / (dlat * 111e3)
+    wspd = np.clip(10.0 + 5.0 * _smooth_field(ny, nx, cells / 4.0, rng), 0.5, None)
+    wdir = math.pi / 2 + 0.8 * _smooth_field(ny, nx, cells / 2.0, rng)
+    # divergence-free current from a stream function, max 1 m/s
+    psi = _smooth_field(ny, nx, cells / 8.0, rng)
+    cu = -(np.roll(psi, -1, 0) - np.roll(psi, 1, 0))
+    cv = np.roll(psi, -1, 1) - np.roll(psi, 1, 1)
+    sc = 1.0 / max(np.abs(cu).max(), np.abs(cv).max(), 1e-30)
+    fields = dict(depth=depth2[sea].astype(np.float32), u=(sc * cu)[sea].astype(np.float32),
+                  v=(sc * cv)[sea].astype(np.float32), u10=wspd[sea].astype(np.float32),
+                  udir=np.mod(wdir[sea], 2 * math.pi).astype(np.float32))
+    assert fields["depth"].shape[0] == n
+    return cfg, mask, fields
+
+
+def frequencies(cfg):
+    fr = np.empty(cfg["ml"], np.float64)
+    fr[0] = cfg["fr1"] * 1.1 ** (1 - cfg["iref"])
+    for m in range(1, cfg["ml"]):
+        fr[m] = fr[m - 1] * 1.1
+    return fr
+
+
+def jonswap_spectrum(cfg, udir, alpha=0.018, fp=0.3, gamma=3.3, sa=0.07, sb=0.09):
+    """Initial spectra F(IJ,K,M) as [ML][KL][NSEA] float32: JONSWAP E(f)
+    (wam_jonswap_module.f90:136-213) times (2/pi) cos^2 spreading about the wind
+    direction (wam_coldstart_module.f90:381-412)."""
+    g = 9.806
+    fr = frequencies(cfg)
+    kl = cfg["kl"]
+    th = (np.arange(kl) + 0.5) * (2 * math.pi / kl)
+    arg = 1.25 * (fp / fr) ** 4
+    et = alpha * g * g / (2 * math.pi) ** 4 * fr ** -5 * np.exp(-np.minimum(arg, 50.0))
+    sig = np.where(fr > fp, sb, sa)
+    et = et * gamma ** np.exp(-((fr / fp - 1.0) ** 2) / (2 * sig * sig))
+    st = (2.0 / math.pi) * np.maximum(0.0, np.cos(th[:, None] - np.asarray(udir, np.float64)[None, :])) ** 2
+    st[st < 1e-9] = 0.0
+    f = et[:, None, None] * st[None, :, :]
+    return f.astype(np.float32)
