@@ -64,6 +64,8 @@ def _lib(fp64):
         lib.orc_api_implsch.argtypes = ([ctypes.c_void_p] + [ctypes.c_void_p] * 9 +
                                         [ctypes.c_int] + [ctypes.c_void_p] * 3 + [ctypes.c_int])
         lib.orc_api_wavnu2.argtypes = [ctypes.c_void_p] * 4
+        lib.orc_api_snonlin.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_double, ctypes.c_double]
+        lib.orc_api_math.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long]
         lib.orc_api_aki.restype = ctypes.c_double
         lib.orc_api_aki.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
         _LIBS[key] = lib
@@ -119,7 +121,7 @@ class Oracle:
         return int(self.scalar("COUNTER")), float(self.scalar("CFL_MAX"))
 
     def propags(self, fl3):
-        f = np.ascontiguousarray(fl3, dtype=self.dtype)
+        f = np.array(fl3, dtype=self.dtype, copy=True).ravel()
         self.lib.orc_api_propags(self.ctx, _p(f))
         return f
 
@@ -150,5 +152,23 @@ class Oracle:
         self.lib.orc_api_wavnu2(self.ctx, _p(d), _p(wn), _p(cg))
         return wn.reshape(self.ml, self.nsea), cg.reshape(self.ml, self.nsea)
 
+    def snonlin(self, F, depth=999.0, akmean=0.0):
+        """SNONLIN alone on one spectrum F[ML][KL]; returns (SL, FL) increments."""
+        f = np.ascontiguousarray(F, dtype=self.dtype).ravel().copy()
+        sl = np.zeros_like(f)
+        fl = np.zeros_like(f)
+        self.lib.orc_api_snonlin(self.ctx, _p(f), _p(sl), _p(fl), float(depth), float(akmean))
+        return sl.reshape(self.ml, self.kl), fl.reshape(self.ml, self.kl)
+
     def aki(self, om, depth):
         return self.lib.orc_api_aki(self.ctx, float(om), float(depth))
+
+
+def shared_math(fn, x, y=None):
+    """Evaluates a wam_math.h function (see orc_api_math) on float32 arrays."""
+    lib = _lib(False)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    yy = None if y is None else np.ascontiguousarray(y, dtype=np.float32)
+    out = np.zeros_like(x)
+    lib.orc_api_math(int(fn), _p(x), _p(yy), _p(out), x.size)
+    return out

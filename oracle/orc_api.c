@@ -89,3 +89,34 @@ void orc_api_implsch(const orc_ctx *c, real *fl3, const real *u10, const real *u
 }
 void orc_api_wavnu2(const orc_ctx *c, const real *depth, real *wn, real *cgg) { orc_wavnu2_all(c, depth, wn, cgg); }
 double orc_api_aki(const orc_ctx *c, double om, double beta) { return orc_aki(c, (real)om, (real)beta); }
+
+void orc_snonlin_point(const orc_ctx *c, real *F, real *SL, real *FL, real depth, real akmean);
+void orc_api_snonlin(const orc_ctx *c, real *F, real *SL, real *FL, double depth, double akmean)
+{
+    orc_snonlin_point(c, F, SL, FL, (real)depth, (real)akmean);
+}
+
+/* The shared single-precision elementary functions (wam_math.h), for the
+   accuracy tests: fn 0 exp, 1 log, 2 log10, 3 tanh, 4 cosh, 5 sinh, 6 cos,
+   7 sin, 8 pow(x, y), 9 nint, 10 ipow(x, (int)y). */
+#include "wam_math.h"
+void orc_api_math(int fn, const float *x, const float *y, float *out, long n)
+{
+    for (long i = 0; i < n; i++) {
+        float a = x[i], b = y ? y[i] : 0.f, r = 0.f;
+        switch (fn) {
+        case 0: r = wam_expf(a); break;
+        case 1: r = wam_logf(a); break;
+        case 2: r = wam_log10f(a); break;
+        case 3: r = wam_tanhf(a); break;
+        case 4: r = wam_coshf(a); break;
+        case 5: r = wam_sinhf(a); break;
+        case 6: r = wam_cosf(a); break;
+        case 7: r = wam_sinf(a); break;
+        case 8: r = wam_powf(a, b); break;
+        case 9: r = (float)wam_nint(a); break;
+        case 10: r = wam_ipowf(a, (int)b); break;
+        }
+        out[i] = r;
+    }
+}

@@ -867,3 +867,12 @@ void orc_wavnu2_all(const orc_ctx *c, const real *depth, real *wn, real *cgg)
         for (int m = 0; m < c->ML; m++) { wn[(size_t)m * c->NSEA + ij] = WN[m]; cgg[(size_t)m * c->NSEA + ij] = CG[m]; }
     }
 }
+
+/* SNONLIN alone on one point's spectrum F[k + KL*m] (SL, FL accumulate). */
+void orc_snonlin_point(const orc_ctx *c, real *F, real *SL, real *FL, real depth, real akmean)
+{
+    orc_work w;
+    memset(&w, 0, sizeof(w));
+    w.F = F; w.SL = SL; w.FL = FL;
+    snonlin(c, &w, depth, akmean);
+}

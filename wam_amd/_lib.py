@@ -34,7 +34,7 @@ SIGNATURES = {
     "wam_gpu_init_host": (_I, [ctypes.POINTER(_P), ctypes.POINTER(WamConfig), _P]),
     "wam_gpu_prepare_propagation": (_I, [_P, _P, _P, _P]),
     "wam_gpu_propags": (_I, [_P, _I, _I, _I, _I, _P]),
-    "wam_gpu_implsch": (_I, [_P, _I, _I, _I] + [_P] * 11 + [_I, _P, _P]),
+    "wam_gpu_implsch": (_I, [_P, _I, _I, _I] + [_P] * 10 + [_I, _P, _P]),
     "wam_gpu_propags_dev": (_I, [_P, _P, _P]),
     "wam_gpu_implsch_dev": (_I, [_P] + [_P] * 8 + [_I, _P, _P, _P]),
     "wam_gpu_set_stream": (_I, [_P, _P]),
@@ -52,6 +52,13 @@ def load():
     """Returns the loaded library; raises if it has not been built."""
     global _lib
     if _lib is None:
+        # torch ships its own libamdhip64.so.7 / libhsa-runtime64.so; loading it
+        # first makes this library bind to that same runtime (same SONAME), so
+        # one process never holds two HIP runtimes competing for the device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: run `python wam_amd/build.py` (no CPU fallback exists)")
         lib = ctypes.CDLL(LIB_PATH)

@@ -53,6 +53,14 @@ def _fractal(ny, nx, beta, rng):
     return f
 
 
+def _smooth_blur(x, cells):
+    """Gaussian blur of a 2-D field (periodic FFT convolution)."""
+    ky = np.fft.fftfreq(x.shape[0])[:, None]
+    kx = np.fft.rfftfreq(x.shape[1])[None, :]
+    g = np.exp(-2.0 * (math.pi * cells) ** 2 * (kx * kx + ky * ky))
+    return np.fft.irfft2(np.fft.rfft2(x) * g, s=x.shape)
+
+
 def _smooth_field(ny, nx, scale_cells, rng):
     """Gaussian random field with a Gaussian correlation of the given length (cells)."""
     ky = np.fft.fftfreq(ny)[:, None]
@@ -85,9 +93,13 @@ def c2_case(ny=901, nx=1600, seed=20261203, iphys=0, kl=36, ml=36, dlat=0.1, dlo
     mask = (land < thr).astype(np.uint8)
     sea = mask.astype(bool)
     n = int(sea.sum())
-    # log-normal fractal bathymetry, shallower towards the coast
-    bath = _fractal(ny, nx, 3.2, rng)
-    depth2 = np.exp(np.log(800.0) + 1.4 * bath - 1.5 * np.clip(land - thr + 1.0, 0.0, None))
+    # log-normal bathymetry: a smooth basin-scale field with weak fractal
+    # detail, shoaling log-linearly over ~200 km shelves towards the coast
+    from scipy.ndimage import distance_transform_edt
+    dist = distance_transform_edt(sea)
+    shelf = np.clip(dist / 18.0, 0.0, 1.0)
+    logd = np.log(2500.0) + 0.5 * _smooth_field(ny, nx, 25.0, rng) + 0.05 * _fractal(ny, nx, 4.0, rng)
+    depth2 = np.exp(np.log(5.0) + (logd - np.log(5.0)) * shelf)
     depth2 = np.clip(depth2, 5.0, 5000.0)
     # wind speed: Gaussian random field, mean 10 and sd 5 m/s, ~500 km scale (0.1 deg ~ 11 km)
     cells = 500e3 / (dlat * 111e3)
