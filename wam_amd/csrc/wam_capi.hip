@@ -19,7 +19,7 @@ __global__ void rescale_kernel(float *coef, size_t nb, int nplanes, float rc);
 __global__ void propags_nocurr_kernel(DevTables t, const float *coef, const float *fin, float *fout);
 __global__ void propags_curr_kernel(DevTables t, const float *coef, const float *fin, float *fout);
 __global__ void wavnu2_kernel(DevTables t, float *wn, float *cgg);
-hipError_t launch_implsch(const DevTables &t, const ImplArgs &a, int nblocks, hipStream_t st);
+hipError_t launch_implsch(const DevTables &t, const ImplArgs &a, int ncu, hipStream_t st);
 }  // namespace wam
 
 using namespace wam;
@@ -40,7 +40,7 @@ struct wam_ctx {
     float *stage = nullptr;               // host drop-in staging: FL3 + per-point fields
     size_t stage_n = 0;
     bool prepared = false;
-    int nblocks_implsch = 0;
+    int ncu = 256;                 // compute units of the device
     std::string err;
 };
 
@@ -82,6 +82,7 @@ static int upload_static(wam_ctx *ctx)
     DevTables &d = ctx->dt;
     d.KL = t.KL; d.ML = t.ML; d.NSEA = t.NSEA; d.NDEPTH = t.cfg.ndepth; d.JUMAX = t.JUMAX;
     d.MFRSTLW = t.MFRSTLW; d.MLSTHG = t.MLSTHG; d.NQ = t.NQ; d.NCMAX = t.NCMAX; d.NK10 = t.NK10;
+    d.CHUNK = t.CHUNK; d.NCHUNK = t.NCHUNK;
     d.iphys = t.cfg.iphys; d.shallow = t.cfg.shallow; d.refr_d = t.cfg.refr_d; d.refr_c = t.cfg.refr_c;
     const Consts &k = t.k;
     d.G = k.G; d.ZPI = k.ZPI; d.ROAIR = k.ROAIR; d.RNUAIRM = k.RNUAIRM; d.ROWATER = k.ROWATER;
@@ -96,7 +97,6 @@ static int upload_static(wam_ctx *ctx)
     d.CONSD = ((-1.33f) * (float)std::pow((double)k.ZPI, 9.0)) / g4;
     d.CONSS = (-1.33f) * k.ZPI;
     d.C5G = 5.0E-07f * k.G;
-    for (int j = 0; j < 19; j++) d.WTAUHF[j] = t.WTAUHF[j];
     const float delpro = (float)t.cfg.idelpro;
     d.DELTH0 = (0.5f * delpro) / t.DELTR;
     d.DELPH0 = (0.5f * delpro) / t.DELPHI;
@@ -109,6 +109,8 @@ static int upload_static(wam_ctx *ctx)
     UP(o, t.GOM, GOM); UP(o, t.TH, TH); UP(o, t.COSTH, COSTH); UP(o, t.SINTH, SINTH);
     UP(o, t.FLMINFR, FLMINFR); UP(o, t.TFAK, TFAK); UP(o, t.TCGOND, TCGOND); UP(o, t.TSIHKD, TSIHKD);
     UP(o, t.SIG10, SIG10); UP(o, t.DSII10, DSII10);
+    std::vector<float> wtauhf(t.WTAUHF, t.WTAUHF + 19);
+    UP(o, wtauhf, WTAUHF);
     UP(o, t.INLCOEF, INLCOEF); UP(o, t.RNLCOEF, RNLCOEF);
     std::vector<float> af(t.MLSTHG);
     for (int mc = 1; mc <= t.MLSTHG; mc++) af[mc - 1] = t.AF11[mc - t.MFRSTLW];
@@ -116,7 +118,7 @@ static int upload_static(wam_ctx *ctx)
     auto zero_based = [](const std::vector<int> &v) { std::vector<int> r(v); for (auto &x : r) x -= 1; return r; };
     std::vector<int> k1 = zero_based(t.K1W), k2 = zero_based(t.K2W), k11 = zero_based(t.K11W), k21 = zero_based(t.K21W);
     UP(o, k1, K1W); UP(o, k2, K2W); UP(o, k11, K11W); UP(o, k21, K21W);
-    UP(o, t.CONTRIB, CONTRIB); UP(o, t.NCONTRIB, NCONTRIB);
+    UP(o, t.CONTRIB_T, CONTRIB_T); UP(o, t.CRANGE, CRANGE); UP(o, t.WSL_T, WSL_T); UP(o, t.WFL_T, WFL_T);
     std::vector<int> ks = zero_based(t.KLAT_S), kn = zero_based(t.KLAT_N), kw = zero_based(t.KLON_W), ke = zero_based(t.KLON_E);
     UP(o, ks, KLAT_S); UP(o, kn, KLAT_N); UP(o, kw, KLON_W); UP(o, ke, KLON_E);
     UP(o, t.DPSN_S, DPSN_S); UP(o, t.DPSN_N, DPSN_N);
@@ -151,8 +153,7 @@ int wam_gpu_init(wam_ctx **out, const wam_config *cfg, const uint8_t *sea_mask)
         int ncu = 256;
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
-        const int want = ncu * 2;
-        ctx->nblocks_implsch = want < ctx->tab.NSEA ? want : ctx->tab.NSEA;
+        ctx->ncu = ncu;
     }
     *out = ctx;
     return rc;
@@ -272,7 +273,7 @@ int wam_gpu_implsch_dev(wam_ctx *ctx, float *d_fl3, const float *d_u10, const fl
     a.fl3 = d_fl3; a.u10 = d_u10; a.udir = d_udir; a.tauw = d_tauw; a.ustar = d_ustar; a.z0 = d_z0;
     a.roairn = d_roairn; a.wstar = d_wstar; a.lcflx = lcflx ? 1 : 0; a.phiaw = d_phiaw; a.fluxes = d_fluxes;
     a.mij = d_mij;
-    HIPCHK(launch_implsch(ctx->dt, a, ctx->nblocks_implsch, ctx->stream));
+    HIPCHK(launch_implsch(ctx->dt, a, ctx->ncu, ctx->stream));
     return WAM_OK;
 }
 

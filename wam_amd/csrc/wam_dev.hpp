@@ -20,18 +20,23 @@ struct DevTables {
     float G, ZPI, ROAIR, RNUAIRM, ROWATER, XKAPPA, XNLEV, BETAMAX, ZALP, ALPHA, CDFAC, EMIN, EPS1, CO;
     float DELTH, INV_LOG_CO, MO_TAIL, MM1_TAIL, MP1_TAIL, DELU, DAL1, DAL2, X0TAUHF, DELT;
     float ZPI4, CONSD, CONSS, C5G;        // folded constants of STRESSO, SDISSIP and the limiter
-    float WTAUHF[19];
     // per frequency / direction
     const float *FR, *DF, *DFIM, *DFIMOFR, *DFIM_FR, *FR5, *FRM5, *RHOWG_DFIM, *GOM, *TH, *COSTH, *SINTH;
     const float *FLMINFR;                 // [ML][JUMAX]
     const float *TFAK, *TCGOND, *TSIHKD;  // [ML][NDEPTH]
     const float *SIG10, *DSII10;          // [NK10]
+    const float *WTAUHF;                  // [19] Simpson weights of TAU_PHI_HF
     // SNONLIN
     const int *INLCOEF;                   // [MLSTHG][5]
     const float *RNLCOEF;                 // [MLSTHG][25]
     const float *AF11;                    // [MLSTHG] (index MC-1)
     const int *K1W, *K2W, *K11W, *K21W;   // [2][KL], 0-based
-    const int *CONTRIB, *NCONTRIB;        // [NB][NCMAX], [NB]
+    // gather lists, [j][NB]: packed quadruplet index | update type << 16, the
+    // SL and FL weight of the update, and per chunk of CHUNK frequencies the
+    // first list position of each bin ([NCHUNK+1][NB])
+    int CHUNK, NCHUNK;
+    const int *CONTRIB_T, *CRANGE;
+    const float *WSL_T, *WFL_T;
     // per sea point (PREPARE_PROPAGATION), 0-based neighbour indices, -1 = land
     const int *KLAT_S, *KLAT_N, *KLON_W, *KLON_E;
     const int *INDEP, *INDEP_G;

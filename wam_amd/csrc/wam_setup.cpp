@@ -367,6 +367,34 @@ static int snonlin_lists(Tables &t, std::string &err)
         t.NCONTRIB[b] = (int)lists[b].size();
         for (size_t j = 0; j < lists[b].size(); j++) t.CONTRIB[(size_t)b * nc + j] = lists[b][j];
     }
+    // RNLCOEF slots of the SL and FL weights of update types 1..8; the centre
+    // update (type 0) is -(2*AD) = AD*(-2), exact in binary arithmetic
+    static const int wsl[9] = {0, 20, 19, 17, 18, 8, 7, 5, 6};
+    static const int wfl[9] = {0, 23, 24, 21, 22, 11, 12, 9, 10};
+    t.NCHUNK = (t.MLSTHG + t.CHUNK - 1) / t.CHUNK;
+    t.CONTRIB_T.assign((size_t)nc * NB, 0);
+    t.WSL_T.assign((size_t)nc * NB, 0.f);
+    t.WFL_T.assign((size_t)nc * NB, 0.f);
+    t.CRANGE.assign((size_t)(t.NCHUNK + 1) * NB, 0);
+    for (int b = 0; b < NB; b++) {
+        const int n = t.NCONTRIB[b];
+        int prev_mc = 0;
+        for (int j = 0; j < n; j++) {
+            const int e = lists[b][j], q = e & 0xffff, ty = e >> 16, mc = q / (2 * KL) + 1;
+            if (mc < prev_mc) { err = "SNONLIN list not ordered by frequency"; return WAM_E_ARG; }
+            prev_mc = mc;
+            t.CONTRIB_T[(size_t)j * NB + b] = e;
+            const float *rc = &t.RNLCOEF[(size_t)(mc - 1) * 25];
+            t.WSL_T[(size_t)j * NB + b] = ty == 0 ? -2.f : rc[wsl[ty]];
+            t.WFL_T[(size_t)j * NB + b] = ty == 0 ? -2.f : rc[wfl[ty]];
+        }
+        for (int c = 0; c <= t.NCHUNK; c++) {
+            const int c0 = 1 + c * t.CHUNK;
+            int j = 0;
+            while (j < n && (lists[b][j] & 0xffff) / (2 * KL) + 1 < c0) j++;
+            t.CRANGE[(size_t)c * NB + b] = j;
+        }
+    }
     return WAM_OK;
 }
 

@@ -60,6 +60,13 @@ struct Tables {
     int NQ = 0, NCMAX = 0;
     std::vector<int32_t> CONTRIB;                        // [NB][NCMAX]
     std::vector<int32_t> NCONTRIB;                       // [NB]
+    // device form: lists transposed to [j][NB] with the SL / FL weight of each
+    // update, and per-chunk start indices for a gather done CHUNK frequencies
+    // of quadruplets at a time
+    int CHUNK = 8, NCHUNK = 0;
+    std::vector<int32_t> CONTRIB_T;                      // [NCMAX][NB]
+    std::vector<float> WSL_T, WFL_T;                     // [NCMAX][NB]
+    std::vector<int32_t> CRANGE;                         // [NCHUNK+1][NB]
 };
 
 // Builds constants, fre_dir, tables, DIA, X0TAUHF and the grid.  Returns a
