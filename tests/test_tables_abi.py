@@ -70,6 +70,43 @@ def test_snonlin_gather_lists_cover_every_scatter():
     assert (interior == 18).all()
 
 
+@pytest.mark.parametrize("kl,ml", [(36, 36), (24, 25), (12, 25)])
+def test_chunked_gather_lists_replay_the_scatter_order(kl, ml):
+    """The banded per-chunk lists the kernel reads, decoded and concatenated
+    chunk by chunk, give back every bin's ordered contribution list."""
+    cfg = W.base_config(kl=kl, ml=ml, nx=4, ny=4)
+    p = WamGpu(cfg, np.ones((4, 4), np.uint8), host_only=True)
+    nb = kl * ml
+    n = np.asarray(p.table("NCONTRIB"))
+    ncmax = int(p.scalar("NCMAX"))
+    lists = np.asarray(p.table("CONTRIB")).view(np.int32).reshape(nb, ncmax)
+    ch = np.asarray(p.table("SN_CH")).view(np.int32).reshape(-1, 5)
+    e = np.asarray(p.table("SN_E")).view(np.int32)
+    ww = np.asarray(p.table("SN_WW")).view(np.int32)
+    chunk = int(p.scalar("CHUNK"))
+    assert 2 * kl * chunk <= 511          # quadruplet slot 511 stays free for the padding entry
+    got = [[] for _ in range(nb)]
+    for c, (blo, bhi, w, off, wwoff) in enumerate(ch):
+        nbc = bhi - blo
+        assert w % 6 == 0
+        for b in range(blo, bhi):
+            ended = False
+            for j in range(w):
+                v = int(e[off + j * nbc + (b - blo)])
+                if v == 0x01FF:
+                    ended = True
+                    continue
+                assert not ended, "entry after the padding"
+                assert j < ww[wwoff + (b - blo) // 64], "wavefront list length too short"
+                ql, wi = v & 511, v >> 9
+                mcl, ty = divmod(wi, 9)
+                assert ql // (2 * kl) == mcl and ql < 511
+                q = c * chunk * 2 * kl + ql
+                got[b].append(q | (ty << 16))
+    for b in range(nb):
+        assert got[b] == list(lists[b, :n[b]]), b
+
+
 def test_bad_options_raise():
     cfg = W.base_config(iphys=1)
     with pytest.raises(WamError) as e:

@@ -76,6 +76,8 @@ static void free_list(std::vector<void *> &v)
         if (r_) return r_;                                                         \
     } while (0)
 
+static_assert(SN_GROUP == SN_GROUP_DEV && SN_NONE == SN_NONE_DEV, "host and device SNONLIN list layouts differ");
+
 static int upload_static(wam_ctx *ctx)
 {
     Tables &t = ctx->tab;
@@ -118,7 +120,7 @@ static int upload_static(wam_ctx *ctx)
     auto zero_based = [](const std::vector<int> &v) { std::vector<int> r(v); for (auto &x : r) x -= 1; return r; };
     std::vector<int> k1 = zero_based(t.K1W), k2 = zero_based(t.K2W), k11 = zero_based(t.K11W), k21 = zero_based(t.K21W);
     UP(o, k1, K1W); UP(o, k2, K2W); UP(o, k11, K11W); UP(o, k21, K21W);
-    UP(o, t.CONTRIB_T, CONTRIB_T); UP(o, t.CRANGE, CRANGE); UP(o, t.WSL_T, WSL_T); UP(o, t.WFL_T, WFL_T);
+    UP(o, t.SN_E, SN_E); UP(o, t.SN_CH, SN_CH); UP(o, t.SN_WW, SN_WW);
     std::vector<int> ks = zero_based(t.KLAT_S), kn = zero_based(t.KLAT_N), kw = zero_based(t.KLON_W), ke = zero_based(t.KLON_E);
     UP(o, ks, KLAT_S); UP(o, kn, KLAT_N); UP(o, kw, KLON_W); UP(o, ke, KLON_E);
     UP(o, t.DPSN_S, DPSN_S); UP(o, t.DPSN_N, DPSN_N);
@@ -407,6 +409,10 @@ int64_t wam_gpu_get_table(wam_ctx *ctx, const char *name, void *out, int64_t n)
     if (s == "DCO") return put(t.DCO);
     if (s == "INDEP") return put(t.INDEP);
     if (s == "NCONTRIB") return put(t.NCONTRIB);
+    if (s == "CONTRIB") return put(t.CONTRIB);
+    if (s == "SN_CH") return put(t.SN_CH);
+    if (s == "SN_WW") return put(t.SN_WW);
+    if (s == "SN_E") { std::vector<int> v(t.SN_E.begin(), t.SN_E.end()); return put(v); }
     if (s == "KLAT" || s == "KLON" || s == "DPSN") {
         // reference layout: KLAT(IJ,IC,ICL) = [ICL][IC][IJ]; KLON(IJ,IC); DPSN(IJ,IC)
         std::vector<int> vi; std::vector<float> vf;
@@ -468,6 +474,8 @@ double wam_gpu_get_scalar(wam_ctx *ctx, const char *name)
     if (s == "MP1_TAIL") return t.MP1_TAIL;
     if (s == "NQ") return t.NQ;
     if (s == "NCMAX") return t.NCMAX;
+    if (s == "CHUNK") return t.CHUNK;
+    if (s == "NCHUNK") return t.NCHUNK;
     if (s == "NK10") return t.NK10;
     if (s == "ALPHA") return t.k.ALPHA;
     if (s == "NCOEF") return ctx->ncoef;

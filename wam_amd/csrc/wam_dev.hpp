@@ -5,6 +5,9 @@
 
 namespace wam {
 
+constexpr int SN_GROUP_DEV = 6;   // SNONLIN list entries a thread loads together
+constexpr unsigned SN_NONE_DEV = 0x01FF;   // padding entry: zero quadruplet slot 511, weight -2
+
 // Coefficient planes of the propagation weights, [plane][M][K][NSEA]:
 // COEF_SUM, COEF_LAT(.,1,1), COEF_LAT(.,2,1), COEF_LON(.,1), COEF_LON(.,2),
 // COEF_THETA(.,1), COEF_THETA(.,2), COEF_SIGD(.,1), COEF_SIGD(.,2)
@@ -31,12 +34,14 @@ struct DevTables {
     const float *RNLCOEF;                 // [MLSTHG][25]
     const float *AF11;                    // [MLSTHG] (index MC-1)
     const int *K1W, *K2W, *K11W, *K21W;   // [2][KL], 0-based
-    // gather lists, [j][NB]: packed quadruplet index | update type << 16, the
-    // SL and FL weight of the update, and per chunk of CHUNK frequencies the
-    // first list position of each bin ([NCHUNK+1][NB])
+    // SNONLIN gather lists by chunks of CHUNK frequencies of quadruplets:
+    // per chunk the first and last+1 bin it feeds, the list width (a multiple
+    // of SN_GROUP_DEV), the offset into SN_E and the offset into SN_WW (list
+    // width per run of 64 bins); entries ql | (MC-offset*9 + update type) << 9,
+    // padded with SN_NONE_DEV
     int CHUNK, NCHUNK;
-    const int *CONTRIB_T, *CRANGE;
-    const float *WSL_T, *WFL_T;
+    const uint16_t *SN_E;
+    const int *SN_CH, *SN_WW;
     // per sea point (PREPARE_PROPAGATION), 0-based neighbour indices, -1 = land
     const int *KLAT_S, *KLAT_N, *KLON_W, *KLON_E;
     const int *INDEP, *INDEP_G;

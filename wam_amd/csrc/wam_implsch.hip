@@ -23,7 +23,6 @@ namespace wam {
 
 constexpr int NT = 256;          // threads per workgroup
 constexpr int MAXF = 64;         // frequencies and directions <= 64
-constexpr int MAXF10 = 128;      // LFACTOR's 10-Hz grid
 
 struct PointScalars {
     float u10, udir, tauw, z0, roairn, wstar, depth;
@@ -225,15 +224,46 @@ __device__ __forceinline__ void sinput(const DevTables &t, const Smem &s, PointS
     __syncthreads();
 }
 
+// value of v in lane i of the wavefront (i uniform)
+__device__ __forceinline__ float lane_at(float v, int i)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), i));
+}
+
+// Two ordered sums at once: sa = a_0 + a_1 + ... + a_{n-1} and sb likewise,
+// added in this order (1 <= n <= 128), where a_i is a0 of lane i for i < 64
+// and a1 of lane i - 64 above (b the same).  The first 64 terms are unrolled
+// without branches; the whole wavefront must be active.
+__device__ __forceinline__ void lane_sum2x(float a0, float a1, float b0, float b1, int n, float &sa, float &sb)
+{
+    float x = lane_at(a0, 0), y = lane_at(b0, 0);
+    const int n0 = n < 64 ? n : 64;
+    for (int i0 = 0; i0 < n0; i0 += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int i = i0 + u;
+            if (i == 0) continue;
+            const float xi = x + lane_at(a0, i), yi = y + lane_at(b0, i);
+            x = i < n0 ? xi : x;
+            y = i < n0 ? yi : y;
+        }
+    }
+    for (int i = 64; i < n; i++) {
+        x = x + lane_at(a1, i - 64);
+        y = y + lane_at(b1, i - 64);
+    }
+    sa = x;
+    sb = y;
+}
+
 // SINPUT_ST6 (:4704-4925) with LFACTOR (:4465-4700) and TAUWINDS (:5555-5587).
-// pf: per-frequency work (>= 6*MAXF + 6*MAXF10 floats).
+// pf: per-frequency work (>= 5*MAXF floats).
 __device__ __forceinline__ void sinput_st6(const DevTables &t, const Smem &s, PointScalars &P, float *pf, const float *wn,
                            const float *cg)
 {
     const int KL = t.KL, NK = t.ML, NB = KL * NK, NK10 = t.NK10, tid = threadIdx.x;
     float *sig = pf, *cinv = pf + MAXF, *sqrtbn = pf + 2 * MAXF;
-    float *sd = pf + 4 * MAXF, *sx = sd + MAXF10, *sy = sx + MAXF10, *cinv10 = sy + MAXF10, *lf10 = cinv10 + MAXF10,
-          *ucinv = lf10 + MAXF10;
+    float *lf10 = pf + 4 * MAXF;
     float *D = s.SCR, *W2 = s.SCR + NB, *SDS = s.SCR + 2 * NB;
     const float ZPI = t.ZPI;
     if (tid < NK) {
@@ -274,88 +304,83 @@ __device__ __forceinline__ void sinput_st6(const DevTables &t, const Smem &s, Po
         SDS[b] = ((d * a) * sig[m]) / cg[m];
     }
     __syncthreads();
-    // LFACTOR: spectral densities of the input (positive parts for the components)
-    if (tid < NK) {
-        const int m = tid;
-        const float *v = SDS + KL * m;
-        float a = 0.f, bx = 0.f, by = 0.f;
-        for (int k = 0; k < KL; k++) {
-            const float vp = wam_maxf(0.f, v[k]);
-            if (k == 0) { a = v[k]; bx = vp * s.costh[k]; by = vp * s.sinth[k]; }
-            else { a = a + v[k]; bx = bx + vp * s.costh[k]; by = by + vp * s.sinth[k]; }
+    // LFACTOR with TAUWINDS on wave 0 alone: lane l holds the 10-Hz grid
+    // points i = l and i = l + 64, and the ordered sums over i read the lanes
+    // in turn, so the stress iteration needs no workgroup barrier.
+    if (tid < 64) {
+        const int l = tid;
+        float sxm = 0.f, sym = 0.f;            // input spectral density components at frequency l
+        if (l < NK) {
+            const float *v = SDS + KL * l;
+            float bx = 0.f, by = 0.f;
+            for (int k = 0; k < KL; k++) {
+                const float vp = wam_maxf(0.f, v[k]);
+                if (k == 0) { bx = vp * s.costh[k]; by = vp * s.sinth[k]; }
+                else { bx = bx + vp * s.costh[k]; by = by + vp * s.sinth[k]; }
+            }
+            sxm = bx * t.DELTH; sym = by * t.DELTH;
         }
-        sd[m] = a * t.DELTH; sx[m] = bx * t.DELTH; sy[m] = by * t.DELTH;
-    }
-    __syncthreads();
-    const float GRW = t.G * t.ROWATER;
-    for (int i = tid; i < NK10; i += NT) {
-        const float ci = i < NK ? cinv[i] : s.sig10[i] * 0.101978f;
-        cinv10[i] = ci;
-        lf10[i] = 1.0f;
-        ucinv[i] = 1.0f - (32.f * P.ust) * ci;
-        if (i >= NK) {
-            const float r2 = wam_ipowf(s.sig10[NK - 1] / s.sig10[i], 2);
-            sd[i] = sd[NK - 1] * r2; sx[i] = sx[NK - 1] * r2; sy[i] = sy[NK - 1] * r2;
+        const float sxl = lane_at(sxm, NK - 1), syl = lane_at(sym, NK - 1);
+        const float ust = P.ust;
+        float ci[2], sxv[2], syv[2], ds[2], uc[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int i = l + 64 * h;
+            ci[h] = 0.f; sxv[h] = 0.f; syv[h] = 0.f; ds[h] = 0.f;
+            if (i < NK) {
+                ci[h] = cinv[i]; sxv[h] = sxm; syv[h] = sym;
+            } else if (i < NK10) {
+                ci[h] = s.sig10[i] * 0.101978f;
+                const float r2 = wam_ipowf(s.sig10[NK - 1] / s.sig10[i], 2);
+                sxv[h] = sxl * r2; syv[h] = syl * r2;
+            }
+            if (i < NK10) ds[h] = s.dsii10[i];
+            uc[h] = 1.0f - (32.f * ust) * ci[h];
         }
-    }
-    __syncthreads();
-    if (tid == 0) {
+        const float GRW = t.G * t.ROWATER;
         const float u10 = P.u10;
-        const float tau_tot = (P.ust * P.ust) * P.roairn;
+        const float tau_tot = (ust * ust) * P.roairn;
         float tau_vis = (wam_maxf(0.0f, -5.0E-5f * u10 + 1.1E-3f) * (u10 * u10)) * P.roairn;
         tau_vis = wam_minf(0.95f * tau_tot, tau_vis);
-        P.tauvx = tau_vis * wam_cosf(P.udir);
-        P.tauvy = tau_vis * wam_sinf(P.udir);
-        float twx = 0.f, twy = 0.f;
-        for (int i = 0; i < NK10; i++) {
-            const float vx = (sx[i] * cinv10[i]) * s.dsii10[i], vy = (sy[i] * cinv10[i]) * s.dsii10[i];
-            twx = i == 0 ? vx : twx + vx; twy = i == 0 ? vy : twy + vy;
-        }
-        twx = GRW * twx; twy = GRW * twy;
-        const float taux = P.tauvx + twx, tauy = P.tauvy + twy;
-        const float tau = sqrtf(taux * taux + tauy * tauy);
-        const float err = (tau - tau_tot) / tau_tot;
-        P.twx = twx; P.twy = twy; P.tau_tot = tau_tot;
-        P.done = !(tau > tau_tot);
-        P.overshot = 0;
-        P.rtau = err / 90.f;
-        P.drtau = 2.0f;
-        P.sign_new = err >= 0.f ? 1 : -1;
-    }
-    __syncthreads();
-    for (int it = 1; it <= 80 && !P.done; it++) {
-        const float rtau = P.rtau;
-        for (int i = tid; i < NK10; i += NT) lf10[i] = wam_minf(1.0f, wam_expf(ucinv[i] * rtau));
-        __syncthreads();
-        // the x and y stress sums on two waves in parallel, combined by thread 0
-        if (tid == 0 || tid == 64) {
-            const float *sv = tid == 0 ? sx : sy;
-            float tw = 0.f;
-            for (int i = 0; i < NK10; i++) {
-                const float v = ((sv[i] * lf10[i]) * cinv10[i]) * s.dsii10[i];
-                tw = i == 0 ? v : tw + v;
+        const float tauvx = tau_vis * wam_cosf(P.udir), tauvy = tau_vis * wam_sinf(P.udir);
+        float twx, twy;
+        lane_sum2x((sxv[0] * ci[0]) * ds[0], (sxv[1] * ci[1]) * ds[1], (syv[0] * ci[0]) * ds[0],
+                   (syv[1] * ci[1]) * ds[1], NK10, twx, twy);
+        twx = GRW * twx;
+        twy = GRW * twy;
+        float taux = tauvx + twx, tauy = tauvy + twy;
+        float tau = sqrtf(taux * taux + tauy * tauy);
+        float err = (tau - tau_tot) / tau_tot;
+        float lf0 = 1.0f, lf1 = 1.0f;
+        if (tau > tau_tot) {
+            bool overshot = false;
+            float rtau = err / 90.f, drtau = 2.0f;
+            int sign_new = err >= 0.f ? 1 : -1;
+            for (int it = 1; it <= 80; it++) {
+                lf0 = wam_minf(1.0f, wam_expf(uc[0] * rtau));
+                lf1 = wam_minf(1.0f, wam_expf(uc[1] * rtau));
+                lane_sum2x(((sxv[0] * lf0) * ci[0]) * ds[0], ((sxv[1] * lf1) * ci[1]) * ds[1],
+                           ((syv[0] * lf0) * ci[0]) * ds[0], ((syv[1] * lf1) * ci[1]) * ds[1], NK10, twx, twy);
+                twx = GRW * twx;
+                twy = GRW * twy;
+                taux = tauvx + twx; tauy = tauvy + twy;
+                tau = sqrtf(taux * taux + tauy * tauy);
+                err = (tau - tau_tot) / tau_tot;
+                const int sign_old = sign_new;
+                sign_new = err >= 0.f ? 1 : -1;
+                if (sign_new != sign_old) overshot = true;
+                if (overshot) drtau = wam_maxf(0.5f * (1.0f + drtau), 1.00010f);
+                rtau = sign_new > 0 ? rtau * drtau : rtau * (1.f / drtau);
+                if (wam_fabsf(err) < 1.54E-4f) break;
             }
-            if (tid == 0) P.twx = GRW * tw; else P.twy = GRW * tw;
         }
-        __syncthreads();
-        if (tid == 0) {
-            const float taux = P.tauvx + P.twx, tauy = P.tauvy + P.twy;
-            const float tau = sqrtf(taux * taux + tauy * tauy);
-            const float err = (tau - P.tau_tot) / P.tau_tot;
-            const int sold = P.sign_new;
-            P.sign_new = err >= 0.f ? 1 : -1;
-            if (P.sign_new != sold) P.overshot = 1;
-            if (P.overshot) P.drtau = wam_maxf(0.5f * (1.0f + P.drtau), 1.00010f);
-            P.rtau = P.sign_new > 0 ? P.rtau * P.drtau : P.rtau * (1.f / P.drtau);
-            if (wam_fabsf(err) < 1.54E-4f) P.done = 1;
+        if (l < NK) lf10[l] = lf0;
+        float ls, ls_unused;
+        lane_sum2x(lf0, lf1, lf0, lf1, NK, ls, ls_unused);
+        if (l == 0) {
+            P.apply = ls < (float)NK;
+            P.tauw = sqrtf(twx * twx + twy * twy);
         }
-        __syncthreads();
-    }
-    if (tid == 0) {
-        float ls = lf10[0];
-        for (int i = 1; i < NK; i++) ls = ls + lf10[i];
-        P.apply = ls < (float)NK;
-        P.tauw = sqrtf(P.twx * P.twx + P.twy * P.twy);
     }
     __syncthreads();
     const bool apply = P.apply != 0;
@@ -520,24 +545,39 @@ __device__ __forceinline__ void imphftail(const DevTables &t, const Smem &s, int
     }
 }
 
-// SNONLIN (:2742-3047) in chunks of CHUNK frequencies: quadruplet terms in
-// parallel, then the ordered gather into SL and FL.
-__device__ __forceinline__ void snonlin(const DevTables &t, const Smem &s, float ftemp_scale)
+// RNLCOEF slots (0-based) of the SL and FL weights of DIA update types 1..8
+// (type 0, the centre bin, has weight -2 for both), and which derivative the
+// FL update takes: 0 = DD, 1 = DM (types 1-4), 2 = DP (types 5-8).
+__constant__ int SLOT_SL[9] = {0, 20, 19, 17, 18, 8, 7, 5, 6};
+__constant__ int SLOT_FL[9] = {0, 23, 24, 21, 22, 11, 12, 9, 10};
+
+// SNONLIN (:2742-3047) in chunks of CHUNK frequencies.  The quadruplet terms
+// of a chunk are formed in parallel, as float4 {AD, DD, DM, DP} per
+// quadruplet (slot 511 is kept zero for the padding entries), with a table
+// {SL weight, FL weight, derivative} per (MC, update type); then every bin
+// the chunk feeds adds its updates in the reference's order.  Adding
+// AD*(-2) is the reference's -(2*AD) exactly.  qslot[i] = MC-offset | r << 8
+// of the quadruplet slot threadIdx.x + i*NT within a chunk (r = KH*KL + K).
+__device__ __forceinline__ void snonlin(const DevTables &t, const Smem &s, float ftemp_scale, const int (&qslot)[2])
 {
-    const int KL = t.KL, NB = KL * t.ML, tid = threadIdx.x;
-    const int nqc = 2 * KL * t.CHUNK;
-    float *QAD = s.SCR, *QDD = s.SCR + nqc, *QDP = s.SCR + 2 * nqc, *QDM = s.SCR + 3 * nqc;
+    constexpr int G = SN_GROUP_DEV;
+    const int KL = t.KL, tid = threadIdx.x, KL2 = 2 * KL;
+    float4 *QV = reinterpret_cast<float4 *>(s.SCR);
+    float4 *WT = QV + 512;
     const float *F = s.F;
-    const int *K1 = s.kw, *K2 = s.kw + 2 * KL, *K11 = s.kw + 4 * KL, *K21 = s.kw + 6 * KL;
+    const int *K1 = s.kw, *K2 = s.kw + KL2, *K11 = s.kw + 2 * KL2, *K21 = s.kw + 3 * KL2;
+    const int lane0 = tid & ~63;
     for (int c = 0; c < t.NCHUNK; c++) {
         const int mc0 = c * t.CHUNK;                  // 0-based first MC of the chunk
-        const int nq = min(t.CHUNK, t.MLSTHG - mc0) * 2 * KL;
-        for (int ql = tid; ql < nq; ql += NT) {
-            const int q = mc0 * 2 * KL + ql;
-            const int mcm1 = q / (2 * KL), r = q - mcm1 * 2 * KL;
+        const int nmc = min(t.CHUNK, t.MLSTHG - mc0);
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            const int mcl = qslot[i] & 255, r = qslot[i] >> 8;
+            if (mcl >= nmc) continue;
+            const int ql = tid + i * NT, mcm1 = mc0 + mcl;
             const int *ic = s.inl + mcm1 * 5;
             const float *rc = s.rnl + mcm1 * 25;
-            const int k1 = K1[r], k2 = K2[r], k11 = K11[r], k21 = K21[r], k = r % KL;
+            const int k1 = K1[r], k2 = K2[r], k11 = K11[r], k21 = K21[r], k = r < KL ? r : r - KL;
             const int IC = (ic[0] - 1) * KL, IP = (ic[1] - 1) * KL, IP1 = (ic[2] - 1) * KL;
             const int IM = (ic[3] - 1) * KL, IM1 = (ic[4] - 1) * KL;
             const float sap = rc[1] * F[k1 + IP] + rc[2] * F[k11 + IP] + rc[3] * F[k1 + IP1] + rc[4] * F[k11 + IP1];
@@ -548,46 +588,76 @@ __device__ __forceinline__ void snonlin(const DevTables &t, const Smem &s, float
             const float fad2 = fad1 - (2.f * sap) * sam;
             fad1 = fad1 + fad2;
             const float fcen = ftemp * fij;
-            QAD[ql] = fad2 * fcen;
-            QDD[ql] = fad1 * ftemp;
-            QDP[ql] = ((fij - 2.f * sam) * t.DAL1) * fcen;
-            QDM[ql] = ((fij - 2.f * sap) * t.DAL2) * fcen;
+            QV[ql] = make_float4(fad2 * fcen, fad1 * ftemp, ((fij - 2.f * sap) * t.DAL2) * fcen,
+                                 ((fij - 2.f * sam) * t.DAL1) * fcen);
         }
+        if (tid >= NT - 9 * nmc) {
+            const int wi = tid - (NT - 9 * nmc), mcl = wi / 9, ty = wi - 9 * mcl;
+            const float *rc = s.rnl + (mc0 + mcl) * 25;
+            const float wsl = ty == 0 ? -2.f : rc[SLOT_SL[ty]], wfl = ty == 0 ? -2.f : rc[SLOT_FL[ty]];
+            WT[wi] = make_float4(wsl, wfl, __int_as_float(ty == 0 ? 0 : (ty <= 4 ? 1 : 2)), 0.f);
+        }
+        if (tid == 0) QV[511] = make_float4(0.f, 0.f, 0.f, 0.f);
         __syncthreads();
-        const int q0 = mc0 * 2 * KL;
-        for (int b = tid; b < NB; b += NT) {
-            const int j0 = t.CRANGE[(size_t)c * NB + b], j1 = t.CRANGE[(size_t)(c + 1) * NB + b];
-            if (j0 == j1) continue;
-            float sl = s.SL[b], fl = s.FL[b];
-            for (int j = j0; j < j1; j++) {
-                const int e = t.CONTRIB_T[(size_t)j * NB + b];
-                const float wsl = t.WSL_T[(size_t)j * NB + b], wfl = t.WFL_T[(size_t)j * NB + b];
-                const int ql = (e & 0xffff) - q0, ty = e >> 16;
-                sl = sl + QAD[ql] * wsl;
-                const float dq = ty == 0 ? QDD[ql] : (ty <= 4 ? QDM[ql] : QDP[ql]);
-                fl = fl + dq * wfl;
+        const int *ch = t.SN_CH + 5 * c;
+        const int blo = ch[0], bhi = ch[1], nbc = bhi - blo;
+        const uint16_t *E = t.SN_E + ch[3];
+        const int *ww = t.SN_WW + ch[4];
+        for (int b0 = blo + lane0; b0 < bhi; b0 += NT) {
+            // one wavefront: bins b0 .. b0 + 63, list length ww of that run
+            const int b = b0 + (tid & 63);
+            const int W = ww[(b0 - blo) >> 6];
+            const bool inb = b < bhi;
+            const uint16_t *e = E + (inb ? b - blo : 0);
+            float sl = inb ? s.SL[b] : 0.f, fl = inb ? s.FL[b] : 0.f;
+            for (int j0 = 0; j0 < W; j0 += G) {
+                unsigned v[G];
+#pragma unroll
+                for (int u = 0; u < G; u++) v[u] = inb ? (unsigned)e[(size_t)(j0 + u) * nbc] : SN_NONE_DEV;
+#pragma unroll
+                for (int u = 0; u < G; u++) {
+                    const float4 q = QV[v[u] & 511u];
+                    const float4 w = WT[v[u] >> 9];
+                    const int cls = __float_as_int(w.z);
+                    const float dq = cls == 0 ? q.y : (cls == 1 ? q.z : q.w);
+                    sl = sl + q.x * w.x;
+                    fl = fl + dq * w.y;
+                }
             }
-            s.SL[b] = sl;
-            s.FL[b] = fl;
+            if (inb) {
+                s.SL[b] = sl;
+                s.FL[b] = fl;
+            }
         }
         __syncthreads();
     }
 }
 
+// Scratch floats: SNONLIN's quadruplet terms (float4 per quadruplet of a
+// chunk) and weight table (float4 per MC and update type), or SINPUT_ST6's
+// three per-bin arrays.
+__host__ __device__ __forceinline__ int scr_floats(int KL, int ML, int CHUNK, int iphys)
+{
+    const int snl = 4 * 512 + 4 * 9 * CHUNK, st6 = iphys == 2 ? 3 * KL * ML : 0;
+    return snl > st6 ? snl : st6;
+}
+
 template <int IPHYS, int SHALLOW>
 __global__ void __launch_bounds__(NT) implsch_kernel(DevTables t, ImplArgs a)
 {
-    extern __shared__ float lds_raw[];
+    extern __shared__ __align__(16) float lds_raw[];
     __shared__ PointScalars P;
     __shared__ float T[MAXF], T2[MAXF], cu[MAXF], sprd[MAXF], t2[MAXF], flm[MAXF], tfk[MAXF], tcg[MAXF];
     __shared__ float wn[MAXF], cgv[MAXF], pm[3 * MAXF + 64], q[6 * MAXF];
-    __shared__ float pf[4 * MAXF + 6 * MAXF10];
+    __shared__ float pf[5 * MAXF];
     const int KL = t.KL, ML = t.ML, NB = KL * ML, NS = t.NSEA, tid = threadIdx.x;
     const int MH = t.MLSTHG;
 
-    // carve the dynamic LDS
+    // carve the dynamic LDS; the scratch area comes first (16-byte aligned)
     Smem s;
     float *p = lds_raw;
+    s.SCR = p;
+    p += scr_floats(KL, ML, t.CHUNK, IPHYS);
     s.fr = p; p += ML; s.dfim = p; p += ML; s.dfimofr = p; p += ML; s.dfim_fr = p; p += ML;
     s.rhowg = p; p += ML; s.fr5 = p; p += ML; s.frm5 = p; p += ML;
     s.th = p; p += KL; s.costh = p; p += KL; s.sinth = p; p += KL;
@@ -598,9 +668,7 @@ __global__ void __launch_bounds__(NT) implsch_kernel(DevTables t, ImplArgs a)
     s.SMIN = nullptr; s.SLBOT = nullptr;
     if (a.lcflx) { s.SMIN = p; p += NB; }
     if (SHALLOW) { s.SLBOT = p; p += NB; }
-    s.SCR = p;
-    const int scr = max(4 * 2 * KL * t.CHUNK, IPHYS == 2 ? 3 * NB : 0);
-    s.LLWS = (unsigned char *)(s.SCR + scr);
+    s.LLWS = (unsigned char *)p;
 
     // point-invariant tables, once per workgroup
     for (int i = tid; i < ML; i += NT) {
@@ -616,6 +684,14 @@ __global__ void __launch_bounds__(NT) implsch_kernel(DevTables t, ImplArgs a)
     }
     if (IPHYS == 2)
         for (int i = tid; i < t.NK10; i += NT) { s.sig10[i] = t.SIG10[i]; s.dsii10[i] = t.DSII10[i]; }
+
+    // SNONLIN quadruplet slots of this thread (MC offset, KH*KL + K); MC
+    // offset 255 marks a slot past the chunk
+    int qslot[2];
+    for (int i = 0; i < 2; i++) {
+        const int ql = tid + i * NT, mcl = ql / (2 * KL);
+        qslot[i] = ql < 511 && mcl < t.CHUNK ? mcl | (ql - mcl * 2 * KL) << 8 : 255;
+    }
 
     // XCD-aware persistent order: the workgroups resident on one XCD walk one
     // contiguous range of sea points side by side, so the bin-major loads of
@@ -720,61 +796,73 @@ __global__ void __launch_bounds__(NT) implsch_kernel(DevTables t, ImplArgs a)
         // dissipation
         if (IPHYS == 2) {
             // SDISSIP_ST6 (:4929-5102) and W3SWL6 (:5106-5252)
-            float *t12 = pm, *ddis = pm + MAXF, *aband = pm + 2 * MAXF, *bn = pf;
-            if (tid < ML) {
+            // per frequency on wave 0 (lane m); the cumulative saturation sum,
+            // the peak search and the B1 integral read the lanes in order
+            float *t12 = pm, *ddis = pm + MAXF;
+            if (tid < 64) {
                 const int m = tid;
-                const float *f = s.F + KL * m;
-                float ed = f[0];
-                for (int k = 1; k < KL; k++) ed = ed + f[k];
-                T[m] = ed * t.DELTH;                              // EDENS
-                const float sigm = t.ZPI * s.fr[m];
-                const float c0 = t.ZPI * sigm;
-                float ab = 0.f, mx = 0.f;
-                for (int k = 0; k < KL; k++) {
-                    const float v = (f[k] * cgv[m]) / c0;
-                    ab = k == 0 ? v : ab + v;
-                    mx = k == 0 ? v : wam_maxf(mx, v);
-                }
-                float sn = 0.f;
-                for (int k = 0; k < KL; k++) {
-                    const float v = (f[k] * cgv[m]) / c0;
-                    const float kn = (mx < 1.0E-34f) ? 1.f : v / mx;
-                    sn = k == 0 ? kn : sn + kn;
-                }
-                const float anar = 1.0f / (sn * t.DELTH);
-                aband[m] = ab;
-                bn[m] = (anar * ((ab * sigm) * t.DELTH)) * wam_ipowf(wn[m], 3);
-                const float dden = (t.ZPI * s.dfim[m]) * sigm;
-                T2[m] = (ab * dden) / cgv[m];
-            }
-            __syncthreads();
-            if (tid == 0) {
-                const float bnt = 0.035f * 0.035f;
-                const float xfac = (1.0f - 1.0f / t.CO) / (t.CO - 1.0f / t.CO);
-                float t2s = 0.f;
-                for (int ik = 0; ik < ML; ik++) {
-                    const float et = (t.ZPI * bnt) / ((1.0f * cgv[ik]) * wam_ipowf(wn[ik], 3));
-                    const float ex = wam_maxf(0.0f, T[ik] - et);
+                const bool on = m < ML;
+                float ab = 0.f, bnm = 0.f, t2m = 0.f, term = 0.f, t1 = 0.f, sigm = 0.f;
+                if (on) {
+                    const float *f = s.F + KL * m;
+                    float ed = f[0];
+                    for (int k = 1; k < KL; k++) ed = ed + f[k];
+                    const float edens = ed * t.DELTH;
+                    sigm = t.ZPI * s.fr[m];
+                    const float c0 = t.ZPI * sigm;
+                    float mx = 0.f;
+                    for (int k = 0; k < KL; k++) {
+                        const float v = (f[k] * cgv[m]) / c0;
+                        ab = k == 0 ? v : ab + v;
+                        mx = k == 0 ? v : wam_maxf(mx, v);
+                    }
+                    float sn = 0.f;
+                    for (int k = 0; k < KL; k++) {
+                        const float v = (f[k] * cgv[m]) / c0;
+                        const float kn = (mx < 1.0E-34f) ? 1.f : v / mx;
+                        sn = k == 0 ? kn : sn + kn;
+                    }
+                    const float anar = 1.0f / (sn * t.DELTH);
+                    bnm = (anar * ((ab * sigm) * t.DELTH)) * wam_ipowf(wn[m], 3);
+                    const float dden = (t.ZPI * s.dfim[m]) * sigm;
+                    t2m = (ab * dden) / cgv[m];
+                    // W3SWL6 terms of frequency m
+                    const float bnt = 0.035f * 0.035f;
+                    const float xfac = (1.0f - 1.0f / t.CO) / (t.CO - 1.0f / t.CO);
+                    const float et = (t.ZPI * bnt) / ((1.0f * cgv[m]) * wam_ipowf(wn[m], 3));
+                    const float ex = wam_maxf(0.0f, edens - et);
                     const float nex = ex / et;
                     const float n4 = wam_ipowf(nex, 4);
-                    const float t1 = ((4.75E-6f * 1.0f) * s.fr[ik]) * n4;
-                    float dfii = t.DF[ik];
-                    if (ik > 0 && ik < ML - 1) dfii = dfii * xfac;
-                    const float term = (1.0f * n4) * dfii;
-                    t2s = ik == 0 ? term : t2s + term;
-                    t12[ik] = -1.0f * (wam_maxf(0.0f, t1) + wam_maxf(0.0f, 7.00E-5f * t2s));
+                    t1 = ((4.75E-6f * 1.0f) * s.fr[m]) * n4;
+                    float dfii = t.DF[m];
+                    if (m > 0 && m < ML - 1) dfii = dfii * xfac;
+                    term = (1.0f * n4) * dfii;
                 }
-            } else if (tid == 64) {
+                // running sum of the terms up to m, the first peak of ABAND, SUM of T2
+                float run = lane_at(term, 0), pre = run;
+                float cur = lane_at(ab, 0), sm = lane_at(t2m, 0);
                 int ikx = 0;
-                for (int m = 1; m < ML; m++) if (aband[m] > aband[ikx]) ikx = m;
-                float sm = T2[0];
-                for (int m = 1; m < ML; m++) sm = sm + T2[m];
-                P.b1 = 0.0041f * ((2.f * sqrtf(sm)) * wn[ikx]);
-            }
-            __syncthreads();
-            if (tid < ML) {
-                const float sigm = t.ZPI * s.fr[tid];
-                ddis[tid] = aband[tid] > 1.E-30f ? (((-(2.f / 3.f)) * P.b1) * sigm) * sqrtf(bn[tid]) : 0.f;
+                for (int i0 = 0; i0 < ML; i0 += 8) {
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        const int i = i0 + u;
+                        if (i == 0) continue;
+                        const float r = run + lane_at(term, i);
+                        run = i < ML ? r : run;
+                        pre = m == i ? run : pre;
+                        const float v = lane_at(ab, i);
+                        const bool take = i < ML && v > cur;
+                        cur = take ? v : cur;
+                        ikx = take ? i : ikx;
+                        const float q = sm + lane_at(t2m, i);
+                        sm = i < ML ? q : sm;
+                    }
+                }
+                const float b1 = 0.0041f * ((2.f * sqrtf(sm)) * wn[ikx]);
+                if (on) {
+                    t12[m] = -1.0f * (wam_maxf(0.0f, t1) + wam_maxf(0.0f, 7.00E-5f * pre));
+                    ddis[m] = ab > 1.E-30f ? (((-(2.f / 3.f)) * b1) * sigm) * sqrtf(bnm) : 0.f;
+                }
             }
             __syncthreads();
             for (int b = tid; b < NB; b += NT) {
@@ -820,7 +908,7 @@ __global__ void __launch_bounds__(NT) implsch_kernel(DevTables t, ImplArgs a)
             __syncthreads();
         }
         // SNONLIN, with the ISNONLIN = 0 shallow-water enhancement
-        snonlin(t, s, SHALLOW ? P.enhfr : 1.f);
+        snonlin(t, s, SHALLOW ? P.enhfr : 1.f, qslot);
         if (SHALLOW) {
             // SBOTTOM (:1527-1589) into SL_BOT
             if (tid < ML) {
@@ -931,7 +1019,7 @@ size_t implsch_lds_bytes(const DevTables &t, int lcflx)
     size_t n = 7 * (size_t)ML + 3 * KL + 26 * (size_t)MH + (t.iphys == 2 ? 2 * (size_t)t.NK10 : 0) + 5 * (size_t)MH +
                8 * (size_t)KL;
     n += 4 * (size_t)NB + (lcflx ? NB : 0) + (t.shallow ? NB : 0);
-    n += (size_t)std::max(4 * 2 * KL * t.CHUNK, t.iphys == 2 ? 3 * NB : 0);
+    n += (size_t)scr_floats(KL, ML, t.CHUNK, t.iphys);
     return n * sizeof(float) + (size_t)NB;
 }
 

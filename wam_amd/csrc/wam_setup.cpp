@@ -367,32 +367,50 @@ static int snonlin_lists(Tables &t, std::string &err)
         t.NCONTRIB[b] = (int)lists[b].size();
         for (size_t j = 0; j < lists[b].size(); j++) t.CONTRIB[(size_t)b * nc + j] = lists[b][j];
     }
-    // RNLCOEF slots of the SL and FL weights of update types 1..8; the centre
-    // update (type 0) is -(2*AD) = AD*(-2), exact in binary arithmetic
-    static const int wsl[9] = {0, 20, 19, 17, 18, 8, 7, 5, 6};
-    static const int wfl[9] = {0, 23, 24, 21, 22, 11, 12, 9, 10};
+    // Chunks of at most 512 quadruplets (two passes of a 256-thread
+    // workgroup).  The update order inside each bin's list is the
+    // reference's, and every list is ordered by MC, so a chunked gather adds
+    // in the same order.
+    t.CHUNK = std::max(1, std::min(14, 511 / (2 * KL)));
     t.NCHUNK = (t.MLSTHG + t.CHUNK - 1) / t.CHUNK;
-    t.CONTRIB_T.assign((size_t)nc * NB, 0);
-    t.WSL_T.assign((size_t)nc * NB, 0.f);
-    t.WFL_T.assign((size_t)nc * NB, 0.f);
-    t.CRANGE.assign((size_t)(t.NCHUNK + 1) * NB, 0);
+    if (2 * KL > 511) { err = "SNONLIN packing needs KL <= 255"; return WAM_E_ARG; }
+    t.SN_E.clear();
+    t.SN_WW.clear();
+    t.SN_CH.assign((size_t)5 * t.NCHUNK, 0);
     for (int b = 0; b < NB; b++) {
-        const int n = t.NCONTRIB[b];
         int prev_mc = 0;
-        for (int j = 0; j < n; j++) {
-            const int e = lists[b][j], q = e & 0xffff, ty = e >> 16, mc = q / (2 * KL) + 1;
+        for (size_t j = 0; j < lists[b].size(); j++) {
+            const int mc = (lists[b][j] & 0xffff) / (2 * KL) + 1;
             if (mc < prev_mc) { err = "SNONLIN list not ordered by frequency"; return WAM_E_ARG; }
             prev_mc = mc;
-            t.CONTRIB_T[(size_t)j * NB + b] = e;
-            const float *rc = &t.RNLCOEF[(size_t)(mc - 1) * 25];
-            t.WSL_T[(size_t)j * NB + b] = ty == 0 ? -2.f : rc[wsl[ty]];
-            t.WFL_T[(size_t)j * NB + b] = ty == 0 ? -2.f : rc[wfl[ty]];
         }
-        for (int c = 0; c <= t.NCHUNK; c++) {
-            const int c0 = 1 + c * t.CHUNK;
-            int j = 0;
-            while (j < n && (lists[b][j] & 0xffff) / (2 * KL) + 1 < c0) j++;
-            t.CRANGE[(size_t)c * NB + b] = j;
+    }
+    for (int c = 0; c < t.NCHUNK; c++) {
+        const int mc_lo = 1 + c * t.CHUNK, mc_hi = std::min(t.MLSTHG, mc_lo + t.CHUNK - 1);
+        std::vector<std::vector<uint16_t>> per(NB);
+        int blo = NB, bhi = 0, w = 0;
+        for (int b = 0; b < NB; b++) {
+            for (int e : lists[b]) {
+                const int q = e & 0xffff, ty = e >> 16, mc = q / (2 * KL) + 1;
+                if (mc < mc_lo || mc > mc_hi) continue;
+                const int ql = q - (mc_lo - 1) * 2 * KL;            // quadruplet index within the chunk
+                per[b].push_back((uint16_t)(ql | (((mc - mc_lo) * 9 + ty) << 9)));
+            }
+            if (!per[b].empty()) { blo = std::min(blo, b); bhi = b + 1; w = std::max(w, (int)per[b].size()); }
+        }
+        if (bhi <= blo) { blo = 0; bhi = 0; w = 0; }
+        w = (w + SN_GROUP - 1) / SN_GROUP * SN_GROUP;                // whole groups of entries
+        const int nbc = bhi - blo, off = (int)t.SN_E.size(), wwoff = (int)t.SN_WW.size();
+        t.SN_CH[5 * c] = blo; t.SN_CH[5 * c + 1] = bhi; t.SN_CH[5 * c + 2] = w; t.SN_CH[5 * c + 3] = off;
+        t.SN_CH[5 * c + 4] = wwoff;
+        t.SN_E.resize((size_t)off + (size_t)w * nbc, SN_NONE);
+        for (int b = blo; b < bhi; b++)
+            for (size_t j = 0; j < per[b].size(); j++) t.SN_E[(size_t)off + j * nbc + (b - blo)] = per[b][j];
+        // list length each run of 64 consecutive bins (one wavefront) needs
+        for (int g0 = blo; g0 < bhi; g0 += 64) {
+            int wg = 0;
+            for (int b = g0; b < std::min(bhi, g0 + 64); b++) wg = std::max(wg, (int)per[b].size());
+            t.SN_WW.push_back((wg + SN_GROUP - 1) / SN_GROUP * SN_GROUP);
         }
     }
     return WAM_OK;
@@ -505,6 +523,7 @@ int build_tables(Tables &t, const wam_config &cfg, const uint8_t *mask, std::str
     if (rc) return rc;
     x0tauhf(t);
     lfactor_grid(t);
+    if (cfg.iphys == 2 && t.NK10 > 128) { err = "LFACTOR 10-Hz grid longer than 128 frequencies (FR(1) too low)"; return WAM_E_ARG; }
     return WAM_OK;
 }
 

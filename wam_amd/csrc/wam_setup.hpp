@@ -14,6 +14,11 @@
 
 namespace wam {
 
+constexpr int SN_GROUP = 6;    // SNONLIN list entries a thread loads together
+// Padding entry of the SNONLIN lists: quadruplet slot 511 (kept zero) with the
+// centre weight -2, so it adds (+0)*(-2) = -0, which leaves every value as it is.
+constexpr uint16_t SN_NONE = 0x01FF;
+
 struct Consts {           // wam_general_module.f90:41-85
     float G, PI, ZPI, DEG, RAD, CIRC, RADIUS, ROAIR, RNUAIR, RNUAIRM, ROWATER, XKAPPA, XNLEV,
         DKMAX, BETAMAX, ZALP, ALPHA, CDFAC, EMIN, EPS1, CO;
@@ -60,13 +65,17 @@ struct Tables {
     int NQ = 0, NCMAX = 0;
     std::vector<int32_t> CONTRIB;                        // [NB][NCMAX]
     std::vector<int32_t> NCONTRIB;                       // [NB]
-    // device form: lists transposed to [j][NB] with the SL / FL weight of each
-    // update, and per-chunk start indices for a gather done CHUNK frequencies
-    // of quadruplets at a time
-    int CHUNK = 8, NCHUNK = 0;
-    std::vector<int32_t> CONTRIB_T;                      // [NCMAX][NB]
-    std::vector<float> WSL_T, WFL_T;                     // [NCMAX][NB]
-    std::vector<int32_t> CRANGE;                         // [NCHUNK+1][NB]
+    // device form, for a gather done CHUNK frequencies of quadruplets at a
+    // time.  Chunk c feeds the bins [SN_CH[5c], SN_CH[5c+1]); its entries are
+    // SN_E[SN_CH[5c+3] + j*(bins of the chunk) + (b - first bin)], j <
+    // SN_CH[5c+2] (a multiple of SN_GROUP), each ql | (MC-offset*9 + update
+    // type) << 9 with ql the quadruplet's index within the chunk (< 511), and
+    // SN_NONE after the last entry of a bin.  SN_WW[SN_CH[5c+4] + g] is the
+    // list length the bins blo + 64g .. blo + 64g + 63 need.
+    int CHUNK = 0, NCHUNK = 0;
+    std::vector<uint16_t> SN_E;
+    std::vector<int32_t> SN_WW;
+    std::vector<int32_t> SN_CH;                          // [NCHUNK][5]
 };
 
 // Builds constants, fre_dir, tables, DIA, X0TAUHF and the grid.  Returns a
