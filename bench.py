@@ -25,6 +25,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+# floating-point work of IMPLSCH per sea point, counted by the oracle's
+# op-counting build on a seeded sub-box of this workload
+OPCOUNT_FILE = os.path.join(ROOT, "tests", "golden", "implsch_opcount.json")
+# HBM bytes per dispatch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# (scaled by a plain copy kernel of known size), per kernel and workload
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_c2.json")
 
 
 def log(*a):
@@ -85,9 +91,11 @@ def main():
     ap.add_argument("--nx", type=int, default=1600)
     ap.add_argument("--iphys", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-ny", type=int, default=180)
-    ap.add_argument("--cpu-nx", type=int, default=240)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-ny", type=int, default=360)
+    ap.add_argument("--cpu-nx", type=int, default=480)
+    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one plain 1 GiB device copy before the timed steps (reference for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -127,6 +135,14 @@ def main():
         if ev:
             ev[2].record(stream)
 
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, device=dev)
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)          # reads 1 GiB, writes 1 GiB
+        torch.cuda.synchronize()
+        log("[bench] copy calibration: 1 GiB read + 1 GiB written by one elementwise kernel")
+        del src, dst
+
     for i in range(args.warmup):
         step()
         torch.cuda.synchronize()
@@ -144,7 +160,10 @@ def main():
     log(f"[bench] {args.steps} steps: {ms_step:.2f} ms/step; PROPAGS {t_prop.mean() * 1e3:.2f} ms "
         f"({counter} sub-steps), IMPLSCH {t_src.mean() * 1e3:.2f} ms")
     finite = bool(torch.isfinite(fa).all().item())
-    hs = 4.0 * torch.sqrt(torch.clamp(fa.view(cfg["ml"], cfg["kl"], n).sum(1).mean(1).sum(), min=0))
+    # significant wave height 4*sqrt(E), E = sum over m of DFIM(m) * sum over k of F (no tail)
+    dfim = torch.tensor(g.table("DFIM"), device=dev)
+    energy = (fa.view(cfg["ml"], cfg["kl"], n).sum(1) * dfim[:, None]).sum(0)
+    hs = 4.0 * torch.sqrt(torch.clamp(energy, min=0))
 
     kl, ml = cfg["kl"], cfg["ml"]
     ncoef = 9
@@ -153,17 +172,27 @@ def main():
     prop_bytes = n * kl * ml * 4 * (2 + ncoef) + n * 4 * 4
     prop_launch_s = t_prop.mean() / counter
     prop_gbs = prop_bytes / prop_launch_s / 1e9
-    # IMPLSCH compulsory bytes: spectrum read + write and the per-point fields
-    src_bytes = n * kl * ml * 8 + n * 4 * 10
-    src_gbs = src_bytes / t_src.mean() / 1e9
+    # IMPLSCH: compute bound (VALU FP32); algorithmic flops per point from the op count
+    opc = json.load(open(OPCOUNT_FILE))[f"c2_sub_iphys{args.iphys}"]
+    src_flops = opc["flops_per_point"] * n
+    src_tflops = src_flops / t_src.mean() / 1e12
+    traffic = {}
+    if os.path.exists(TRAFFIC_FILE):
+        traffic = json.load(open(TRAFFIC_FILE))
     dominant = "IMPLSCH" if t_src.mean() > t_prop.mean() else "PROPAGS"
     roof_prop = dict(kernel="propags_curr_kernel", bound="hbm", achieved=round(prop_gbs, 1), peak=HBM_PEAK_GBS,
-                     unit="GB/s", frac=round(prop_gbs / HBM_PEAK_GBS, 4), traffic=None,
+                     unit="GB/s", frac=round(prop_gbs / HBM_PEAK_GBS, 4),
+                     traffic=traffic.get("propags_curr_kernel", {}).get("bytes_per_dispatch"),
                      algorithmic_bytes_per_launch=prop_bytes, avg_launch_ms=round(prop_launch_s * 1e3, 3))
-    roof_src = dict(kernel="implsch_kernel<0,1>", bound="hbm", achieved=round(src_gbs, 1), peak=HBM_PEAK_GBS,
-                    unit="GB/s", frac=round(src_gbs / HBM_PEAK_GBS, 4), traffic=None,
-                    algorithmic_bytes_per_launch=src_bytes, avg_launch_ms=round(t_src.mean() * 1e3, 3),
-                    note="compute-bound kernel; HBM fraction shown for its compulsory bytes")
+    roof_src = dict(kernel=f"implsch_kernel<{args.iphys},1>", bound="mfma", achieved=round(src_tflops, 3),
+                    peak=FP32_PEAK_TFLOPS, unit="TFLOP/s", frac=round(src_tflops / FP32_PEAK_TFLOPS, 4),
+                    traffic=traffic.get(f"implsch_kernel<{args.iphys},1>", {}).get("bytes_per_dispatch"),
+                    algorithmic_flops_per_launch=round(src_flops),
+                    flops_per_point=round(opc["flops_per_point"], 1),
+                    transcendentals_per_point=round(opc["trans_per_point"], 1),
+                    avg_launch_ms=round(t_src.mean() * 1e3, 3),
+                    note="FP32 VALU work (no GEMM-shaped arithmetic); peak is MI355X's FP32 157.3 TFLOP/s, "
+                         "unfused add/mul/div as counted by the oracle's op counter")
     out = {
         "metric": "sea-point spectrum updates/s per step (src+prop); batched & adjoint vs plain",
         "value": n / (ms_step * 1e-3),
@@ -179,7 +208,7 @@ def main():
         "t_propags_ms": round(t_prop.mean() * 1e3, 3), "t_implsch_ms": round(t_src.mean() * 1e3, 3),
         "roofline": roof_src if dominant == "IMPLSCH" else roof_prop,
         "roofline_other": roof_prop if dominant == "IMPLSCH" else roof_src,
-        "state_finite": finite, "mean_hs_m": round(float(hs), 4),
+        "state_finite": finite, "mean_hs_m": round(float(hs.mean()), 4),
     }
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle) ...")

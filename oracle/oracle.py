@@ -43,8 +43,8 @@ def build():
 _LIBS = {}
 
 
-def _lib(fp64):
-    key = "liborc64.so" if fp64 else "liborc32.so"
+def _lib(fp64, opcount=False):
+    key = "liborc_opcount.so" if opcount else ("liborc64.so" if fp64 else "liborc32.so")
     if key not in _LIBS:
         path = os.path.join(_HERE, key)
         if not os.path.exists(path):
@@ -68,6 +68,9 @@ def _lib(fp64):
         lib.orc_api_math.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long]
         lib.orc_api_aki.restype = ctypes.c_double
         lib.orc_api_aki.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        if opcount:
+            lib.orc_opcount_implsch.argtypes = ([ctypes.c_void_p] + [ctypes.c_void_p] * 9 +
+                                                [ctypes.c_int] + [ctypes.c_void_p] * 4)
         _LIBS[key] = lib
     return _LIBS[key]
 
@@ -79,11 +82,12 @@ def _p(a):
 class Oracle:
     """One model configuration on one regular grid (sea mask nx*ny, row-major from the south-west)."""
 
-    def __init__(self, cfg, mask, fp64=False):
+    def __init__(self, cfg, mask, fp64=False, opcount=False):
         self.cfg = dict(cfg)
-        self.fp64 = fp64
-        self.dtype = np.float64 if fp64 else np.float32
-        self.lib = _lib(fp64)
+        self.fp64 = fp64 and not opcount
+        self.dtype = np.float64 if self.fp64 else np.float32
+        self.opcount = opcount
+        self.lib = _lib(self.fp64, opcount)
         self._c = make_config(cfg)
         m = np.ascontiguousarray(mask, dtype=np.uint8).ravel()
         st = ctypes.c_int(0)
@@ -144,6 +148,34 @@ class Oracle:
                                  _p(roairn), _p(wstar), _p(depth), int(bool(lcflx)), _p(phiaw),
                                  _p(flux), _p(mij), int(nthreads))
         return dict(fl3=f, tauw=tauw, ustar=ustar, z0=z0, phiaw=phiaw, flux=flux, mij=mij)
+
+    def implsch_opcount(self, fl3, u10, udir, tauw, z0, depth, roairn=None, wstar=None, lcflx=False):
+        """IMPLSCH of the REAL*4 restatement with every floating-point
+        operation counted (orc_opcount.h), on one thread.  Returns the same
+        fields as implsch() and the totals over all sea points:
+        add, mul, div, sqrt, trans (elementary-function calls)."""
+        if not self.opcount:
+            raise ValueError("construct the Oracle with opcount=True")
+        n = self.nsea
+        dt = np.float32
+        f = np.ascontiguousarray(fl3, dtype=dt).copy()
+        u10 = np.ascontiguousarray(u10, dtype=dt)
+        udir = np.ascontiguousarray(udir, dtype=dt)
+        tauw = np.ascontiguousarray(tauw, dtype=dt).copy()
+        z0 = np.ascontiguousarray(z0, dtype=dt).copy()
+        depth = np.ascontiguousarray(depth, dtype=dt)
+        roairn = np.full(n, 1.225, dt) if roairn is None else np.ascontiguousarray(roairn, dtype=dt)
+        wstar = np.zeros(n, dt) if wstar is None else np.ascontiguousarray(wstar, dtype=dt)
+        ustar = np.zeros(n, dt)
+        phiaw = np.zeros(n, dt)
+        flux = np.zeros((6, n), dt)
+        mij = np.zeros(n, np.int32)
+        counts = np.zeros(5, np.uint64)
+        self.lib.orc_opcount_implsch(self.ctx, _p(f), _p(u10), _p(udir), _p(tauw), _p(ustar), _p(z0),
+                                     _p(roairn), _p(wstar), _p(depth), int(bool(lcflx)), _p(phiaw),
+                                     _p(flux), _p(mij), _p(counts))
+        res = dict(fl3=f, tauw=tauw, ustar=ustar, z0=z0, phiaw=phiaw, flux=flux, mij=mij)
+        return res, dict(zip(["add", "mul", "div", "sqrt", "trans"], (int(x) for x in counts)))
 
     def wavnu2(self, depth):
         d = np.ascontiguousarray(depth, dtype=self.dtype)

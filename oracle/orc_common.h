@@ -9,7 +9,8 @@
  * fl3[(m*KL + k)*NIJ + ij] (0-based), sea point fastest.
  *
  * Build FP32 (default, the reference's REAL) or FP64 (-DORC_FP64: the
- * reference built with 8-byte reals).  The FP32 build evaluates the Fortran
+ * reference built with 8-byte reals); -DORC_OPCOUNT (C++) is the FP32
+ * arithmetic with every operation counted (orc_opcount.h).  The FP32 build evaluates the Fortran
  * intrinsics with wam_math.h (deterministic single precision); the FP64 build
  * uses libm double.
  */
@@ -47,6 +48,28 @@ static inline int INTR(real x) { return (int)trunc(x); }
 static inline real FLOORR(real x) { return floor(x); }
 static inline real CEILR(real x) { return ceil(x); }
 static inline real SIGNR(real a, real b) { return b >= 0 ? fabs(a) : -fabs(a); }
+#elif defined(ORC_OPCOUNT)
+#include "orc_opcount.h"
+typedef oc_real real;
+#define R(x) (x##f)
+#define EXP(x) oc_t(wam_expf((x).v))
+#define LOG(x) oc_t(wam_logf((x).v))
+#define LOG10(x) oc_t(wam_log10f((x).v))
+#define TANH(x) oc_t(wam_tanhf((x).v))
+#define COSH(x) oc_t(wam_coshf((x).v))
+#define SINH(x) oc_t(wam_sinhf((x).v))
+#define COS(x) oc_t(wam_cosf(oc_real(x).v))
+#define SIN(x) oc_t(wam_sinf(oc_real(x).v))
+#define SQRT(x) oc_sqrt(x)
+#define ACOS(x) oc_t((float)acos((double)(x)))
+#define POW(x, y) oc_t(wam_powf(oc_real(x).v, oc_real(y).v))
+#define ABS(x) oc_real(wam_fabsf(oc_real(x).v))
+#define IPOW(x, n) oc_ipow(x, n)
+static inline int NINT(real x) { return wam_nint(x.v); }
+static inline int INTR(real x) { return wam_int(x.v); }
+static inline real FLOORR(real x) { return wam_floorf(x.v); }
+static inline real CEILR(real x) { return wam_ceilf(x.v); }
+static inline real SIGNR(real a, real b) { return wam_signf(a.v, b.v); }
 #else
 #include "wam_math.h"
 typedef float real;

@@ -1,0 +1,54 @@
+"""Writes implsch_opcount.json: the floating-point work of one IMPLSCH call
+per sea point, counted by the oracle's op-counting build (oracle/orc_opcount.h:
+add/sub, mul, div, sqrt one each; elementary-function calls separately), on
+seeded sub-boxes of the production workload (0.1 deg, 36 x 36, shallow water,
+currents) for both physics options, and on the 60 x 60 deep-water case.
+
+    python tests/golden/make_opcount.py
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+CASES = {
+    "c2_sub_iphys0": dict(kind="c2", ny=60, nx=90, seed=20261203, iphys=0),
+    "c2_sub_iphys2": dict(kind="c2", ny=60, nx=90, seed=20261203, iphys=2),
+    "c1": dict(kind="c1"),
+}
+
+
+def count(case):
+    from oracle.oracle import Oracle
+    from wam_amd import workloads as W
+    if case["kind"] == "c2":
+        cfg, mask, f = W.c2_case(ny=case["ny"], nx=case["nx"], seed=case["seed"], iphys=case["iphys"])
+        depth = f["depth"]
+    else:
+        cfg, mask, f = W.c1_case()
+        depth = f["depth"]
+    o = Oracle(cfg, mask, opcount=True)
+    n = o.nsea
+    fl3 = W.jonswap_spectrum(cfg, f["udir"]).ravel()
+    z = np.zeros(n, np.float32)
+    _, c = o.implsch_opcount(fl3, f["u10"], f["udir"], z, z, depth)
+    flops = c["add"] + c["mul"] + c["div"] + c["sqrt"]
+    return dict(nsea=n, kl=cfg["kl"], ml=cfg["ml"], iphys=cfg["iphys"], shallow=cfg["shallow"],
+                totals=c, flops_per_point=flops / n, trans_per_point=c["trans"] / n)
+
+
+def main():
+    out = {name: dict(case=case, **count(case)) for name, case in CASES.items()}
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "implsch_opcount.json")
+    with open(path, "w") as fh:
+        json.dump(out, fh, indent=1, sort_keys=True)
+    for k, v in out.items():
+        print(k, v["nsea"], round(v["flops_per_point"]), round(v["trans_per_point"]))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,39 @@
+"""The op-counting build of the oracle (oracle/orc_opcount.h) evaluates the
+same REAL*4 arithmetic as the oracle, so the work it counts is the work of
+the restated IMPLSCH; and the frozen per-point counts the benchmark's
+roofline uses (tests/golden/implsch_opcount.json) are what it counts."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+from wam_amd import workloads as W
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.mark.parametrize("iphys", [0, 2])
+def test_counted_build_is_bit_identical_to_the_oracle(iphys):
+    cfg, mask, f = W.c2_case(ny=30, nx=40, seed=11, iphys=iphys)
+    o = Oracle(cfg, mask)
+    oc = Oracle(cfg, mask, opcount=True)
+    n = o.nsea
+    fl3 = W.jonswap_spectrum(cfg, f["udir"]).ravel()
+    z = np.zeros(n, np.float32)
+    r1 = o.implsch(fl3, f["u10"], f["udir"], z, z, f["depth"], nthreads=1)
+    r2, c = oc.implsch_opcount(fl3, f["u10"], f["udir"], z, z, f["depth"])
+    for k in ("fl3", "tauw", "ustar", "z0", "phiaw"):
+        assert np.array_equal(r1[k].view(np.int32), r2[k].view(np.int32)), k
+    assert np.array_equal(r1["mij"], r2["mij"])
+    assert c["add"] > 0 and c["mul"] > 0 and c["trans"] > 0
+
+
+def test_frozen_counts_reproduce():
+    from tests.golden.make_opcount import CASES, count
+    ref = json.load(open(os.path.join(HERE, "golden", "implsch_opcount.json")))
+    for name in ("c2_sub_iphys0", "c1"):
+        got = count(CASES[name])
+        assert got["totals"] == ref[name]["totals"], name
+        assert got["nsea"] == ref[name]["nsea"]
