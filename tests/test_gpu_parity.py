@@ -38,6 +38,7 @@ CASES = {
     "c2cut_janssen_shallow_nocurr": lambda: _with(W.c2_case(ny=30, nx=50, seed=8, kl=24, ml=30), refr_c=0),
     "c2cut_janssen_shallow_norefr": lambda: _with(W.c2_case(ny=30, nx=50, seed=8, kl=12, ml=25), refr_c=0,
                                                     refr_d=0),
+    "c2cut_janssen_shallow_curr_kl12": lambda: W.c2_case(ny=30, nx=40, seed=6, kl=12, ml=25),
     "deep_current": lambda: _with(W.c2_case(ny=30, nx=40, seed=4, kl=24, ml=25), shallow=0, deep_depth=True),
 }
 

@@ -18,6 +18,8 @@ __global__ void minmax_kernel(const float *x, size_t n, float *out);
 __global__ void rescale_kernel(float *coef, size_t nb, int nplanes, float rc);
 __global__ void propags_nocurr_kernel(DevTables t, const float *coef, const float *fin, float *fout);
 __global__ void propags_curr_kernel(DevTables t, const float *coef, const float *fin, float *fout);
+template <int KLT>
+__global__ void propags_curr_rows_kernel(DevTables t, const float *coef, const float *fin, float *fout);
 __global__ void wavnu2_kernel(DevTables t, float *wn, float *cgg);
 hipError_t launch_implsch(const DevTables &t, const ImplArgs &a, int ncu, hipStream_t st);
 }  // namespace wam
@@ -252,7 +254,11 @@ int wam_gpu_propags_dev(wam_ctx *ctx, const float *d_in, float *d_out)
     if (ctx->counter > 1 && !ctx->fscratch) HIPCHK(hipMalloc(&ctx->fscratch, nb * sizeof(float)));
     const int nch = (t.NSEA + 255) / 256;
     const int grid = ((nch + 7) / 8) * 8;
-    auto kern = t.cfg.refr_c ? propags_curr_kernel : propags_nocurr_kernel;
+    // KPM is cyclic (K+1, K-1 modulo KL), so for the common direction counts
+    // the register-row form of PROP_CURR applies
+    auto kern = propags_nocurr_kernel;
+    if (t.cfg.refr_c)
+        kern = t.KL == 36 ? propags_curr_rows_kernel<36> : t.KL == 24 ? propags_curr_rows_kernel<24> : propags_curr_kernel;
     // sub-steps alternate between out and scratch so that the last one lands in d_out
     const float *src = d_in;
     for (int i = 0; i < ctx->counter; i++) {

@@ -204,6 +204,13 @@ __device__ __forceinline__ int xcd_chunk(int nchunks)
     return (blockIdx.x % 8) * per + blockIdx.x / 8;
 }
 
+// Weights are read exactly once per sub-step: non-temporal loads keep them
+// from evicting the spectrum lines that neighbouring points re-read from L2.
+__device__ __forceinline__ float ld_stream(const float *__restrict__ p, size_t i)
+{
+    return __builtin_nontemporal_load(p + i);
+}
+
 __device__ __forceinline__ float ld_or0(const float *__restrict__ p, int j)
 {
     return j >= 0 ? p[j] : 0.f;
@@ -231,11 +238,11 @@ __global__ void __launch_bounds__(256) propags_nocurr_kernel(DevTables t, const 
             const float *fkp = fin + ((size_t)m * KL + (t.KPM_P[k] - 1)) * NS;
             const float *fkm = fin + ((size_t)m * KL + (t.KPM_M[k] - 1)) * NS;
             const size_t at = row + ij;
-            float s = coef[P_SUM * NB + at] * f[ij] + coef[1 * NB + at] * ld_or0(f, nlat);
-            s = s + coef[2 * NB + at] * ld_or0(f, nlon);
-            s = s + coef[3 * NB + at] * fkp[ij];
-            s = s + coef[4 * NB + at] * fkm[ij];
-            fout[at] = s;
+            float s = ld_stream(coef, P_SUM * NB + at) * f[ij] + ld_stream(coef, 1 * NB + at) * ld_or0(f, nlat);
+            s = s + ld_stream(coef, 2 * NB + at) * ld_or0(f, nlon);
+            s = s + ld_stream(coef, 3 * NB + at) * fkp[ij];
+            s = s + ld_stream(coef, 4 * NB + at) * fkm[ij];
+            __builtin_nontemporal_store(s, &fout[at]);
         }
     }
 }
@@ -263,19 +270,75 @@ __global__ void __launch_bounds__(256) propags_curr_kernel(DevTables t, const fl
             const float *fkm = fin + ((size_t)m * KL + (t.KPM_M[k] - 1)) * NS;
             const float *fmp = fin + ((size_t)mp * KL + k) * NS;
             const float *fmm = fin + ((size_t)mm * KL + k) * NS;
-            float s = coef[P_SUM * NB + at] * f[ij];
-            s = s + coef[P_LAT2 * NB + at] * ld_or0(f, nn);
-            s = s + coef[P_LAT1 * NB + at] * ld_or0(f, ns);
-            s = s + coef[P_LON2 * NB + at] * ld_or0(f, ne);
-            s = s + coef[P_LON1 * NB + at] * ld_or0(f, nw);
-            s = s + coef[P_TH1 * NB + at] * fkp[ij];
-            s = s + coef[P_TH2 * NB + at] * fkm[ij];
-            s = s + coef[P_SIG1 * NB + at] * fmp[ij];
-            s = s + coef[P_SIG2 * NB + at] * fmm[ij];
-            fout[at] = s;
+            float s = ld_stream(coef, P_SUM * NB + at) * f[ij];
+            s = s + ld_stream(coef, P_LAT2 * NB + at) * ld_or0(f, nn);
+            s = s + ld_stream(coef, P_LAT1 * NB + at) * ld_or0(f, ns);
+            s = s + ld_stream(coef, P_LON2 * NB + at) * ld_or0(f, ne);
+            s = s + ld_stream(coef, P_LON1 * NB + at) * ld_or0(f, nw);
+            s = s + ld_stream(coef, P_TH1 * NB + at) * fkp[ij];
+            s = s + ld_stream(coef, P_TH2 * NB + at) * fkm[ij];
+            s = s + ld_stream(coef, P_SIG1 * NB + at) * fmp[ij];
+            s = s + ld_stream(coef, P_SIG2 * NB + at) * fmm[ij];
+            __builtin_nontemporal_store(s, &fout[at]);
         }
     }
 }
+
+// PROP_CURR for KL = KLT directions: the point's spectrum rows M-1, M and
+// M+1 (all directions) are held in registers, so the direction and
+// frequency neighbours (cyclic K+1, K-1; M+1, M-1 clamped at the ends) cost
+// no memory access; only the four spatial neighbours and the weights are
+// loaded.  Same sums in the same order as propags_curr_kernel.
+template <int KLT>
+__global__ void __launch_bounds__(256) propags_curr_rows_kernel(DevTables t, const float *__restrict__ coef,
+                                                                 const float *__restrict__ fin,
+                                                                 float *__restrict__ fout)
+{
+    const int nch = (t.NSEA + 255) / 256;
+    const int ch = xcd_chunk(nch);
+    if (ch >= nch) return;
+    const int ij = ch * 256 + threadIdx.x;
+    if (ij >= t.NSEA) return;
+    const int ML = t.ML, NS = t.NSEA;
+    const size_t NB = (size_t)KLT * ML * NS;
+    const int ns = t.KLAT_S[ij], nn = t.KLAT_N[ij], nw = t.KLON_W[ij], ne = t.KLON_E[ij];
+    float prev[KLT], cur[KLT], nxt[KLT];
+#pragma unroll
+    for (int k = 0; k < KLT; k++) cur[k] = fin[(size_t)k * NS + ij];
+#pragma unroll
+    for (int k = 0; k < KLT; k++) prev[k] = cur[k];
+    for (int m = 0; m < ML; m++) {
+        if (m < ML - 1) {
+#pragma unroll
+            for (int k = 0; k < KLT; k++) nxt[k] = fin[((size_t)(m + 1) * KLT + k) * NS + ij];
+        } else {
+#pragma unroll
+            for (int k = 0; k < KLT; k++) nxt[k] = cur[k];
+        }
+#pragma unroll
+        for (int k = 0; k < KLT; k++) {
+            const int kp = k + 1 < KLT ? k + 1 : 0, km = k > 0 ? k - 1 : KLT - 1;
+            const size_t row = ((size_t)m * KLT + k) * NS;
+            const float *f = fin + row;
+            const size_t at = row + ij;
+            float s = ld_stream(coef, P_SUM * NB + at) * cur[k];
+            s = s + ld_stream(coef, P_LAT2 * NB + at) * ld_or0(f, nn);
+            s = s + ld_stream(coef, P_LAT1 * NB + at) * ld_or0(f, ns);
+            s = s + ld_stream(coef, P_LON2 * NB + at) * ld_or0(f, ne);
+            s = s + ld_stream(coef, P_LON1 * NB + at) * ld_or0(f, nw);
+            s = s + ld_stream(coef, P_TH1 * NB + at) * cur[kp];
+            s = s + ld_stream(coef, P_TH2 * NB + at) * cur[km];
+            s = s + ld_stream(coef, P_SIG1 * NB + at) * nxt[k];
+            s = s + ld_stream(coef, P_SIG2 * NB + at) * prev[k];
+            __builtin_nontemporal_store(s, &fout[at]);
+        }
+#pragma unroll
+        for (int k = 0; k < KLT; k++) { prev[k] = cur[k]; cur[k] = nxt[k]; }
+    }
+}
+
+template __global__ void propags_curr_rows_kernel<24>(DevTables, const float *, const float *, float *);
+template __global__ void propags_curr_rows_kernel<36>(DevTables, const float *, const float *, float *);
 
 // WAVNU2 for every (sea point, frequency): WN and CGG as [ML][NSEA].
 __global__ void wavnu2_kernel(DevTables t, float *__restrict__ wn, float *__restrict__ cgg)
