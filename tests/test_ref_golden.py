@@ -1,0 +1,103 @@
+"""The oracle against the reference's own IMPLSCH: golden vectors written by
+tests/golden/make_ref_implsch.py, which runs wam_source_module.f90:374-700
+built from the reference sources (oracle/Makefile.ref).  CPU only.
+
+Tolerances.  The oracle evaluates the Fortran intrinsics with wam_math.h
+(deterministic single precision, <= 2 ulp for exp/log/tanh); the reference
+used flang's runtime and glibc libm, so individual elementary-function
+values can differ by an ulp or two, which the implicit scheme carries into
+the spectrum.  Bars:
+  * spectrum after one step: |F - F_ref| <= 2e-5 * max(|F_ref|, 1e-6 max|F_ref|)
+    per bin, and the 99.9th percentile of that ratio <= 1e-5;
+  * per-point outputs (USTAR, TAUW, Z0, PHIAW, energy fluxes, and the stress
+    vectors measured against their magnitude) after one step <= 1e-5 rel;
+  * Hs, mean period and USTAR after 24 h of source integration (288 steps of
+    300 s) <= 1e-4 rel, spectrum <= 1e-4."""
+import glob
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIXTURES = sorted(glob.glob(os.path.join(HERE, "golden", "ref_implsch_*.npz")))
+OUT_FLUX = ["phioc", "tauoc_x", "tauoc_y", "phibot", "taubot_x", "taubot_y"]
+
+
+def load(path):
+    z = np.load(path)
+    return dict(z), json.loads(str(z["cfg"]))
+
+
+def spec_err(f, ref):
+    ref = np.asarray(ref, np.float64)
+    den = np.maximum(np.abs(ref), 1e-6 * np.abs(ref).max())
+    return np.abs(np.asarray(f, np.float64) - ref) / den
+
+
+def rel(a, b, floor=1e-30):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor)))
+
+
+def integrated(cfg, dfim, fr, f, n):
+    e = f.reshape(cfg["ml"], cfg["kl"], n).sum(1).astype(np.float64)
+    tot = (e * dfim[:, None]).sum(0)
+    return 4.0 * np.sqrt(tot), tot / (e * (dfim / fr)[:, None]).sum(0)
+
+
+def run_oracle(z, cfg):
+    o = Oracle(cfg, z["mask"])
+    o.prepare_propagation(z["in_depth"], None, None)
+    fl3, tw, z0 = z["in_fl3"].copy(), z["in_tauw"].copy(), z["in_z0"].copy()
+    for _ in range(int(z["nstep"])):
+        q = o.implsch(fl3, z["in_u10"], z["in_udir"], tw, z0, z["in_depth"], roairn=z["in_roairn"],
+                      wstar=z["in_wstar"], lcflx=bool(int(z["lcflx"])))
+        fl3, tw, z0 = q["fl3"], q["tauw"], q["z0"]
+    return o, q
+
+
+def check(z, cfg, get, dfim, fr):
+    """get(name) returns the implementation's output of that name."""
+    n = z["in_u10"].size
+    nstep = int(z["nstep"])
+    err = spec_err(get("fl3"), z["out_fl3"])
+    hs, tm = integrated(cfg, dfim, fr, get("fl3"), n)
+    hs_r, tm_r = integrated(cfg, dfim, fr, z["out_fl3"], n)
+    if nstep == 1:
+        assert err.max() <= 2e-5 and np.quantile(err, 0.999) <= 1e-5, (err.max(), np.quantile(err, 0.999))
+        for k, floor in (("ustar", 1e-6), ("tauw", 1e-9), ("z0", 1e-12), ("phiaw", 1e-9)):
+            assert rel(get(k), z["out_" + k], floor) <= 1e-5, k
+        if int(z["lcflx"]):
+            for k in ("phioc", "phibot"):
+                assert rel(get(k), z["out_" + k], 1e-6 * np.abs(z["out_" + k]).max() + 1e-30) <= 1e-5, k
+            # stresses as vectors: a component near zero is a cancellation of
+            # terms of the vector's size, so its error is measured against that
+            for kx, ky in (("tauoc_x", "tauoc_y"), ("taubot_x", "taubot_y")):
+                dx = np.asarray(get(kx), np.float64) - z["out_" + kx]
+                dy = np.asarray(get(ky), np.float64) - z["out_" + ky]
+                mag = np.hypot(z["out_" + kx].astype(np.float64), z["out_" + ky].astype(np.float64))
+                assert np.max(np.hypot(dx, dy) / np.maximum(mag, 1e-30)) <= 1e-5, kx
+    else:
+        assert err.max() <= 1e-4, err.max()
+        assert rel(hs, hs_r) <= 1e-4 and rel(tm, tm_r) <= 1e-4
+        assert rel(get("ustar"), z["out_ustar"], 1e-6) <= 1e-4
+    assert rel(hs, hs_r) <= 1e-4
+
+
+@pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(p)[12:-4] for p in FIXTURES])
+def test_oracle_matches_reference_implsch(path):
+    z, cfg = load(path)
+    o, q = run_oracle(z, cfg)
+    flux = dict(zip(OUT_FLUX, q["flux"]))
+    check(z, cfg, lambda k: flux[k] if k in flux else q[k], o.get("DFIM"), o.get("FR"))
+
+
+def test_fixtures_present():
+    names = {os.path.basename(p) for p in FIXTURES}
+    assert {"ref_implsch_c2sub_janssen_1step.npz", "ref_implsch_c2sub_st6_1step.npz",
+            "ref_implsch_c1_st6_1step.npz", "ref_implsch_c2sub_janssen_24h.npz"} <= names

@@ -325,7 +325,8 @@ int wam_gpu_implsch(wam_ctx *ctx, int32_t nsea, int32_t kl, int32_t ml, float *f
     (void)depth;
     (void)indep;
     const size_t nb = (size_t)t.KL * t.ML * nsea, ns = nsea;
-    int rc = ensure_stage(ctx, nb + 13 * ns);
+    // FL3, then 8 per-point arrays (U10 UDIR TAUW USTAR Z0 ROAIRN WSTAR PHIAW) and 6 flux rows
+    int rc = ensure_stage(ctx, nb + 14 * ns);
     if (rc) return rc;
     float *d = ctx->stage;
     float *dfl = d, *du = d + nb, *dd = du + ns, *dt = dd + ns, *dus = dt + ns, *dz = dus + ns, *dr = dz + ns,
