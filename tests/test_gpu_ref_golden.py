@@ -8,7 +8,7 @@ import os
 import numpy as np
 import pytest
 
-from test_ref_golden import FIXTURES, OUT_FLUX, check, load
+from test_ref_golden import FIXTURES, OUT_FLUX, PROP_FIXTURES, check, check_propags, load
 
 pytestmark = pytest.mark.gpu
 
@@ -29,4 +29,19 @@ def test_gpu_matches_reference_implsch(path):
         fl3, tw, z0 = q["fl3"], q["tauw"], q["z0"]
     flux = dict(zip(OUT_FLUX, q["flux"]))
     check(z, cfg, lambda k: flux[k] if k in flux else q[k], g.table("DFIM"), g.table("FR"))
+    g.close()
+
+
+@pytest.mark.parametrize("path", PROP_FIXTURES, ids=[os.path.basename(p)[12:-4] for p in PROP_FIXTURES])
+def test_gpu_matches_reference_propags(path):
+    from wam_amd import WamGpu
+    z, cfg = load(path)
+    print(f"[case] {os.path.basename(path)}", flush=True)
+    g = WamGpu(cfg, z["mask"])
+    cur = (z["in_u"], z["in_v"]) if cfg["refr_c"] else (None, None)
+    counter, cfl = g.prepare_propagation(z["in_depth"], *cur)
+    f = z["in_fl3"]
+    for _ in range(int(z["nprop"])):
+        f = g.propags(f)
+    check_propags(z, counter, cfl, g.table("INDEP") if cfg["shallow"] else None, f)
     g.close()
