@@ -8,6 +8,15 @@
  * Arrays follow the Fortran layout of the reference: FL3(IJ,K,M) is
  * fl3[(m*KL + k)*NIJ + ij] (0-based), sea point fastest.
  *
+ * What pins it (tests/): IMPLSCH (every routine it calls, both physics) and
+ * PREPARE_PROPAGATION + PROPAGS against golden vectors written by running the
+ * reference's own Fortran (oracle/Makefile.ref, ref_*_driver.f90); the DIA
+ * tables against the reference's logged tables; grids, shallow tables,
+ * FLMINFR and neighbour tables against closed forms, the dispersion relation
+ * and brute force.  Parity unpinned: the elementary functions themselves
+ * (wam_math.h vs the reference's libm), beyond the end-to-end bars of those
+ * tests.
+ *
  * Build FP32 (default, the reference's REAL) or FP64 (-DORC_FP64: the
  * reference built with 8-byte reals); -DORC_OPCOUNT (C++) is the FP32
  * arithmetic with every operation counted (orc_opcount.h).  The FP32 build evaluates the Fortran
