@@ -81,6 +81,16 @@ int wam_gpu_init_host(wam_ctx **ctx, const wam_config *cfg, const uint8_t *sea_m
    depth only).  Must precede IMPLSCH and PROPAGS. */
 int wam_gpu_prepare_propagation(wam_ctx *ctx, const float *depth, const float *u, const float *v);
 
+/* Where PROPAGS's weights live.  No counterpart in the reference, which
+   always stores its COEF_* arrays (wam_propagation_module.f90:75-110):
+   stored = 1 keeps the weight planes in HBM (5 or 9 planes of NSEA*KL*ML
+   floats) and streams them in every sub-step; stored = 0 computes them
+   inside the stencil from the per-point fields; stored = -1, the default,
+   computes them for shallow water with current refraction and stores them
+   otherwise.  Results are bit-identical in every mode.  Takes effect at the
+   next wam_gpu_prepare_propagation. */
+int wam_gpu_set_propagation_weights(wam_ctx *ctx, int32_t stored);
+
 /* Drop-in PROPAGS: F3(NIJS:NIJL,KL,ML) in host memory, updated in place. */
 int wam_gpu_propags(wam_ctx *ctx, int32_t nijs, int32_t nijl, int32_t kl, int32_t ml, float *f3);
 

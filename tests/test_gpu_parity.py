@@ -61,25 +61,31 @@ def case(request):
     print(f"case {request.param}: iphys={cfg['iphys']} shallow={cfg['shallow']} refr_c={cfg['refr_c']} "
           f"{cfg['kl']}x{cfg['ml']}", flush=True)
     o = Oracle(cfg, mask)
-    g = WamGpu(cfg, mask)
+    g = WamGpu(cfg, mask)                   # weights computed in the PROPAGS stencil
+    g.set_propagation_weights(False)
+    gs = WamGpu(cfg, mask)                  # weight planes stored in HBM
+    gs.set_propagation_weights(True)
     co = o.prepare_propagation(f["depth"], f["u"], f["v"])
     cg = g.prepare_propagation(f["depth"], f["u"], f["v"])
+    cs = gs.prepare_propagation(f["depth"], f["u"], f["v"])
     fl3 = W.jonswap_spectrum(cfg, f["udir"]).ravel()
-    return dict(name=request.param, cfg=cfg, f=f, o=o, g=g, co=co, cg=cg, fl3=fl3)
+    return dict(name=request.param, cfg=cfg, f=f, o=o, g=g, gs=gs, co=co, cg=cg, cs=cs, fl3=fl3)
 
 
 def test_propagation_weights_and_cfl(case):
-    o, g = case["o"], case["g"]
-    assert case["co"][0] == case["cg"][0]                      # COUNTER
-    assert np.float32(case["co"][1]) == np.float32(case["cg"][1])   # max CFL
-    assert bits_equal(g.table("COEF"), o.get("COEF"))
+    o, gs = case["o"], case["gs"]
+    for c in (case["cg"], case["cs"]):                          # both weight modes
+        assert case["co"][0] == c[0]                            # COUNTER
+        assert np.float32(case["co"][1]) == np.float32(c[1])    # max CFL
+    assert bits_equal(gs.table("COEF"), o.get("COEF"))
     if case["cfg"]["shallow"]:
-        assert np.array_equal(g.table("INDEP"), o.get("INDEP"))
+        assert np.array_equal(case["g"].table("INDEP"), o.get("INDEP"))
 
 
-def test_propags_bit_exact(case):
+@pytest.mark.parametrize("mode", ["computed", "stored"])
+def test_propags_bit_exact(case, mode):
     a = case["o"].propags(case["fl3"])
-    b = case["g"].propags(case["fl3"])
+    b = (case["g"] if mode == "computed" else case["gs"]).propags(case["fl3"])
     assert bits_equal(b, a)
 
 

@@ -20,6 +20,11 @@ __global__ void propags_nocurr_kernel(DevTables t, const float *coef, const floa
 __global__ void propags_curr_kernel(DevTables t, const float *coef, const float *fin, float *fout);
 template <int KLT>
 __global__ void propags_curr_rows_kernel(DevTables t, const float *coef, const float *fin, float *fout);
+template <bool CURR>
+__global__ void cfl_minmax_kernel(DevTables t, float *out);
+template <bool CURR>
+__global__ void propags_fused_kernel(DevTables t, int counter, const float *fin, float *fout);
+__global__ void propags_fused_sc_kernel(DevTables t, int counter, const float *fin, float *fout);
 __global__ void wavnu2_kernel(DevTables t, float *wn, float *cgg);
 hipError_t launch_implsch(const DevTables &t, const ImplArgs &a, int ncu, hipStream_t st);
 }  // namespace wam
@@ -33,8 +38,9 @@ struct wam_ctx {
     hipStream_t own_stream = nullptr;
     std::vector<void *> dev_allocs;       // static tables (freed at finalize)
     std::vector<void *> point_allocs;     // per depth field
-    float *coef = nullptr;
+    float *coef = nullptr;          // stored weight planes (NULL: computed in the stencil)
     int ncoef = 0;
+    int weights_mode = -1;          // wam_gpu_set_propagation_weights: -1 auto, 0 computed, 1 stored
     int counter = 1;
     float cfl_max = 0.f, cfl_min = 0.f;
     float *wn = nullptr, *cgg = nullptr;
@@ -183,29 +189,52 @@ int wam_gpu_prepare_propagation(wam_ctx *ctx, const float *depth, const float *u
 
     const size_t nb = (size_t)t.KL * t.ML * ns;
     ctx->ncoef = t.cfg.refr_c ? 9 : 5;
-    void *p = nullptr;
-    HIPCHK(hipMalloc(&p, nb * ctx->ncoef * sizeof(float)));
-    o.push_back(p);
-    ctx->coef = (float *)p;
     hipStream_t st = ctx->stream;
-    hipLaunchKernelGGL(prep_coef_kernel, dim3((ns + 255) / 256, t.KL * t.ML), dim3(256), 0, st, ctx->dt, ctx->coef);
-    HIPCHK(hipGetLastError());
-    // CHECK_CFL
-    const int nblk = 1024;
-    float *mm = nullptr;
-    HIPCHK(hipMalloc(&mm, 2 * nblk * sizeof(float) * ctx->ncoef));
-    for (int pl = 0; pl < ctx->ncoef; pl++)
-        hipLaunchKernelGGL(minmax_kernel, dim3(nblk), dim3(256), 0, st, ctx->coef + pl * nb, nb, mm + 2 * nblk * pl);
-    std::vector<float> h(2 * nblk * ctx->ncoef);
-    HIPCHK(hipMemcpyAsync(h.data(), mm, h.size() * sizeof(float), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    (void)hipFree(mm);
     std::vector<float> lo(ctx->ncoef, 3.4e38f), hi(ctx->ncoef, -3.4e38f);
-    for (int pl = 0; pl < ctx->ncoef; pl++)
-        for (int b = 0; b < nblk; b++) {
-            lo[pl] = fminf(lo[pl], h[2 * nblk * pl + 2 * b]);
-            hi[pl] = fmaxf(hi[pl], h[2 * nblk * pl + 2 * b + 1]);
-        }
+    void *p = nullptr;
+    // auto: computed in the stencil where the hoisted shallow-water + current
+    // kernel applies (it is faster than streaming nine planes), else stored
+    const bool stored = ctx->weights_mode == 1 ||
+                        (ctx->weights_mode == -1 && !(t.cfg.shallow && t.cfg.refr_c));
+    if (stored) {
+        // the reference's COEF_* planes in HBM, then CHECK_CFL over them
+        HIPCHK(hipMalloc(&p, nb * ctx->ncoef * sizeof(float)));
+        o.push_back(p);
+        ctx->coef = (float *)p;
+        hipLaunchKernelGGL(prep_coef_kernel, dim3((ns + 255) / 256, t.KL * t.ML), dim3(256), 0, st, ctx->dt, ctx->coef);
+        HIPCHK(hipGetLastError());
+        const int nblk = 1024;
+        float *mm = nullptr;
+        HIPCHK(hipMalloc(&mm, 2 * nblk * sizeof(float) * ctx->ncoef));
+        for (int pl = 0; pl < ctx->ncoef; pl++)
+            hipLaunchKernelGGL(minmax_kernel, dim3(nblk), dim3(256), 0, st, ctx->coef + pl * nb, nb, mm + 2 * nblk * pl);
+        std::vector<float> h(2 * nblk * ctx->ncoef);
+        HIPCHK(hipMemcpyAsync(h.data(), mm, h.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        (void)hipFree(mm);
+        for (int pl = 0; pl < ctx->ncoef; pl++)
+            for (int b = 0; b < nblk; b++) {
+                lo[pl] = fminf(lo[pl], h[2 * nblk * pl + 2 * b]);
+                hi[pl] = fmaxf(hi[pl], h[2 * nblk * pl + 2 * b + 1]);
+            }
+    } else {
+        // CHECK_CFL over weights computed on the fly; PROPAGS recomputes them
+        const int nblk = (ns + 255) / 256;
+        float *mm = nullptr;
+        HIPCHK(hipMalloc(&mm, (size_t)nblk * 2 * P_NPLANES * sizeof(float)));
+        if (t.cfg.refr_c) hipLaunchKernelGGL(cfl_minmax_kernel<true>, dim3(nblk), dim3(256), 0, st, ctx->dt, mm);
+        else hipLaunchKernelGGL(cfl_minmax_kernel<false>, dim3(nblk), dim3(256), 0, st, ctx->dt, mm);
+        HIPCHK(hipGetLastError());
+        std::vector<float> h((size_t)nblk * 2 * P_NPLANES);
+        HIPCHK(hipMemcpyAsync(h.data(), mm, h.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        (void)hipFree(mm);
+        for (int b = 0; b < nblk; b++)
+            for (int pl = 0; pl < ctx->ncoef; pl++) {
+                lo[pl] = fminf(lo[pl], h[((size_t)b * P_NPLANES + pl) * 2]);
+                hi[pl] = fmaxf(hi[pl], h[((size_t)b * P_NPLANES + pl) * 2 + 1]);
+            }
+    }
     // plane order: no current SUM LAT1 LON1 TH1 TH2; current SUM LAT1 LAT2 LON1 LON2 TH1 TH2 SIG1 SIG2
     float lat11mx, lat11mn, lon2mx = 0.1f, lon2mn = 0.1f, s1mx = 0.1f, s1mn = 0.1f, s2mx = 0.1f, s2mn = 0.1f;
     float lon1mx, th1mx, th1mn, th2mx, th2mn;
@@ -229,8 +258,11 @@ int wam_gpu_prepare_propagation(wam_ctx *ctx, const float *depth, const float *u
     ctx->counter = 1;
     if (cmax > 0.999f) {
         ctx->counter = (int)std::ceil(cmax);
-        hipLaunchKernelGGL(rescale_kernel, dim3(4096), dim3(256), 0, st, ctx->coef, nb, ctx->ncoef, (float)ctx->counter);
-        HIPCHK(hipGetLastError());
+        if (ctx->coef) {
+            hipLaunchKernelGGL(rescale_kernel, dim3(4096), dim3(256), 0, st, ctx->coef, nb, ctx->ncoef,
+                               (float)ctx->counter);
+            HIPCHK(hipGetLastError());
+        }
     }
     if (t.cfg.iphys == 2) {
         HIPCHK(hipMalloc(&p, 2 * (size_t)t.ML * ns * sizeof(float)));
@@ -264,7 +296,17 @@ int wam_gpu_propags_dev(wam_ctx *ctx, const float *d_in, float *d_out)
     for (int i = 0; i < ctx->counter; i++) {
         const int left = ctx->counter - 1 - i;
         float *dst = (left % 2 == 0) ? d_out : ctx->fscratch;
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, ctx->stream, ctx->dt, ctx->coef, src, dst);
+        if (ctx->coef)
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, ctx->stream, ctx->dt, ctx->coef, src, dst);
+        else if (t.cfg.refr_c && t.cfg.shallow)
+            hipLaunchKernelGGL(propags_fused_sc_kernel, dim3(grid), dim3(256), 0, ctx->stream, ctx->dt, ctx->counter,
+                               src, dst);
+        else if (t.cfg.refr_c)
+            hipLaunchKernelGGL(propags_fused_kernel<true>, dim3(grid), dim3(256), 0, ctx->stream, ctx->dt, ctx->counter,
+                               src, dst);
+        else
+            hipLaunchKernelGGL(propags_fused_kernel<false>, dim3(grid), dim3(256), 0, ctx->stream, ctx->dt,
+                               ctx->counter, src, dst);
         HIPCHK(hipGetLastError());
         src = dst;
     }
@@ -348,6 +390,15 @@ int wam_gpu_implsch(wam_ctx *ctx, int32_t nsea, int32_t kl, int32_t ml, float *f
     if (phiaw) HIPCHK(hipMemcpyAsync(phiaw, dph, ns * 4, hipMemcpyDeviceToHost, st));
     if (fluxes && lcflx) HIPCHK(hipMemcpyAsync(fluxes, dfx, 6 * ns * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    return WAM_OK;
+}
+
+int wam_gpu_set_propagation_weights(wam_ctx *ctx, int32_t stored)
+{
+    if (!ctx) return WAM_E_ARG;
+    if (stored < -1 || stored > 1) { ctx->err = "stored must be -1, 0 or 1"; return WAM_E_ARG; }
+    ctx->weights_mode = stored;
+    ctx->prepared = false;      // takes effect at the next wam_gpu_prepare_propagation
     return WAM_OK;
 }
 

@@ -54,17 +54,16 @@ __device__ __forceinline__ float sidc(const DevTables &t, int ij, int k, int m)
     return (s0 * cg_at(t, ij, m) + tempd * t.TSIHKD[ig]) * t.TFAK[ig];
 }
 
-__global__ void prep_coef_kernel(DevTables t, float *__restrict__ coef)
+// The propagation weights of one (sea point, direction, frequency), in the
+// plane order of wam_dev.hpp (without currents: SUM, LAT1, LON1, TH1, TH2).
+// CURR selects the current-refraction formulas (t.refr_c) at compile time.
+template <bool CURR>
+__device__ __forceinline__ void coef_bin(const DevTables &t, int ij, int m, int k, float *__restrict__ c)
 {
-    const int ij = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ij >= t.NSEA) return;
-    const int KL = t.KL, ML = t.ML;
-    const int bin = blockIdx.y, m = bin / KL, k = bin % KL;
-    const size_t NB = (size_t)KL * ML * t.NSEA;
-    const size_t at = (size_t)bin * t.NSEA + ij;
+    const int ML = t.ML;
     const int kp = t.KPM_P[k] - 1, km = t.KPM_M[k] - 1;
     const float sd = t.SINTH[k], cd = t.COSTH[k];
-    if (!t.refr_c && !t.shallow) {                         // PREP_SPHER_DEEP
+    if (!CURR && !t.shallow) {                             // PREP_SPHER_DEEP
         const float sda = wam_fabsf(sd), cda = wam_fabsf(cd * t.DELPH0);
         const int ic = t.JYO1[k], id = t.JYO2[k];
         const float sp = t.DELTH0 * (t.SINTH[k] + t.SINTH[kp]);
@@ -75,12 +74,12 @@ __global__ void prep_coef_kernel(DevTables t, float *__restrict__ coef)
         float dtc = cda * ((id == 1 ? t.DPSN_S[ij] : t.DPSN_N[ij]) + 1.f) + dle;
         const float dtp = t.DRGM[ij] * sp, dtm = t.DRGM[ij] * sm;
         dtc = (dtc + wam_maxf(0.f, dtp)) - wam_minf(0.f, dtm);
-        coef[0 * NB + at] = 1.f - gom * dtc;
-        coef[1 * NB + at] = gom * dpn;
-        coef[2 * NB + at] = gom * dle;
-        coef[3 * NB + at] = -(gom * wam_minf(0.f, dtp));
-        coef[4 * NB + at] = gom * wam_maxf(0.f, dtm);
-    } else if (!t.refr_c) {                                // PREP_SPHER_SHALLOW
+        c[0] = 1.f - gom * dtc;
+        c[1] = gom * dpn;
+        c[2] = gom * dle;
+        c[3] = -(gom * wam_minf(0.f, dtp));
+        c[4] = gom * wam_maxf(0.f, dtm);
+    } else if (!CURR) {                                    // PREP_SPHER_SHALLOW
         const float sda = wam_fabsf(sd), cda = wam_fabsf(cd * t.DELPH0);
         const float sp = t.DELTH0 * (t.SINTH[k] + t.SINTH[kp]);
         const float sm = t.DELTH0 * (t.SINTH[k] + t.SINTH[km]);
@@ -101,11 +100,11 @@ __global__ void prep_coef_kernel(DevTables t, float *__restrict__ coef)
             dtp = dtp * sp + (thk + thdd(t, ij, kp, m)) * t.DELTHR;
         } else { dtm = dtp * sm; dtp = dtp * sp; }
         dtc = (dtc - wam_maxf(0.f, dtp)) + wam_minf(0.f, dtm);
-        coef[0 * NB + at] = dtc;
-        coef[1 * NB + at] = lat;
-        coef[2 * NB + at] = lon;
-        coef[3 * NB + at] = -wam_minf(0.f, dtp);
-        coef[4 * NB + at] = wam_maxf(0.f, dtm);
+        c[0] = dtc;
+        c[1] = lat;
+        c[2] = lon;
+        c[3] = -wam_minf(0.f, dtp);
+        c[4] = wam_maxf(0.f, dtm);
     } else {                                               // current refraction, regular grid
         const int mm = (m > 0 ? m - 1 : 0), mp = (m < ML - 1 ? m + 1 : ML - 1);
         const float dfp = t.DELFR0 / t.FR[m], dfm = t.DELFR0 / t.FR[mm];
@@ -153,15 +152,35 @@ __global__ void prep_coef_kernel(DevTables t, float *__restrict__ coef)
             s1 = -wam_minf(0.f, dom) / 1.1f;
             s2 = wam_maxf(0.f, dom) * 1.1f;
         }
-        coef[P_SUM * NB + at] = dtc;
-        coef[P_LAT1 * NB + at] = wam_maxf(0.f, dps);
-        coef[P_LAT2 * NB + at] = -wam_minf(0.f, dpn);
-        coef[P_LON1 * NB + at] = wam_maxf(0.f, dlw);
-        coef[P_LON2 * NB + at] = -wam_minf(0.f, dle);
-        coef[P_TH1 * NB + at] = -wam_minf(0.f, dtp);
-        coef[P_TH2 * NB + at] = wam_maxf(0.f, dtm);
-        coef[P_SIG1 * NB + at] = s1;
-        coef[P_SIG2 * NB + at] = s2;
+        c[P_SUM] = dtc;
+        c[P_LAT1] = wam_maxf(0.f, dps);
+        c[P_LAT2] = -wam_minf(0.f, dpn);
+        c[P_LON1] = wam_maxf(0.f, dlw);
+        c[P_LON2] = -wam_minf(0.f, dle);
+        c[P_TH1] = -wam_minf(0.f, dtp);
+        c[P_TH2] = wam_maxf(0.f, dtm);
+        c[P_SIG1] = s1;
+        c[P_SIG2] = s2;
+    }
+}
+
+__global__ void prep_coef_kernel(DevTables t, float *__restrict__ coef)
+{
+    const int ij = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ij >= t.NSEA) return;
+    const int KL = t.KL;
+    const int bin = blockIdx.y, m = bin / KL, k = bin % KL;
+    const size_t NB = (size_t)KL * t.ML * t.NSEA;
+    const size_t at = (size_t)bin * t.NSEA + ij;
+    float c[P_NPLANES];
+    if (t.refr_c) {
+        coef_bin<true>(t, ij, m, k, c);
+#pragma unroll
+        for (int p = 0; p < P_NPLANES; p++) coef[p * NB + at] = c[p];
+    } else {
+        coef_bin<false>(t, ij, m, k, c);
+#pragma unroll
+        for (int p = 0; p < 5; p++) coef[p * NB + at] = c[p];
     }
 }
 
@@ -339,6 +358,219 @@ __global__ void __launch_bounds__(256) propags_curr_rows_kernel(DevTables t, con
 
 template __global__ void propags_curr_rows_kernel<24>(DevTables, const float *, const float *, float *);
 template __global__ void propags_curr_rows_kernel<36>(DevTables, const float *, const float *, float *);
+
+// CHECK_CFL's minima and maxima of every weight plane without storing the
+// planes: a thread computes its sea point's weights bin by bin, the workgroup
+// reduces, out[blk][plane][2] = {min, max} (order-free, exact).
+template <bool CURR>
+__global__ void __launch_bounds__(256) cfl_minmax_kernel(DevTables t, float *__restrict__ out)
+{
+    __shared__ float red[2 * P_NPLANES][256];
+    const int ij = blockIdx.x * 256 + threadIdx.x;
+    constexpr int np = CURR ? P_NPLANES : 5;
+    float lo[P_NPLANES], hi[P_NPLANES];
+#pragma unroll
+    for (int p = 0; p < P_NPLANES; p++) { lo[p] = 3.4e38f; hi[p] = -3.4e38f; }
+    if (ij < t.NSEA) {
+        for (int m = 0; m < t.ML; m++)
+            for (int k = 0; k < t.KL; k++) {
+                float c[P_NPLANES];
+                coef_bin<CURR>(t, ij, m, k, c);
+#pragma unroll
+                for (int p = 0; p < np; p++) { lo[p] = fminf(lo[p], c[p]); hi[p] = fmaxf(hi[p], c[p]); }
+            }
+    }
+#pragma unroll
+    for (int p = 0; p < P_NPLANES; p++) { red[2 * p][threadIdx.x] = lo[p]; red[2 * p + 1][threadIdx.x] = hi[p]; }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int p = 0; p < P_NPLANES; p++) {
+                red[2 * p][threadIdx.x] = fminf(red[2 * p][threadIdx.x], red[2 * p][threadIdx.x + w]);
+                red[2 * p + 1][threadIdx.x] = fmaxf(red[2 * p + 1][threadIdx.x], red[2 * p + 1][threadIdx.x + w]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 2 * P_NPLANES) out[(size_t)blockIdx.x * 2 * P_NPLANES + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// PROP_CURR / PROP_NO_CURR with the weights computed in place (coef_bin, then
+// CHECK_CFL's rescale for COUNTER > 1 with the same expressions as
+// rescale_kernel), so no weight planes are read: the stencil's HBM traffic is
+// the spectrum alone.  Same sums in the same order as the stored-weight kernels.
+template <bool CURR>
+__global__ void __launch_bounds__(256) propags_fused_kernel(DevTables t, int counter, const float *__restrict__ fin,
+                                                             float *__restrict__ fout)
+{
+    const int nch = (t.NSEA + 255) / 256;
+    const int ch = xcd_chunk(nch);
+    if (ch >= nch) return;
+    const int ij = ch * 256 + threadIdx.x;
+    if (ij >= t.NSEA) return;
+    const int KL = t.KL, ML = t.ML, NS = t.NSEA;
+    const float rc = (float)counter;
+    const int ns = t.KLAT_S[ij], nn = t.KLAT_N[ij], nw = t.KLON_W[ij], ne = t.KLON_E[ij];
+    for (int m = 0; m < ML; m++) {
+        const int mm = m > 0 ? m - 1 : 0, mp = m < ML - 1 ? m + 1 : ML - 1;
+        for (int k = 0; k < KL; k++) {
+            float c[P_NPLANES];
+            coef_bin<CURR>(t, ij, m, k, c);
+            if (counter > 1) {
+                c[0] = 1.f - (1.f - c[0]) / rc;
+#pragma unroll
+                for (int p = 1; p < (CURR ? P_NPLANES : 5); p++) c[p] = c[p] / rc;
+            }
+            const size_t row = ((size_t)m * KL + k) * NS;
+            const float *f = fin + row;
+            const float *fkp = fin + ((size_t)m * KL + (t.KPM_P[k] - 1)) * NS;
+            const float *fkm = fin + ((size_t)m * KL + (t.KPM_M[k] - 1)) * NS;
+            float s;
+            if (CURR) {
+                const float *fmp = fin + ((size_t)mp * KL + k) * NS;
+                const float *fmm = fin + ((size_t)mm * KL + k) * NS;
+                s = c[P_SUM] * f[ij];
+                s = s + c[P_LAT2] * ld_or0(f, nn);
+                s = s + c[P_LAT1] * ld_or0(f, ns);
+                s = s + c[P_LON2] * ld_or0(f, ne);
+                s = s + c[P_LON1] * ld_or0(f, nw);
+                s = s + c[P_TH1] * fkp[ij];
+                s = s + c[P_TH2] * fkm[ij];
+                s = s + c[P_SIG1] * fmp[ij];
+                s = s + c[P_SIG2] * fmm[ij];
+            } else {
+                const int nlat = t.JYO1[k] == 1 ? ns : nn;
+                const int nlon = t.JXO1[k] == 1 ? nw : ne;
+                s = c[0] * f[ij] + c[1] * ld_or0(f, nlat);
+                s = s + c[2] * ld_or0(f, nlon);
+                s = s + c[3] * fkp[ij];
+                s = s + c[4] * fkm[ij];
+            }
+            __builtin_nontemporal_store(s, &fout[row + ij]);
+        }
+    }
+}
+
+// PROP_CURR with PREP_SPHER_SHALLOW_CURR_REG's weights computed in place, for
+// shallow water with current refraction.  The same values as coef_bin<true>
+// (every weight is the same sequence of operations), with the work hoisted:
+// per-point fields once, the per-frequency group velocities and depth-table
+// factors carried across the M loop, the per-direction refraction terms in a
+// window carried across the (cyclic) K loop.  CHECK_CFL's rescale divides by
+// COUNTER; for a power of two that is the exact multiplication by 1/COUNTER.
+__global__ void __launch_bounds__(256) propags_fused_sc_kernel(DevTables t, int counter,
+                                                                const float *__restrict__ fin,
+                                                                float *__restrict__ fout)
+{
+    const int nch = (t.NSEA + 255) / 256;
+    const int ch = xcd_chunk(nch);
+    if (ch >= nch) return;
+    const int ij = ch * 256 + threadIdx.x;
+    if (ij >= t.NSEA) return;
+    const int KL = t.KL, ML = t.ML, NS = t.NSEA, ND = t.NDEPTH;
+    const int ns = t.KLAT_S[ij], nn = t.KLAT_N[ij], nw = t.KLON_W[ij], ne = t.KLON_E[ij];
+    // per point
+    const float u = t.U[ij], v = t.V[ij];
+    const float uw = nw < 0 ? 0.f : t.U[nw], ue = ne < 0 ? 0.f : t.U[ne];
+    const float vs = ns < 0 ? 0.f : t.V[ns], vn = nn < 0 ? 0.f : t.V[nn];
+    const int i0 = t.INDEP[ij] - 1;
+    const int iw = nw < 0 ? ND - 1 : t.INDEP[nw] - 1, ie = ne < 0 ? ND - 1 : t.INDEP[ne] - 1;
+    const int is = ns < 0 ? ND - 1 : t.INDEP[ns] - 1, in = nn < 0 ? ND - 1 : t.INDEP[nn] - 1;
+    const int ig = t.INDEP_G[ij] - 1;
+    const float ddphi = t.DDPHI[ij], ddlam = t.DDLAM[ij];
+    const float duphi = t.DUPHI[ij], dulam = t.DULAM[ij], dvphi = t.DVPHI[ij], dvlam = t.DVLAM[ij];
+    const float drgm = t.DRGM[ij], della0 = t.DELLA0[ij], dpsn_s = t.DPSN_S[ij], dpsn_n = t.DPSN_N[ij];
+    const float tempd = v * ddphi + u * ddlam;
+    const bool pow2 = (counter & (counter - 1)) == 0;
+    const float rc = (float)counter, rinv = 1.0f / rc;
+    // group velocity, TSIHKD and TFAK of the point's depth rows at frequency m
+    auto cgv = [&](int idx, int m) { return t.TCGOND[m * ND + idx]; };
+    float cg_m = cgv(i0, 0), ts_m = t.TSIHKD[ig], tf_m = t.TFAK[ig];
+    float cg_mm = cg_m, ts_mm = ts_m, tf_mm = tf_m;                 // M-1 clamps to 1
+    for (int m = 0; m < ML; m++) {
+        const int mp = m < ML - 1 ? m + 1 : ML - 1;
+        const float cg_mp = cgv(i0, mp), ts_mp = t.TSIHKD[mp * ND + ig], tf_mp = t.TFAK[mp * ND + ig];
+        const int mm = m > 0 ? m - 1 : 0;
+        const float dfp = t.DELFR0 / t.FR[m], dfm = t.DELFR0 / t.FR[mm];
+        const float cgw = cgv(iw, m), cge = cgv(ie, m), cgs = cgv(is, m), cgn = cgv(in, m);
+        // THDD(K, M) = temp(K)*TSIHKD(M) + THDC(K); window over K-1, K, K+1
+        auto thdd_k = [&](int k) {
+            const float s = t.SINTH[k], c = t.COSTH[k];
+            const float temp = s * ddphi - c * ddlam;
+            const float ss = s * s, sc = s * c, cc = c * c;
+            const float thdc = ((ss * duphi + sc * dvphi) - sc * dulam) - cc * dvlam;
+            return temp * ts_m + thdc;
+        };
+        float th_km = thdd_k(KL - 1), th_k = thdd_k(0), th_kp = thdd_k(KL > 1 ? 1 : 0);
+        for (int k = 0; k < KL; k++) {
+            const int kp = k + 1 < KL ? k + 1 : 0, km = k > 0 ? k - 1 : KL - 1;
+            const float sd = t.SINTH[k], cd = t.COSTH[k];
+            const float sp = t.DELTH0 * (t.SINTH[k] + t.SINTH[kp]);
+            const float sm = t.DELTH0 * (t.SINTH[k] + t.SINTH[km]);
+            const float w1 = u + sd * cg_m;
+            const float w2 = (v + cd * cg_m) * t.DELPH0;
+            const float dlw = (w1 + (uw + sd * cgw)) * della0;
+            const float dle = (w1 + (ue + sd * cge)) * della0;
+            float dtc = (1.f - wam_maxf(0.f, dle)) + wam_minf(0.f, dlw);
+            const float dps = w2 + dpsn_s * ((vs + cd * cgs) * t.DELPH0);
+            const float dpn = w2 + dpsn_n * ((vn + cd * cgn) * t.DELPH0);
+            dtc = (dtc - wam_maxf(0.f, dpn)) + wam_minf(0.f, dps);
+            const float dtp = ((sp * drgm) * cg_m) + (th_k + th_kp) * t.DELTHR;
+            const float dtm = ((sm * drgm) * cg_m) + (th_k + th_km) * t.DELTHR;
+            dtc = (dtc - wam_maxf(0.f, dtp)) + wam_minf(0.f, dtm);
+            // SIDC(K, M') = (S0(K)*CG(M') + TEMPD*TSIHKD(M'))*TFAK(M')
+            const float ss = sd * sd, sc = sd * cd, cc = cd * cd;
+            const float s0 = (((-(sc * duphi)) - cc * dvphi) - ss * dulam) - sc * dvlam;
+            const float sk = (s0 * cg_m + tempd * ts_m) * tf_m;
+            const float dop = (sk + (s0 * cg_mp + tempd * ts_mp) * tf_mp) * dfp;
+            const float dom = (sk + (s0 * cg_mm + tempd * ts_mm) * tf_mm) * dfm;
+            dtc = (dtc - wam_maxf(0.f, dop)) + wam_minf(0.f, dom);
+            float c[P_NPLANES];
+            c[P_SUM] = dtc;
+            c[P_LAT1] = wam_maxf(0.f, dps);
+            c[P_LAT2] = -wam_minf(0.f, dpn);
+            c[P_LON1] = wam_maxf(0.f, dlw);
+            c[P_LON2] = -wam_minf(0.f, dle);
+            c[P_TH1] = -wam_minf(0.f, dtp);
+            c[P_TH2] = wam_maxf(0.f, dtm);
+            c[P_SIG1] = -wam_minf(0.f, dop) / 1.1f;
+            c[P_SIG2] = wam_maxf(0.f, dom) * 1.1f;
+            if (counter > 1) {
+                if (pow2) {
+                    c[0] = 1.f - (1.f - c[0]) * rinv;
+#pragma unroll
+                    for (int p = 1; p < P_NPLANES; p++) c[p] = c[p] * rinv;
+                } else {
+                    c[0] = 1.f - (1.f - c[0]) / rc;
+#pragma unroll
+                    for (int p = 1; p < P_NPLANES; p++) c[p] = c[p] / rc;
+                }
+            }
+            const int mmi = mm, mpi = mp;
+            const size_t row = ((size_t)m * KL + k) * NS;
+            const float *f = fin + row;
+            float sacc = c[P_SUM] * f[ij];
+            sacc = sacc + c[P_LAT2] * ld_or0(f, nn);
+            sacc = sacc + c[P_LAT1] * ld_or0(f, ns);
+            sacc = sacc + c[P_LON2] * ld_or0(f, ne);
+            sacc = sacc + c[P_LON1] * ld_or0(f, nw);
+            sacc = sacc + c[P_TH1] * fin[((size_t)m * KL + kp) * NS + ij];
+            sacc = sacc + c[P_TH2] * fin[((size_t)m * KL + km) * NS + ij];
+            sacc = sacc + c[P_SIG1] * fin[((size_t)mpi * KL + k) * NS + ij];
+            sacc = sacc + c[P_SIG2] * fin[((size_t)mmi * KL + k) * NS + ij];
+            __builtin_nontemporal_store(sacc, &fout[row + ij]);
+            th_km = th_k;
+            th_k = th_kp;
+            th_kp = thdd_k(k + 2 < KL ? k + 2 : k + 2 - KL);
+        }
+        cg_mm = cg_m; ts_mm = ts_m; tf_mm = tf_m;
+        cg_m = cg_mp; ts_m = ts_mp; tf_m = tf_mp;
+    }
+}
+
+template __global__ void cfl_minmax_kernel<true>(DevTables, float *);
+template __global__ void cfl_minmax_kernel<false>(DevTables, float *);
+template __global__ void propags_fused_kernel<true>(DevTables, int, const float *, float *);
+template __global__ void propags_fused_kernel<false>(DevTables, int, const float *, float *);
 
 // WAVNU2 for every (sea point, frequency): WN and CGG as [ML][NSEA].
 __global__ void wavnu2_kernel(DevTables t, float *__restrict__ wn, float *__restrict__ cgg)

@@ -66,6 +66,14 @@ class WamGpu:
             pass
 
     # -- set-up -------------------------------------------------------------
+    def set_propagation_weights(self, stored):
+        """stored=True: weight planes in HBM (the reference's COEF_* arrays);
+        False: computed inside the PROPAGS stencil; None (default): computed
+        for shallow water with currents, stored otherwise.  Applies from the
+        next prepare_propagation."""
+        mode = -1 if stored is None else int(bool(stored))
+        self._check(self.lib.wam_gpu_set_propagation_weights(self.ctx, mode))
+
     def prepare_propagation(self, depth, u=None, v=None):
         d = np.ascontiguousarray(depth, dtype=np.float32)
         uu = None if u is None else np.ascontiguousarray(u, dtype=np.float32)
