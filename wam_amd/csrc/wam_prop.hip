@@ -7,6 +7,9 @@
 //    frequency); THDD/SIDC (:400-560) are recomputed inline from per-point
 //    gradients instead of being stored.
 //  * minmax_kernel / rescale_kernel: CHECK_CFL (:1917-2138).
+//  * propags_fused_sc_kernel / propags_fused_kernel: PROPAGS with the
+//    weights computed inside the stencil (no weight planes in HBM), with
+//    cfl_minmax_kernel as their CHECK_CFL.
 //  * propags_*_kernel: PROP_NO_CURR (:750-785) and PROP_CURR (:794-825).
 //    A thread owns one sea point and walks all (M, K) planes, so the
 //    neighbour indices are read once and every plane access is coalesced;
