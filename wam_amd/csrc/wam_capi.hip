@@ -27,6 +27,7 @@ __global__ void propags_fused_kernel(DevTables t, int counter, const float *fin,
 __global__ void propags_fused_sc_kernel(DevTables t, int counter, const float *fin, float *fout);
 __global__ void wavnu2_kernel(DevTables t, float *wn, float *cgg);
 hipError_t launch_implsch(const DevTables &t, const ImplArgs &a, int ncu, hipStream_t st);
+hipError_t launch_implsch_split(const DevTables &t, const ImplArgs &a, const ImplScratch &x, int ncu, hipStream_t st);
 }  // namespace wam
 
 using namespace wam;
@@ -45,6 +46,7 @@ struct wam_ctx {
     float cfl_max = 0.f, cfl_min = 0.f;
     float *wn = nullptr, *cgg = nullptr;
     float *fscratch = nullptr;            // COUNTER > 1 ping-pong
+    ImplScratch isx{};                    // IMPLSCH work arrays (IPHYS = 0), allocated at the first call
     float *stage = nullptr;               // host drop-in staging: FL3 + per-point fields
     size_t stage_n = 0;
     bool prepared = false;
@@ -313,6 +315,27 @@ int wam_gpu_propags_dev(wam_ctx *ctx, const float *d_in, float *d_out)
     return WAM_OK;
 }
 
+// Work arrays of the three-kernel IMPLSCH: tiled F, SL, FL (and SL - SPOS
+// with LCFLX), STRESSO's direction sums, per-point scalars, LLWS bits.
+static int ensure_impl_scratch(wam_ctx *ctx, int lcflx)
+{
+    const Tables &t = ctx->tab;
+    ImplScratch &x = ctx->isx;
+    const size_t ns = t.NSEA, nb = (size_t)t.KL * t.ML, nw = (nb + 31) / 32;
+    const size_t tiled = (ns + 63) / 64 * 64 * nb;      // whole 64-point tiles
+    if (!x.ft) {
+        x.NB4 = (int)(nb / 4);
+        HIPCHK(hipMalloc(&x.ft, tiled * sizeof(float)));
+        HIPCHK(hipMalloc(&x.slt, tiled * sizeof(float)));
+        HIPCHK(hipMalloc(&x.flt, tiled * sizeof(float)));
+        HIPCHK(hipMalloc(&x.axy, 3 * (size_t)t.ML * ns * sizeof(float)));
+        HIPCHK(hipMalloc(&x.pp, 4 * ns * sizeof(float)));
+        HIPCHK(hipMalloc(&x.llws, nw * ns * sizeof(unsigned)));
+    }
+    if (lcflx && !x.smt) HIPCHK(hipMalloc(&x.smt, tiled * sizeof(float)));
+    return WAM_OK;
+}
+
 int wam_gpu_implsch_dev(wam_ctx *ctx, float *d_fl3, const float *d_u10, const float *d_udir, float *d_tauw,
                         float *d_ustar, float *d_z0, const float *d_roairn, const float *d_wstar, int32_t lcflx,
                         float *d_phiaw, float *d_fluxes, int32_t *d_mij)
@@ -323,7 +346,13 @@ int wam_gpu_implsch_dev(wam_ctx *ctx, float *d_fl3, const float *d_u10, const fl
     a.fl3 = d_fl3; a.u10 = d_u10; a.udir = d_udir; a.tauw = d_tauw; a.ustar = d_ustar; a.z0 = d_z0;
     a.roairn = d_roairn; a.wstar = d_wstar; a.lcflx = lcflx ? 1 : 0; a.phiaw = d_phiaw; a.fluxes = d_fluxes;
     a.mij = d_mij;
-    HIPCHK(launch_implsch(ctx->dt, a, ctx->ncu, ctx->stream));
+    if (ctx->dt.iphys == 0 && ctx->dt.KL % 4 == 0) {      // three kernels; the tiles need KL % 4 == 0
+        const int rc = ensure_impl_scratch(ctx, a.lcflx);
+        if (rc) return rc;
+        HIPCHK(launch_implsch_split(ctx->dt, a, ctx->isx, ctx->ncu, ctx->stream));
+    } else {
+        HIPCHK(launch_implsch(ctx->dt, a, ctx->ncu, ctx->stream));
+    }
     return WAM_OK;
 }
 
@@ -549,6 +578,9 @@ void wam_gpu_finalize(wam_ctx *ctx)
     free_list(ctx->dev_allocs);
     if (ctx->fscratch) (void)hipFree(ctx->fscratch);
     if (ctx->stage) (void)hipFree(ctx->stage);
+    for (void *p : {(void *)ctx->isx.ft, (void *)ctx->isx.slt, (void *)ctx->isx.flt, (void *)ctx->isx.smt,
+                    (void *)ctx->isx.axy, (void *)ctx->isx.pp, (void *)ctx->isx.llws})
+        if (p) (void)hipFree(p);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
 }

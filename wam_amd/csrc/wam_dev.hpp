@@ -60,4 +60,23 @@ struct ImplArgs {
     int lcflx;
 };
 
+// Device work arrays of the three-kernel IMPLSCH (IPHYS = 0).  Spectra are
+// in point-quad tiles: 64 sea points x 4 consecutive bins of one direction
+// row, each point's 4 bins contiguous (16 bytes), so the per-lane kernels
+// move 1 KB per wavefront access and the per-point kernel reads a point's
+// bins as float4 (KL % 4 == 0).
+struct ImplScratch {
+    int NB4;                       // KL*ML/4
+    float *ft, *slt, *flt, *smt;   // F, SL, FL, SL - SPOS of the second SINPUT (LCFLX only)
+    float *axy;                    // [3][ML][NSEA]: STRESSO's direction sums of the first SINPUT
+    float *pp;                     // [4][NSEA]: ENHFR, MAX(FMEANWS, FMEAN), MIJ, PHIAW
+    unsigned *llws;                // [(KL*ML+31)/32][NSEA]: LLWS bits of the second SINPUT
+};
+
+// offset of the quad of bins 4q..4q+3 of sea point pt in the tiled arrays
+__host__ __device__ __forceinline__ size_t quad_offset(int pt, int q, int nb4)
+{
+    return ((size_t)(pt >> 6) * nb4 + q) * 256 + (size_t)(pt & 63) * 4;
+}
+
 }  // namespace wam

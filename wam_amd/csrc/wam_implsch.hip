@@ -135,21 +135,21 @@ __device__ __forceinline__ float wsigstar(const DevTables &t, float ust, float z
 }
 
 // FRCUTINDEX (:3569-3647) and FRCUTINDEX_ST6 (:5447-5515)
-__device__ __forceinline__ int frcut(const DevTables &t, const Smem &s, float fm, float fmws, float ust)
+__device__ __forceinline__ int frcut(const DevTables &t, const float *fr, float fm, float fmws, float ust)
 {
     if (t.iphys == 2) {
         const int nk = t.ML;
-        const float fxfm = 6.f * t.ZPI, fxpm = (4.0f * t.G) / 28.f, signk = t.ZPI * s.fr[nk - 1];
+        const float fxfm = 6.f * t.ZPI, fxpm = (4.0f * t.G) / 28.f, signk = t.ZPI * fr[nk - 1];
         float fhigh = wam_maxf(fxfm * fm, fxpm / ust);
         const float f1 = 1.f / wam_logf(t.CO);
-        const float f2 = 1.f - wam_logf(t.ZPI * s.fr[0]) * f1;
+        const float f2 = 1.f - wam_logf(t.ZPI * fr[0]) * f1;
         int nkh = min(nk, wam_int(f2 + f1 * wam_logf(wam_maxf(1.E-7f, fhigh))));
         const int nkh1 = min(nk, nkh + 1);
         fhigh = wam_minf(signk, wam_maxf(fxfm * fm, fxpm / ust));
         return max(2, min(nkh1, wam_int(f2 + f1 * wam_logf(wam_maxf(1.E-7f, fhigh)))));
     }
-    const float fpmh = 2.5f / s.fr[0];
-    const float fppm = (3.0f * t.G) / ((28.0f * t.ZPI) * s.fr[0]);
+    const float fpmh = 2.5f / fr[0];
+    const float fppm = (3.0f * t.G) / ((28.0f * t.ZPI) * fr[0]);
     const float fm2 = wam_maxf(fmws, fm) * fpmh;
     const float fpm = fppm / wam_maxf(ust, 1.0E-6f);
     const float fpm4 = wam_maxf(fm2, fpm);
@@ -157,64 +157,81 @@ __device__ __forceinline__ int frcut(const DevTables &t, const Smem &s, float fm
     return min(max(1, mij), t.ML);
 }
 
+// SINPUT's factors of one frequency (:2111-2308): CNSN, UCN1, UCN2, the two
+// XV1D terms and ZCN.
+struct SinFreq {
+    float cnsn, ucn1, ucn2, xv1, xv2, zcn;
+};
+
+template <int SHALLOW>
+__device__ __forceinline__ SinFreq sin_freq(const DevTables &t, float fr_m, float tfk_m, float usn1, float usn2,
+                                            float z0, float roairn)
+{
+    const float xk2 = t.XKAPPA * t.XKAPPA;
+    const float fac = t.ZPI * fr_m;
+    const float cst = ((fac * t.BETAMAX) / xk2) / t.ROWATER;
+    float xk, cm, cnsn;
+    if (SHALLOW) {
+        xk = tfk_m;
+        cm = xk / fac;
+        cnsn = ((cst * wam_ipowf(fac, 2)) / (t.G * xk)) * roairn;
+    } else {
+        xk = wam_ipowf(fac, 2) / t.G;
+        cm = fac / t.G;
+        cnsn = cst * roairn;
+    }
+    SinFreq r;
+    r.zcn = wam_logf(xk * z0);
+    r.cnsn = cnsn;
+    r.ucn1 = usn1 * cm + t.ZALP;
+    r.ucn2 = usn2 * cm + t.ZALP;
+    r.xv1 = -1.f / (((usn1 / t.XKAPPA) * r.zcn) * cm);
+    r.xv2 = -1.f / (((usn2 / t.XKAPPA) * r.zcn) * cm);
+    return r;
+}
+
+// SINPUT of one bin with TEMPD = COS(TH(K) - UDIR) = tk: the positive input
+// SPOS/F (sp), the derivative FL = SL/F (fl) and LLWS (ll).
+__device__ __forceinline__ void sin_bin(const DevTables &t, const SinFreq &r, float tk, float const3, float &sp,
+                                        float &fl, bool &ll)
+{
+    float ufac1 = 0.f;
+    ll = false;
+    if (tk > 0.01f) {
+#pragma unroll
+        for (int is = 0; is < 2; is++) {
+            const float x = tk * (is == 0 ? r.ucn1 : r.ucn2);
+            const float zlog = r.zcn + t.XKAPPA / x;
+            if (zlog < 0.f) {
+                const float z2x = (zlog * zlog) * x;
+                ufac1 = ufac1 + ((0.5f * wam_expf(zlog)) * z2x) * z2x;
+                ll = true;
+            }
+        }
+    }
+    float ufac2 = (const3 * (tk - r.xv1)) * wam_ipowf(r.ucn1, 2);
+    ufac2 = 0.5f * ufac2 + ((0.5f * const3) * (tk - r.xv2)) * wam_ipowf(r.ucn2, 2);
+    sp = r.cnsn * ufac1;
+    fl = sp + r.cnsn * ufac2;
+}
+
 // SINPUT (:2111-2308), NSIN = 2: SL, SPOS, FL, LLWS for every bin.
-// q[6*m..] = CNSN, UCN1, UCN2, XV1D1, XV1D2, ZCN per frequency.
 template <int SHALLOW>
 __device__ __forceinline__ void sinput(const DevTables &t, const Smem &s, PointScalars &P, const float *cu, const float *tfk,
-                       float *q)
+                       SinFreq *q)
 {
     const int KL = t.KL, ML = t.ML, NB = KL * ML, tid = threadIdx.x;
     if (tid == 0) P.sig_n = wsigstar(t, P.ust, P.z0, P.wstar);
     __syncthreads();
-    const float xk2 = t.XKAPPA * t.XKAPPA;
     const float usn1 = P.ust * (1.0f - P.sig_n), usn2 = P.ust * (1.0f + P.sig_n);
-    if (tid < ML) {
-        const int m = tid;
-        const float fac = t.ZPI * s.fr[m];
-        const float cst = ((fac * t.BETAMAX) / xk2) / t.ROWATER;
-        float xk, cm, cnsn;
-        if (SHALLOW) {
-            xk = tfk[m];
-            cm = xk / fac;
-            cnsn = ((cst * wam_ipowf(fac, 2)) / (t.G * xk)) * P.roairn;
-        } else {
-            xk = wam_ipowf(fac, 2) / t.G;
-            cm = fac / t.G;
-            cnsn = cst * P.roairn;
-        }
-        const float zcn = wam_logf(xk * P.z0);
-        float *r = q + 6 * m;
-        r[0] = cnsn;
-        r[1] = usn1 * cm + t.ZALP;
-        r[2] = usn2 * cm + t.ZALP;
-        r[3] = -1.f / (((usn1 / t.XKAPPA) * zcn) * cm);
-        r[4] = -1.f / (((usn2 / t.XKAPPA) * zcn) * cm);
-        r[5] = zcn;
-    }
+    if (tid < ML) q[tid] = sin_freq<SHALLOW>(t, s.fr[tid], SHALLOW ? tfk[tid] : 0.f, usn1, usn2, P.z0, P.roairn);
     __syncthreads();
-    const float const3 = ((2.f * t.XKAPPA) / t.BETAMAX) * xk2;
+    const float const3 = ((2.f * t.XKAPPA) / t.BETAMAX) * (t.XKAPPA * t.XKAPPA);
     for (int b = tid; b < NB; b += NT) {
         const int m = b / KL, k = b - KL * m;
-        const float *r = q + 6 * m;
-        const float tk = cu[k];
-        float ufac1 = 0.f;
-        unsigned char ll = 0;
-        if (tk > 0.01f) {
-#pragma unroll
-            for (int is = 0; is < 2; is++) {
-                const float x = tk * r[1 + is];
-                const float zlog = r[5] + t.XKAPPA / x;
-                if (zlog < 0.f) {
-                    const float z2x = (zlog * zlog) * x;
-                    ufac1 = ufac1 + ((0.5f * wam_expf(zlog)) * z2x) * z2x;
-                    ll = 1;
-                }
-            }
-        }
-        float ufac2 = (const3 * (tk - r[3])) * wam_ipowf(r[1], 2);
-        ufac2 = 0.5f * ufac2 + ((0.5f * const3) * (tk - r[4])) * wam_ipowf(r[2], 2);
-        const float sp = r[0] * ufac1;
-        const float fl = sp + r[0] * ufac2;
+        float sp, fl;
+        bool ll;
+        sin_bin(t, q[m], cu[k], const3, sp, fl, ll);
         const float f = s.F[b];
         s.SPOS[b] = sp * f;
         s.FL[b] = fl;
@@ -402,11 +419,11 @@ __device__ __forceinline__ void sinput_st6(const DevTables &t, const Smem &s, Po
 }
 
 // TAU_PHI_HF with XLEVTAIL = 0 (:3853-3966), sequential form.
-__device__ __forceinline__ void tau_phi_hf_seq(const DevTables &t, const Smem &s, int mij, float ust0, float z0, float &tauhf,
+__device__ __forceinline__ void tau_phi_hf_seq(const DevTables &t, const float *fr, int mij, float ust0, float z0, float &tauhf,
                                float &phihf)
 {
     const float gm1 = 1.0f / t.G;
-    const float omegac = t.ZPI * s.fr[mij - 1];
+    const float omegac = t.ZPI * fr[mij - 1];
     const float tauw0 = ust0 * ust0;
     const float xloggz0 = wam_logf(t.G * z0);
     const float omegacc = wam_maxf(omegac, (t.X0TAUHF * t.G) / ust0);
@@ -511,7 +528,7 @@ __device__ __forceinline__ void stresso(const DevTables &t, const Smem &s, Point
         float tau1 = th, phi1 = hf[40] * ph;
         // a non-finite node feeds back into the running stress of the later
         // nodes in the sequential form: evaluate that form then
-        if (!fin) tau_phi_hf_seq(t, s, mij, P.ust, P.z0, tau1, phi1);
+        if (!fin) tau_phi_hf_seq(t, s.fr, mij, P.ust, P.z0, tau1, phi1);
         const float tauhf = (c1 * hf[38]) * tau1, phihf = (c2 * hf[39]) * phi1;
         phi = phi + phihf;
         xs = xs + tauhf * wam_sinf(P.udir);
@@ -633,6 +650,39 @@ __device__ __forceinline__ void snonlin(const DevTables &t, const Smem &s, float
     }
 }
 
+// WNFLUXES (:3970-4054) on SSOURCE = SL - SMIN (in s.SMIN) and SL_BOT, one
+// thread: the six flux components out[c * ns], c = 0..5.
+template <int SHALLOW>
+__device__ __forceinline__ void wnfluxes(const DevTables &t, const Smem &s, float phiaw, float ust, float udir, int mij,
+                                         const float *tfk, float *out, int ns)
+{
+    const int KL = t.KL;
+    const float rog = t.ROWATER * t.G;
+    float phioc = phiaw, tau = t.ROAIR * (ust * ust);
+    float tx = tau * wam_sinf(udir), ty = tau * wam_cosf(udir);
+    float pb = 0.f, bx = 0.f, by = 0.f;
+    const float scdfm = (0.5f * t.DELTH) * (1.f - 1.f / t.CO);
+    for (int m = 0; m < mij; m++) {
+        const float cfm = (m < mij - 1) ? rog * s.dfim[m] : (rog * scdfm) * s.fr[mij - 1];
+        const float cm = SHALLOW ? tfk[m] / (t.ZPI * s.fr[m]) : (t.ZPI * s.fr[m]) / t.G;
+        for (int k = 0; k < KL; k++) {
+            const int b = k + KL * m;
+            const float ss = s.SMIN[b];
+            phioc = phioc - ss * cfm;
+            tx = tx - ((cm * ss) * cfm) * s.sinth[k];
+            ty = ty - ((cm * ss) * cfm) * s.costh[k];
+            if (SHALLOW) {
+                const float sb = s.SLBOT[b];
+                pb = pb - sb * cfm;
+                bx = bx - ((cm * sb) * cfm) * s.sinth[k];
+                by = by - ((cm * sb) * cfm) * s.costh[k];
+            }
+        }
+    }
+    out[0] = phioc; out[(size_t)ns] = tx; out[2 * (size_t)ns] = ty;
+    out[3 * (size_t)ns] = pb; out[4 * (size_t)ns] = bx; out[5 * (size_t)ns] = by;
+}
+
 // Scratch floats: SNONLIN's quadruplet terms (float4 per quadruplet of a
 // chunk) and weight table (float4 per MC and update type), or SINPUT_ST6's
 // three per-bin arrays.
@@ -648,7 +698,8 @@ __global__ void __launch_bounds__(NT) implsch_kernel(DevTables t, ImplArgs a)
     extern __shared__ __align__(16) float lds_raw[];
     __shared__ PointScalars P;
     __shared__ float T[MAXF], T2[MAXF], cu[MAXF], sprd[MAXF], t2[MAXF], flm[MAXF], tfk[MAXF], tcg[MAXF];
-    __shared__ float wn[MAXF], cgv[MAXF], pm[3 * MAXF + 64], q[6 * MAXF];
+    __shared__ float wn[MAXF], cgv[MAXF], pm[3 * MAXF + 64];
+    __shared__ SinFreq q[MAXF];
     __shared__ float pf[5 * MAXF];
     const int KL = t.KL, ML = t.ML, NB = KL * ML, NS = t.NSEA, tid = threadIdx.x;
     const int MH = t.MLSTHG;
@@ -770,7 +821,7 @@ __global__ void __launch_bounds__(NT) implsch_kernel(DevTables t, ImplArgs a)
             const float e = wam_maxf(P.s_e, t.EMIN);
             P.emeanws = e;
             P.fmeanws = e / wam_maxf(P.s_fm, t.EMIN);
-            P.mij = frcut(t, s, P.fmean, P.fmeanws, P.ust);
+            P.mij = frcut(t, s.fr, P.fmean, P.fmeanws, P.ust);
         }
         __syncthreads();
         stresso<SHALLOW>(t, s, P, cu, tfk, pm, IPHYS != 2);
@@ -928,36 +979,7 @@ __global__ void __launch_bounds__(NT) implsch_kernel(DevTables t, ImplArgs a)
             // WNFLUXES (:3970-4054) on SSOURCE = SL - SMIN
             for (int b = tid; b < NB; b += NT) s.SMIN[b] = s.SL[b] - s.SMIN[b];
             __syncthreads();
-            if (tid == 0) {
-                const float rog = t.ROWATER * t.G;
-                float phioc = P.phiaw, tau = t.ROAIR * (P.ust * P.ust);
-                float tx = tau * wam_sinf(P.udir), ty = tau * wam_cosf(P.udir);
-                float pb = 0.f, bx = 0.f, by = 0.f;
-                const float scdfm = (0.5f * t.DELTH) * (1.f - 1.f / t.CO);
-                const int mij = P.mij;
-                for (int m = 0; m < mij; m++) {
-                    const float cfm = (m < mij - 1) ? rog * s.dfim[m] : (rog * scdfm) * s.fr[mij - 1];
-                    const float cm = SHALLOW ? tfk[m] / (t.ZPI * s.fr[m]) : (t.ZPI * s.fr[m]) / t.G;
-                    for (int k = 0; k < KL; k++) {
-                        const int b = k + KL * m;
-                        const float ss = s.SMIN[b];
-                        phioc = phioc - ss * cfm;
-                        tx = tx - ((cm * ss) * cfm) * s.sinth[k];
-                        ty = ty - ((cm * ss) * cfm) * s.costh[k];
-                        if (SHALLOW) {
-                            const float sb = s.SLBOT[b];
-                            pb = pb - sb * cfm;
-                            bx = bx - ((cm * sb) * cfm) * s.sinth[k];
-                            by = by - ((cm * sb) * cfm) * s.costh[k];
-                        }
-                    }
-                }
-                if (a.fluxes) {
-                    a.fluxes[0 * (size_t)NS + pt] = phioc; a.fluxes[1 * (size_t)NS + pt] = tx;
-                    a.fluxes[2 * (size_t)NS + pt] = ty; a.fluxes[3 * (size_t)NS + pt] = pb;
-                    a.fluxes[4 * (size_t)NS + pt] = bx; a.fluxes[5 * (size_t)NS + pt] = by;
-                }
-            }
+            if (tid == 0 && a.fluxes) wnfluxes<SHALLOW>(t, s, P.phiaw, P.ust, P.udir, P.mij, tfk, a.fluxes + pt, NS);
         }
         // 4. limiter and implicit update
         if (tid < ML) {
@@ -991,7 +1013,7 @@ __global__ void __launch_bounds__(NT) implsch_kernel(DevTables t, ImplArgs a)
             const float ew = wam_maxf(P.s_w1, t.EMIN);
             P.emeanws = ew;
             P.fmeanws = ew / wam_maxf(P.s_w2, t.EMIN);
-            P.mij = frcut(t, s, P.fmean, P.fmeanws, P.ust);
+            P.mij = frcut(t, s.fr, P.fmean, P.fmeanws, P.ust);
         }
         __syncthreads();
         imphftail<SHALLOW>(t, s, P.mij, tfk, tcg, t2, 0);
@@ -1011,6 +1033,520 @@ __global__ void __launch_bounds__(NT) implsch_kernel(DevTables t, ImplArgs a)
         }
         __syncthreads();
     }
+}
+
+// ---------------------------------------------------------------------------
+// IMPLSCH for IPHYS = 0 in three kernels.  The scalar chain of a sea point
+// (AIRSEA's Newton iteration, the ordered frequency integrals, STRESSO's
+// sums, TAU_PHI_HF, FRCUTINDEX) is sequential within the point but
+// independent across points, so the two per-lane kernels give every sea
+// point its own lane (PW points per wavefront; FL3's point-fastest rows load
+// and store coalesced) and run each stage at full width in the reference's
+// order.  SNONLIN's DIA couples the bins of one point and stays one
+// workgroup per point in between, on point-major copies of F, SL and FL.
+//   implsch_pre_kernel:  AIRSEA, mean parameters, SINPUT, FRCUTINDEX,
+//                        STRESSO, AIRSEA, IMPHFTAIL, SINPUT, STRESSO, SDISSIP
+//   implsch_snl_kernel:  SNONLIN, SBOTTOM, WNFLUXES, limiter and update
+//   implsch_post_kernel: mean parameters, FRCUTINDEX, IMPHFTAIL, FL3 out
+// ---------------------------------------------------------------------------
+
+constexpr int PW = 64;    // sea points per workgroup of the per-lane kernels
+
+// STRESSO's closing part (:3164-3356) for one point: the direction sums phi,
+// xs, ys over M < MIJ, the tail moments h1, h2 of F(., MIJ) and TAU_PHI_HF in
+// its sequential form.  Returns TAUW; PHIAW in phiaw.
+__device__ __forceinline__ float stresso_close(const DevTables &t, int mij, float ust, float z0, float udir, float phi,
+                                               float xs, float ys, float h1, float h2, float &phiaw)
+{
+    const float gm1 = 1.0f / t.G;
+    const float cst = (t.DELTH * t.ZPI4) * gm1;
+    const float c1 = (cst * t.FR5[mij - 1]) * gm1;
+    const float c2 = (t.ROAIR * cst) * t.FR5[mij - 1];
+    xs = xs / wam_maxf(t.ROAIR, 1.f);
+    ys = ys / wam_maxf(t.ROAIR, 1.f);
+    float tau1, phi1;
+    tau_phi_hf_seq(t, t.FR, mij, ust, z0, tau1, phi1);
+    const float tauhf = (c1 * h1) * tau1, phihf = (c2 * h2) * phi1;
+    phi = phi + phihf;
+    xs = xs + tauhf * wam_sinf(udir);
+    ys = ys + tauhf * wam_cosf(udir);
+    float tw = sqrtf(xs * xs + ys * ys);
+    tw = wam_minf(tw, ust * ust - t.EPS1);
+    tw = wam_maxf(tw, 0.f);
+    phiaw = phi;
+    return tw;
+}
+
+// STRESSO's weight of frequency m: RHOWG_DFIM, halved at MIJ below ML.
+__device__ __forceinline__ float stresso_rw(const DevTables &t, int m, int mij)
+{
+    float rw = t.RHOWG_DFIM[m];
+    if (m == mij - 1 && mij != t.ML) rw = 0.5f * rw;
+    return rw;
+}
+
+template <int SHALLOW>
+__device__ __forceinline__ float stresso_cm(const DevTables &t, int m, float tfk_m)
+{
+    return SHALLOW ? tfk_m * (1.0f / (t.ZPI * t.FR[m])) : (t.ZPI * t.FR[m]) * (1.0f / t.G);
+}
+
+template <int SHALLOW>
+__global__ void __launch_bounds__(PW) implsch_pre_kernel(DevTables t, ImplArgs a, ImplScratch x)
+{
+    extern __shared__ float cu_lds[];            // [KL][PW]: COS(TH(K) - UDIR) of each lane's point
+    const int KL = t.KL, ML = t.ML, NB = KL * ML, NS = t.NSEA, NB4 = x.NB4, lane = threadIdx.x;
+    const int pt0 = blockIdx.x * PW + lane;
+    const bool on = pt0 < NS;
+    const int pt = on ? pt0 : NS - 1;
+    float *cu = cu_lds + lane;
+    const float *F = a.fl3 + pt;                  // F(b) = F[b * NS]
+    const float u10 = a.u10[pt], udir = a.udir[pt], roairn = a.roairn[pt], wstar = a.wstar[pt];
+    const int indep = SHALLOW ? t.INDEP[pt] : 1;
+    float tauw = a.tauw[pt], ust = 0.f, z0 = a.z0[pt];
+    for (int k = 0; k < KL; k++) cu[k * PW] = wam_cosf(t.TH[k] - udir);
+    airsea(t, u10, tauw, ust, z0);
+
+    const float const3 = ((2.f * t.XKAPPA) / t.BETAMAX) * (t.XKAPPA * t.XKAPPA);
+    const float d1 = sqrtf(t.G) / t.ZPI, d2 = t.ZPI / sqrtf(t.G);
+    // 1. one pass over the spectrum: the mean parameters' integrals, the
+    //    first SINPUT's LLWS-masked integrals and STRESSO's direction sums.
+    //    Frequency ML goes first, since it seeds every integral; its own
+    //    terms are added last.  Bins go four at a time (KL % 4 == 0).
+    float s_e = 0.f, s_fm = 0.f, s_tm = 0.f, s_w1 = 0.f, s_w2 = 0.f, q_e = 0.f, q_fm = 0.f, Tl = 0.f, Ql = 0.f;
+    {
+        const float sig_n = wsigstar(t, ust, z0, wstar);
+        const float usn1 = ust * (1.0f - sig_n), usn2 = ust * (1.0f + sig_n);
+        for (int i = 0; i < ML; i++) {
+            const int m = i == 0 ? ML - 1 : i - 1;
+            const float tfk = SHALLOW ? t.TFAK[tb(t, m, indep)] : 0.f;
+            const SinFreq r = sin_freq<SHALLOW>(t, t.FR[m], tfk, usn1, usn2, z0, roairn);
+            float T = 0.f, Q = 0.f, sa = 0.f, sx = 0.f, sy = 0.f;
+            bool qfirst = true;
+            for (int k0 = 0; k0 < KL; k0 += 4) {
+                float f4[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) f4[u] = F[(size_t)(k0 + u + KL * m) * NS];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int k = k0 + u;
+                    const float f = f4[u];
+                    T = k == 0 ? f : T + f;
+                    float sp, fl;
+                    bool ll;
+                    sin_bin(t, r, cu[k * PW], const3, sp, fl, ll);
+                    if (ll) {
+                        Q = qfirst ? f : Q + f;
+                        qfirst = false;
+                    }
+                    const float spp = wam_maxf(sp * f, 0.f);
+                    if (k == 0) { sa = spp; sx = spp * t.SINTH[k]; sy = spp * t.COSTH[k]; }
+                    else { sa = sa + spp; sx = sx + spp * t.SINTH[k]; sy = sy + spp * t.COSTH[k]; }
+                }
+            }
+            if (on) {
+                float *axy = x.axy + (size_t)m * NS + pt;
+                axy[0] = sa;
+                axy[(size_t)ML * NS] = sx;
+                axy[(size_t)2 * ML * NS] = sy;
+            }
+            if (i == 0) {
+                Tl = T; Ql = Q;
+                s_e = t.MO_TAIL * T; s_fm = t.MM1_TAIL * T; s_tm = t.MP1_TAIL * T;
+                if (SHALLOW) { s_w1 = (t.MM1_TAIL * d1) * T; s_w2 = (t.MP1_TAIL * d2) * T; }
+                q_e = t.MO_TAIL * Q; q_fm = t.MM1_TAIL * Q;
+            } else {
+                s_e = s_e + T * t.DFIM[m]; s_fm = s_fm + T * t.DFIMOFR[m]; s_tm = s_tm + T * t.DFIM_FR[m];
+                if (SHALLOW) {
+                    const float sk = sqrtf(tfk);
+                    s_w1 = s_w1 + (T / sk) * t.DFIM[m];
+                    s_w2 = s_w2 + (T * sk) * t.DFIM[m];
+                }
+                q_e = q_e + Q * t.DFIM[m]; q_fm = q_fm + Q * t.DFIMOFR[m];
+            }
+        }
+        const int m = ML - 1;
+        s_e = s_e + Tl * t.DFIM[m]; s_fm = s_fm + Tl * t.DFIMOFR[m]; s_tm = s_tm + Tl * t.DFIM_FR[m];
+        if (SHALLOW) {
+            const float sk = sqrtf(t.TFAK[tb(t, m, indep)]);
+            s_w1 = s_w1 + (Tl / sk) * t.DFIM[m];
+            s_w2 = s_w2 + (Tl * sk) * t.DFIM[m];
+        }
+        q_e = q_e + Ql * t.DFIM[m]; q_fm = q_fm + Ql * t.DFIMOFR[m];
+    }
+    const float emean = wam_maxf(s_e, t.EMIN);
+    const float fmean = emean / wam_maxf(s_fm, t.EMIN);
+    const float f1mean = 1.f / (emean > t.EMIN ? emean / s_tm : 1.f);
+    float akmean = 1.f, xkmean = 1.f;
+    if (SHALLOW) {
+        akmean = emean > t.EMIN ? wam_ipowf(emean / s_w1, 2) : 1.f;
+        xkmean = emean > t.EMIN ? wam_ipowf(s_w2 / emean, 2) : 1.f;
+    }
+    const float ews = wam_maxf(q_e, t.EMIN);
+    const float fmeanws = ews / wam_maxf(q_fm, t.EMIN);
+    const int mij = frcut(t, t.FR, fmean, fmeanws, ust);
+
+    // 2. STRESSO on the first SINPUT.  The moments of F(., MIJ) serve both
+    //    STRESSO calls: IMPHFTAIL changes the frequencies above MIJ only.
+    float h1 = 0.f, h2 = 0.f;
+    for (int k = 0; k < KL; k++) {
+        const float cosw = wam_maxf(cu[k * PW], 0.f);
+        const float f = F[(size_t)(k + KL * (mij - 1)) * NS];
+        if (k == 0) { h2 = f * wam_ipowf(cosw, 2); h1 = h2 * cosw; }
+        else { h1 = h1 + f * wam_ipowf(cosw, 3); h2 = h2 + f * wam_ipowf(cosw, 2); }
+    }
+    {
+        float phi = 0.f, xs = 0.f, ys = 0.f;
+        for (int m = 0; m < mij; m++) {
+            const float *axy = x.axy + (size_t)m * NS + pt;
+            const float rw = stresso_rw(t, m, mij);
+            const float cmrw = stresso_cm<SHALLOW>(t, m, SHALLOW ? t.TFAK[tb(t, m, indep)] : 0.f) * rw;
+            phi = phi + axy[0] * rw;
+            xs = xs + axy[(size_t)ML * NS] * cmrw;
+            ys = ys + axy[(size_t)2 * ML * NS] * cmrw;
+        }
+        float phiaw;
+        tauw = stresso_close(t, mij, ust, z0, udir, phi, xs, ys, h1, h2, phiaw);
+    }
+    airsea(t, u10, tauw, ust, z0);
+
+    // 3. IMPHFTAIL, the second SINPUT, STRESSO's sums and SDISSIP in one pass;
+    //    F, SL and FL go to the tiled work arrays, LLWS to bits.
+    float tden;
+    if (SHALLOW) {
+        const float akm0 = 1.f / t.TFAK[tb(t, mij - 1, indep)];
+        tden = wam_ipowf(akm0, 3) / t.TCGOND[tb(t, mij - 1, indep)];
+    } else {
+        tden = t.FRM5[mij - 1];
+    }
+    const float e2 = emean * emean;
+    const float sds = SHALLOW ? ((t.CONSS * f1mean) * e2) * wam_ipowf(xkmean, 4) : (t.CONSD * e2) * wam_ipowf(f1mean, 9);
+    float phi = 0.f, xs = 0.f, ys = 0.f;
+    {
+        const float sig_n = wsigstar(t, ust, z0, wstar);
+        const float usn1 = ust * (1.0f - sig_n), usn2 = ust * (1.0f + sig_n);
+        const size_t q0 = quad_offset(pt, 0, NB4);
+        unsigned *LLW = x.llws + pt;
+        unsigned word = 0;
+        for (int m = 0; m < ML; m++) {
+            const float tfk = SHALLOW ? t.TFAK[tb(t, m, indep)] : 0.f;
+            const SinFreq r = sin_freq<SHALLOW>(t, t.FR[m], tfk, usn1, usn2, z0, roairn);
+            const bool tail = m >= mij;
+            float ttail = 0.f;
+            if (tail) {
+                const float an = SHALLOW ? wam_ipowf(1.f / tfk, 3) / t.TCGOND[tb(t, m, indep)] : t.FRM5[m];
+                ttail = an / tden;
+            }
+            const float xd = SHALLOW ? tfk / xkmean : wam_ipowf(t.FR[m] / f1mean, 2);
+            const float t2d = sds * (0.5f * xd + 0.5f * wam_ipowf(xd, 2));
+            const int mrow = tail ? mij - 1 : m;
+            float sa = 0.f, sx = 0.f, sy = 0.f;
+            for (int k0 = 0; k0 < KL; k0 += 4) {
+                float f4[4], sl4[4], fl4[4], sm4[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) f4[u] = F[(size_t)(k0 + u + KL * mrow) * NS];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int k = k0 + u, b = k + KL * m;
+                    const float f = tail ? ttail * f4[u] : f4[u];
+                    float sp, fl;
+                    bool ll;
+                    sin_bin(t, r, cu[k * PW], const3, sp, fl, ll);
+                    const float sl = fl * f, spos = sp * f;
+                    f4[u] = f;
+                    sl4[u] = sl + t2d * f;
+                    fl4[u] = fl + t2d;
+                    sm4[u] = sl - spos;
+                    word |= (unsigned)ll << (b & 31);
+                    if ((b & 31) == 31 || b == NB - 1) {
+                        if (on) LLW[(size_t)(b >> 5) * NS] = word;
+                        word = 0;
+                    }
+                    const float spp = wam_maxf(spos, 0.f);
+                    if (k == 0) { sa = spp; sx = spp * t.SINTH[k]; sy = spp * t.COSTH[k]; }
+                    else { sa = sa + spp; sx = sx + spp * t.SINTH[k]; sy = sy + spp * t.COSTH[k]; }
+                }
+                if (on) {
+                    const size_t o = q0 + (size_t)((k0 + KL * m) >> 2) * 256;
+                    *reinterpret_cast<float4 *>(x.ft + o) = make_float4(f4[0], f4[1], f4[2], f4[3]);
+                    *reinterpret_cast<float4 *>(x.slt + o) = make_float4(sl4[0], sl4[1], sl4[2], sl4[3]);
+                    *reinterpret_cast<float4 *>(x.flt + o) = make_float4(fl4[0], fl4[1], fl4[2], fl4[3]);
+                    if (a.lcflx)
+                        *reinterpret_cast<float4 *>(x.smt + o) = make_float4(sm4[0], sm4[1], sm4[2], sm4[3]);
+                }
+            }
+            if (m < mij) {
+                const float rw = stresso_rw(t, m, mij);
+                const float cmrw = stresso_cm<SHALLOW>(t, m, tfk) * rw;
+                phi = phi + sa * rw;
+                xs = xs + sx * cmrw;
+                ys = ys + sy * cmrw;
+            }
+        }
+    }
+    float phiaw;
+    tauw = stresso_close(t, mij, ust, z0, udir, phi, xs, ys, h1, h2, phiaw);
+    if (on) {
+        a.tauw[pt] = tauw;
+        a.ustar[pt] = ust;
+        a.z0[pt] = z0;
+        if (a.phiaw) a.phiaw[pt] = phiaw;
+        float enhfr = 1.f;
+        if (SHALLOW) {
+            const float enh = wam_maxf((0.75f * t.DEPTH[pt]) * akmean, .5f);
+            enhfr = 1.f + ((5.5f / enh) * (1.f - .833f * enh)) * wam_expf(-1.25f * enh);
+        }
+        x.pp[pt] = enhfr;
+        x.pp[(size_t)NS + pt] = wam_maxf(fmeanws, fmean);
+        x.pp[(size_t)2 * NS + pt] = (float)mij;
+        x.pp[(size_t)3 * NS + pt] = phiaw;
+    }
+}
+
+template <int SHALLOW>
+__global__ void __launch_bounds__(NT) implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x)
+{
+    extern __shared__ __align__(16) float lds_raw[];
+    __shared__ float cu[MAXF], sprd[MAXF], flm[MAXF], tfk[MAXF], tlim[MAXF], tbot[MAXF];
+    __shared__ float Ps[5];
+    __shared__ int Pmij;
+    const int KL = t.KL, ML = t.ML, NB = KL * ML, NS = t.NSEA, NB4 = x.NB4, tid = threadIdx.x;
+    const int MH = t.MLSTHG;
+
+    Smem s{};
+    float *p = lds_raw;
+    s.SCR = p; p += scr_floats(KL, ML, t.CHUNK, 0);
+    s.F = p; p += NB; s.SL = p; p += NB; s.FL = p; p += NB;
+    if (a.lcflx) {
+        s.SMIN = p; p += NB;
+        if (SHALLOW) { s.SLBOT = p; p += NB; }
+    }
+    s.fr = p; p += ML; s.dfim = p; p += ML;
+    s.th = p; p += KL; s.costh = p; p += KL; s.sinth = p; p += KL;
+    s.rnl = p; p += 25 * MH; s.af11 = p; p += MH;
+    s.inl = (int *)p; p += 5 * MH; s.kw = (int *)p; p += 8 * KL;
+
+    for (int i = tid; i < ML; i += NT) { s.fr[i] = t.FR[i]; s.dfim[i] = t.DFIM[i]; }
+    for (int i = tid; i < KL; i += NT) { s.th[i] = t.TH[i]; s.costh[i] = t.COSTH[i]; s.sinth[i] = t.SINTH[i]; }
+    for (int i = tid; i < 25 * MH; i += NT) s.rnl[i] = t.RNLCOEF[i];
+    for (int i = tid; i < 5 * MH; i += NT) s.inl[i] = t.INLCOEF[i];
+    for (int i = tid; i < MH; i += NT) s.af11[i] = t.AF11[i];
+    for (int i = tid; i < 2 * KL; i += NT) {
+        s.kw[i] = t.K1W[i]; s.kw[2 * KL + i] = t.K2W[i]; s.kw[4 * KL + i] = t.K11W[i]; s.kw[6 * KL + i] = t.K21W[i];
+    }
+    int qslot[2];
+    for (int i = 0; i < 2; i++) {
+        const int ql = tid + i * NT, mcl = ql / (2 * KL);
+        qslot[i] = ql < 511 && mcl < t.CHUNK ? mcl | (ql - mcl * 2 * KL) << 8 : 255;
+    }
+
+    const int nblk = gridDim.x, xcd = blockIdx.x % 8, lb = blockIdx.x / 8;
+    const int nb_x = (nblk + 7 - xcd) / 8;
+    const int per = (NS + 7) / 8;
+    const int p0 = xcd * per, p1 = min(NS, p0 + per);
+    const float delt = t.DELT;
+    for (int pt = p0 + lb; pt < p1; pt += nb_x) {
+        const size_t q0 = quad_offset(pt, 0, NB4);
+        {
+            float4 *F4 = reinterpret_cast<float4 *>(s.F), *SL4 = reinterpret_cast<float4 *>(s.SL);
+            float4 *FL4 = reinterpret_cast<float4 *>(s.FL), *SM4 = reinterpret_cast<float4 *>(s.SMIN);
+            for (int q = tid; q < NB4; q += NT) {
+                const size_t o = q0 + (size_t)q * 256;
+                F4[q] = *reinterpret_cast<const float4 *>(x.ft + o);
+                SL4[q] = *reinterpret_cast<const float4 *>(x.slt + o);
+                FL4[q] = *reinterpret_cast<const float4 *>(x.flt + o);
+                if (a.lcflx) SM4[q] = *reinterpret_cast<const float4 *>(x.smt + o);
+            }
+        }
+        const float udir = a.udir[pt];
+        if (tid == 0) {
+            Ps[0] = a.ustar[pt]; Ps[1] = x.pp[pt]; Ps[2] = x.pp[(size_t)NS + pt]; Ps[3] = x.pp[(size_t)3 * NS + pt];
+            Ps[4] = t.DEPTH[pt];
+            Pmij = (int)x.pp[(size_t)2 * NS + pt];
+        }
+        if (tid < KL) {
+            const float c = wam_cosf(s.th[tid] - udir);
+            cu[tid] = c;
+            sprd[tid] = wam_ipowf(wam_maxf(0.f, c), 2);
+        }
+        if (tid >= 64 && tid - 64 < ML) {
+            const int m = tid - 64;
+            const int ju = min(t.JUMAX, max(wam_nint(a.u10[pt] / t.DELU), 1));
+            flm[m] = t.FLMINFR[(size_t)m * t.JUMAX + ju - 1];
+            if (SHALLOW) tfk[m] = t.TFAK[tb(t, m, t.INDEP[pt])];
+        }
+        __syncthreads();
+        // SNONLIN, with the ISNONLIN = 0 shallow-water enhancement
+        snonlin(t, s, SHALLOW ? Ps[1] : 1.f, qslot);
+        // SBOTTOM's factor (:1527-1589) and the limiter's (:652-690)
+        if (tid < ML) {
+            if (SHALLOW) {
+                const float wav = tfk[tid];
+                const float sbo = wam_minf((2.f * Ps[4]) * wav, 50.f);
+                tbot[tid] = (((-2.0f * 0.038f) / t.G) * wav) / wam_sinhf(sbo);
+            }
+            const float delfl = (t.C5G / wam_ipowf(s.fr[tid], 4)) * delt;
+            tlim[tid] = (Ps[0] * delfl) * Ps[2];
+        }
+        __syncthreads();
+        if (a.lcflx) {
+            for (int b = tid; b < NB; b += NT) {
+                const int m = b / KL;
+                if (SHALLOW) {
+                    s.SLBOT[b] = 0.f + tbot[m] * s.F[b];
+                    s.FL[b] = s.FL[b] + tbot[m];
+                }
+                s.SMIN[b] = s.SL[b] - s.SMIN[b];
+            }
+            __syncthreads();
+            if (tid == 0 && a.fluxes) wnfluxes<SHALLOW>(t, s, Ps[3], Ps[0], udir, Pmij, tfk, a.fluxes + pt, NS);
+        }
+        // limiter and implicit update, a quad of bins (one direction row) per
+        // thread; SBOTTOM's terms are added in place when SL_BOT is not kept
+        // for the fluxes
+        for (int q = tid; q < NB4; q += NT) {
+            const int m = (4 * q) / KL;
+            float r[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int b = 4 * q + u, k = b - KL * m;
+                const float fo = s.F[b];
+                float sl = s.SL[b], fl = s.FL[b];
+                if (SHALLOW) {
+                    if (a.lcflx) {
+                        sl = sl + s.SLBOT[b];
+                    } else {
+                        const float slb = 0.f + tbot[m] * fo;
+                        fl = fl + tbot[m];
+                        sl = sl + slb;
+                    }
+                }
+                const float temp1 = (delt * sl) / wam_maxf(1.f, 1.f - delt * fl);
+                const float temp3 = wam_minf(wam_fabsf(temp1), tlim[m]);
+                const float f = fo + wam_signf(temp3, temp1);
+                r[u] = wam_maxf(f, flm[m] * sprd[k]);
+            }
+            *reinterpret_cast<float4 *>(x.ft + q0 + (size_t)q * 256) = make_float4(r[0], r[1], r[2], r[3]);
+        }
+        __syncthreads();
+    }
+}
+
+template <int SHALLOW>
+__global__ void __launch_bounds__(PW) implsch_post_kernel(DevTables t, ImplArgs a, ImplScratch x)
+{
+    const int KL = t.KL, ML = t.ML, NS = t.NSEA, NB4 = x.NB4, lane = threadIdx.x;
+    const int pt0 = blockIdx.x * PW + lane;
+    const bool on = pt0 < NS;
+    const int pt = on ? pt0 : NS - 1;
+    const float *FQ = x.ft + quad_offset(pt, 0, NB4);   // bins 4q..4q+3 at FQ[q * 256]
+    const unsigned *LLW = x.llws + pt;
+    const float ust = a.ustar[pt];
+    // direction sums (plain and LLWS-masked) and their integrals; frequency ML first
+    float s_e = 0.f, s_fm = 0.f, s_w1 = 0.f, s_w2 = 0.f, Tl = 0.f, Ql = 0.f;
+    for (int i = 0; i < ML; i++) {
+        const int m = i == 0 ? ML - 1 : i - 1;
+        float T = 0.f, Q = 0.f;
+        bool qfirst = true;
+        for (int k0 = 0; k0 < KL; k0 += 4) {
+            const int b0 = k0 + KL * m;
+            const float4 f4 = *reinterpret_cast<const float4 *>(FQ + (size_t)(b0 >> 2) * 256);
+            const unsigned w = LLW[(size_t)(b0 >> 5) * NS] >> (b0 & 31);   // 4 | 32 and b0 % 4 == 0
+            const float fv[4] = {f4.x, f4.y, f4.z, f4.w};
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int k = k0 + u;
+                const float f = fv[u];
+                T = k == 0 ? f : T + f;
+                if ((w >> u) & 1u) {
+                    Q = qfirst ? f : Q + f;
+                    qfirst = false;
+                }
+            }
+        }
+        if (i == 0) {
+            Tl = T; Ql = Q;
+            s_e = t.MO_TAIL * T; s_fm = t.MM1_TAIL * T; s_w1 = t.MO_TAIL * Q; s_w2 = t.MM1_TAIL * Q;
+        } else {
+            s_e = s_e + T * t.DFIM[m]; s_fm = s_fm + T * t.DFIMOFR[m];
+            s_w1 = s_w1 + Q * t.DFIM[m]; s_w2 = s_w2 + Q * t.DFIMOFR[m];
+        }
+    }
+    {
+        const int m = ML - 1;
+        s_e = s_e + Tl * t.DFIM[m]; s_fm = s_fm + Tl * t.DFIMOFR[m];
+        s_w1 = s_w1 + Ql * t.DFIM[m]; s_w2 = s_w2 + Ql * t.DFIMOFR[m];
+    }
+    const float e = wam_maxf(s_e, t.EMIN);
+    const float fmean = e / wam_maxf(s_fm, t.EMIN);
+    const float ew = wam_maxf(s_w1, t.EMIN);
+    const float fmeanws = ew / wam_maxf(s_w2, t.EMIN);
+    const int mij = frcut(t, t.FR, fmean, fmeanws, ust);
+    const int indep = SHALLOW ? t.INDEP[pt] : 1;
+    float tden;
+    if (SHALLOW) {
+        const float akm0 = 1.f / t.TFAK[tb(t, mij - 1, indep)];
+        tden = wam_ipowf(akm0, 3) / t.TCGOND[tb(t, mij - 1, indep)];
+    } else {
+        tden = t.FRM5[mij - 1];
+    }
+    for (int m = 0; m < ML; m++) {
+        const bool tail = m >= mij;
+        float ttail = 0.f;
+        if (tail) {
+            const float an = SHALLOW ? wam_ipowf(1.f / t.TFAK[tb(t, m, indep)], 3) / t.TCGOND[tb(t, m, indep)]
+                                     : t.FRM5[m];
+            ttail = an / tden;
+        }
+        const int mrow = tail ? mij - 1 : m;
+        for (int k0 = 0; k0 < KL; k0 += 4) {
+            const float4 f4 = *reinterpret_cast<const float4 *>(FQ + (size_t)((k0 + KL * mrow) >> 2) * 256);
+            const float fv[4] = {f4.x, f4.y, f4.z, f4.w};
+            if (on) {
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    a.fl3[(size_t)(k0 + u + KL * m) * NS + pt] = tail ? ttail * fv[u] : fv[u];
+            }
+        }
+    }
+    if (on && a.mij) a.mij[pt] = mij;
+}
+
+size_t implsch_snl_lds_bytes(const DevTables &t, int lcflx)
+{
+    const int ML = t.ML, KL = t.KL, MH = t.MLSTHG, NB = KL * ML;
+    size_t n = (size_t)scr_floats(KL, ML, t.CHUNK, 0) + 3 * (size_t)NB;
+    if (lcflx) n += (t.shallow ? 2 : 1) * (size_t)NB;
+    n += 2 * (size_t)ML + 3 * (size_t)KL + 26 * (size_t)MH + 5 * (size_t)MH + 8 * (size_t)KL;
+    return n * sizeof(float);
+}
+
+// The three kernels of IPHYS = 0 on one stream; the middle one persistent,
+// one workgroup per resident slot.
+hipError_t launch_implsch_split(const DevTables &t, const ImplArgs &a, const ImplScratch &x, int ncu, hipStream_t st)
+{
+    const int nwg = (t.NSEA + PW - 1) / PW;
+    const size_t lds1 = (size_t)t.KL * PW * sizeof(float);
+    if (t.shallow) hipLaunchKernelGGL(implsch_pre_kernel<1>, dim3(nwg), dim3(PW), lds1, st, t, a, x);
+    else hipLaunchKernelGGL(implsch_pre_kernel<0>, dim3(nwg), dim3(PW), lds1, st, t, a, x);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    void (*k)(DevTables, ImplArgs, ImplScratch) = t.shallow ? implsch_snl_kernel<1> : implsch_snl_kernel<0>;
+    const size_t lds = implsch_snl_lds_bytes(t, a.lcflx);
+    e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    int per_cu = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, NT, lds);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) return hipErrorInvalidConfiguration;
+    const long want = (long)per_cu * ncu;
+    const int nblocks = (int)(want < t.NSEA ? want : t.NSEA);
+    hipLaunchKernelGGL(k, dim3(nblocks), dim3(NT), lds, st, t, a, x);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (t.shallow) hipLaunchKernelGGL(implsch_post_kernel<1>, dim3(nwg), dim3(PW), 0, st, t, a, x);
+    else hipLaunchKernelGGL(implsch_post_kernel<0>, dim3(nwg), dim3(PW), 0, st, t, a, x);
+    return hipGetLastError();
 }
 
 size_t implsch_lds_bytes(const DevTables &t, int lcflx)
