@@ -31,6 +31,8 @@ OPCOUNT_FILE = os.path.join(ROOT, "tests", "golden", "implsch_opcount.json")
 # HBM bytes per dispatch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # (scaled by a plain copy kernel of known size), per kernel and workload
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_c2.json")
+# kernels the library times (wam_gpu_get_timing)
+KERNELS = ("propags", "implsch_pre", "implsch_snl", "implsch_post", "implsch")
 
 
 def log(*a):
@@ -148,12 +150,19 @@ def main():
         torch.cuda.synchronize()
         log(f"[bench] warm-up step {i + 1} done")
     evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    g.set_timing(True)            # HIP events around every kernel of the timed steps, on the library's stream
     torch.cuda.synchronize()
     w0 = time.perf_counter()
     for i in range(args.steps):
         step(evs[i])
     torch.cuda.synchronize()
     wall = time.perf_counter() - w0
+    ktime = {}
+    for name in KERNELS:
+        ms, calls = g.kernel_time(name)
+        if calls:
+            ktime[name] = ms / calls
+    g.set_timing(False)
     t_prop = np.array([a.elapsed_time(b) for a, b, c in evs]) * 1e-3
     t_src = np.array([b.elapsed_time(c) for a, b, c in evs]) * 1e-3
     ms_step = wall / args.steps * 1e3
@@ -166,36 +175,48 @@ def main():
     hs = 4.0 * torch.sqrt(torch.clamp(energy, min=0))
 
     kl, ml = cfg["kl"], cfg["ml"]
+    traffic = {}
+    if os.path.exists(TRAFFIC_FILE):
+        traffic = json.load(open(TRAFFIC_FILE))
     # PROPAGS algorithmic bytes per sub-step.  The weights are computed inside
     # the stencil (the library's default for shallow water with currents), so
     # the traffic is the spectrum read and written once plus the per-point
     # fields: 4 neighbour indices, U, V, INDEP, INDEP_G, the six depth and
     # current gradients, DRGM, DELLA0, DPSN_S, DPSN_N (19 x 4 B).
     prop_bytes = n * kl * ml * 4 * 2 + n * 4 * 19
-    prop_launch_s = t_prop.mean() / counter
-    prop_gbs = prop_bytes / prop_launch_s / 1e9
-    # IMPLSCH: compute bound (VALU FP32); algorithmic flops per point from the op count
+    prop_ms = ktime["propags"]
+    prop_gbs = prop_bytes / (prop_ms * 1e-3) / 1e9
+    roofs = {"propags": dict(kernel="propags_fused_sc_kernel", bound="hbm", achieved=round(prop_gbs, 1),
+                             peak=HBM_PEAK_GBS, unit="GB/s", frac=round(prop_gbs / HBM_PEAK_GBS, 4),
+                             traffic=traffic.get("propags_fused_sc_kernel", {}).get("bytes_per_dispatch"),
+                             algorithmic_bytes_per_launch=prop_bytes, avg_launch_ms=round(prop_ms, 3),
+                             note="one sub-step; weights computed in the stencil (no 9-plane weight stream)")}
+    # IMPLSCH: compute bound (VALU FP32).  Algorithmic flops per sea point
+    # from the oracle's op count, per stage for the three kernels of IPHYS=0
+    # (pre: up to SDISSIP; snl: SNONLIN, SBOTTOM and the limited update;
+    # post: new mean parameters and tail).
     opc = json.load(open(OPCOUNT_FILE))[f"c2_sub_iphys{args.iphys}"]
-    src_flops = opc["flops_per_point"] * n
-    src_tflops = src_flops / t_src.mean() / 1e12
-    traffic = {}
-    if os.path.exists(TRAFFIC_FILE):
-        traffic = json.load(open(TRAFFIC_FILE))
-    dominant = "IMPLSCH" if t_src.mean() > t_prop.mean() else "PROPAGS"
-    roof_prop = dict(kernel="propags_fused_sc_kernel", bound="hbm", achieved=round(prop_gbs, 1), peak=HBM_PEAK_GBS,
-                     unit="GB/s", frac=round(prop_gbs / HBM_PEAK_GBS, 4),
-                     traffic=traffic.get("propags_fused_sc_kernel", {}).get("bytes_per_dispatch"),
-                     algorithmic_bytes_per_launch=prop_bytes, avg_launch_ms=round(prop_launch_s * 1e3, 3),
-                     note="weights computed in the stencil (no 9-plane weight stream); VALU-limited")
-    roof_src = dict(kernel=f"implsch_kernel<{args.iphys},1>", bound="mfma", achieved=round(src_tflops, 3),
-                    peak=FP32_PEAK_TFLOPS, unit="TFLOP/s", frac=round(src_tflops / FP32_PEAK_TFLOPS, 4),
-                    traffic=traffic.get(f"implsch_kernel<{args.iphys},1>", {}).get("bytes_per_dispatch"),
-                    algorithmic_flops_per_launch=round(src_flops),
-                    flops_per_point=round(opc["flops_per_point"], 1),
-                    transcendentals_per_point=round(opc["trans_per_point"], 1),
-                    avg_launch_ms=round(t_src.mean() * 1e3, 3),
-                    note="FP32 VALU work (no GEMM-shaped arithmetic); peak is MI355X's FP32 157.3 TFLOP/s, "
-                         "unfused add/mul/div as counted by the oracle's op counter")
+    stage_of = {"implsch_pre": "pre", "implsch_snl": "snl", "implsch_post": "post"}
+    for name, ms in ktime.items():
+        if name == "propags":
+            continue
+        if name == "implsch":
+            fpp, tpp, kname = opc["flops_per_point"], opc["trans_per_point"], f"implsch_kernel<{args.iphys},1>"
+        else:
+            st = opc["stages"][stage_of[name]]
+            fpp, tpp, kname = st["flops_per_point"], st["trans_per_point"], f"{name}_kernel<1>"
+        tflops = fpp * n / (ms * 1e-3) / 1e12
+        roofs[name] = dict(kernel=kname, bound="mfma", achieved=round(tflops, 3), peak=FP32_PEAK_TFLOPS,
+                           unit="TFLOP/s", frac=round(tflops / FP32_PEAK_TFLOPS, 4),
+                           traffic=traffic.get(kname, {}).get("bytes_per_dispatch"),
+                           algorithmic_flops_per_launch=round(fpp * n), flops_per_point=round(fpp, 1),
+                           transcendentals_per_point=round(tpp, 1), avg_launch_ms=round(ms, 3),
+                           note="FP32 VALU work (no GEMM-shaped arithmetic); peak is MI355X's FP32 157.3 "
+                                "TFLOP/s; unfused add/mul/div/sqrt as counted by the oracle's op counter")
+    # the dominant kernel: the largest share of the step's GPU time
+    share = {k: v * (counter if k == "propags" else 1) for k, v in ktime.items()}
+    dominant = max(share, key=share.get)
+    others = {k: v for k, v in roofs.items() if k != dominant}
     out = {
         "metric": "sea-point spectrum updates/s per step (src+prop); batched & adjoint vs plain",
         "value": n / (ms_step * 1e-3),
@@ -209,8 +230,10 @@ def main():
                    "nsea": n, "kl": kl, "ml": ml, "iphys": args.iphys, "counter": counter,
                    "max_cfl": round(cfl, 4), "parallelism": "single GPU"},
         "t_propags_ms": round(t_prop.mean() * 1e3, 3), "t_implsch_ms": round(t_src.mean() * 1e3, 3),
-        "roofline": roof_src if dominant == "IMPLSCH" else roof_prop,
-        "roofline_other": roof_prop if dominant == "IMPLSCH" else roof_src,
+        "kernel_ms": {k: round(v, 3) for k, v in ktime.items()},
+        "roofline": roofs[dominant],
+        "roofline_other": others.get("propags", next(iter(others.values()), None)),
+        "roofline_all": others,
         "state_finite": finite, "mean_hs_m": round(float(hs.mean()), 4),
     }
     if not args.no_cpu_baseline:

@@ -115,6 +115,19 @@ int wam_gpu_implsch_dev(wam_ctx *ctx, float *d_fl3, const float *d_u10, const fl
 
 /* Stream used by the context (a hipStream_t; NULL = the context's own). */
 int wam_gpu_set_stream(wam_ctx *ctx, void *stream);
+
+/* Per-kernel timing on the context stream.  No counterpart in the
+   reference (its timing is host wall-clock around whole calls).  on = 1
+   records a HIP event pair around every kernel launch of PROPAGS and IMPLSCH
+   from now on and drops earlier records; on = 0 stops recording. */
+int wam_gpu_set_timing(wam_ctx *ctx, int32_t on);
+
+/* Total GPU milliseconds and number of launches of one kernel since timing
+   was switched on: name is "propags" (one PROPAGS sub-step), "implsch_pre",
+   "implsch_snl", "implsch_post" (the three IMPLSCH kernels of IPHYS = 0) or
+   "implsch" (the single IMPLSCH kernel of the other configurations).  Waits
+   for the context stream. */
+int wam_gpu_get_timing(wam_ctx *ctx, const char *name, double *total_ms, int64_t *launches);
 int wam_gpu_sync(wam_ctx *ctx);
 
 /* Introspection for parity checks: copies a named table into out (n elements

@@ -70,7 +70,7 @@ def _lib(fp64, opcount=False):
         lib.orc_api_aki.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
         if opcount:
             lib.orc_opcount_implsch.argtypes = ([ctypes.c_void_p] + [ctypes.c_void_p] * 9 +
-                                                [ctypes.c_int] + [ctypes.c_void_p] * 4)
+                                                [ctypes.c_int] + [ctypes.c_void_p] * 5)
         _LIBS[key] = lib
     return _LIBS[key]
 
@@ -153,7 +153,9 @@ class Oracle:
         """IMPLSCH of the REAL*4 restatement with every floating-point
         operation counted (orc_opcount.h), on one thread.  Returns the same
         fields as implsch() and the totals over all sea points:
-        add, mul, div, sqrt, trans (elementary-function calls)."""
+        add, mul, div, sqrt, trans (elementary-function calls), with the same
+        per stage in "stages" (0: up to SDISSIP; 1: SNONLIN, SBOTTOM,
+        WNFLUXES and the limited update; 2: new mean parameters and tail)."""
         if not self.opcount:
             raise ValueError("construct the Oracle with opcount=True")
         n = self.nsea
@@ -171,11 +173,15 @@ class Oracle:
         flux = np.zeros((6, n), dt)
         mij = np.zeros(n, np.int32)
         counts = np.zeros(5, np.uint64)
+        stages = np.zeros((3, 5), np.uint64)
         self.lib.orc_opcount_implsch(self.ctx, _p(f), _p(u10), _p(udir), _p(tauw), _p(ustar), _p(z0),
                                      _p(roairn), _p(wstar), _p(depth), int(bool(lcflx)), _p(phiaw),
-                                     _p(flux), _p(mij), _p(counts))
+                                     _p(flux), _p(mij), _p(counts), _p(stages))
         res = dict(fl3=f, tauw=tauw, ustar=ustar, z0=z0, phiaw=phiaw, flux=flux, mij=mij)
-        return res, dict(zip(["add", "mul", "div", "sqrt", "trans"], (int(x) for x in counts)))
+        names = ["add", "mul", "div", "sqrt", "trans"]
+        tot = dict(zip(names, (int(x) for x in counts)))
+        tot["stages"] = [dict(zip(names, (int(x) for x in row))) for row in stages]
+        return res, tot
 
     def wavnu2(self, depth):
         d = np.ascontiguousarray(depth, dtype=self.dtype)

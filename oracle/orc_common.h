@@ -19,7 +19,8 @@
  *
  * Build FP32 (default, the reference's REAL) or FP64 (-DORC_FP64: the
  * reference built with 8-byte reals); -DORC_OPCOUNT (C++) is the FP32
- * arithmetic with every operation counted (orc_opcount.h).  The FP32 build evaluates the Fortran
+ * arithmetic with every operation counted (orc_opcount.h), per stage at the
+ * ORC_STAGE markers.  The FP32 build evaluates the Fortran
  * intrinsics with wam_math.h (deterministic single precision); the FP64 build
  * uses libm double.
  */
@@ -101,6 +102,13 @@ static inline int INTR(real x) { return wam_int(x); }
 static inline real FLOORR(real x) { return wam_floorf(x); }
 static inline real CEILR(real x) { return wam_ceilf(x); }
 static inline real SIGNR(real a, real b) { return wam_signf(a, b); }
+#endif
+
+/* Stage markers of the op-counting build (no-ops otherwise). */
+#ifdef ORC_OPCOUNT
+#define ORC_STAGE(n) oc_stage(n)
+#else
+#define ORC_STAGE(n) ((void)0)
 #endif
 
 static inline real MAXR(real a, real b) { return a > b ? a : b; }

@@ -738,6 +738,7 @@ static void implsch_point(const orc_ctx *c, orc_work *w, real U10, real UDIR, re
 {
     int KL = c->KL, ML = c->ML, NB = KL * ML, iphys = c->cfg.iphys;
     real SPRD[64], T[64];
+    ORC_STAGE(0);   /* op-count stages: 0 up to SDISSIP, 1 SNONLIN to the update, 2 the new mean parameters and tail */
     for (int k = 0; k < KL; k++) SPRD[k] = IPOW(MAXR(R(0.), COS(c->TH[k] - UDIR)), 2);
     int JU = MINI(c->JUMAX, MAXI(NINT(U10 / c->DELU), 1));
 
@@ -777,6 +778,7 @@ static void implsch_point(const orc_ctx *c, orc_work *w, real U10, real UDIR, re
     if (iphys == 2) { sdissip_st6(c, w); w3swl6(c, w); }
     else sdissip(c, w, EMEAN, F1MEAN, XKMEAN, indep);
 
+    ORC_STAGE(1);
     snonlin(c, w, DEPTH, AKMEAN);
 
     if (c->cfg.shallow) {
@@ -807,6 +809,7 @@ static void implsch_point(const orc_ctx *c, orc_work *w, real U10, real UDIR, re
         }
     }
     /* 5. new mean parameters, cut-off and tail */
+    ORC_STAGE(2);
     dirsum(c, w->F, NULL, T);
     EMEAN = total_energy(c, T);
     FMEAN = femean(c, T, EMEAN);
@@ -819,6 +822,7 @@ static void implsch_point(const orc_ctx *c, orc_work *w, real U10, real UDIR, re
     *TAUW = tauw; *USTAR = ust; *Z0 = z0;
     *phiaw_out = phiaw;
     *mij_out = MIJ;
+    ORC_STAGE(0);
 }
 
 /* Block driver: fl3[(m*KL+k)*NSEA + ij].  out_flux: [6][NSEA] if lcflx. */

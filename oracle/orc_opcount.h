@@ -23,6 +23,21 @@
 struct oc_counts { uint64_t add, mul, div, sqrt, trans; };
 extern oc_counts oc_cnt;
 
+/* Per-stage totals: ORC_STAGE(n) in the restatement closes the running stage
+   (its operations since the last marker go to oc_stage_cnt[stage]) and opens
+   stage n. */
+constexpr int OC_NSTAGE = 3;
+extern oc_counts oc_stage_cnt[OC_NSTAGE], oc_mark;
+extern int oc_cur_stage;
+inline void oc_stage(int n)
+{
+    oc_counts &s = oc_stage_cnt[oc_cur_stage];
+    s.add += oc_cnt.add - oc_mark.add; s.mul += oc_cnt.mul - oc_mark.mul; s.div += oc_cnt.div - oc_mark.div;
+    s.sqrt += oc_cnt.sqrt - oc_mark.sqrt; s.trans += oc_cnt.trans - oc_mark.trans;
+    oc_mark = oc_cnt;
+    oc_cur_stage = n;
+}
+
 struct oc_real {
     float v;
     oc_real() = default;

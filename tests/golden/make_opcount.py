@@ -1,5 +1,5 @@
 """Writes implsch_opcount.json: the floating-point work of one IMPLSCH call
-per sea point, counted by the oracle's op-counting build (oracle/orc_opcount.h:
+per sea point, in total and per stage, counted by the oracle's op-counting build (oracle/orc_opcount.h:
 add/sub, mul, div, sqrt one each; elementary-function calls separately), on
 seeded sub-boxes of the production workload (0.1 deg, 36 x 36, shallow water,
 currents) for both physics options, and on the 60 x 60 deep-water case.
@@ -36,9 +36,14 @@ def count(case):
     fl3 = W.jonswap_spectrum(cfg, f["udir"]).ravel()
     z = np.zeros(n, np.float32)
     _, c = o.implsch_opcount(fl3, f["u10"], f["udir"], z, z, depth)
-    flops = c["add"] + c["mul"] + c["div"] + c["sqrt"]
+    stages = c.pop("stages")
+    fl = lambda x: x["add"] + x["mul"] + x["div"] + x["sqrt"]
+    # the stages of the three-kernel IMPLSCH: up to SDISSIP ("pre"), SNONLIN
+    # to the limited update ("snl"), new mean parameters and tail ("post")
+    st = {nm: dict(totals=x, flops_per_point=fl(x) / n, trans_per_point=x["trans"] / n)
+          for nm, x in zip(("pre", "snl", "post"), stages)}
     return dict(nsea=n, kl=cfg["kl"], ml=cfg["ml"], iphys=cfg["iphys"], shallow=cfg["shallow"],
-                totals=c, flops_per_point=flops / n, trans_per_point=c["trans"] / n)
+                totals=c, flops_per_point=fl(c) / n, trans_per_point=c["trans"] / n, stages=st)
 
 
 def main():

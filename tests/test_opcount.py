@@ -37,3 +37,12 @@ def test_frozen_counts_reproduce():
         got = count(CASES[name])
         assert got["totals"] == ref[name]["totals"], name
         assert got["nsea"] == ref[name]["nsea"]
+        assert got["stages"] == ref[name]["stages"], name
+
+
+def test_stages_partition_the_totals():
+    ref = json.load(open(os.path.join(HERE, "golden", "implsch_opcount.json")))
+    for name, v in ref.items():
+        for key in ("add", "mul", "div", "sqrt", "trans"):
+            assert sum(s["totals"][key] for s in v["stages"].values()) == v["totals"][key], (name, key)
+        assert v["stages"]["snl"]["flops_per_point"] > 0

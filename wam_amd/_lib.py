@@ -39,6 +39,9 @@ SIGNATURES = {
     "wam_gpu_implsch_dev": (_I, [_P] + [_P] * 8 + [_I, _P, _P, _P]),
     "wam_gpu_set_propagation_weights": (_I, [_P, _I]),
     "wam_gpu_set_stream": (_I, [_P, _P]),
+    "wam_gpu_set_timing": (_I, [_P, _I]),
+    "wam_gpu_get_timing": (_I, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
+                                ctypes.POINTER(ctypes.c_int64)]),
     "wam_gpu_sync": (_I, [_P]),
     "wam_gpu_get_table": (ctypes.c_int64, [_P, ctypes.c_char_p, _P, ctypes.c_int64]),
     "wam_gpu_get_scalar": (ctypes.c_double, [_P, ctypes.c_char_p]),

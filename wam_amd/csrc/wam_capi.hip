@@ -27,7 +27,8 @@ __global__ void propags_fused_kernel(DevTables t, int counter, const float *fin,
 __global__ void propags_fused_sc_kernel(DevTables t, int counter, const float *fin, float *fout);
 __global__ void wavnu2_kernel(DevTables t, float *wn, float *cgg);
 hipError_t launch_implsch(const DevTables &t, const ImplArgs &a, int ncu, hipStream_t st);
-hipError_t launch_implsch_split(const DevTables &t, const ImplArgs &a, const ImplScratch &x, int ncu, hipStream_t st);
+hipError_t launch_implsch_split(const DevTables &t, const ImplArgs &a, const ImplScratch &x, int ncu, hipStream_t st,
+                                const hipEvent_t *ev);
 }  // namespace wam
 
 using namespace wam;
@@ -47,6 +48,10 @@ struct wam_ctx {
     float *wn = nullptr, *cgg = nullptr;
     float *fscratch = nullptr;            // COUNTER > 1 ping-pong
     ImplScratch isx{};                    // IMPLSCH work arrays (IPHYS = 0), allocated at the first call
+    // per-kernel timing (wam_gpu_set_timing): an event pair per launch
+    bool timing = false;
+    struct Timed { std::string name; hipEvent_t a, b; };
+    std::vector<Timed> timed;
     float *stage = nullptr;               // host drop-in staging: FL3 + per-point fields
     size_t stage_n = 0;
     bool prepared = false;
@@ -62,6 +67,35 @@ struct wam_ctx {
             return WAM_E_HIP;                                                      \
         }                                                                          \
     } while (0)
+
+// n events recorded around launches when timing is on (ev[i] = nullptr otherwise)
+static int timing_events(wam_ctx *ctx, hipEvent_t *ev, int n)
+{
+    for (int i = 0; i < n; i++) {
+        ev[i] = nullptr;
+        if (ctx->timing) HIPCHK(hipEventCreate(&ev[i]));
+    }
+    return WAM_OK;
+}
+
+static void timing_record(wam_ctx *ctx, hipEvent_t e)
+{
+    if (e) (void)hipEventRecord(e, ctx->stream);
+}
+
+static void timing_keep(wam_ctx *ctx, const char *name, hipEvent_t a, hipEvent_t b)
+{
+    if (a && b) ctx->timed.push_back({name, a, b});
+}
+
+static void timing_clear(wam_ctx *ctx)
+{
+    for (auto &r : ctx->timed) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    ctx->timed.clear();
+}
 
 template <class T>
 static int upload(wam_ctx *ctx, std::vector<void *> &owner, const T *host, size_t n, const T **dev)
@@ -298,6 +332,10 @@ int wam_gpu_propags_dev(wam_ctx *ctx, const float *d_in, float *d_out)
     for (int i = 0; i < ctx->counter; i++) {
         const int left = ctx->counter - 1 - i;
         float *dst = (left % 2 == 0) ? d_out : ctx->fscratch;
+        hipEvent_t ev[2];
+        const int rc = timing_events(ctx, ev, 2);
+        if (rc) return rc;
+        timing_record(ctx, ev[0]);
         if (ctx->coef)
             hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, ctx->stream, ctx->dt, ctx->coef, src, dst);
         else if (t.cfg.refr_c && t.cfg.shallow)
@@ -310,6 +348,8 @@ int wam_gpu_propags_dev(wam_ctx *ctx, const float *d_in, float *d_out)
             hipLaunchKernelGGL(propags_fused_kernel<false>, dim3(grid), dim3(256), 0, ctx->stream, ctx->dt,
                                ctx->counter, src, dst);
         HIPCHK(hipGetLastError());
+        timing_record(ctx, ev[1]);
+        timing_keep(ctx, "propags", ev[0], ev[1]);
         src = dst;
     }
     return WAM_OK;
@@ -346,12 +386,23 @@ int wam_gpu_implsch_dev(wam_ctx *ctx, float *d_fl3, const float *d_u10, const fl
     a.fl3 = d_fl3; a.u10 = d_u10; a.udir = d_udir; a.tauw = d_tauw; a.ustar = d_ustar; a.z0 = d_z0;
     a.roairn = d_roairn; a.wstar = d_wstar; a.lcflx = lcflx ? 1 : 0; a.phiaw = d_phiaw; a.fluxes = d_fluxes;
     a.mij = d_mij;
+    hipEvent_t ev[4];
+    int rc = timing_events(ctx, ev, 4);
+    if (rc) return rc;
     if (ctx->dt.iphys == 0 && ctx->dt.KL % 4 == 0) {      // three kernels; the tiles need KL % 4 == 0
-        const int rc = ensure_impl_scratch(ctx, a.lcflx);
+        rc = ensure_impl_scratch(ctx, a.lcflx);
         if (rc) return rc;
-        HIPCHK(launch_implsch_split(ctx->dt, a, ctx->isx, ctx->ncu, ctx->stream));
+        HIPCHK(launch_implsch_split(ctx->dt, a, ctx->isx, ctx->ncu, ctx->stream, ev));
+        timing_keep(ctx, "implsch_pre", ev[0], ev[1]);
+        timing_keep(ctx, "implsch_snl", ev[1], ev[2]);
+        timing_keep(ctx, "implsch_post", ev[2], ev[3]);
     } else {
+        timing_record(ctx, ev[0]);
         HIPCHK(launch_implsch(ctx->dt, a, ctx->ncu, ctx->stream));
+        timing_record(ctx, ev[1]);
+        timing_keep(ctx, "implsch", ev[0], ev[1]);
+        for (int i = 2; i < 4; i++)
+            if (ev[i]) (void)hipEventDestroy(ev[i]);
     }
     return WAM_OK;
 }
@@ -428,6 +479,32 @@ int wam_gpu_set_propagation_weights(wam_ctx *ctx, int32_t stored)
     if (stored < -1 || stored > 1) { ctx->err = "stored must be -1, 0 or 1"; return WAM_E_ARG; }
     ctx->weights_mode = stored;
     ctx->prepared = false;      // takes effect at the next wam_gpu_prepare_propagation
+    return WAM_OK;
+}
+
+int wam_gpu_set_timing(wam_ctx *ctx, int32_t on)
+{
+    if (!ctx) return WAM_E_ARG;
+    timing_clear(ctx);
+    ctx->timing = on != 0;
+    return WAM_OK;
+}
+
+int wam_gpu_get_timing(wam_ctx *ctx, const char *name, double *total_ms, int64_t *launches)
+{
+    if (!ctx || !name || !total_ms || !launches) return WAM_E_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    double sum = 0.0;
+    int64_t n = 0;
+    for (auto &r : ctx->timed) {
+        if (r.name != name) continue;
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+        sum += ms;
+        n++;
+    }
+    *total_ms = sum;
+    *launches = n;
     return WAM_OK;
 }
 
@@ -578,6 +655,7 @@ void wam_gpu_finalize(wam_ctx *ctx)
     free_list(ctx->dev_allocs);
     if (ctx->fscratch) (void)hipFree(ctx->fscratch);
     if (ctx->stage) (void)hipFree(ctx->stage);
+    timing_clear(ctx);
     for (void *p : {(void *)ctx->isx.ft, (void *)ctx->isx.slt, (void *)ctx->isx.flt, (void *)ctx->isx.smt,
                     (void *)ctx->isx.axy, (void *)ctx->isx.pp, (void *)ctx->isx.llws})
         if (p) (void)hipFree(p);

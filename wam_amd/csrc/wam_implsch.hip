@@ -1523,14 +1523,18 @@ size_t implsch_snl_lds_bytes(const DevTables &t, int lcflx)
 
 // The three kernels of IPHYS = 0 on one stream; the middle one persistent,
 // one workgroup per resident slot.
-hipError_t launch_implsch_split(const DevTables &t, const ImplArgs &a, const ImplScratch &x, int ncu, hipStream_t st)
+hipError_t launch_implsch_split(const DevTables &t, const ImplArgs &a, const ImplScratch &x, int ncu, hipStream_t st,
+                                const hipEvent_t *ev)
 {
     const int nwg = (t.NSEA + PW - 1) / PW;
+    auto mark = [&](int i) { if (ev && ev[i]) (void)hipEventRecord(ev[i], st); };
+    mark(0);
     const size_t lds1 = (size_t)t.KL * PW * sizeof(float);
     if (t.shallow) hipLaunchKernelGGL(implsch_pre_kernel<1>, dim3(nwg), dim3(PW), lds1, st, t, a, x);
     else hipLaunchKernelGGL(implsch_pre_kernel<0>, dim3(nwg), dim3(PW), lds1, st, t, a, x);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    mark(1);
     void (*k)(DevTables, ImplArgs, ImplScratch) = t.shallow ? implsch_snl_kernel<1> : implsch_snl_kernel<0>;
     const size_t lds = implsch_snl_lds_bytes(t, a.lcflx);
     e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -1544,9 +1548,12 @@ hipError_t launch_implsch_split(const DevTables &t, const ImplArgs &a, const Imp
     hipLaunchKernelGGL(k, dim3(nblocks), dim3(NT), lds, st, t, a, x);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
+    mark(2);
     if (t.shallow) hipLaunchKernelGGL(implsch_post_kernel<1>, dim3(nwg), dim3(PW), 0, st, t, a, x);
     else hipLaunchKernelGGL(implsch_post_kernel<0>, dim3(nwg), dim3(PW), 0, st, t, a, x);
-    return hipGetLastError();
+    e = hipGetLastError();
+    mark(3);
+    return e;
 }
 
 size_t implsch_lds_bytes(const DevTables &t, int lcflx)

@@ -97,6 +97,19 @@ class WamGpu:
     def scalar(self, name):
         return self.lib.wam_gpu_get_scalar(self.ctx, name.encode())
 
+    def set_timing(self, on):
+        """Records a HIP event pair around every PROPAGS / IMPLSCH kernel
+        launch from now on (on=True; earlier records are dropped) or stops."""
+        self._check(self.lib.wam_gpu_set_timing(self.ctx, int(bool(on))))
+
+    def kernel_time(self, name):
+        """(total GPU ms, launches) of one kernel since set_timing(True):
+        "propags", "implsch_pre", "implsch_snl", "implsch_post" or "implsch"."""
+        ms = ctypes.c_double(0.0)
+        n = ctypes.c_int64(0)
+        self._check(self.lib.wam_gpu_get_timing(self.ctx, name.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
     def set_stream(self, stream_handle):
         self._check(self.lib.wam_gpu_set_stream(self.ctx, ctypes.c_void_p(stream_handle)))
 
