@@ -29,6 +29,7 @@ __global__ void wavnu2_kernel(DevTables t, float *wn, float *cgg);
 hipError_t launch_implsch(const DevTables &t, const ImplArgs &a, int ncu, hipStream_t st);
 hipError_t launch_implsch_split(const DevTables &t, const ImplArgs &a, const ImplScratch &x, int ncu, hipStream_t st,
                                 const hipEvent_t *ev);
+bool implsch_split_supported(const DevTables &t);
 }  // namespace wam
 
 using namespace wam;
@@ -121,6 +122,7 @@ static void free_list(std::vector<void *> &v)
     } while (0)
 
 static_assert(SN_GROUP == SN_GROUP_DEV && SN_NONE == SN_NONE_DEV, "host and device SNONLIN list layouts differ");
+static_assert(SP_QPT == SP_QPT_DEV && SP_NONE == SP_NONE_DEV, "host and device SNONLIN products layouts differ");
 
 static int upload_static(wam_ctx *ctx)
 {
@@ -165,6 +167,8 @@ static int upload_static(wam_ctx *ctx)
     std::vector<int> k1 = zero_based(t.K1W), k2 = zero_based(t.K2W), k11 = zero_based(t.K11W), k21 = zero_based(t.K21W);
     UP(o, k1, K1W); UP(o, k2, K2W); UP(o, k11, K11W); UP(o, k21, K21W);
     UP(o, t.SN_E, SN_E); UP(o, t.SN_CH, SN_CH); UP(o, t.SN_WW, SN_WW);
+    d.SP_CHUNK = t.SP_CHUNK; d.SP_NCHUNK = t.SP_NCHUNK; d.SP_NRUN = t.SP_NRUN;
+    UP(o, t.SP_E, SP_E); UP(o, t.SP_RW, SP_RW);
     std::vector<int> ks = zero_based(t.KLAT_S), kn = zero_based(t.KLAT_N), kw = zero_based(t.KLON_W), ke = zero_based(t.KLON_E);
     UP(o, ks, KLAT_S); UP(o, kn, KLAT_N); UP(o, kw, KLON_W); UP(o, ke, KLON_E);
     UP(o, t.DPSN_S, DPSN_S); UP(o, t.DPSN_N, DPSN_N);
@@ -389,7 +393,7 @@ int wam_gpu_implsch_dev(wam_ctx *ctx, float *d_fl3, const float *d_u10, const fl
     hipEvent_t ev[4];
     int rc = timing_events(ctx, ev, 4);
     if (rc) return rc;
-    if (ctx->dt.iphys == 0 && ctx->dt.KL % 4 == 0) {      // three kernels; the tiles need KL % 4 == 0
+    if (implsch_split_supported(ctx->dt)) {
         rc = ensure_impl_scratch(ctx, a.lcflx);
         if (rc) return rc;
         HIPCHK(launch_implsch_split(ctx->dt, a, ctx->isx, ctx->ncu, ctx->stream, ev));
@@ -577,6 +581,8 @@ int64_t wam_gpu_get_table(wam_ctx *ctx, const char *name, void *out, int64_t n)
     if (s == "SN_CH") return put(t.SN_CH);
     if (s == "SN_WW") return put(t.SN_WW);
     if (s == "SN_E") { std::vector<int> v(t.SN_E.begin(), t.SN_E.end()); return put(v); }
+    if (s == "SP_RW") return put(t.SP_RW);
+    if (s == "SP_E") { std::vector<int> v(t.SP_E.begin(), t.SP_E.end()); return put(v); }
     if (s == "KLAT" || s == "KLON" || s == "DPSN") {
         // reference layout: KLAT(IJ,IC,ICL) = [ICL][IC][IJ]; KLON(IJ,IC); DPSN(IJ,IC)
         std::vector<int> vi; std::vector<float> vf;
@@ -640,6 +646,9 @@ double wam_gpu_get_scalar(wam_ctx *ctx, const char *name)
     if (s == "NCMAX") return t.NCMAX;
     if (s == "CHUNK") return t.CHUNK;
     if (s == "NCHUNK") return t.NCHUNK;
+    if (s == "SP_CHUNK") return t.SP_CHUNK;
+    if (s == "SP_NCHUNK") return t.SP_NCHUNK;
+    if (s == "SP_NRUN") return t.SP_NRUN;
     if (s == "NK10") return t.NK10;
     if (s == "ALPHA") return t.k.ALPHA;
     if (s == "NCOEF") return ctx->ncoef;

@@ -568,6 +568,32 @@ __device__ __forceinline__ void imphftail(const DevTables &t, const Smem &s, int
 __constant__ int SLOT_SL[9] = {0, 20, 19, 17, 18, 8, 7, 5, 6};
 __constant__ int SLOT_FL[9] = {0, 23, 24, 21, 22, 11, 12, 9, 10};
 
+// The terms of DIA quadruplet (MC = mcm1 + 1, KH*KL + K = r) of SNONLIN
+// (:2890-2960): {AD, DD, DM, DP} = the SL update of the centre bin over -2,
+// the FL update of the centre bin over -2, and the two FL update factors of
+// the MM/MM1 and MP/MP1 bins.
+__device__ __forceinline__ float4 dia_terms(const DevTables &t, const Smem &s, int mcm1, int r, float ftemp_scale)
+{
+    const int KL = t.KL, KL2 = 2 * KL;
+    const float *F = s.F;
+    const int *ic = s.inl + mcm1 * 5;
+    const float *rc = s.rnl + mcm1 * 25;
+    const int k1 = s.kw[r], k2 = s.kw[KL2 + r], k11 = s.kw[2 * KL2 + r], k21 = s.kw[3 * KL2 + r];
+    const int k = r < KL ? r : r - KL;
+    const int IC = (ic[0] - 1) * KL, IP = (ic[1] - 1) * KL, IP1 = (ic[2] - 1) * KL;
+    const int IM = (ic[3] - 1) * KL, IM1 = (ic[4] - 1) * KL;
+    const float sap = rc[1] * F[k1 + IP] + rc[2] * F[k11 + IP] + rc[3] * F[k1 + IP1] + rc[4] * F[k11 + IP1];
+    const float sam = rc[13] * F[k2 + IM] + rc[14] * F[k21 + IM] + rc[15] * F[k2 + IM1] + rc[16] * F[k21 + IM1];
+    const float fij = F[k + IC] * rc[0];
+    const float ftemp = s.af11[mcm1] * ftemp_scale;
+    float fad1 = fij * (sap + sam);
+    const float fad2 = fad1 - (2.f * sap) * sam;
+    fad1 = fad1 + fad2;
+    const float fcen = ftemp * fij;
+    return make_float4(fad2 * fcen, fad1 * ftemp, ((fij - 2.f * sap) * t.DAL2) * fcen,
+                       ((fij - 2.f * sam) * t.DAL1) * fcen);
+}
+
 // SNONLIN (:2742-3047) in chunks of CHUNK frequencies.  The quadruplet terms
 // of a chunk are formed in parallel, as float4 {AD, DD, DM, DP} per
 // quadruplet (slot 511 is kept zero for the padding entries), with a table
@@ -581,8 +607,6 @@ __device__ __forceinline__ void snonlin(const DevTables &t, const Smem &s, float
     const int KL = t.KL, tid = threadIdx.x, KL2 = 2 * KL;
     float4 *QV = reinterpret_cast<float4 *>(s.SCR);
     float4 *WT = QV + 512;
-    const float *F = s.F;
-    const int *K1 = s.kw, *K2 = s.kw + KL2, *K11 = s.kw + 2 * KL2, *K21 = s.kw + 3 * KL2;
     const int lane0 = tid & ~63;
     for (int c = 0; c < t.NCHUNK; c++) {
         const int mc0 = c * t.CHUNK;                  // 0-based first MC of the chunk
@@ -591,22 +615,7 @@ __device__ __forceinline__ void snonlin(const DevTables &t, const Smem &s, float
         for (int i = 0; i < 2; i++) {
             const int mcl = qslot[i] & 255, r = qslot[i] >> 8;
             if (mcl >= nmc) continue;
-            const int ql = tid + i * NT, mcm1 = mc0 + mcl;
-            const int *ic = s.inl + mcm1 * 5;
-            const float *rc = s.rnl + mcm1 * 25;
-            const int k1 = K1[r], k2 = K2[r], k11 = K11[r], k21 = K21[r], k = r < KL ? r : r - KL;
-            const int IC = (ic[0] - 1) * KL, IP = (ic[1] - 1) * KL, IP1 = (ic[2] - 1) * KL;
-            const int IM = (ic[3] - 1) * KL, IM1 = (ic[4] - 1) * KL;
-            const float sap = rc[1] * F[k1 + IP] + rc[2] * F[k11 + IP] + rc[3] * F[k1 + IP1] + rc[4] * F[k11 + IP1];
-            const float sam = rc[13] * F[k2 + IM] + rc[14] * F[k21 + IM] + rc[15] * F[k2 + IM1] + rc[16] * F[k21 + IM1];
-            const float fij = F[k + IC] * rc[0];
-            const float ftemp = s.af11[mcm1] * ftemp_scale;
-            float fad1 = fij * (sap + sam);
-            const float fad2 = fad1 - (2.f * sap) * sam;
-            fad1 = fad1 + fad2;
-            const float fcen = ftemp * fij;
-            QV[ql] = make_float4(fad2 * fcen, fad1 * ftemp, ((fij - 2.f * sap) * t.DAL2) * fcen,
-                                 ((fij - 2.f * sam) * t.DAL1) * fcen);
+            QV[tid + i * NT] = dia_terms(t, s, mc0 + mcl, r, ftemp_scale);
         }
         if (tid >= NT - 9 * nmc) {
             const int wi = tid - (NT - 9 * nmc), mcl = wi / 9, ty = wi - 9 * mcl;
@@ -681,6 +690,74 @@ __device__ __forceinline__ void wnfluxes(const DevTables &t, const Smem &s, floa
     }
     out[0] = phioc; out[(size_t)ns] = tx; out[2 * (size_t)ns] = ty;
     out[3 * (size_t)ns] = pb; out[4 * (size_t)ns] = bx; out[5 * (size_t)ns] = by;
+}
+
+// SNONLIN (:2742-3047) in the products form, for the bins this thread owns,
+// b = threadIdx.x + 256 j, whose SL and FL run in sl[j], fl[j].  Per chunk
+// of SP_CHUNK frequencies every quadruplet's thread forms its nine updates,
+// U[type*S + ql] = {AD * SL weight, (DD|DM|DP) * FL weight} (type 0 the
+// centre bin, weights -2; types 1-4 the MM/MM1 bins with DM, 5-8 the MP/MP1
+// bins with DP), each product rounded as in the reference's update; then
+// every bin adds its updates from the chunk in the reference's order
+// (host-built lists, two entries per 32-bit word).  U[S - 1] = (-0, -0) pads.
+template <int NJ>
+__device__ __forceinline__ void snonlin_products(const DevTables &t, const Smem &s, float2 *U, float ftemp_scale,
+                                                 const int (&qslot)[SP_QPT_DEV], float (&sl)[NJ], float (&fl)[NJ])
+{
+    constexpr int S = 256 * SP_QPT_DEV;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    for (int c = 0; c < t.SP_NCHUNK; c++) {
+        const int mc0 = c * t.SP_CHUNK;
+        const int nmc = min(t.SP_CHUNK, t.MLSTHG - mc0);
+#pragma unroll
+        for (int i = 0; i < SP_QPT_DEV; i++) {
+            const int mcl = qslot[i] & 255, r = qslot[i] >> 8;
+            if (mcl >= nmc) continue;
+            const int ql = tid + i * NT, mcm1 = mc0 + mcl;
+            const float4 q = dia_terms(t, s, mcm1, r, ftemp_scale);
+            const float *rc = s.rnl + mcm1 * 25;
+            U[ql] = make_float2(q.x * -2.f, q.y * -2.f);
+            U[1 * S + ql] = make_float2(q.x * rc[20], q.z * rc[23]);
+            U[2 * S + ql] = make_float2(q.x * rc[19], q.z * rc[24]);
+            U[3 * S + ql] = make_float2(q.x * rc[17], q.z * rc[21]);
+            U[4 * S + ql] = make_float2(q.x * rc[18], q.z * rc[22]);
+            U[5 * S + ql] = make_float2(q.x * rc[8], q.w * rc[11]);
+            U[6 * S + ql] = make_float2(q.x * rc[7], q.w * rc[12]);
+            U[7 * S + ql] = make_float2(q.x * rc[5], q.w * rc[9]);
+            U[8 * S + ql] = make_float2(q.x * rc[6], q.w * rc[10]);
+        }
+        __syncthreads();
+        const int *rw = t.SP_RW + 2 * c * t.SP_NRUN;
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            const int r = 4 * j + wave;
+            if (r >= t.SP_NRUN) continue;
+            const int off = rw[2 * r], np = rw[2 * r + 1] >> 1;
+            const uint32_t *e = reinterpret_cast<const uint32_t *>(t.SP_E + off) + lane;
+            float a = sl[j], b = fl[j];
+            int i = 0;
+            for (; i + 4 <= np; i += 4) {
+                uint32_t v[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) v[u] = e[(i + u) * 64];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const float2 u0 = U[v[u] & 0xffffu], u1 = U[v[u] >> 16];
+                    a = a + u0.x; b = b + u0.y;
+                    a = a + u1.x; b = b + u1.y;
+                }
+            }
+            for (; i < np; i++) {
+                const uint32_t v = e[i * 64];
+                const float2 u0 = U[v & 0xffffu], u1 = U[v >> 16];
+                a = a + u0.x; b = b + u0.y;
+                a = a + u1.x; b = b + u1.y;
+            }
+            sl[j] = a;
+            fl[j] = b;
+        }
+        __syncthreads();
+    }
 }
 
 // Scratch floats: SNONLIN's quadruplet terms (float4 per quadruplet of a
@@ -1303,6 +1380,8 @@ __global__ void __launch_bounds__(PW) implsch_pre_kernel(DevTables t, ImplArgs a
     }
 }
 
+constexpr int SNL_NJ = 6;   // bins per thread of implsch_snl_kernel: KL*ML <= 256 * SNL_NJ
+
 template <int SHALLOW>
 __global__ void __launch_bounds__(NT) implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x)
 {
@@ -1310,13 +1389,15 @@ __global__ void __launch_bounds__(NT) implsch_snl_kernel(DevTables t, ImplArgs a
     __shared__ float cu[MAXF], sprd[MAXF], flm[MAXF], tfk[MAXF], tlim[MAXF], tbot[MAXF];
     __shared__ float Ps[5];
     __shared__ int Pmij;
+    constexpr int S = 256 * SP_QPT_DEV;
     const int KL = t.KL, ML = t.ML, NB = KL * ML, NS = t.NSEA, NB4 = x.NB4, tid = threadIdx.x;
     const int MH = t.MLSTHG;
 
+    // LDS: the products U [9][S] float2, F, (SMIN, SL_BOT with LCFLX), tables
     Smem s{};
     float *p = lds_raw;
-    s.SCR = p; p += scr_floats(KL, ML, t.CHUNK, 0);
-    s.F = p; p += NB; s.SL = p; p += NB; s.FL = p; p += NB;
+    float2 *U = reinterpret_cast<float2 *>(p); p += 2 * 9 * S;
+    s.F = p; p += NB;
     if (a.lcflx) {
         s.SMIN = p; p += NB;
         if (SHALLOW) { s.SLBOT = p; p += NB; }
@@ -1334,10 +1415,13 @@ __global__ void __launch_bounds__(NT) implsch_snl_kernel(DevTables t, ImplArgs a
     for (int i = tid; i < 2 * KL; i += NT) {
         s.kw[i] = t.K1W[i]; s.kw[2 * KL + i] = t.K2W[i]; s.kw[4 * KL + i] = t.K11W[i]; s.kw[6 * KL + i] = t.K21W[i];
     }
-    int qslot[2];
-    for (int i = 0; i < 2; i++) {
+    if (tid == 0) U[S - 1] = make_float2(-0.f, -0.f);
+    // this thread's quadruplet slots (MC offset, KH*KL + K); MC offset 255
+    // marks a slot past the chunk
+    int qslot[SP_QPT_DEV];
+    for (int i = 0; i < SP_QPT_DEV; i++) {
         const int ql = tid + i * NT, mcl = ql / (2 * KL);
-        qslot[i] = ql < 511 && mcl < t.CHUNK ? mcl | (ql - mcl * 2 * KL) << 8 : 255;
+        qslot[i] = ql < S - 1 && mcl < t.SP_CHUNK ? mcl | (ql - mcl * 2 * KL) << 8 : 255;
     }
 
     const int nblk = gridDim.x, xcd = blockIdx.x % 8, lb = blockIdx.x / 8;
@@ -1348,14 +1432,24 @@ __global__ void __launch_bounds__(NT) implsch_snl_kernel(DevTables t, ImplArgs a
     for (int pt = p0 + lb; pt < p1; pt += nb_x) {
         const size_t q0 = quad_offset(pt, 0, NB4);
         {
-            float4 *F4 = reinterpret_cast<float4 *>(s.F), *SL4 = reinterpret_cast<float4 *>(s.SL);
-            float4 *FL4 = reinterpret_cast<float4 *>(s.FL), *SM4 = reinterpret_cast<float4 *>(s.SMIN);
+            float4 *F4 = reinterpret_cast<float4 *>(s.F), *SM4 = reinterpret_cast<float4 *>(s.SMIN);
             for (int q = tid; q < NB4; q += NT) {
                 const size_t o = q0 + (size_t)q * 256;
                 F4[q] = *reinterpret_cast<const float4 *>(x.ft + o);
-                SL4[q] = *reinterpret_cast<const float4 *>(x.slt + o);
-                FL4[q] = *reinterpret_cast<const float4 *>(x.flt + o);
                 if (a.lcflx) SM4[q] = *reinterpret_cast<const float4 *>(x.smt + o);
+            }
+        }
+        // SL and FL of the bins tid + 256 j
+        float sl[SNL_NJ], fl[SNL_NJ];
+#pragma unroll
+        for (int j = 0; j < SNL_NJ; j++) {
+            const int b = tid + 256 * j;
+            sl[j] = 0.f;
+            fl[j] = 0.f;
+            if (b < NB) {
+                const size_t o = q0 + (size_t)(b >> 2) * 256 + (b & 3);
+                sl[j] = x.slt[o];
+                fl[j] = x.flt[o];
             }
         }
         const float udir = a.udir[pt];
@@ -1377,7 +1471,7 @@ __global__ void __launch_bounds__(NT) implsch_snl_kernel(DevTables t, ImplArgs a
         }
         __syncthreads();
         // SNONLIN, with the ISNONLIN = 0 shallow-water enhancement
-        snonlin(t, s, SHALLOW ? Ps[1] : 1.f, qslot);
+        snonlin_products<SNL_NJ>(t, s, U, SHALLOW ? Ps[1] : 1.f, qslot, sl, fl);
         // SBOTTOM's factor (:1527-1589) and the limiter's (:652-690)
         if (tid < ML) {
             if (SHALLOW) {
@@ -1390,43 +1484,42 @@ __global__ void __launch_bounds__(NT) implsch_snl_kernel(DevTables t, ImplArgs a
         }
         __syncthreads();
         if (a.lcflx) {
-            for (int b = tid; b < NB; b += NT) {
-                const int m = b / KL;
-                if (SHALLOW) {
-                    s.SLBOT[b] = 0.f + tbot[m] * s.F[b];
-                    s.FL[b] = s.FL[b] + tbot[m];
+            // WNFLUXES on SSOURCE = SL - SMIN and SL_BOT = 0 + factor * F (SBOTTOM)
+#pragma unroll
+            for (int j = 0; j < SNL_NJ; j++) {
+                const int b = tid + 256 * j;
+                if (b < NB) {
+                    if (SHALLOW) s.SLBOT[b] = 0.f + tbot[b / KL] * s.F[b];
+                    s.SMIN[b] = sl[j] - s.SMIN[b];
                 }
-                s.SMIN[b] = s.SL[b] - s.SMIN[b];
             }
             __syncthreads();
             if (tid == 0 && a.fluxes) wnfluxes<SHALLOW>(t, s, Ps[3], Ps[0], udir, Pmij, tfk, a.fluxes + pt, NS);
         }
-        // limiter and implicit update, a quad of bins (one direction row) per
-        // thread; SBOTTOM's terms are added in place when SL_BOT is not kept
-        // for the fluxes
-        for (int q = tid; q < NB4; q += NT) {
-            const int m = (4 * q) / KL;
-            float r[4];
+        // SBOTTOM (SL_BOT = 0 + factor * F, FL + factor), the limiter and the
+        // implicit update, into F in place (nothing reads F after SNONLIN's
+        // last quadruplet pass)
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int b = 4 * q + u, k = b - KL * m;
-                const float fo = s.F[b];
-                float sl = s.SL[b], fl = s.FL[b];
+        for (int j = 0; j < SNL_NJ; j++) {
+            const int b = tid + 256 * j;
+            if (b < NB) {
+                const int m = b / KL, k = b - KL * m;
+                float sv = sl[j], fv = fl[j];
                 if (SHALLOW) {
-                    if (a.lcflx) {
-                        sl = sl + s.SLBOT[b];
-                    } else {
-                        const float slb = 0.f + tbot[m] * fo;
-                        fl = fl + tbot[m];
-                        sl = sl + slb;
-                    }
+                    const float slb = 0.f + tbot[m] * s.F[b];
+                    fv = fv + tbot[m];
+                    sv = sv + slb;
                 }
-                const float temp1 = (delt * sl) / wam_maxf(1.f, 1.f - delt * fl);
+                const float temp1 = (delt * sv) / wam_maxf(1.f, 1.f - delt * fv);
                 const float temp3 = wam_minf(wam_fabsf(temp1), tlim[m]);
-                const float f = fo + wam_signf(temp3, temp1);
-                r[u] = wam_maxf(f, flm[m] * sprd[k]);
+                const float f = s.F[b] + wam_signf(temp3, temp1);
+                s.F[b] = wam_maxf(f, flm[m] * sprd[k]);
             }
-            *reinterpret_cast<float4 *>(x.ft + q0 + (size_t)q * 256) = make_float4(r[0], r[1], r[2], r[3]);
+        }
+        __syncthreads();
+        {
+            const float4 *F4 = reinterpret_cast<const float4 *>(s.F);
+            for (int q = tid; q < NB4; q += NT) *reinterpret_cast<float4 *>(x.ft + q0 + (size_t)q * 256) = F4[q];
         }
         __syncthreads();
     }
@@ -1515,10 +1608,18 @@ __global__ void __launch_bounds__(PW) implsch_post_kernel(DevTables t, ImplArgs 
 size_t implsch_snl_lds_bytes(const DevTables &t, int lcflx)
 {
     const int ML = t.ML, KL = t.KL, MH = t.MLSTHG, NB = KL * ML;
-    size_t n = (size_t)scr_floats(KL, ML, t.CHUNK, 0) + 3 * (size_t)NB;
+    size_t n = 2 * 9 * 256 * (size_t)SP_QPT_DEV + (size_t)NB;
     if (lcflx) n += (t.shallow ? 2 : 1) * (size_t)NB;
     n += 2 * (size_t)ML + 3 * (size_t)KL + 26 * (size_t)MH + 5 * (size_t)MH + 8 * (size_t)KL;
     return n * sizeof(float);
+}
+
+// Whether the three-kernel form applies: Janssen physics, whole 4-bin quads
+// per direction row, at most SNL_NJ bins per thread, one chunk's quadruplets
+// of one frequency within the slots.
+bool implsch_split_supported(const DevTables &t)
+{
+    return t.iphys == 0 && t.KL % 4 == 0 && t.KL * t.ML <= 256 * SNL_NJ && 2 * t.KL < 256 * SP_QPT_DEV;
 }
 
 // The three kernels of IPHYS = 0 on one stream; the middle one persistent,

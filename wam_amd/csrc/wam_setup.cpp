@@ -413,6 +413,36 @@ static int snonlin_lists(Tables &t, std::string &err)
             t.SN_WW.push_back((wg + SN_GROUP - 1) / SN_GROUP * SN_GROUP);
         }
     }
+    // products form: chunks of at most 255 quadruplets, per run of 64 bins
+    t.SP_CHUNK = std::max(1, (256 * SP_QPT - 1) / (2 * KL));
+    t.SP_NCHUNK = (t.MLSTHG + t.SP_CHUNK - 1) / t.SP_CHUNK;
+    t.SP_NRUN = (NB + 63) / 64;
+    if (2 * KL > 256 * SP_QPT - 1) { err = "SNONLIN products form needs more quadruplet slots"; return WAM_E_ARG; }
+    t.SP_E.clear();
+    t.SP_RW.assign((size_t)2 * t.SP_NCHUNK * t.SP_NRUN, 0);
+    for (int c = 0; c < t.SP_NCHUNK; c++) {
+        const int mc_lo = 1 + c * t.SP_CHUNK, mc_hi = std::min(t.MLSTHG, mc_lo + t.SP_CHUNK - 1);
+        for (int r = 0; r < t.SP_NRUN; r++) {
+            std::vector<std::vector<uint16_t>> per(64);
+            size_t w = 0;
+            for (int l = 0; l < 64 && 64 * r + l < NB; l++) {
+                for (int e : lists[64 * r + l]) {
+                    const int q = e & 0xffff, ty = e >> 16, mc = q / (2 * KL) + 1;
+                    if (mc < mc_lo || mc > mc_hi) continue;
+                    per[l].push_back((uint16_t)(ty * 256 * SP_QPT + (q - (mc_lo - 1) * 2 * KL)));
+                }
+                w = std::max(w, per[l].size());
+            }
+            w = (w + 1) / 2 * 2;                                  // whole pairs
+            const size_t off = t.SP_E.size();
+            t.SP_RW[2 * ((size_t)c * t.SP_NRUN + r)] = (int32_t)off;
+            t.SP_RW[2 * ((size_t)c * t.SP_NRUN + r) + 1] = (int32_t)w;
+            t.SP_E.resize(off + w * 64, SP_NONE);
+            // pairs of entries lane-interleaved: entries 2i, 2i+1 of lane l at (i*64 + l)*2 + {0, 1}
+            for (int l = 0; l < 64; l++)
+                for (size_t i = 0; i < per[l].size(); i++) t.SP_E[off + ((i / 2) * 64 + l) * 2 + (i % 2)] = per[l][i];
+        }
+    }
     return WAM_OK;
 }
 
