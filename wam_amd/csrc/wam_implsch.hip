@@ -692,28 +692,30 @@ __device__ __forceinline__ void wnfluxes(const DevTables &t, const Smem &s, floa
     out[3 * (size_t)ns] = pb; out[4 * (size_t)ns] = bx; out[5 * (size_t)ns] = by;
 }
 
-// SNONLIN (:2742-3047) in the products form, for the bins this thread owns,
-// b = threadIdx.x + 256 j, whose SL and FL run in sl[j], fl[j].  Per chunk
+// SNONLIN (:2742-3047) in the products form for a workgroup of NTH threads,
+// for the bins this thread owns, b = threadIdx.x + NTH j, whose SL and FL
+// run in sl[j], fl[j].  Per chunk
 // of SP_CHUNK frequencies every quadruplet's thread forms its nine updates,
 // U[type*S + ql] = {AD * SL weight, (DD|DM|DP) * FL weight} (type 0 the
 // centre bin, weights -2; types 1-4 the MM/MM1 bins with DM, 5-8 the MP/MP1
 // bins with DP), each product rounded as in the reference's update; then
 // every bin adds its updates from the chunk in the reference's order
 // (host-built lists, two entries per 32-bit word).  U[S - 1] = (-0, -0) pads.
-template <int NJ>
+template <int NTH, int NJ>
 __device__ __forceinline__ void snonlin_products(const DevTables &t, const Smem &s, float2 *U, float ftemp_scale,
-                                                 const int (&qslot)[SP_QPT_DEV], float (&sl)[NJ], float (&fl)[NJ])
+                                                 const int (&qslot)[256 * SP_QPT_DEV / NTH], float (&sl)[NJ],
+                                                 float (&fl)[NJ])
 {
-    constexpr int S = 256 * SP_QPT_DEV;
+    constexpr int S = 256 * SP_QPT_DEV, QT = S / NTH, NW = NTH / 64;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     for (int c = 0; c < t.SP_NCHUNK; c++) {
         const int mc0 = c * t.SP_CHUNK;
         const int nmc = min(t.SP_CHUNK, t.MLSTHG - mc0);
 #pragma unroll
-        for (int i = 0; i < SP_QPT_DEV; i++) {
+        for (int i = 0; i < QT; i++) {
             const int mcl = qslot[i] & 255, r = qslot[i] >> 8;
             if (mcl >= nmc) continue;
-            const int ql = tid + i * NT, mcm1 = mc0 + mcl;
+            const int ql = tid + i * NTH, mcm1 = mc0 + mcl;
             const float4 q = dia_terms(t, s, mcm1, r, ftemp_scale);
             const float *rc = s.rnl + mcm1 * 25;
             U[ql] = make_float2(q.x * -2.f, q.y * -2.f);
@@ -730,7 +732,7 @@ __device__ __forceinline__ void snonlin_products(const DevTables &t, const Smem 
         const int *rw = t.SP_RW + 2 * c * t.SP_NRUN;
 #pragma unroll
         for (int j = 0; j < NJ; j++) {
-            const int r = 4 * j + wave;
+            const int r = NW * j + wave;
             if (r >= t.SP_NRUN) continue;
             const int off = rw[2 * r], np = rw[2 * r + 1] >> 1;
             const uint32_t *e = reinterpret_cast<const uint32_t *>(t.SP_E + off) + lane;
@@ -1380,10 +1382,14 @@ __global__ void __launch_bounds__(PW) implsch_pre_kernel(DevTables t, ImplArgs a
     }
 }
 
-constexpr int SNL_NJ = 6;   // bins per thread of implsch_snl_kernel: KL*ML <= 256 * SNL_NJ
+constexpr int SNL_NT = 512;  // threads of implsch_snl_kernel: one quadruplet of a chunk each
+constexpr int SNL_NJ = 3;    // bins per thread: KL*ML <= SNL_NT * SNL_NJ
+constexpr int SNL_QT = 256 * SP_QPT_DEV / SNL_NT;
 
+// Three workgroups (LDS-limited) of 8 waves per CU: registers for 6 waves per SIMD.
 template <int SHALLOW>
-__global__ void __launch_bounds__(NT) implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x)
+__global__ void __launch_bounds__(SNL_NT) __attribute__((amdgpu_waves_per_eu(6)))
+implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x)
 {
     extern __shared__ __align__(16) float lds_raw[];
     __shared__ float cu[MAXF], sprd[MAXF], flm[MAXF], tfk[MAXF], tlim[MAXF], tbot[MAXF];
@@ -1407,20 +1413,20 @@ __global__ void __launch_bounds__(NT) implsch_snl_kernel(DevTables t, ImplArgs a
     s.rnl = p; p += 25 * MH; s.af11 = p; p += MH;
     s.inl = (int *)p; p += 5 * MH; s.kw = (int *)p; p += 8 * KL;
 
-    for (int i = tid; i < ML; i += NT) { s.fr[i] = t.FR[i]; s.dfim[i] = t.DFIM[i]; }
-    for (int i = tid; i < KL; i += NT) { s.th[i] = t.TH[i]; s.costh[i] = t.COSTH[i]; s.sinth[i] = t.SINTH[i]; }
-    for (int i = tid; i < 25 * MH; i += NT) s.rnl[i] = t.RNLCOEF[i];
-    for (int i = tid; i < 5 * MH; i += NT) s.inl[i] = t.INLCOEF[i];
-    for (int i = tid; i < MH; i += NT) s.af11[i] = t.AF11[i];
-    for (int i = tid; i < 2 * KL; i += NT) {
+    for (int i = tid; i < ML; i += SNL_NT) { s.fr[i] = t.FR[i]; s.dfim[i] = t.DFIM[i]; }
+    for (int i = tid; i < KL; i += SNL_NT) { s.th[i] = t.TH[i]; s.costh[i] = t.COSTH[i]; s.sinth[i] = t.SINTH[i]; }
+    for (int i = tid; i < 25 * MH; i += SNL_NT) s.rnl[i] = t.RNLCOEF[i];
+    for (int i = tid; i < 5 * MH; i += SNL_NT) s.inl[i] = t.INLCOEF[i];
+    for (int i = tid; i < MH; i += SNL_NT) s.af11[i] = t.AF11[i];
+    for (int i = tid; i < 2 * KL; i += SNL_NT) {
         s.kw[i] = t.K1W[i]; s.kw[2 * KL + i] = t.K2W[i]; s.kw[4 * KL + i] = t.K11W[i]; s.kw[6 * KL + i] = t.K21W[i];
     }
     if (tid == 0) U[S - 1] = make_float2(-0.f, -0.f);
     // this thread's quadruplet slots (MC offset, KH*KL + K); MC offset 255
     // marks a slot past the chunk
-    int qslot[SP_QPT_DEV];
-    for (int i = 0; i < SP_QPT_DEV; i++) {
-        const int ql = tid + i * NT, mcl = ql / (2 * KL);
+    int qslot[SNL_QT];
+    for (int i = 0; i < SNL_QT; i++) {
+        const int ql = tid + i * SNL_NT, mcl = ql / (2 * KL);
         qslot[i] = ql < S - 1 && mcl < t.SP_CHUNK ? mcl | (ql - mcl * 2 * KL) << 8 : 255;
     }
 
@@ -1433,17 +1439,17 @@ __global__ void __launch_bounds__(NT) implsch_snl_kernel(DevTables t, ImplArgs a
         const size_t q0 = quad_offset(pt, 0, NB4);
         {
             float4 *F4 = reinterpret_cast<float4 *>(s.F), *SM4 = reinterpret_cast<float4 *>(s.SMIN);
-            for (int q = tid; q < NB4; q += NT) {
+            for (int q = tid; q < NB4; q += SNL_NT) {
                 const size_t o = q0 + (size_t)q * 256;
                 F4[q] = *reinterpret_cast<const float4 *>(x.ft + o);
                 if (a.lcflx) SM4[q] = *reinterpret_cast<const float4 *>(x.smt + o);
             }
         }
-        // SL and FL of the bins tid + 256 j
+        // SL and FL of the bins tid + SNL_NT j
         float sl[SNL_NJ], fl[SNL_NJ];
 #pragma unroll
         for (int j = 0; j < SNL_NJ; j++) {
-            const int b = tid + 256 * j;
+            const int b = tid + SNL_NT * j;
             sl[j] = 0.f;
             fl[j] = 0.f;
             if (b < NB) {
@@ -1471,7 +1477,7 @@ __global__ void __launch_bounds__(NT) implsch_snl_kernel(DevTables t, ImplArgs a
         }
         __syncthreads();
         // SNONLIN, with the ISNONLIN = 0 shallow-water enhancement
-        snonlin_products<SNL_NJ>(t, s, U, SHALLOW ? Ps[1] : 1.f, qslot, sl, fl);
+        snonlin_products<SNL_NT, SNL_NJ>(t, s, U, SHALLOW ? Ps[1] : 1.f, qslot, sl, fl);
         // SBOTTOM's factor (:1527-1589) and the limiter's (:652-690)
         if (tid < ML) {
             if (SHALLOW) {
@@ -1487,7 +1493,7 @@ __global__ void __launch_bounds__(NT) implsch_snl_kernel(DevTables t, ImplArgs a
             // WNFLUXES on SSOURCE = SL - SMIN and SL_BOT = 0 + factor * F (SBOTTOM)
 #pragma unroll
             for (int j = 0; j < SNL_NJ; j++) {
-                const int b = tid + 256 * j;
+                const int b = tid + SNL_NT * j;
                 if (b < NB) {
                     if (SHALLOW) s.SLBOT[b] = 0.f + tbot[b / KL] * s.F[b];
                     s.SMIN[b] = sl[j] - s.SMIN[b];
@@ -1501,7 +1507,7 @@ __global__ void __launch_bounds__(NT) implsch_snl_kernel(DevTables t, ImplArgs a
         // last quadruplet pass)
 #pragma unroll
         for (int j = 0; j < SNL_NJ; j++) {
-            const int b = tid + 256 * j;
+            const int b = tid + SNL_NT * j;
             if (b < NB) {
                 const int m = b / KL, k = b - KL * m;
                 float sv = sl[j], fv = fl[j];
@@ -1519,7 +1525,7 @@ __global__ void __launch_bounds__(NT) implsch_snl_kernel(DevTables t, ImplArgs a
         __syncthreads();
         {
             const float4 *F4 = reinterpret_cast<const float4 *>(s.F);
-            for (int q = tid; q < NB4; q += NT) *reinterpret_cast<float4 *>(x.ft + q0 + (size_t)q * 256) = F4[q];
+            for (int q = tid; q < NB4; q += SNL_NT) *reinterpret_cast<float4 *>(x.ft + q0 + (size_t)q * 256) = F4[q];
         }
         __syncthreads();
     }
@@ -1619,7 +1625,7 @@ size_t implsch_snl_lds_bytes(const DevTables &t, int lcflx)
 // of one frequency within the slots.
 bool implsch_split_supported(const DevTables &t)
 {
-    return t.iphys == 0 && t.KL % 4 == 0 && t.KL * t.ML <= 256 * SNL_NJ && 2 * t.KL < 256 * SP_QPT_DEV;
+    return t.iphys == 0 && t.KL % 4 == 0 && t.KL * t.ML <= SNL_NT * SNL_NJ && 2 * t.KL < 256 * SP_QPT_DEV;
 }
 
 // The three kernels of IPHYS = 0 on one stream; the middle one persistent,
@@ -1641,12 +1647,12 @@ hipError_t launch_implsch_split(const DevTables &t, const ImplArgs &a, const Imp
     e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     int per_cu = 0;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, NT, lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, SNL_NT, lds);
     if (e != hipSuccess) return e;
     if (per_cu < 1) return hipErrorInvalidConfiguration;
     const long want = (long)per_cu * ncu;
     const int nblocks = (int)(want < t.NSEA ? want : t.NSEA);
-    hipLaunchKernelGGL(k, dim3(nblocks), dim3(NT), lds, st, t, a, x);
+    hipLaunchKernelGGL(k, dim3(nblocks), dim3(SNL_NT), lds, st, t, a, x);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     mark(2);
