@@ -107,6 +107,46 @@ def test_chunked_gather_lists_replay_the_scatter_order(kl, ml):
         assert got[b] == list(lists[b, :n[b]]), b
 
 
+@pytest.mark.parametrize("kl,ml", [(36, 36), (24, 25), (12, 25)])
+def test_products_gather_lists_replay_the_scatter_order(kl, ml):
+    """The per-run lists of the products form (byte offsets of update slots,
+    two entries per 32-bit word), decoded chunk by chunk, give back every
+    bin's ordered contribution list; padding is the zero slot only."""
+    cfg = W.base_config(kl=kl, ml=ml, nx=4, ny=4)
+    p = WamGpu(cfg, np.ones((4, 4), np.uint8), host_only=True)
+    nb = kl * ml
+    n = np.asarray(p.table("NCONTRIB"))
+    ncmax = int(p.scalar("NCMAX"))
+    lists = np.asarray(p.table("CONTRIB")).view(np.int32).reshape(nb, ncmax)
+    rw = np.asarray(p.table("SP_RW")).view(np.int32).reshape(-1, 2)
+    e = np.asarray(p.table("SP_E")).view(np.int32)
+    chunk, nchunk, nrun = int(p.scalar("SP_CHUNK")), int(p.scalar("SP_NCHUNK")), int(p.scalar("SP_NRUN"))
+    slots = 512                                   # update slots per type (two quadruplets per thread of 256)
+    assert 2 * kl * chunk < slots                 # the last slot of type 0 stays free for the padding
+    pad = (slots - 1) * 8
+    assert nrun == (nb + 63) // 64 and len(rw) == nchunk * nrun
+    got = [[] for _ in range(nb)]
+    for c in range(nchunk):
+        for r in range(nrun):
+            off, w = rw[c * nrun + r]
+            assert w % 2 == 0
+            for l in range(64):
+                b = 64 * r + l
+                ended = False
+                for i in range(w):
+                    v = int(e[off + ((i // 2) * 64 + l) * 2 + (i % 2)])
+                    if v == pad:
+                        ended = True
+                        continue
+                    assert not ended, "entry after the padding"
+                    assert b < nb and v % 8 == 0
+                    ty, ql = divmod(v // 8, slots)
+                    assert ty < 9 and ql < 2 * kl * chunk
+                    got[b].append((c * chunk * 2 * kl + ql) | (ty << 16))
+    for b in range(nb):
+        assert got[b] == list(lists[b, :n[b]]), b
+
+
 def test_bad_options_raise():
     cfg = W.base_config(iphys=1)
     with pytest.raises(WamError) as e:

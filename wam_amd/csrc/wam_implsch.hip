@@ -701,41 +701,116 @@ __device__ __forceinline__ void wnfluxes(const DevTables &t, const Smem &s, floa
 // bins with DP), each product rounded as in the reference's update; then
 // every bin adds its updates from the chunk in the reference's order
 // (host-built lists, two entries per 32-bit word).  U[S - 1] = (-0, -0) pads.
+// The nine updates of one quadruplet, u[type * S] (see snonlin_products),
+// from the MC's update weights w[0..3] (snl_coef_block); the stores cannot
+// alias the weights, so their loads go first.
+template <int S>
+__device__ __forceinline__ void dia_products(float2 *__restrict__ u, const float4 *__restrict__ w, float qx, float qy,
+                                             float qz, float qw)
+{
+    u[0] = make_float2(qx * -2.f, qy * -2.f);
+    const float4 a = w[0], b = w[1], c = w[2], d = w[3];
+    u[1 * S] = make_float2(qx * a.x, qz * a.y);
+    u[2 * S] = make_float2(qx * a.z, qz * a.w);
+    u[3 * S] = make_float2(qx * b.x, qz * b.y);
+    u[4 * S] = make_float2(qx * b.z, qz * b.w);
+    u[5 * S] = make_float2(qx * c.x, qw * c.y);
+    u[6 * S] = make_float2(qx * c.z, qw * c.w);
+    u[7 * S] = make_float2(qx * d.x, qw * d.y);
+    u[8 * S] = make_float2(qx * d.z, qw * d.w);
+}
+
+// The LDS block of one MC for implsch_snl_kernel, 36 words (9 x 16 bytes):
+// RNLCOEF(1..5, 14..17) = rc[0..4], rc[13..16]; AF11; the byte offsets
+// (INLCOEF - 1)*KL*4 of rows IC, IP, IP1, IM, IM1; then the update weights
+// (SL, FL) of types 1..8: rc[20, 23], rc[19, 24], rc[17, 21], rc[18, 22],
+// rc[8, 11], rc[7, 12], rc[5, 9], rc[6, 10].
+__device__ __forceinline__ float snl_coef_block(const DevTables &t, int mc, int u)
+{
+    const float *rc = t.RNLCOEF + 25 * mc;
+    switch (u) {
+    case 0: case 1: case 2: case 3: case 4: return rc[u];
+    case 5: case 6: case 7: case 8: return rc[u + 8];
+    case 9: return t.AF11[mc];
+    case 12: case 13: case 14: case 15: case 16:
+        return __int_as_float((t.INLCOEF[5 * mc + u - 12] - 1) * t.KL * 4);
+    case 20: return rc[20]; case 21: return rc[23]; case 22: return rc[19]; case 23: return rc[24];
+    case 24: return rc[17]; case 25: return rc[21]; case 26: return rc[18]; case 27: return rc[22];
+    case 28: return rc[8]; case 29: return rc[11]; case 30: return rc[7]; case 31: return rc[12];
+    case 32: return rc[5]; case 33: return rc[9]; case 34: return rc[6]; case 35: return rc[10];
+    default: return 0.f;
+    }
+}
+
 template <int NTH, int NJ>
 __device__ __forceinline__ void snonlin_products(const DevTables &t, const Smem &s, float2 *U, float ftemp_scale,
-                                                 const int (&qslot)[256 * SP_QPT_DEV / NTH], float (&sl)[NJ],
+                                                 const int (&qslot)[256 * SP_QPT_DEV / NTH],
+                                                 const unsigned (&qdir)[256 * SP_QPT_DEV / NTH],
+                                                 const unsigned (&qdk)[256 * SP_QPT_DEV / NTH], float (&sl)[NJ],
                                                  float (&fl)[NJ])
 {
     constexpr int S = 256 * SP_QPT_DEV, QT = S / NTH, NW = NTH / 64;
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const float *F = s.F;
     for (int c = 0; c < t.SP_NCHUNK; c++) {
         const int mc0 = c * t.SP_CHUNK;
         const int nmc = min(t.SP_CHUNK, t.MLSTHG - mc0);
 #pragma unroll
         for (int i = 0; i < QT; i++) {
-            const int mcl = qslot[i] & 255, r = qslot[i] >> 8;
+            const int mcl = qslot[i] & 255;
             if (mcl >= nmc) continue;
             const int ql = tid + i * NTH, mcm1 = mc0 + mcl;
-            const float4 q = dia_terms(t, s, mcm1, r, ftemp_scale);
-            const float *rc = s.rnl + mcm1 * 25;
-            U[ql] = make_float2(q.x * -2.f, q.y * -2.f);
-            U[1 * S + ql] = make_float2(q.x * rc[20], q.z * rc[23]);
-            U[2 * S + ql] = make_float2(q.x * rc[19], q.z * rc[24]);
-            U[3 * S + ql] = make_float2(q.x * rc[17], q.z * rc[21]);
-            U[4 * S + ql] = make_float2(q.x * rc[18], q.z * rc[22]);
-            U[5 * S + ql] = make_float2(q.x * rc[8], q.w * rc[11]);
-            U[6 * S + ql] = make_float2(q.x * rc[7], q.w * rc[12]);
-            U[7 * S + ql] = make_float2(q.x * rc[5], q.w * rc[9]);
-            U[8 * S + ql] = make_float2(q.x * rc[6], q.w * rc[10]);
+            // byte offsets into F: frequency rows IC, IP, IP1, IM, IM1 of
+            // this MC (s.inl holds (INLCOEF - 1) * KL * 4) plus the
+            // quadruplet's directions K1, K11, K2, K21 (bytes of qdir) and K
+            // the MC's block of 9 float4 (snl_coef_block): the terms'
+            // coefficients, the row offsets, the update weights
+            const float4 *cb = reinterpret_cast<const float4 *>(s.rnl) + mcm1 * 9;
+            const float4 ca = cb[0], cc = cb[1], ce = cb[2];
+            const int4 ic4 = reinterpret_cast<const int4 *>(cb)[3];
+            const int ic[5] = {ic4.x, ic4.y, ic4.z, ic4.w, reinterpret_cast<const int4 *>(cb)[4].x};
+            const float w[17] = {ca.x, ca.y, ca.z, ca.w, cc.x, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
+                                 cc.y, cc.z, cc.w, ce.x};
+            const float af = ce.y;
+            const unsigned d = qdir[i];
+            const unsigned k1 = d & 255u, k11 = (d >> 8) & 255u, k2 = (d >> 16) & 255u, k21 = d >> 24;
+            const unsigned k = qdk[i];
+            const char *Fb = reinterpret_cast<const char *>(F);
+            auto f_at = [&](unsigned off) { return *reinterpret_cast<const float *>(Fb + off); };
+            const unsigned IC = ic[0], IP = ic[1], IP1 = ic[2], IM = ic[3], IM1 = ic[4];
+            // dia_terms with the coefficients in registers (same operations)
+            const float sap = w[1] * f_at(k1 + IP) + w[2] * f_at(k11 + IP) + w[3] * f_at(k1 + IP1) + w[4] * f_at(k11 + IP1);
+            const float sam =
+                w[13] * f_at(k2 + IM) + w[14] * f_at(k21 + IM) + w[15] * f_at(k2 + IM1) + w[16] * f_at(k21 + IM1);
+            const float fij = f_at(k + IC) * w[0];
+            const float ftemp = af * ftemp_scale;
+            float fad1 = fij * (sap + sam);
+            const float fad2 = fad1 - (2.f * sap) * sam;
+            fad1 = fad1 + fad2;
+            const float fcen = ftemp * fij;
+            const float qx = fad2 * fcen, qy = fad1 * ftemp, qz = ((fij - 2.f * sap) * t.DAL2) * fcen;
+            const float qw = ((fij - 2.f * sam) * t.DAL1) * fcen;
+            dia_products<S>(U + ql, cb + 5, qx, qy, qz, qw);
         }
         __syncthreads();
-        const int *rw = t.SP_RW + 2 * c * t.SP_NRUN;
+        // this wave's runs of the chunk: entry offset and pair count (uniform)
+        const int2 *rw = reinterpret_cast<const int2 *>(t.SP_RW) + (size_t)c * t.SP_NRUN;
+        int roff[NJ], rnp[NJ];
 #pragma unroll
         for (int j = 0; j < NJ; j++) {
             const int r = NW * j + wave;
-            if (r >= t.SP_NRUN) continue;
-            const int off = rw[2 * r], np = rw[2 * r + 1] >> 1;
-            const uint32_t *e = reinterpret_cast<const uint32_t *>(t.SP_E + off) + lane;
+            const int2 v = r < t.SP_NRUN ? rw[r] : make_int2(0, 0);
+            roff[j] = v.x;
+            rnp[j] = v.y >> 1;
+        }
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            const int np = rnp[j];
+            if (np == 0) continue;
+            const uint32_t *e = reinterpret_cast<const uint32_t *>(t.SP_E + roff[j]) + lane;
+            const char *Ub = reinterpret_cast<const char *>(U);
+            auto u_at = [&](uint32_t off) { return *reinterpret_cast<const float2 *>(Ub + off); };
             float a = sl[j], b = fl[j];
             int i = 0;
             for (; i + 4 <= np; i += 4) {
@@ -744,14 +819,14 @@ __device__ __forceinline__ void snonlin_products(const DevTables &t, const Smem 
                 for (int u = 0; u < 4; u++) v[u] = e[(i + u) * 64];
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
-                    const float2 u0 = U[v[u] & 0xffffu], u1 = U[v[u] >> 16];
+                    const float2 u0 = u_at(v[u] & 0xffffu), u1 = u_at(v[u] >> 16);
                     a = a + u0.x; b = b + u0.y;
                     a = a + u1.x; b = b + u1.y;
                 }
             }
             for (; i < np; i++) {
                 const uint32_t v = e[i * 64];
-                const float2 u0 = U[v & 0xffffu], u1 = U[v >> 16];
+                const float2 u0 = u_at(v & 0xffffu), u1 = u_at(v >> 16);
                 a = a + u0.x; b = b + u0.y;
                 a = a + u1.x; b = b + u1.y;
             }
@@ -1408,26 +1483,29 @@ implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x)
         s.SMIN = p; p += NB;
         if (SHALLOW) { s.SLBOT = p; p += NB; }
     }
+    s.rnl = p; p += 36 * MH;      // per MC a 16-byte aligned block (snl_coef_block)
     s.fr = p; p += ML; s.dfim = p; p += ML;
     s.th = p; p += KL; s.costh = p; p += KL; s.sinth = p; p += KL;
-    s.rnl = p; p += 25 * MH; s.af11 = p; p += MH;
-    s.inl = (int *)p; p += 5 * MH; s.kw = (int *)p; p += 8 * KL;
 
     for (int i = tid; i < ML; i += SNL_NT) { s.fr[i] = t.FR[i]; s.dfim[i] = t.DFIM[i]; }
     for (int i = tid; i < KL; i += SNL_NT) { s.th[i] = t.TH[i]; s.costh[i] = t.COSTH[i]; s.sinth[i] = t.SINTH[i]; }
-    for (int i = tid; i < 25 * MH; i += SNL_NT) s.rnl[i] = t.RNLCOEF[i];
-    for (int i = tid; i < 5 * MH; i += SNL_NT) s.inl[i] = t.INLCOEF[i];
-    for (int i = tid; i < MH; i += SNL_NT) s.af11[i] = t.AF11[i];
-    for (int i = tid; i < 2 * KL; i += SNL_NT) {
-        s.kw[i] = t.K1W[i]; s.kw[2 * KL + i] = t.K2W[i]; s.kw[4 * KL + i] = t.K11W[i]; s.kw[6 * KL + i] = t.K21W[i];
-    }
+    for (int i = tid; i < 36 * MH; i += SNL_NT) s.rnl[i] = snl_coef_block(t, i / 36, i % 36);
     if (tid == 0) U[S - 1] = make_float2(-0.f, -0.f);
     // this thread's quadruplet slots (MC offset, KH*KL + K); MC offset 255
     // marks a slot past the chunk
+    // and its directions as byte offsets: K1W, K11W, K2W, K21W (one byte
+    // each of qdir, KL <= 64) and K (qdk)
     int qslot[SNL_QT];
+    unsigned qdir[SNL_QT], qdk[SNL_QT];
     for (int i = 0; i < SNL_QT; i++) {
         const int ql = tid + i * SNL_NT, mcl = ql / (2 * KL);
-        qslot[i] = ql < S - 1 && mcl < t.SP_CHUNK ? mcl | (ql - mcl * 2 * KL) << 8 : 255;
+        const int r = ql - mcl * 2 * KL;
+        const bool in = ql < S - 1 && mcl < t.SP_CHUNK;
+        qslot[i] = in ? mcl | r << 8 : 255;
+        qdir[i] = in ? (unsigned)(4 * t.K1W[r]) | (unsigned)(4 * t.K11W[r]) << 8 | (unsigned)(4 * t.K2W[r]) << 16 |
+                           (unsigned)(4 * t.K21W[r]) << 24
+                     : 0u;
+        qdk[i] = in ? (unsigned)(4 * (r < KL ? r : r - KL)) : 0u;
     }
 
     const int nblk = gridDim.x, xcd = blockIdx.x % 8, lb = blockIdx.x / 8;
@@ -1477,7 +1555,7 @@ implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x)
         }
         __syncthreads();
         // SNONLIN, with the ISNONLIN = 0 shallow-water enhancement
-        snonlin_products<SNL_NT, SNL_NJ>(t, s, U, SHALLOW ? Ps[1] : 1.f, qslot, sl, fl);
+        snonlin_products<SNL_NT, SNL_NJ>(t, s, U, SHALLOW ? Ps[1] : 1.f, qslot, qdir, qdk, sl, fl);
         // SBOTTOM's factor (:1527-1589) and the limiter's (:652-690)
         if (tid < ML) {
             if (SHALLOW) {
@@ -1616,7 +1694,7 @@ size_t implsch_snl_lds_bytes(const DevTables &t, int lcflx)
     const int ML = t.ML, KL = t.KL, MH = t.MLSTHG, NB = KL * ML;
     size_t n = 2 * 9 * 256 * (size_t)SP_QPT_DEV + (size_t)NB;
     if (lcflx) n += (t.shallow ? 2 : 1) * (size_t)NB;
-    n += 2 * (size_t)ML + 3 * (size_t)KL + 26 * (size_t)MH + 5 * (size_t)MH + 8 * (size_t)KL;
+    n += 2 * (size_t)ML + 3 * (size_t)KL + 36 * (size_t)MH;
     return n * sizeof(float);
 }
 
