@@ -121,7 +121,7 @@ def test_products_gather_lists_replay_the_scatter_order(kl, ml):
     rw = np.asarray(p.table("SP_RW")).view(np.int32).reshape(-1, 2)
     e = np.asarray(p.table("SP_E")).view(np.int32)
     chunk, nchunk, nrun = int(p.scalar("SP_CHUNK")), int(p.scalar("SP_NCHUNK")), int(p.scalar("SP_NRUN"))
-    slots = 512                                   # update slots per type (two quadruplets per thread of 256)
+    slots, stride = 512, 516                      # quadruplet slots of a chunk, update slots per type
     assert 2 * kl * chunk < slots                 # the last slot of type 0 stays free for the padding
     pad = (slots - 1) * 8
     assert nrun == (nb + 63) // 64 and len(rw) == nchunk * nrun
@@ -140,7 +140,7 @@ def test_products_gather_lists_replay_the_scatter_order(kl, ml):
                         continue
                     assert not ended, "entry after the padding"
                     assert b < nb and v % 8 == 0
-                    ty, ql = divmod(v // 8, slots)
+                    ty, ql = divmod(v // 8, stride)
                     assert ty < 9 and ql < 2 * kl * chunk
                     got[b].append((c * chunk * 2 * kl + ql) | (ty << 16))
     for b in range(nb):

@@ -122,7 +122,8 @@ static void free_list(std::vector<void *> &v)
     } while (0)
 
 static_assert(SN_GROUP == SN_GROUP_DEV && SN_NONE == SN_NONE_DEV, "host and device SNONLIN list layouts differ");
-static_assert(SP_QPT == SP_QPT_DEV && SP_NONE == SP_NONE_DEV, "host and device SNONLIN products layouts differ");
+static_assert(SP_QPT == SP_QPT_DEV && SP_NONE == SP_NONE_DEV && SP_TSTRIDE == SP_TSTRIDE_DEV,
+              "host and device SNONLIN products layouts differ");
 
 static int upload_static(wam_ctx *ctx)
 {

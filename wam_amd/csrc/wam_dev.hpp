@@ -8,6 +8,7 @@ namespace wam {
 constexpr int SN_GROUP_DEV = 6;   // SNONLIN list entries a thread loads together
 constexpr unsigned SN_NONE_DEV = 0x01FF;   // padding entry: zero quadruplet slot 511, weight -2
 constexpr int SP_QPT_DEV = 2;              // products form: quadruplets per thread of a chunk
+constexpr int SP_TSTRIDE_DEV = 256 * SP_QPT_DEV + 4;           // products form: update slots per type
 constexpr unsigned SP_NONE_DEV = (256 * SP_QPT_DEV - 1) * 8;   // products form: byte offset of the (-0, -0) slot
 
 // Coefficient planes of the propagation weights, [plane][M][K][NSEA]:
@@ -45,7 +46,7 @@ struct DevTables {
     const uint16_t *SN_E;
     const int *SN_CH, *SN_WW;
     // the products form of the lists (entries the byte offsets of the
-    // update slots, (type*S + ql)*8; per run of 64 bins a block of entry
+    // update slots, (type*SP_TSTRIDE_DEV + ql)*8; per run of 64 bins a block of entry
     // pairs at the SP_RW offset)
     int SP_CHUNK, SP_NCHUNK, SP_NRUN;
     const uint16_t *SP_E;

@@ -701,7 +701,7 @@ __device__ __forceinline__ void wnfluxes(const DevTables &t, const Smem &s, floa
 // bins with DP), each product rounded as in the reference's update; then
 // every bin adds its updates from the chunk in the reference's order
 // (host-built lists, two entries per 32-bit word).  U[S - 1] = (-0, -0) pads.
-// The nine updates of one quadruplet, u[type * S] (see snonlin_products),
+// The nine updates of one quadruplet, u[type * S], S the type stride (see snonlin_products),
 // from the MC's update weights w[0..3] (snl_coef_block); the stores cannot
 // alias the weights, so their loads go first.
 template <int S>
@@ -791,7 +791,7 @@ __device__ __forceinline__ void snonlin_products(const DevTables &t, const Smem 
             const float fcen = ftemp * fij;
             const float qx = fad2 * fcen, qy = fad1 * ftemp, qz = ((fij - 2.f * sap) * t.DAL2) * fcen;
             const float qw = ((fij - 2.f * sam) * t.DAL1) * fcen;
-            dia_products<S>(U + ql, cb + 5, qx, qy, qz, qw);
+            dia_products<SP_TSTRIDE_DEV>(U + ql, cb + 5, qx, qy, qz, qw);
         }
         __syncthreads();
         // this wave's runs of the chunk: entry offset and pair count (uniform)
@@ -1477,7 +1477,7 @@ implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x)
     // LDS: the products U [9][S] float2, F, (SMIN, SL_BOT with LCFLX), tables
     Smem s{};
     float *p = lds_raw;
-    float2 *U = reinterpret_cast<float2 *>(p); p += 2 * 9 * S;
+    float2 *U = reinterpret_cast<float2 *>(p); p += 2 * 9 * SP_TSTRIDE_DEV;
     s.F = p; p += NB;
     if (a.lcflx) {
         s.SMIN = p; p += NB;
@@ -1692,7 +1692,7 @@ __global__ void __launch_bounds__(PW) implsch_post_kernel(DevTables t, ImplArgs 
 size_t implsch_snl_lds_bytes(const DevTables &t, int lcflx)
 {
     const int ML = t.ML, KL = t.KL, MH = t.MLSTHG, NB = KL * ML;
-    size_t n = 2 * 9 * 256 * (size_t)SP_QPT_DEV + (size_t)NB;
+    size_t n = 2 * 9 * (size_t)SP_TSTRIDE_DEV + (size_t)NB;
     if (lcflx) n += (t.shallow ? 2 : 1) * (size_t)NB;
     n += 2 * (size_t)ML + 3 * (size_t)KL + 36 * (size_t)MH;
     return n * sizeof(float);

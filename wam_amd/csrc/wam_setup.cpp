@@ -429,7 +429,7 @@ static int snonlin_lists(Tables &t, std::string &err)
                 for (int e : lists[64 * r + l]) {
                     const int q = e & 0xffff, ty = e >> 16, mc = q / (2 * KL) + 1;
                     if (mc < mc_lo || mc > mc_hi) continue;
-                    per[l].push_back((uint16_t)((ty * 256 * SP_QPT + (q - (mc_lo - 1) * 2 * KL)) * 8));
+                    per[l].push_back((uint16_t)((ty * SP_TSTRIDE + (q - (mc_lo - 1) * 2 * KL)) * 8));
                 }
                 w = std::max(w, per[l].size());
             }

@@ -19,6 +19,7 @@ constexpr int SN_GROUP = 6;    // SNONLIN list entries a thread loads together
 // centre weight -2, so it adds (+0)*(-2) = -0, which leaves every value as it is.
 constexpr uint16_t SN_NONE = 0x01FF;
 constexpr int SP_QPT = 2;               // products form: quadruplets per thread of a chunk
+constexpr int SP_TSTRIDE = 256 * SP_QPT + 4;    // products form: update slots per type (padded against LDS bank conflicts)
 constexpr uint16_t SP_NONE = (256 * SP_QPT - 1) * 8;   // products form: byte offset of the unused last slot of type 0
 
 struct Consts {           // wam_general_module.f90:41-85
@@ -81,10 +82,10 @@ struct Tables {
     // products form, for a gather in which thread t of a 256-thread
     // workgroup owns the bins t + 256 j.  Chunk c covers SP_CHUNK
     // frequencies of quadruplets (fewer than S = 256*SP_QPT, SP_QPT per
-    // thread); its updates sit at type*S + ql (type 0..8, ql the
-    // quadruplet's index within the chunk), float2 each, and slot S - 1
-    // holds a zero.  Entries are byte offsets of these slots, (type*S + ql)*8,
-    // and SP_NONE = (S - 1)*8.  For run r of 64 bins (64 r .. 64 r + 63) chunk c has W =
+    // thread); its updates sit at type*SP_TSTRIDE + ql (type 0..8, ql the
+    // quadruplet's index within the chunk), float2 each, and slot S - 1 of
+    // type 0 holds a zero.  Entries are byte offsets of these slots,
+    // (type*SP_TSTRIDE + ql)*8, and SP_NONE = (S - 1)*8.  For run r of 64 bins (64 r .. 64 r + 63) chunk c has W =
     // SP_RW[2(c*SP_NRUN + r) + 1] entries per bin (even, 0 when the run gets
     // nothing from the chunk), padded with SP_NONE; entries 2i and 2i+1 of
     // bin 64 r + l sit at SP_E[SP_RW[2(c*SP_NRUN + r)] + (i*64 + l)*2 + {0,1}].
