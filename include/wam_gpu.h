@@ -116,6 +116,49 @@ int wam_gpu_implsch_dev(wam_ctx *ctx, float *d_fl3, const float *d_u10, const fl
 /* Stream used by the context (a hipStream_t; NULL = the context's own). */
 int wam_gpu_set_stream(wam_ctx *ctx, void *stream);
 
+/* Integrated output parameters (COMPUTE_OUTPUT_PARAMETER,
+   wam_output_module.f90:481-601, with TOTAL_ENERGY, FEMEAN,
+   TM1_TM2_PERIODS, PEAK_PERIOD, MEAN_DIRECTION and CHARNOCK_PAR of
+   wam_interface_module.f90) for every sea point: out[p * NSEA + ij], p one
+   of the WAM_OUT_* indices below, which follow the reference's parameter
+   numbers 1-16 and 32.  orient = 1 gives directions as DEG * angle
+   ("coming from"), orient = 0 as MOD(DEG * angle + 180, 360) ("going to"),
+   the reference's orientation_of_directions.  USTAR is taken as given (the
+   reference evaluates AIRSEA first when every USTAR is zero).  Depth and
+   currents are the fields of wam_gpu_prepare_propagation. */
+enum {
+    WAM_OUT_U10 = 0,        /*  1 wind speed [m/s] */
+    WAM_OUT_UDIR,           /*  2 wind direction [deg] */
+    WAM_OUT_USTAR,          /*  3 friction velocity [m/s] */
+    WAM_OUT_CD,             /*  4 drag coefficient */
+    WAM_OUT_CHARNOCK,       /*  5 Charnock parameter */
+    WAM_OUT_DEPTH,          /*  6 water depth [m] (999 in deep water) */
+    WAM_OUT_CURRENT_SPEED,  /*  7 current speed [m/s] */
+    WAM_OUT_CURRENT_DIR,    /*  8 current direction [deg] */
+    WAM_OUT_HS,             /*  9 significant wave height [m] */
+    WAM_OUT_TP,             /* 10 peak period [s] */
+    WAM_OUT_TM,             /* 11 mean period [s] */
+    WAM_OUT_TM1,            /* 12 TM1 period [s] */
+    WAM_OUT_TM2,            /* 13 TM2 period [s] */
+    WAM_OUT_MDIR,           /* 14 mean wave direction [deg] */
+    WAM_OUT_SPREAD,         /* 15 directional spread [deg] */
+    WAM_OUT_TAUWN,          /* 16 normalised wave stress */
+    WAM_OUT_Z0,             /* 32 roughness length [m] */
+    WAM_NOUT
+};
+
+/* Device-resident form: HBM pointers, asynchronous on the context stream;
+   d_out holds WAM_NOUT * NSEA floats. */
+int wam_gpu_output_parameters_dev(wam_ctx *ctx, const float *d_fl3, const float *d_u10, const float *d_udir,
+                                  const float *d_ustar, const float *d_tauw, const float *d_z0, int32_t orient,
+                                  float *d_out);
+
+/* Host-memory form: FL3(NSEA,KL,ML) and the per-point fields in host
+   memory, out (WAM_NOUT * NSEA floats) filled on return. */
+int wam_gpu_output_parameters(wam_ctx *ctx, int32_t nsea, int32_t kl, int32_t ml, const float *fl3, const float *u10,
+                              const float *udir, const float *ustar, const float *tauw, const float *z0,
+                              int32_t orient, float *out);
+
 /* Per-kernel timing on the context stream.  No counterpart in the
    reference (its timing is host wall-clock around whole calls).  on = 1
    records a HIP event pair around every kernel launch of PROPAGS and IMPLSCH

@@ -64,6 +64,7 @@ def _lib(fp64, opcount=False):
         lib.orc_api_implsch.argtypes = ([ctypes.c_void_p] + [ctypes.c_void_p] * 9 +
                                         [ctypes.c_int] + [ctypes.c_void_p] * 3 + [ctypes.c_int])
         lib.orc_api_wavnu2.argtypes = [ctypes.c_void_p] * 4
+        lib.orc_api_output_parameters.argtypes = [ctypes.c_void_p] * 10 + [ctypes.c_int, ctypes.c_void_p]
         lib.orc_api_snonlin.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_double, ctypes.c_double]
         lib.orc_api_math.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long]
         lib.orc_api_aki.restype = ctypes.c_double
@@ -182,6 +183,16 @@ class Oracle:
         tot = dict(zip(names, (int(x) for x in counts)))
         tot["stages"] = [dict(zip(names, (int(x) for x in row))) for row in stages]
         return res, tot
+
+    def output_parameters(self, fl3, u10, udir, ustar, tauw, z0, depth, u=None, v=None, orient=1):
+        """COMPUTE_OUTPUT_PARAMETER's integrated parameters (orc_output.c):
+        [17][NSEA] in the order of the WAM_OUT_* indices of include/wam_gpu.h."""
+        n = self.nsea
+        a = lambda x: None if x is None else np.ascontiguousarray(x, dtype=self.dtype)
+        args = [a(x) for x in (fl3, u10, udir, ustar, tauw, z0, depth, u, v)]   # alive across the call
+        out = np.zeros((17, n), self.dtype)
+        self.lib.orc_api_output_parameters(self.ctx, *[_p(x) for x in args], int(orient), _p(out))
+        return out
 
     def wavnu2(self, depth):
         d = np.ascontiguousarray(depth, dtype=self.dtype)

@@ -88,6 +88,15 @@ void orc_api_implsch(const orc_ctx *c, real *fl3, const real *u10, const real *u
                 phiaw, out_flux, mij, nthreads);
 }
 void orc_api_wavnu2(const orc_ctx *c, const real *depth, real *wn, real *cgg) { orc_wavnu2_all(c, depth, wn, cgg); }
+void orc_output_parameters(const orc_ctx *c, const real *fl3, const real *u10, const real *udir, const real *ustar,
+                           const real *tauw, const real *z0, const real *depth, const real *u, const real *v,
+                           int orient, real *out);
+void orc_api_output_parameters(const orc_ctx *c, const real *fl3, const real *u10, const real *udir,
+                               const real *ustar, const real *tauw, const real *z0, const real *depth, const real *u,
+                               const real *v, int orient, real *out)
+{
+    orc_output_parameters(c, fl3, u10, udir, ustar, tauw, z0, depth, u, v, orient, out);
+}
 double orc_api_aki(const orc_ctx *c, double om, double beta) { return orc_aki(c, (real)om, (real)beta); }
 
 void orc_snonlin_point(const orc_ctx *c, real *F, real *SL, real *FL, real depth, real akmean);
@@ -98,7 +107,7 @@ void orc_api_snonlin(const orc_ctx *c, real *F, real *SL, real *FL, double depth
 
 /* The shared single-precision elementary functions (wam_math.h), for the
    accuracy tests: fn 0 exp, 1 log, 2 log10, 3 tanh, 4 cosh, 5 sinh, 6 cos,
-   7 sin, 8 pow(x, y), 9 nint, 10 ipow(x, (int)y). */
+   7 sin, 8 pow(x, y), 9 nint, 10 ipow(x, (int)y), 11 atan2(x, y). */
 #include "wam_math.h"
 void orc_api_math(int fn, const float *x, const float *y, float *out, long n)
 {
@@ -116,6 +125,7 @@ void orc_api_math(int fn, const float *x, const float *y, float *out, long n)
         case 8: r = wam_powf(a, b); break;
         case 9: r = (float)wam_nint(a); break;
         case 10: r = wam_ipowf(a, (int)b); break;
+        case 11: r = wam_atan2f(a, b); break;
         }
         out[i] = r;
     }

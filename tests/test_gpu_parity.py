@@ -103,6 +103,33 @@ def test_implsch_one_step(case):
         assert bits_equal(rg[k], ro[k]), (k, maxrel(rg[k], ro[k]))
 
 
+def test_output_parameters_bit_exact(case):
+    """The integrated output parameters on the GPU (host and device-resident
+    forms) against the oracle, both direction conventions."""
+    o, g, f = case["o"], case["g"], case["f"]
+    n = g.nsea
+    rng = np.random.default_rng(3)
+    fl3 = o.propags(case["fl3"])
+    ust = (0.05 + rng.random(n)).astype(np.float32)
+    tauw = (0.5 * ust * ust * rng.random(n)).astype(np.float32)
+    z0 = (1e-4 + 1e-3 * rng.random(n)).astype(np.float32)
+    for orient in (1, 0):
+        ro = o.output_parameters(fl3, f["u10"], f["udir"], ust, tauw, z0, f["depth"], f["u"], f["v"], orient)
+        rg = g.output_parameters(fl3, f["u10"], f["udir"], ust, tauw, z0, orient)
+        for i, name in enumerate(g.OUT_NAMES):
+            assert bits_equal(rg[i], ro[i]), (orient, name, maxrel(rg[i], ro[i]))
+    torch = pytest.importorskip("torch")
+    dev = torch.device("cuda:0")
+    t = lambda x: torch.from_numpy(np.ascontiguousarray(x, np.float32)).to(dev)
+    out = torch.empty(g.NOUT * n, device=dev)
+    g.set_stream(torch.cuda.current_stream().cuda_stream)
+    g.output_parameters_dev(t(fl3), t(f["u10"]), t(f["udir"]), t(ust), t(tauw), t(z0), out, orient=1)
+    torch.cuda.synchronize()
+    g.set_stream(0)
+    ro = o.output_parameters(fl3, f["u10"], f["udir"], ust, tauw, z0, f["depth"], f["u"], f["v"], 1)
+    assert bits_equal(out.cpu().numpy().reshape(g.NOUT, n), ro)
+
+
 def test_multi_step_device_resident(case):
     """Six propagation + source steps on torch tensors resident in HBM,
     through wam_gpu_propags_dev / wam_gpu_implsch_dev, against the oracle."""

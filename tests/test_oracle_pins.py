@@ -264,7 +264,8 @@ def test_propags_constant_spectrum_and_land_slot():
 
 
 def test_shared_math_accuracy():
-    """wam_math.h single-precision functions vs float64 numpy: <= 2.5 ulp."""
+    """wam_math.h single-precision functions vs float64 numpy: <= 2.5 ulp
+    (atan2 <= 3 ulp)."""
     rng = np.random.default_rng(5)
 
     def ulp_err(got, ref):
@@ -289,6 +290,13 @@ def test_shared_math_accuracy():
     assert ulp_err(shared_math(6, xs)[keep], c[keep]) <= 2.0
     keep = np.abs(s) > 1e-3
     assert ulp_err(shared_math(7, xs)[keep], s[keep]) <= 2.0
+    ya = (rng.normal(size=200000) * 10.0 ** rng.uniform(-3, 3, 200000)).astype(np.float32)
+    xa = (rng.normal(size=200000) * 10.0 ** rng.uniform(-3, 3, 200000)).astype(np.float32)
+    a = np.arctan2(ya.astype(np.float64), xa.astype(np.float64))
+    keep = np.abs(a) > 1e-6
+    assert ulp_err(shared_math(11, ya, xa)[keep], a[keep]) <= 3.0
+    axes = shared_math(11, np.array([0, 1, -1, 0, 0], np.float32), np.array([1, 0, 0, -1, 0], np.float32))
+    assert list(axes) == [0.0, np.float32(np.pi / 2), np.float32(-np.pi / 2), np.float32(np.pi), 0.0]
     # Fortran NINT: halves away from zero
     v = np.array([2.5, -2.5, 0.49999997, -0.5, 1.5, 3.4999998], np.float32)
     assert list(shared_math(9, v)) == [3, -3, 0, -1, 2, 3]

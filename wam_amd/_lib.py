@@ -39,6 +39,8 @@ SIGNATURES = {
     "wam_gpu_implsch_dev": (_I, [_P] + [_P] * 8 + [_I, _P, _P, _P]),
     "wam_gpu_set_propagation_weights": (_I, [_P, _I]),
     "wam_gpu_set_stream": (_I, [_P, _P]),
+    "wam_gpu_output_parameters_dev": (_I, [_P] * 7 + [_I, _P]),
+    "wam_gpu_output_parameters": (_I, [_P, _I, _I, _I] + [_P] * 6 + [_I, _P]),
     "wam_gpu_set_timing": (_I, [_P, _I]),
     "wam_gpu_get_timing": (_I, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                 ctypes.POINTER(ctypes.c_int64)]),

@@ -30,6 +30,9 @@ hipError_t launch_implsch(const DevTables &t, const ImplArgs &a, int ncu, hipStr
 hipError_t launch_implsch_split(const DevTables &t, const ImplArgs &a, const ImplScratch &x, int ncu, hipStream_t st,
                                 const hipEvent_t *ev);
 bool implsch_split_supported(const DevTables &t);
+hipError_t launch_output_params(const DevTables &t, const float *fl3, const float *u10, const float *udir,
+                                const float *ustar, const float *tauw, const float *z0, int orient, float *out,
+                                hipStream_t st);
 }  // namespace wam
 
 using namespace wam;
@@ -138,8 +141,9 @@ static int upload_static(wam_ctx *ctx)
     d.XKAPPA = k.XKAPPA; d.XNLEV = k.XNLEV; d.BETAMAX = k.BETAMAX; d.ZALP = k.ZALP; d.ALPHA = k.ALPHA;
     d.CDFAC = k.CDFAC; d.EMIN = k.EMIN; d.EPS1 = k.EPS1; d.CO = k.CO;
     d.DELTH = t.DELTH; d.INV_LOG_CO = t.INV_LOG_CO; d.MO_TAIL = t.MO_TAIL; d.MM1_TAIL = t.MM1_TAIL;
-    d.MP1_TAIL = t.MP1_TAIL; d.DELU = t.DELU; d.DAL1 = t.DAL1; d.DAL2 = t.DAL2; d.X0TAUHF = t.X0TAUHF;
+    d.MP1_TAIL = t.MP1_TAIL; d.MP2_TAIL = t.MP2_TAIL; d.DELU = t.DELU; d.DAL1 = t.DAL1; d.DAL2 = t.DAL2; d.X0TAUHF = t.X0TAUHF;
     d.DELT = (float)t.cfg.idelt;
+    d.DEG = k.DEG;
     // constant expressions the Fortran compiler folds: one correct rounding each
     d.ZPI4 = (float)std::pow((double)k.ZPI, 4.0);
     const float g4 = (float)std::pow((double)k.G, 4.0);
@@ -154,7 +158,7 @@ static int upload_static(wam_ctx *ctx)
 
     auto &o = ctx->dev_allocs;
     UP(o, t.FR, FR); UP(o, t.DF, DF); UP(o, t.DFIM, DFIM); UP(o, t.DFIMOFR, DFIMOFR);
-    UP(o, t.DFIM_FR, DFIM_FR); UP(o, t.FR5, FR5); UP(o, t.FRM5, FRM5); UP(o, t.RHOWG_DFIM, RHOWG_DFIM);
+    UP(o, t.DFIM_FR, DFIM_FR); UP(o, t.DFIM_FR2, DFIM_FR2); UP(o, t.FR5, FR5); UP(o, t.FRM5, FRM5); UP(o, t.RHOWG_DFIM, RHOWG_DFIM);
     UP(o, t.GOM, GOM); UP(o, t.TH, TH); UP(o, t.COSTH, COSTH); UP(o, t.SINTH, SINTH);
     UP(o, t.FLMINFR, FLMINFR); UP(o, t.TFAK, TFAK); UP(o, t.TCGOND, TCGOND); UP(o, t.TSIHKD, TSIHKD);
     UP(o, t.SIG10, SIG10); UP(o, t.DSII10, DSII10);
@@ -478,6 +482,45 @@ int wam_gpu_implsch(wam_ctx *ctx, int32_t nsea, int32_t kl, int32_t ml, float *f
     return WAM_OK;
 }
 
+int wam_gpu_output_parameters_dev(wam_ctx *ctx, const float *d_fl3, const float *d_u10, const float *d_udir,
+                                  const float *d_ustar, const float *d_tauw, const float *d_z0, int32_t orient,
+                                  float *d_out)
+{
+    if (!ctx || !d_fl3 || !d_u10 || !d_udir || !d_ustar || !d_tauw || !d_z0 || !d_out) return WAM_E_ARG;
+    if (!ctx->prepared) { ctx->err = "PREPARE_PROPAGATION not done"; return WAM_E_STATE; }
+    if (ctx->tab.KL > 64 || ctx->tab.ML > 64) { ctx->err = "output parameters need KL, ML <= 64"; return WAM_E_ARG; }
+    HIPCHK(launch_output_params(ctx->dt, d_fl3, d_u10, d_udir, d_ustar, d_tauw, d_z0, orient ? 1 : 0, d_out,
+                                ctx->stream));
+    return WAM_OK;
+}
+
+int wam_gpu_output_parameters(wam_ctx *ctx, int32_t nsea, int32_t kl, int32_t ml, const float *fl3, const float *u10,
+                              const float *udir, const float *ustar, const float *tauw, const float *z0,
+                              int32_t orient, float *out)
+{
+    if (!ctx || !fl3 || !u10 || !udir || !ustar || !tauw || !z0 || !out) return WAM_E_ARG;
+    const Tables &t = ctx->tab;
+    if (nsea != t.NSEA || kl != t.KL || ml != t.ML) { ctx->err = "output parameters: size mismatch"; return WAM_E_ARG; }
+    const size_t nb = (size_t)t.KL * t.ML * nsea, ns = nsea;
+    // FL3, five per-point fields, WAM_NOUT output rows
+    int rc = ensure_stage(ctx, nb + (5 + WAM_NOUT) * ns);
+    if (rc) return rc;
+    float *d = ctx->stage;
+    float *dfl = d, *du = d + nb, *dd = du + ns, *dus = dd + ns, *dt = dus + ns, *dz = dt + ns, *dout = dz + ns;
+    hipStream_t st = ctx->stream;
+    HIPCHK(hipMemcpyAsync(dfl, fl3, nb * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(du, u10, ns * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dd, udir, ns * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dus, ustar, ns * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dt, tauw, ns * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dz, z0, ns * 4, hipMemcpyHostToDevice, st));
+    rc = wam_gpu_output_parameters_dev(ctx, dfl, du, dd, dus, dt, dz, orient, dout);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(out, dout, WAM_NOUT * ns * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return WAM_OK;
+}
+
 int wam_gpu_set_propagation_weights(wam_ctx *ctx, int32_t stored)
 {
     if (!ctx) return WAM_E_ARG;
@@ -542,6 +585,7 @@ int64_t wam_gpu_get_table(wam_ctx *ctx, const char *name, void *out, int64_t n)
     if (s == "DFIM") return put(t.DFIM);
     if (s == "DFIMOFR") return put(t.DFIMOFR);
     if (s == "DFIM_FR") return put(t.DFIM_FR);
+    if (s == "DFIM_FR2") return put(t.DFIM_FR2);
     if (s == "FR5") return put(t.FR5);
     if (s == "FRM5") return put(t.FRM5);
     if (s == "RHOWG_DFIM") return put(t.RHOWG_DFIM);

@@ -24,10 +24,11 @@ struct DevTables {
     int KL, ML, NSEA, NDEPTH, JUMAX, MFRSTLW, MLSTHG, NQ, NCMAX, NK10;
     int iphys, shallow, refr_d, refr_c;
     float G, ZPI, ROAIR, RNUAIRM, ROWATER, XKAPPA, XNLEV, BETAMAX, ZALP, ALPHA, CDFAC, EMIN, EPS1, CO;
-    float DELTH, INV_LOG_CO, MO_TAIL, MM1_TAIL, MP1_TAIL, DELU, DAL1, DAL2, X0TAUHF, DELT;
+    float DELTH, INV_LOG_CO, MO_TAIL, MM1_TAIL, MP1_TAIL, MP2_TAIL, DELU, DAL1, DAL2, X0TAUHF, DELT;
+    float DEG;                            // 180/PI
     float ZPI4, CONSD, CONSS, C5G;        // folded constants of STRESSO, SDISSIP and the limiter
     // per frequency / direction
-    const float *FR, *DF, *DFIM, *DFIMOFR, *DFIM_FR, *FR5, *FRM5, *RHOWG_DFIM, *GOM, *TH, *COSTH, *SINTH;
+    const float *FR, *DF, *DFIM, *DFIMOFR, *DFIM_FR, *DFIM_FR2, *FR5, *FRM5, *RHOWG_DFIM, *GOM, *TH, *COSTH, *SINTH;
     const float *FLMINFR;                 // [ML][JUMAX]
     const float *TFAK, *TCGOND, *TSIHKD;  // [ML][NDEPTH]
     const float *SIG10, *DSII10;          // [NK10]

@@ -13,7 +13,8 @@
  *
  * Algorithms: Cody-Waite range reduction followed by the minimax polynomials
  * published with the Cephes library (S. L. Moshier, "Methods and Programs
- * for Mathematical Functions", 1989): expf, logf, sinf/cosf, tanhf, sinhf.
+ * for Mathematical Functions", 1989): expf, logf, sinf/cosf, tanhf, sinhf,
+ * atanf (and atan2f from it).
  * Accuracy (checked against double-precision libm in the CPU test-suite):
  * <= 2 ulp over the argument ranges WAM uses.
  */
@@ -254,6 +255,41 @@ WAM_HD float wam_powf(float x, float y)
 {
     if (x == 0.0f) return y > 0.0f ? 0.0f : (y == 0.0f ? 1.0f : wam_i2f(0x7f800000));
     return wam_expf(y * wam_logf(x));
+}
+
+/* atan(x): reduction to |x| <= tan(pi/8) by atan(x) = pi/2 - atan(1/x) and
+   pi/4 + atan((x-1)/(x+1)), then Cephes' odd minimax polynomial. */
+WAM_HD float wam_atanf(float x)
+{
+    const float ax = wam_fabsf(x);
+    float y = 0.0f, r = ax;
+    if (ax > 2.414213562373095f) {           /* tan(3 pi/8) */
+        y = 1.5707963267948966f;
+        r = -(1.0f / ax);
+    } else if (ax > 0.4142135623730950f) {  /* tan(pi/8) */
+        y = 0.7853981633974483f;
+        r = (ax - 1.0f) / (ax + 1.0f);
+    }
+    const float z = r * r;
+    y = y + ((((8.05374449538e-2f * z - 1.38776856032e-1f) * z + 1.99777106478e-1f) * z - 3.33329491539e-1f) * z *
+                 r +
+             r);
+    return x < 0.0f ? -y : y;
+}
+
+/* atan2(y, x) in (-pi, pi], by quadrant from wam_atanf(y/x). */
+WAM_HD float wam_atan2f(float y, float x)
+{
+    const float pi = 3.14159265358979323846f;
+    if (x == 0.0f) {
+        if (y > 0.0f) return 1.5707963267948966f;
+        if (y < 0.0f) return -1.5707963267948966f;
+        return 0.0f;
+    }
+    if (y == 0.0f) return x > 0.0f ? 0.0f : pi;
+    const float w = wam_atanf(y / x);
+    if (x > 0.0f) return w;
+    return y < 0.0f ? w - pi : w + pi;
 }
 
 #endif

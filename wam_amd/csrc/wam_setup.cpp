@@ -53,7 +53,7 @@ static void fre_dir(Tables &t)
     const Consts &c = t.k;
     const float CO = c.CO;
     auto z = [&](std::vector<float> &v, int n) { v.assign(n, 0.f); };
-    z(t.FR, ML); z(t.DF, ML); z(t.DFIM, ML); z(t.DFIMOFR, ML); z(t.DFIM_FR, ML); z(t.FR5, ML);
+    z(t.FR, ML); z(t.DF, ML); z(t.DFIM, ML); z(t.DFIMOFR, ML); z(t.DFIM_FR, ML); z(t.DFIM_FR2, ML); z(t.FR5, ML);
     z(t.FRM5, ML); z(t.RHOWG_DFIM, ML); z(t.GOM, ML); z(t.TH, KL); z(t.COSTH, KL); z(t.SINTH, KL);
     int n = 1 - t.cfg.iref;
     float cof = n >= 0 ? wam_ipowf(CO, n) : 1.f / wam_ipowf(CO, -n);
@@ -82,6 +82,7 @@ static void fre_dir(Tables &t)
         t.DFIM[m] = t.DF[m] * t.DELTH;
         t.DFIMOFR[m] = t.DFIM[m] / t.FR[m];
         t.DFIM_FR[m] = dffr * t.DELTH;
+        t.DFIM_FR2[m] = (dffr * t.FR[m]) * t.DELTH;
         t.RHOWG_DFIM[m] = (((c.ROWATER * c.G) * t.DELTH) * lco) * t.FR[m];
     }
     t.RHOWG_DFIM[0] = 0.5f * t.RHOWG_DFIM[0];

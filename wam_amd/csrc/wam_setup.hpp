@@ -33,7 +33,7 @@ struct Tables {
     Consts k{};
     // frequency / direction grid (MAKE_FRE_DIR)
     float INV_LOG_CO = 0, DELTH = 0, DELTR = 0, MO_TAIL = 0, MM1_TAIL = 0, MP1_TAIL = 0, MP2_TAIL = 0;
-    std::vector<float> FR, DF, DFIM, DFIMOFR, DFIM_FR, FR5, FRM5, RHOWG_DFIM, GOM, TH, COSTH, SINTH;
+    std::vector<float> FR, DF, DFIM, DFIMOFR, DFIM_FR, DFIM_FR2, FR5, FRM5, RHOWG_DFIM, GOM, TH, COSTH, SINTH;
     std::vector<int> KPM_m, KPM_p, MPM_m, MPM_p, JXO1, JXO2, JYO1, JYO2;   // 1-based values
     // MIN_ENERGY / MAKE_SHALLOW_TABLES
     int JUMAX = 300;

@@ -97,6 +97,26 @@ class WamGpu:
     def scalar(self, name):
         return self.lib.wam_gpu_get_scalar(self.ctx, name.encode())
 
+    NOUT = 17
+    OUT_NAMES = ("u10", "udir", "ustar", "cd", "charnock", "depth", "current_speed", "current_dir", "hs", "tp", "tm",
+                 "tm1", "tm2", "mdir", "spread", "tauwn", "z0")
+
+    def output_parameters(self, fl3, u10, udir, ustar, tauw, z0, orient=1):
+        """Integrated output parameters (host arrays): [NOUT][NSEA] in the
+        order of OUT_NAMES (WAM_OUT_* of include/wam_gpu.h)."""
+        n = self.nsea
+        a = [np.ascontiguousarray(x, dtype=np.float32) for x in (fl3, u10, udir, ustar, tauw, z0)]
+        out = np.zeros((self.NOUT, n), np.float32)
+        self._check(self.lib.wam_gpu_output_parameters(self.ctx, n, self.cfg["kl"], self.cfg["ml"],
+                                                       *[_ptr(x) for x in a], int(orient), _ptr(out)))
+        return out
+
+    def output_parameters_dev(self, fl3, u10, udir, ustar, tauw, z0, out, orient=1):
+        """Device-resident form on torch CUDA tensors; out holds NOUT * NSEA floats."""
+        p = lambda x: ctypes.c_void_p(x.data_ptr())
+        self._check(self.lib.wam_gpu_output_parameters_dev(self.ctx, p(fl3), p(u10), p(udir), p(ustar), p(tauw), p(z0),
+                                                           int(orient), p(out)))
+
     def set_timing(self, on):
         """Records a HIP event pair around every PROPAGS / IMPLSCH kernel
         launch from now on (on=True; earlier records are dropped) or stops."""
