@@ -51,6 +51,12 @@ INTERFACE
       TYPE(C_PTR), VALUE :: ctx, fl3, u10, udir, tauw, ustar, z0, roairn, wstar, depth, indep, phiaw, fluxes
       INTEGER(C_INT32_T), VALUE :: nsea, kl, ml, lcflx
    END FUNCTION
+   INTEGER(C_INT) FUNCTION c_wam_gpu_output_parameters(ctx, nsea, kl, ml, fl3, u10, udir, ustar, tauw, z0, &
+&     orient, out) BIND(C, NAME='wam_gpu_output_parameters')
+      IMPORT :: C_INT, C_INT32_T, C_PTR
+      TYPE(C_PTR), VALUE :: ctx, fl3, u10, udir, ustar, tauw, z0, out
+      INTEGER(C_INT32_T), VALUE :: nsea, kl, ml, orient
+   END FUNCTION
    FUNCTION c_wam_gpu_last_error(ctx) BIND(C, NAME='wam_gpu_last_error')
       IMPORT :: C_PTR
       TYPE(C_PTR), VALUE :: ctx
@@ -75,7 +81,8 @@ END INTERFACE
 TYPE(C_PTR), SAVE :: CTX = C_NULL_PTR
 INTEGER, SAVE, PUBLIC :: NSEA_GPU = 0
 
-PUBLIC :: WAM_GPU_SETUP, WAM_GPU_PREPARE_PROPAGATION, PROPAGS_GPU, IMPLSCH_GPU, WAM_GPU_END
+PUBLIC :: WAM_GPU_SETUP, WAM_GPU_PREPARE_PROPAGATION, PROPAGS_GPU, IMPLSCH_GPU, OUTPUT_PARAMETERS_GPU, &
+&         WAM_GPU_END
 
 CONTAINS
 
@@ -130,6 +137,23 @@ SUBROUTINE IMPLSCH_GPU (FL3, U10, UDIR, TAUW, USTAR, Z0, ROAIRN, WSTAR, DEPTH, I
 &             C_LOC(Z0), C_LOC(ROAIRN), C_LOC(WSTAR), C_LOC(DEPTH), C_LOC(INDEP), 0_C_INT32_T, C_NULL_PTR, &
 &             C_NULL_PTR), 'IMPLSCH_GPU')
 END SUBROUTINE IMPLSCH_GPU
+
+!  Integrated parameters of COMPUTE_OUTPUT_PARAMETER (wam_output_module.f90:481-601):
+!  BOUT(IJ,P) for the 17 parameters 1-16 and 32, in that order.  ORIENT = 1 gives
+!  directions "coming from", 0 "going to" (orientation_of_directions).
+SUBROUTINE OUTPUT_PARAMETERS_GPU (FL3, U10, UDIR, USTAR, TAUW, Z0, ORIENT, BOUT)
+   REAL(C_FLOAT), INTENT(IN),  TARGET, CONTIGUOUS :: FL3(:,:,:)
+   REAL(C_FLOAT), INTENT(IN),  TARGET, CONTIGUOUS :: U10(:), UDIR(:), USTAR(:), TAUW(:), Z0(:)
+   INTEGER,       INTENT(IN)                      :: ORIENT
+   REAL(C_FLOAT), INTENT(OUT), TARGET, CONTIGUOUS :: BOUT(:,:)
+   IF (SIZE(BOUT,1) .NE. SIZE(FL3,1) .OR. SIZE(BOUT,2) .LT. 17) THEN
+      WRITE (*,*) ' *** WAM GPU ERROR in OUTPUT_PARAMETERS_GPU: BOUT must be (NSEA, 17)'
+      ERROR STOP 1
+   END IF
+   CALL CHECK(c_wam_gpu_output_parameters(CTX, INT(SIZE(FL3,1), C_INT32_T), INT(SIZE(FL3,2), C_INT32_T), &
+&             INT(SIZE(FL3,3), C_INT32_T), C_LOC(FL3), C_LOC(U10), C_LOC(UDIR), C_LOC(USTAR), C_LOC(TAUW), &
+&             C_LOC(Z0), INT(ORIENT, C_INT32_T), C_LOC(BOUT)), 'OUTPUT_PARAMETERS_GPU')
+END SUBROUTINE OUTPUT_PARAMETERS_GPU
 
 SUBROUTINE WAM_GPU_END
    IF (C_ASSOCIATED(CTX)) CALL c_wam_gpu_finalize(CTX)

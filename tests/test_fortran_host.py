@@ -1,7 +1,8 @@
 """The Fortran host (fortran/): ISO_C_BINDING shims with the reference's
 IMPLSCH/PROPAGS argument lists driving the HIP library, as WAMODEL does
 (wamodel.f90:286-360).  The CPU test checks that flang builds and links it;
-the GPU test runs a few steps and compares with the oracle bit-for-bit."""
+the GPU test runs a few steps, then the integrated output parameters, and
+compares both with the oracle bit-for-bit."""
 import os
 import subprocess
 
@@ -74,3 +75,9 @@ def test_fortran_host_matches_oracle(tmp_path, which):
         fo, tw, z0 = r["fl3"], r["tauw"], r["z0"]
     assert np.array_equal(out.view(np.int32), fo.view(np.int32))
     assert np.array_equal(ust.view(np.int32), r["ustar"].view(np.int32))
+    # BOUT(NSEA,17) of OUTPUT_PARAMETERS_GPU (directions "coming from")
+    bout = np.fromfile(os.path.join(d, "bout.f32"), np.float32).reshape(17, n)
+    cur = cfg["refr_c"] != 0
+    ro = o.output_parameters(fo, f["u10"], f["udir"], r["ustar"], tw, z0, f["depth"],
+                             f["u"] if cur else None, f["v"] if cur else None, orient=1)
+    assert np.array_equal(bout.view(np.int32), ro.view(np.int32))
