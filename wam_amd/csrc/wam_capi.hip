@@ -125,7 +125,7 @@ static void free_list(std::vector<void *> &v)
     } while (0)
 
 static_assert(SN_GROUP == SN_GROUP_DEV && SN_NONE == SN_NONE_DEV, "host and device SNONLIN list layouts differ");
-static_assert(SP_QPT == SP_QPT_DEV && SP_NONE == SP_NONE_DEV && SP_TSTRIDE == SP_TSTRIDE_DEV,
+static_assert(SP_QPT == SP_QPT_DEV && SP_NONE == SP_NONE_DEV && SP_TSTRIDE == SP_TSTRIDE_DEV && SP_GROUP == SP_GROUP_DEV,
               "host and device SNONLIN products layouts differ");
 
 static int upload_static(wam_ctx *ctx)
@@ -172,7 +172,7 @@ static int upload_static(wam_ctx *ctx)
     std::vector<int> k1 = zero_based(t.K1W), k2 = zero_based(t.K2W), k11 = zero_based(t.K11W), k21 = zero_based(t.K21W);
     UP(o, k1, K1W); UP(o, k2, K2W); UP(o, k11, K11W); UP(o, k21, K21W);
     UP(o, t.SN_E, SN_E); UP(o, t.SN_CH, SN_CH); UP(o, t.SN_WW, SN_WW);
-    d.SP_CHUNK = t.SP_CHUNK; d.SP_NCHUNK = t.SP_NCHUNK; d.SP_NRUN = t.SP_NRUN;
+    d.SP_CHUNK = t.SP_CHUNK; d.SP_NCHUNK = t.SP_NCHUNK; d.SP_NRUN = t.SP_NRUN; d.SP_PAD = t.SP_PAD;
     UP(o, t.SP_E, SP_E); UP(o, t.SP_RW, SP_RW);
     std::vector<int> ks = zero_based(t.KLAT_S), kn = zero_based(t.KLAT_N), kw = zero_based(t.KLON_W), ke = zero_based(t.KLON_E);
     UP(o, ks, KLAT_S); UP(o, kn, KLAT_N); UP(o, kw, KLON_W); UP(o, ke, KLON_E);
@@ -694,6 +694,7 @@ double wam_gpu_get_scalar(wam_ctx *ctx, const char *name)
     if (s == "SP_CHUNK") return t.SP_CHUNK;
     if (s == "SP_NCHUNK") return t.SP_NCHUNK;
     if (s == "SP_NRUN") return t.SP_NRUN;
+    if (s == "SP_PAD") return t.SP_PAD;
     if (s == "NK10") return t.NK10;
     if (s == "ALPHA") return t.k.ALPHA;
     if (s == "NCOEF") return ctx->ncoef;

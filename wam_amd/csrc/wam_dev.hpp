@@ -10,6 +10,7 @@ constexpr unsigned SN_NONE_DEV = 0x01FF;   // padding entry: zero quadruplet slo
 constexpr int SP_QPT_DEV = 2;              // products form: quadruplets per thread of a chunk
 constexpr int SP_TSTRIDE_DEV = 256 * SP_QPT_DEV + 4;           // products form: update slots per type
 constexpr unsigned SP_NONE_DEV = (256 * SP_QPT_DEV - 1) * 8;   // products form: byte offset of the (-0, -0) slot
+constexpr int SP_GROUP_DEV = 4;            // products form: entry pairs per gather step (lists are whole groups)
 
 // Coefficient planes of the propagation weights, [plane][M][K][NSEA]:
 // COEF_SUM, COEF_LAT(.,1,1), COEF_LAT(.,2,1), COEF_LON(.,1), COEF_LON(.,2),
@@ -48,8 +49,9 @@ struct DevTables {
     const int *SN_CH, *SN_WW;
     // the products form of the lists (entries the byte offsets of the
     // update slots, (type*SP_TSTRIDE_DEV + ql)*8; per run of 64 bins a block of entry
-    // pairs at the SP_RW offset)
-    int SP_CHUNK, SP_NCHUNK, SP_NRUN;
+    // pairs at the SP_RW offset, whole groups of SP_GROUP_DEV pairs; runs
+    // with no entries point at the padding group SP_PAD)
+    int SP_CHUNK, SP_NCHUNK, SP_NRUN, SP_PAD;
     const uint16_t *SP_E;
     const int *SP_RW;
     // per sea point (PREPARE_PROPAGATION), 0-based neighbour indices, -1 = land

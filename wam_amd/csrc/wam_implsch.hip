@@ -39,6 +39,7 @@ struct Smem {
     // point-invariant tables
     float *fr, *dfim, *dfimofr, *dfim_fr, *rhowg, *fr5, *frm5, *th, *costh, *sinth, *rnl, *af11, *sig10, *dsii10;
     int *inl, *kw;                                       // kw: K1W, K2W, K11W, K21W, each [2][KL], 0-based
+    int *sprw;                                           // SP_RW (products-form run table)
     // per point
     float *F, *SL, *FL, *SPOS, *SMIN, *SLBOT, *SCR;
     unsigned char *LLWS;
@@ -701,23 +702,21 @@ __device__ __forceinline__ void wnfluxes(const DevTables &t, const Smem &s, floa
 // bins with DP), each product rounded as in the reference's update; then
 // every bin adds its updates from the chunk in the reference's order
 // (host-built lists, two entries per 32-bit word).  U[S - 1] = (-0, -0) pads.
-// The nine updates of one quadruplet, u[type * S], S the type stride (see snonlin_products),
-// from the MC's update weights w[0..3] (snl_coef_block); the stores cannot
-// alias the weights, so their loads go first.
+// The nine updates of one quadruplet, u[type * S], S the type stride (see
+// snonlin_products), from the MC's update weights wa..wd (snl_coef_block).
 template <int S>
-__device__ __forceinline__ void dia_products(float2 *__restrict__ u, const float4 *__restrict__ w, float qx, float qy,
-                                             float qz, float qw)
+__device__ __forceinline__ void dia_products(float2 *u, float4 wa, float4 wb, float4 wc, float4 wd, float qx,
+                                             float qy, float qz, float qw)
 {
     u[0] = make_float2(qx * -2.f, qy * -2.f);
-    const float4 a = w[0], b = w[1], c = w[2], d = w[3];
-    u[1 * S] = make_float2(qx * a.x, qz * a.y);
-    u[2 * S] = make_float2(qx * a.z, qz * a.w);
-    u[3 * S] = make_float2(qx * b.x, qz * b.y);
-    u[4 * S] = make_float2(qx * b.z, qz * b.w);
-    u[5 * S] = make_float2(qx * c.x, qw * c.y);
-    u[6 * S] = make_float2(qx * c.z, qw * c.w);
-    u[7 * S] = make_float2(qx * d.x, qw * d.y);
-    u[8 * S] = make_float2(qx * d.z, qw * d.w);
+    u[1 * S] = make_float2(qx * wa.x, qz * wa.y);
+    u[2 * S] = make_float2(qx * wa.z, qz * wa.w);
+    u[3 * S] = make_float2(qx * wb.x, qz * wb.y);
+    u[4 * S] = make_float2(qx * wb.z, qz * wb.w);
+    u[5 * S] = make_float2(qx * wc.x, qw * wc.y);
+    u[6 * S] = make_float2(qx * wc.z, qw * wc.w);
+    u[7 * S] = make_float2(qx * wd.x, qw * wd.y);
+    u[8 * S] = make_float2(qx * wd.z, qw * wd.w);
 }
 
 // The LDS block of one MC for implsch_snl_kernel, 36 words (9 x 16 bytes):
@@ -742,6 +741,24 @@ __device__ __forceinline__ float snl_coef_block(const DevTables &t, int mc, int 
     }
 }
 
+// One gather step of a bin: SP_GROUP_DEV entry pairs (byte offsets into U,
+// two per word), all eight LDS reads issued before the adds, which keep the
+// entries' order.
+__device__ __forceinline__ void gather_group(const char *Ub, const uint32_t (&v)[SP_GROUP_DEV], float &a, float &b)
+{
+    float2 x0[SP_GROUP_DEV], x1[SP_GROUP_DEV];
+#pragma unroll
+    for (int u = 0; u < SP_GROUP_DEV; u++) {
+        x0[u] = *reinterpret_cast<const float2 *>(Ub + (v[u] & 0xffffu));
+        x1[u] = *reinterpret_cast<const float2 *>(Ub + (v[u] >> 16));
+    }
+#pragma unroll
+    for (int u = 0; u < SP_GROUP_DEV; u++) {
+        a = a + x0[u].x; b = b + x0[u].y;
+        a = a + x1[u].x; b = b + x1[u].y;
+    }
+}
+
 template <int NTH, int NJ>
 __device__ __forceinline__ void snonlin_products(const DevTables &t, const Smem &s, float2 *U, float ftemp_scale,
                                                  const int (&qslot)[256 * SP_QPT_DEV / NTH],
@@ -749,86 +766,82 @@ __device__ __forceinline__ void snonlin_products(const DevTables &t, const Smem 
                                                  const unsigned (&qdk)[256 * SP_QPT_DEV / NTH], float (&sl)[NJ],
                                                  float (&fl)[NJ])
 {
-    constexpr int S = 256 * SP_QPT_DEV, QT = S / NTH, NW = NTH / 64;
+    constexpr int S = 256 * SP_QPT_DEV, QT = S / NTH, NW = NTH / 64, G = SP_GROUP_DEV;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const float *F = s.F;
+    const char *Fb = reinterpret_cast<const char *>(s.F);
+    const char *Ub = reinterpret_cast<const char *>(U);
+    auto f_at = [&](unsigned off) { return *reinterpret_cast<const float *>(Fb + off); };
     for (int c = 0; c < t.SP_NCHUNK; c++) {
         const int mc0 = c * t.SP_CHUNK;
         const int nmc = min(t.SP_CHUNK, t.MLSTHG - mc0);
+        // this wave's runs of the chunk (entry offset, pair count: uniform,
+        // from the LDS copy of SP_RW) and the first group of each run's
+        // list, loaded now so that the loads overlap the quadruplet phase
+        const int2 *rw = reinterpret_cast<const int2 *>(s.sprw) + (size_t)c * t.SP_NRUN;
+        int roff[NJ], rnp[NJ];
+        uint32_t pv[NJ][G];
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            const int r = NW * j + wave;
+            const int2 v = r < t.SP_NRUN ? rw[r] : make_int2(t.SP_PAD, 0);
+            roff[j] = __builtin_amdgcn_readfirstlane(v.x);
+            rnp[j] = __builtin_amdgcn_readfirstlane(v.y) >> 1;
+            const uint32_t *e = reinterpret_cast<const uint32_t *>(t.SP_E + roff[j]) + lane;
+#pragma unroll
+            for (int u = 0; u < G; u++) pv[j][u] = e[u * 64];
+        }
 #pragma unroll
         for (int i = 0; i < QT; i++) {
             const int mcl = qslot[i] & 255;
             if (mcl >= nmc) continue;
             const int ql = tid + i * NTH, mcm1 = mc0 + mcl;
-            // byte offsets into F: frequency rows IC, IP, IP1, IM, IM1 of
-            // this MC (s.inl holds (INLCOEF - 1) * KL * 4) plus the
-            // quadruplet's directions K1, K11, K2, K21 (bytes of qdir) and K
             // the MC's block of 9 float4 (snl_coef_block): the terms'
-            // coefficients, the row offsets, the update weights
+            // coefficients, the byte offsets into F of the frequency rows
+            // IC, IP, IP1, IM, IM1 (s.rnl holds (INLCOEF - 1) * KL * 4), the
+            // update weights
             const float4 *cb = reinterpret_cast<const float4 *>(s.rnl) + mcm1 * 9;
             const float4 ca = cb[0], cc = cb[1], ce = cb[2];
             const int4 ic4 = reinterpret_cast<const int4 *>(cb)[3];
-            const int ic[5] = {ic4.x, ic4.y, ic4.z, ic4.w, reinterpret_cast<const int4 *>(cb)[4].x};
-            const float w[17] = {ca.x, ca.y, ca.z, ca.w, cc.x, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
-                                 cc.y, cc.z, cc.w, ce.x};
-            const float af = ce.y;
+            const unsigned IM1 = reinterpret_cast<const int4 *>(cb)[4].x;
+            const float4 wa = cb[5], wb = cb[6], wc = cb[7], wd = cb[8];
+            const unsigned IC = ic4.x, IP = ic4.y, IP1 = ic4.z, IM = ic4.w;
+            // the quadruplet's directions K1, K11, K2, K21 (bytes of qdir)
+            // and K, as byte offsets
             const unsigned d = qdir[i];
             const unsigned k1 = d & 255u, k11 = (d >> 8) & 255u, k2 = (d >> 16) & 255u, k21 = d >> 24;
             const unsigned k = qdk[i];
-            const char *Fb = reinterpret_cast<const char *>(F);
-            auto f_at = [&](unsigned off) { return *reinterpret_cast<const float *>(Fb + off); };
-            const unsigned IC = ic[0], IP = ic[1], IP1 = ic[2], IM = ic[3], IM1 = ic[4];
-            // dia_terms with the coefficients in registers (same operations)
-            const float sap = w[1] * f_at(k1 + IP) + w[2] * f_at(k11 + IP) + w[3] * f_at(k1 + IP1) + w[4] * f_at(k11 + IP1);
-            const float sam =
-                w[13] * f_at(k2 + IM) + w[14] * f_at(k21 + IM) + w[15] * f_at(k2 + IM1) + w[16] * f_at(k21 + IM1);
-            const float fij = f_at(k + IC) * w[0];
-            const float ftemp = af * ftemp_scale;
+            // the nine spectral values first, then dia_terms' arithmetic
+            const float fp0 = f_at(k1 + IP), fp1 = f_at(k11 + IP), fp2 = f_at(k1 + IP1), fp3 = f_at(k11 + IP1);
+            const float fm0 = f_at(k2 + IM), fm1 = f_at(k21 + IM), fm2 = f_at(k2 + IM1), fm3 = f_at(k21 + IM1);
+            const float fc = f_at(k + IC);
+            __builtin_amdgcn_sched_barrier(0);
+            const float sap = ca.y * fp0 + ca.z * fp1 + ca.w * fp2 + cc.x * fp3;
+            const float sam = cc.y * fm0 + cc.z * fm1 + cc.w * fm2 + ce.x * fm3;
+            const float fij = fc * ca.x;
+            const float ftemp = ce.y * ftemp_scale;
             float fad1 = fij * (sap + sam);
             const float fad2 = fad1 - (2.f * sap) * sam;
             fad1 = fad1 + fad2;
             const float fcen = ftemp * fij;
             const float qx = fad2 * fcen, qy = fad1 * ftemp, qz = ((fij - 2.f * sap) * t.DAL2) * fcen;
             const float qw = ((fij - 2.f * sam) * t.DAL1) * fcen;
-            dia_products<SP_TSTRIDE_DEV>(U + ql, cb + 5, qx, qy, qz, qw);
+            dia_products<SP_TSTRIDE_DEV>(U + ql, wa, wb, wc, wd, qx, qy, qz, qw);
         }
         __syncthreads();
-        // this wave's runs of the chunk: entry offset and pair count (uniform)
-        const int2 *rw = reinterpret_cast<const int2 *>(t.SP_RW) + (size_t)c * t.SP_NRUN;
-        int roff[NJ], rnp[NJ];
-#pragma unroll
-        for (int j = 0; j < NJ; j++) {
-            const int r = NW * j + wave;
-            const int2 v = r < t.SP_NRUN ? rw[r] : make_int2(0, 0);
-            roff[j] = v.x;
-            rnp[j] = v.y >> 1;
-        }
+        // every bin adds its updates from the chunk, in the reference's order
 #pragma unroll
         for (int j = 0; j < NJ; j++) {
             const int np = rnp[j];
             if (np == 0) continue;
             const uint32_t *e = reinterpret_cast<const uint32_t *>(t.SP_E + roff[j]) + lane;
-            const char *Ub = reinterpret_cast<const char *>(U);
-            auto u_at = [&](uint32_t off) { return *reinterpret_cast<const float2 *>(Ub + off); };
             float a = sl[j], b = fl[j];
-            int i = 0;
-            for (; i + 4 <= np; i += 4) {
-                uint32_t v[4];
+            gather_group(Ub, pv[j], a, b);
+            for (int i = G; i < np; i += G) {          // np is a whole number of groups
+                uint32_t v[G];
 #pragma unroll
-                for (int u = 0; u < 4; u++) v[u] = e[(i + u) * 64];
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const float2 u0 = u_at(v[u] & 0xffffu), u1 = u_at(v[u] >> 16);
-                    a = a + u0.x; b = b + u0.y;
-                    a = a + u1.x; b = b + u1.y;
-                }
-            }
-            for (; i < np; i++) {
-                const uint32_t v = e[i * 64];
-                const float2 u0 = u_at(v & 0xffffu), u1 = u_at(v >> 16);
-                a = a + u0.x; b = b + u0.y;
-                a = a + u1.x; b = b + u1.y;
+                for (int u = 0; u < G; u++) v[u] = e[(i + u) * 64];
+                gather_group(Ub, v, a, b);
             }
             sl[j] = a;
             fl[j] = b;
@@ -1484,12 +1497,14 @@ implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x)
         if (SHALLOW) { s.SLBOT = p; p += NB; }
     }
     s.rnl = p; p += 36 * MH;      // per MC a 16-byte aligned block (snl_coef_block)
+    s.sprw = reinterpret_cast<int *>(p); p += 2 * t.SP_NCHUNK * t.SP_NRUN;
     s.fr = p; p += ML; s.dfim = p; p += ML;
     s.th = p; p += KL; s.costh = p; p += KL; s.sinth = p; p += KL;
 
     for (int i = tid; i < ML; i += SNL_NT) { s.fr[i] = t.FR[i]; s.dfim[i] = t.DFIM[i]; }
     for (int i = tid; i < KL; i += SNL_NT) { s.th[i] = t.TH[i]; s.costh[i] = t.COSTH[i]; s.sinth[i] = t.SINTH[i]; }
     for (int i = tid; i < 36 * MH; i += SNL_NT) s.rnl[i] = snl_coef_block(t, i / 36, i % 36);
+    for (int i = tid; i < 2 * t.SP_NCHUNK * t.SP_NRUN; i += SNL_NT) s.sprw[i] = t.SP_RW[i];
     if (tid == 0) U[S - 1] = make_float2(-0.f, -0.f);
     // this thread's quadruplet slots (MC offset, KH*KL + K); MC offset 255
     // marks a slot past the chunk
@@ -1694,7 +1709,7 @@ size_t implsch_snl_lds_bytes(const DevTables &t, int lcflx)
     const int ML = t.ML, KL = t.KL, MH = t.MLSTHG, NB = KL * ML;
     size_t n = 2 * 9 * (size_t)SP_TSTRIDE_DEV + (size_t)NB;
     if (lcflx) n += (t.shallow ? 2 : 1) * (size_t)NB;
-    n += 2 * (size_t)ML + 3 * (size_t)KL + 36 * (size_t)MH;
+    n += 2 * (size_t)ML + 3 * (size_t)KL + 36 * (size_t)MH + 2 * (size_t)t.SP_NCHUNK * t.SP_NRUN;
     return n * sizeof(float);
 }
 

@@ -434,7 +434,7 @@ static int snonlin_lists(Tables &t, std::string &err)
                 }
                 w = std::max(w, per[l].size());
             }
-            w = (w + 1) / 2 * 2;                                  // whole pairs
+            w = (w + 2 * SP_GROUP - 1) / (2 * SP_GROUP) * (2 * SP_GROUP);   // whole groups of pairs
             const size_t off = t.SP_E.size();
             t.SP_RW[2 * ((size_t)c * t.SP_NRUN + r)] = (int32_t)off;
             t.SP_RW[2 * ((size_t)c * t.SP_NRUN + r) + 1] = (int32_t)w;
@@ -444,6 +444,11 @@ static int snonlin_lists(Tables &t, std::string &err)
                 for (size_t i = 0; i < per[l].size(); i++) t.SP_E[off + ((i / 2) * 64 + l) * 2 + (i % 2)] = per[l][i];
         }
     }
+    // one group of padding pairs for the runs with no entries
+    t.SP_PAD = (int)t.SP_E.size();
+    t.SP_E.resize(t.SP_E.size() + (size_t)SP_GROUP * 64 * 2, SP_NONE);
+    for (size_t i = 0; i < t.SP_RW.size(); i += 2)
+        if (t.SP_RW[i + 1] == 0) t.SP_RW[i] = t.SP_PAD;
     return WAM_OK;
 }
 

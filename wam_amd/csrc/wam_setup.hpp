@@ -21,6 +21,7 @@ constexpr uint16_t SN_NONE = 0x01FF;
 constexpr int SP_QPT = 2;               // products form: quadruplets per thread of a chunk
 constexpr int SP_TSTRIDE = 256 * SP_QPT + 4;    // products form: update slots per type (padded against LDS bank conflicts)
 constexpr uint16_t SP_NONE = (256 * SP_QPT - 1) * 8;   // products form: byte offset of the unused last slot of type 0
+constexpr int SP_GROUP = 4;             // products form: entry pairs a gather step takes per bin
 
 struct Consts {           // wam_general_module.f90:41-85
     float G, PI, ZPI, DEG, RAD, CIRC, RADIUS, ROAIR, RNUAIR, RNUAIRM, ROWATER, XKAPPA, XNLEV,
@@ -86,10 +87,13 @@ struct Tables {
     // quadruplet's index within the chunk), float2 each, and slot S - 1 of
     // type 0 holds a zero.  Entries are byte offsets of these slots,
     // (type*SP_TSTRIDE + ql)*8, and SP_NONE = (S - 1)*8.  For run r of 64 bins (64 r .. 64 r + 63) chunk c has W =
-    // SP_RW[2(c*SP_NRUN + r) + 1] entries per bin (even, 0 when the run gets
-    // nothing from the chunk), padded with SP_NONE; entries 2i and 2i+1 of
-    // bin 64 r + l sit at SP_E[SP_RW[2(c*SP_NRUN + r)] + (i*64 + l)*2 + {0,1}].
-    int SP_CHUNK = 0, SP_NCHUNK = 0, SP_NRUN = 0;
+    // SP_RW[2(c*SP_NRUN + r) + 1] entries per bin (a multiple of 2*SP_GROUP,
+    // 0 when the run gets nothing from the chunk), padded with SP_NONE;
+    // entries 2i and 2i+1 of bin 64 r + l sit at
+    // SP_E[SP_RW[2(c*SP_NRUN + r)] + (i*64 + l)*2 + {0,1}].  A run with no
+    // entries points at SP_PAD, a block of SP_GROUP pairs per lane of SP_NONE,
+    // so that a group can always be loaded.
+    int SP_CHUNK = 0, SP_NCHUNK = 0, SP_NRUN = 0, SP_PAD = 0;
     std::vector<uint16_t> SP_E;
     std::vector<int32_t> SP_RW;                          // [SP_NCHUNK][SP_NRUN][2]
 };
