@@ -24,6 +24,7 @@ template <bool CURR>
 __global__ void cfl_minmax_kernel(DevTables t, float *out);
 template <bool CURR>
 __global__ void propags_fused_kernel(DevTables t, int counter, const float *fin, float *fout);
+template <int RESCALE>
 __global__ void propags_fused_sc_kernel(DevTables t, int counter, const float *fin, float *fout);
 __global__ void wavnu2_kernel(DevTables t, float *wn, float *cgg);
 hipError_t launch_implsch(const DevTables &t, const ImplArgs &a, int ncu, hipStream_t st);
@@ -179,6 +180,17 @@ static int upload_static(wam_ctx *ctx)
     UP(o, t.DPSN_S, DPSN_S); UP(o, t.DPSN_N, DPSN_N);
     UP(o, t.JXO1, JXO1); UP(o, t.JXO2, JXO2); UP(o, t.JYO1, JYO1); UP(o, t.JYO2, JYO2);
     UP(o, t.KPM_p, KPM_P); UP(o, t.KPM_m, KPM_M);
+    std::vector<float> kdir((size_t)8 * t.KL, 0.f);
+    for (int k = 0; k < t.KL; k++) {
+        const int kp = k + 1 < t.KL ? k + 1 : 0, km = k > 0 ? k - 1 : t.KL - 1;
+        const float s = t.SINTH[k], c = t.COSTH[k];
+        float *kd = kdir.data() + 8 * k;
+        kd[0] = s; kd[1] = c;
+        kd[2] = d.DELTH0 * (s + t.SINTH[kp]);
+        kd[3] = d.DELTH0 * (s + t.SINTH[km]);
+        kd[4] = s * s; kd[5] = s * c; kd[6] = c * c;
+    }
+    UP(o, kdir, KDIR);
     return WAM_OK;
 }
 
@@ -347,9 +359,13 @@ int wam_gpu_propags_dev(wam_ctx *ctx, const float *d_in, float *d_out)
         timing_record(ctx, ev[0]);
         if (ctx->coef)
             hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, ctx->stream, ctx->dt, ctx->coef, src, dst);
-        else if (t.cfg.refr_c && t.cfg.shallow)
-            hipLaunchKernelGGL(propags_fused_sc_kernel, dim3(grid), dim3(256), 0, ctx->stream, ctx->dt, ctx->counter,
-                               src, dst);
+        else if (t.cfg.refr_c && t.cfg.shallow && 3ull * 4 * t.KL * t.NSEA < (1ull << 32)) {
+            // (32-bit buffer offsets over three frequency planes)
+            const int c = ctx->counter;
+            auto k = c == 1 ? propags_fused_sc_kernel<0>
+                            : (c & (c - 1)) == 0 ? propags_fused_sc_kernel<1> : propags_fused_sc_kernel<2>;
+            hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, ctx->stream, ctx->dt, c, src, dst);
+        }
         else if (t.cfg.refr_c)
             hipLaunchKernelGGL(propags_fused_kernel<true>, dim3(grid), dim3(256), 0, ctx->stream, ctx->dt, ctx->counter,
                                src, dst);

@@ -62,6 +62,9 @@ struct DevTables {
     const float *WN, *CGG;                // [ML][NSEA] (IPHYS = 2)
     const int *JXO1, *JYO1, *JXO2, *JYO2, *KPM_P, *KPM_M;   // 1-based
     float DELTH0, DELPH0, DELTHR, DELFR0;
+    // per direction K (cyclic K+1, K-1): SINTH, COSTH, DELTH0*(SINTH(K)+SINTH(K+1)),
+    // DELTH0*(SINTH(K)+SINTH(K-1)), SINTH**2, SINTH*COSTH, COSTH**2, 0
+    const float *KDIR;
 };
 
 struct ImplArgs {

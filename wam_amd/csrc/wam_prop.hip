@@ -458,33 +458,61 @@ __global__ void __launch_bounds__(256) propags_fused_kernel(DevTables t, int cou
 // (every weight is the same sequence of operations), with the work hoisted:
 // per-point fields once, the per-frequency group velocities and depth-table
 // factors carried across the M loop, the per-direction refraction terms in a
-// window carried across the (cyclic) K loop.  CHECK_CFL's rescale divides by
-// COUNTER; for a power of two that is the exact multiplication by 1/COUNTER.
+// window carried across the (cyclic) K loop, the per-direction constants from
+// KDIR (host-computed with the same operations).  Rows of F are addressed as
+// a uniform row pointer plus a per-lane 32-bit byte offset; a land neighbour
+// reads the point's own value and takes 0 instead (the reference's land slot).
+// RESCALE is CHECK_CFL's division of the weights by COUNTER: 0 none
+// (COUNTER = 1), 1 a power of two (the exact multiplication by 1/COUNTER),
+// 2 a true division.
+template <int RESCALE>
 __global__ void __launch_bounds__(256) propags_fused_sc_kernel(DevTables t, int counter,
                                                                 const float *__restrict__ fin,
                                                                 float *__restrict__ fout)
 {
-    const int nch = (t.NSEA + 255) / 256;
+    // KDIR in LDS (KL <= 64): uniform reads of it are broadcasts
+    __shared__ float4 kdir4[2 * 64];
+    const int KL = t.KL, ML = t.ML, NS = t.NSEA, ND = t.NDEPTH;
+    for (int i = threadIdx.x; i < 2 * KL; i += 256) kdir4[i] = reinterpret_cast<const float4 *>(t.KDIR)[i];
+    __syncthreads();
+    const float *kdir = reinterpret_cast<const float *>(kdir4);
+    const int nch = (NS + 255) / 256;
     const int ch = xcd_chunk(nch);
     if (ch >= nch) return;
     const int ij = ch * 256 + threadIdx.x;
-    if (ij >= t.NSEA) return;
-    const int KL = t.KL, ML = t.ML, NS = t.NSEA, ND = t.NDEPTH;
+    if (ij >= NS) return;
     const int ns = t.KLAT_S[ij], nn = t.KLAT_N[ij], nw = t.KLON_W[ij], ne = t.KLON_E[ij];
+    const bool hs = ns >= 0, hn = nn >= 0, hw = nw >= 0, he = ne >= 0;
+    const unsigned o_c = 4u * ij, o_s = 4u * (hs ? ns : ij), o_n = 4u * (hn ? nn : ij);
+    const unsigned o_w = 4u * (hw ? nw : ij), o_e = 4u * (he ? ne : ij);
     // per point
     const float u = t.U[ij], v = t.V[ij];
-    const float uw = nw < 0 ? 0.f : t.U[nw], ue = ne < 0 ? 0.f : t.U[ne];
-    const float vs = ns < 0 ? 0.f : t.V[ns], vn = nn < 0 ? 0.f : t.V[nn];
+    const float uw = hw ? t.U[nw] : 0.f, ue = he ? t.U[ne] : 0.f;
+    const float vs = hs ? t.V[ns] : 0.f, vn = hn ? t.V[nn] : 0.f;
     const int i0 = t.INDEP[ij] - 1;
-    const int iw = nw < 0 ? ND - 1 : t.INDEP[nw] - 1, ie = ne < 0 ? ND - 1 : t.INDEP[ne] - 1;
-    const int is = ns < 0 ? ND - 1 : t.INDEP[ns] - 1, in = nn < 0 ? ND - 1 : t.INDEP[nn] - 1;
+    const int iw = hw ? t.INDEP[nw] - 1 : ND - 1, ie = he ? t.INDEP[ne] - 1 : ND - 1;
+    const int is = hs ? t.INDEP[ns] - 1 : ND - 1, in = hn ? t.INDEP[nn] - 1 : ND - 1;
     const int ig = t.INDEP_G[ij] - 1;
     const float ddphi = t.DDPHI[ij], ddlam = t.DDLAM[ij];
     const float duphi = t.DUPHI[ij], dulam = t.DULAM[ij], dvphi = t.DVPHI[ij], dvlam = t.DVLAM[ij];
     const float drgm = t.DRGM[ij], della0 = t.DELLA0[ij], dpsn_s = t.DPSN_S[ij], dpsn_n = t.DPSN_N[ij];
     const float tempd = v * ddphi + u * ddlam;
-    const bool pow2 = (counter & (counter - 1)) == 0;
     const float rc = (float)counter, rinv = 1.0f / rc;
+    // F through one buffer resource per frequency M, based at plane M-1
+    // (clamped): rows (M', K) at scalar byte offsets (M' - M-1)*planeb +
+    // K*rowb, the sea point (or neighbour) as the lane's byte offset; the
+    // host keeps 3*planeb below 2**32
+    const unsigned rowb = 4u * NS, planeb = rowb * KL;
+    auto rsrc = [&](const float *base, int m, unsigned bytes) {
+        // the base as a wave-uniform value (kept in scalar registers)
+        const uint64_t a = reinterpret_cast<uint64_t>(base + (size_t)m * KL * NS);
+        const uint64_t u = (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)a) |
+                           (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(a >> 32)) << 32;
+        return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void *>(u), (short)0, (int)bytes, 0x00020000);
+    };
+    auto ld = [&](__amdgpu_buffer_rsrc_t r, unsigned off, unsigned soff) {
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
+    };
     // group velocity, TSIHKD and TFAK of the point's depth rows at frequency m
     auto cgv = [&](int idx, int m) { return t.TCGOND[m * ND + idx]; };
     float cg_m = cgv(i0, 0), ts_m = t.TSIHKD[ig], tf_m = t.TFAK[ig];
@@ -495,20 +523,28 @@ __global__ void __launch_bounds__(256) propags_fused_sc_kernel(DevTables t, int 
         const int mm = m > 0 ? m - 1 : 0;
         const float dfp = t.DELFR0 / t.FR[m], dfm = t.DELFR0 / t.FR[mm];
         const float cgw = cgv(iw, m), cge = cgv(ie, m), cgs = cgv(is, m), cgn = cgv(in, m);
+        const __amdgpu_buffer_rsrc_t pf = rsrc(fin, mm, 3 * planeb), po = rsrc(fout, m, planeb);
+        const unsigned sm0 = (m - mm) * planeb, sp0 = (mp - mm) * planeb;   // planes M, M+1 from plane M-1
         // THDD(K, M) = temp(K)*TSIHKD(M) + THDC(K); window over K-1, K, K+1
         auto thdd_k = [&](int k) {
-            const float s = t.SINTH[k], c = t.COSTH[k];
+            const float *kd = kdir + 8 * k;
+            const float s = kd[0], c = kd[1];
             const float temp = s * ddphi - c * ddlam;
-            const float ss = s * s, sc = s * c, cc = c * c;
+            const float ss = kd[4], sc = kd[5], cc = kd[6];
             const float thdc = ((ss * duphi + sc * dvphi) - sc * dulam) - cc * dvlam;
             return temp * ts_m + thdc;
         };
         float th_km = thdd_k(KL - 1), th_k = thdd_k(0), th_kp = thdd_k(KL > 1 ? 1 : 0);
         for (int k = 0; k < KL; k++) {
             const int kp = k + 1 < KL ? k + 1 : 0, km = k > 0 ? k - 1 : KL - 1;
-            const float sd = t.SINTH[k], cd = t.COSTH[k];
-            const float sp = t.DELTH0 * (t.SINTH[k] + t.SINTH[kp]);
-            const float sm = t.DELTH0 * (t.SINTH[k] + t.SINTH[km]);
+            const float *kd = kdir + 8 * k;
+            const float sd = kd[0], cd = kd[1], sp = kd[2], sm = kd[3];
+            // the nine spectral values of the stencil
+            const unsigned rk = sm0 + k * rowb;
+            const float f_c = ld(pf, o_c, rk);
+            const float l_n = ld(pf, o_n, rk), l_s = ld(pf, o_s, rk), l_e = ld(pf, o_e, rk), l_w = ld(pf, o_w, rk);
+            const float f_kp = ld(pf, o_c, sm0 + kp * rowb), f_km = ld(pf, o_c, sm0 + km * rowb);
+            const float f_mp = ld(pf, o_c, sp0 + k * rowb), f_mm = ld(pf, o_c, k * rowb);
             const float w1 = u + sd * cg_m;
             const float w2 = (v + cd * cg_m) * t.DELPH0;
             const float dlw = (w1 + (uw + sd * cgw)) * della0;
@@ -521,7 +557,7 @@ __global__ void __launch_bounds__(256) propags_fused_sc_kernel(DevTables t, int 
             const float dtm = ((sm * drgm) * cg_m) + (th_k + th_km) * t.DELTHR;
             dtc = (dtc - wam_maxf(0.f, dtp)) + wam_minf(0.f, dtm);
             // SIDC(K, M') = (S0(K)*CG(M') + TEMPD*TSIHKD(M'))*TFAK(M')
-            const float ss = sd * sd, sc = sd * cd, cc = cd * cd;
+            const float ss = kd[4], sc = kd[5], cc = kd[6];
             const float s0 = (((-(sc * duphi)) - cc * dvphi) - ss * dulam) - sc * dvlam;
             const float sk = (s0 * cg_m + tempd * ts_m) * tf_m;
             const float dop = (sk + (s0 * cg_mp + tempd * ts_mp) * tf_mp) * dfp;
@@ -537,30 +573,25 @@ __global__ void __launch_bounds__(256) propags_fused_sc_kernel(DevTables t, int 
             c[P_TH2] = wam_maxf(0.f, dtm);
             c[P_SIG1] = -wam_minf(0.f, dop) / 1.1f;
             c[P_SIG2] = wam_maxf(0.f, dom) * 1.1f;
-            if (counter > 1) {
-                if (pow2) {
-                    c[0] = 1.f - (1.f - c[0]) * rinv;
+            if (RESCALE == 1) {
+                c[0] = 1.f - (1.f - c[0]) * rinv;
 #pragma unroll
-                    for (int p = 1; p < P_NPLANES; p++) c[p] = c[p] * rinv;
-                } else {
-                    c[0] = 1.f - (1.f - c[0]) / rc;
+                for (int p = 1; p < P_NPLANES; p++) c[p] = c[p] * rinv;
+            } else if (RESCALE == 2) {
+                c[0] = 1.f - (1.f - c[0]) / rc;
 #pragma unroll
-                    for (int p = 1; p < P_NPLANES; p++) c[p] = c[p] / rc;
-                }
+                for (int p = 1; p < P_NPLANES; p++) c[p] = c[p] / rc;
             }
-            const int mmi = mm, mpi = mp;
-            const size_t row = ((size_t)m * KL + k) * NS;
-            const float *f = fin + row;
-            float sacc = c[P_SUM] * f[ij];
-            sacc = sacc + c[P_LAT2] * ld_or0(f, nn);
-            sacc = sacc + c[P_LAT1] * ld_or0(f, ns);
-            sacc = sacc + c[P_LON2] * ld_or0(f, ne);
-            sacc = sacc + c[P_LON1] * ld_or0(f, nw);
-            sacc = sacc + c[P_TH1] * fin[((size_t)m * KL + kp) * NS + ij];
-            sacc = sacc + c[P_TH2] * fin[((size_t)m * KL + km) * NS + ij];
-            sacc = sacc + c[P_SIG1] * fin[((size_t)mpi * KL + k) * NS + ij];
-            sacc = sacc + c[P_SIG2] * fin[((size_t)mmi * KL + k) * NS + ij];
-            __builtin_nontemporal_store(sacc, &fout[row + ij]);
+            float sacc = c[P_SUM] * f_c;
+            sacc = sacc + c[P_LAT2] * (hn ? l_n : 0.f);
+            sacc = sacc + c[P_LAT1] * (hs ? l_s : 0.f);
+            sacc = sacc + c[P_LON2] * (he ? l_e : 0.f);
+            sacc = sacc + c[P_LON1] * (hw ? l_w : 0.f);
+            sacc = sacc + c[P_TH1] * f_kp;
+            sacc = sacc + c[P_TH2] * f_km;
+            sacc = sacc + c[P_SIG1] * f_mp;
+            sacc = sacc + c[P_SIG2] * f_mm;
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(sacc), po, o_c, k * rowb, 2);   // nt
             th_km = th_k;
             th_k = th_kp;
             th_kp = thdd_k(k + 2 < KL ? k + 2 : k + 2 - KL);
@@ -570,6 +601,9 @@ __global__ void __launch_bounds__(256) propags_fused_sc_kernel(DevTables t, int 
     }
 }
 
+template __global__ void propags_fused_sc_kernel<0>(DevTables, int, const float *, float *);
+template __global__ void propags_fused_sc_kernel<1>(DevTables, int, const float *, float *);
+template __global__ void propags_fused_sc_kernel<2>(DevTables, int, const float *, float *);
 template __global__ void cfl_minmax_kernel<true>(DevTables, float *);
 template __global__ void cfl_minmax_kernel<false>(DevTables, float *);
 template __global__ void propags_fused_kernel<true>(DevTables, int, const float *, float *);
