@@ -396,6 +396,7 @@ static int ensure_impl_scratch(wam_ctx *ctx, int lcflx)
         HIPCHK(hipMalloc(&x.axy, 3 * (size_t)t.ML * ns * sizeof(float)));
         HIPCHK(hipMalloc(&x.pp, 4 * ns * sizeof(float)));
         HIPCHK(hipMalloc(&x.llws, nw * ns * sizeof(unsigned)));
+        HIPCHK(hipMalloc(&x.tick, 8 * 32 * sizeof(unsigned)));
     }
     if (lcflx && !x.smt) HIPCHK(hipMalloc(&x.smt, tiled * sizeof(float)));
     return WAM_OK;
@@ -728,7 +729,7 @@ void wam_gpu_finalize(wam_ctx *ctx)
     if (ctx->stage) (void)hipFree(ctx->stage);
     timing_clear(ctx);
     for (void *p : {(void *)ctx->isx.ft, (void *)ctx->isx.slt, (void *)ctx->isx.flt, (void *)ctx->isx.smt,
-                    (void *)ctx->isx.axy, (void *)ctx->isx.pp, (void *)ctx->isx.llws})
+                    (void *)ctx->isx.axy, (void *)ctx->isx.pp, (void *)ctx->isx.llws, (void *)ctx->isx.tick})
         if (p) (void)hipFree(p);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;

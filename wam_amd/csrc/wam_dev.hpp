@@ -86,6 +86,7 @@ struct ImplScratch {
     float *axy;                    // [3][ML][NSEA]: STRESSO's direction sums of the first SINPUT
     float *pp;                     // [4][NSEA]: ENHFR, MAX(FMEANWS, FMEAN), MIJ, PHIAW
     unsigned *llws;                // [(KL*ML+31)/32][NSEA]: LLWS bits of the second SINPUT
+    unsigned *tick;                // [8][32]: point counters of implsch_snl_kernel, one per XCD (own 128-byte line)
 };
 
 // offset of the quad of bins 4q..4q+3 of sea point pt in the tiled arrays

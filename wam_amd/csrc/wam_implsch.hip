@@ -1468,6 +1468,8 @@ __global__ void __launch_bounds__(PW) implsch_pre_kernel(DevTables t, ImplArgs a
         x.pp[(size_t)2 * NS + pt] = (float)mij;
         x.pp[(size_t)3 * NS + pt] = phiaw;
     }
+    // implsch_snl_kernel's point tickets start from zero
+    if (blockIdx.x == 0 && lane < 8) x.tick[32 * lane] = 0u;
 }
 
 constexpr int SNL_NT = 512;  // threads of implsch_snl_kernel: one quadruplet of a chunk each
@@ -1482,7 +1484,7 @@ implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x)
     extern __shared__ __align__(16) float lds_raw[];
     __shared__ float cu[MAXF], sprd[MAXF], flm[MAXF], tfk[MAXF], tlim[MAXF], tbot[MAXF];
     __shared__ float Ps[5];
-    __shared__ int Pmij;
+    __shared__ int Pmij, Pnext;
     constexpr int S = 256 * SP_QPT_DEV;
     const int KL = t.KL, ML = t.ML, NB = KL * ML, NS = t.NSEA, NB4 = x.NB4, tid = threadIdx.x;
     const int MH = t.MLSTHG;
@@ -1523,13 +1525,22 @@ implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x)
         qdk[i] = in ? (unsigned)(4 * (r < KL ? r : r - KL)) : 0u;
     }
 
-    const int nblk = gridDim.x, xcd = blockIdx.x % 8, lb = blockIdx.x / 8;
-    const int nb_x = (nblk + 7 - xcd) / 8;
+    // Points by tickets: each XCD (blockIdx % 8) owns a contiguous range of
+    // sea points and hands them out in order from its counter, so the
+    // workgroups of an XCD work on neighbouring points at the same time and
+    // share the 128-byte lines of the tiled arrays in its L2.  The ticket for
+    // the next point is taken at the start of the current one.
+    const int xcd = blockIdx.x % 8;
     const int per = (NS + 7) / 8;
     const int p0 = xcd * per, p1 = min(NS, p0 + per);
+    unsigned *tick = x.tick + 32 * xcd;
     const float delt = t.DELT;
-    for (int pt = p0 + lb; pt < p1; pt += nb_x) {
+    if (tid == 0) Pnext = p0 + (int)atomicAdd(tick, 1u);
+    __syncthreads();
+    for (int pt = Pnext; pt < p1; pt = Pnext) {
         const size_t q0 = quad_offset(pt, 0, NB4);
+        int tnext = 0;
+        if (tid == 0) tnext = (int)atomicAdd(tick, 1u);
         {
             float4 *F4 = reinterpret_cast<float4 *>(s.F), *SM4 = reinterpret_cast<float4 *>(s.SMIN);
             for (int q = tid; q < NB4; q += SNL_NT) {
@@ -1620,6 +1631,7 @@ implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x)
             const float4 *F4 = reinterpret_cast<const float4 *>(s.F);
             for (int q = tid; q < NB4; q += SNL_NT) *reinterpret_cast<float4 *>(x.ft + q0 + (size_t)q * 256) = F4[q];
         }
+        if (tid == 0) Pnext = p0 + tnext;
         __syncthreads();
     }
 }
