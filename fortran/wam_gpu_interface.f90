@@ -40,6 +40,11 @@ INTERFACE
       IMPORT :: C_INT, C_PTR
       TYPE(C_PTR), VALUE :: ctx, depth, u, v
    END FUNCTION
+   INTEGER(C_INT) FUNCTION c_wam_gpu_prepare_propagation_dev(ctx, d_depth, d_u, d_v) &
+&     BIND(C, NAME='wam_gpu_prepare_propagation_dev')
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx, d_depth, d_u, d_v
+   END FUNCTION
    INTEGER(C_INT) FUNCTION c_wam_gpu_propags(ctx, nijs, nijl, kl, ml, f3) BIND(C, NAME='wam_gpu_propags')
       IMPORT :: C_INT, C_INT32_T, C_PTR
       TYPE(C_PTR), VALUE :: ctx, f3
@@ -81,8 +86,8 @@ END INTERFACE
 TYPE(C_PTR), SAVE :: CTX = C_NULL_PTR
 INTEGER, SAVE, PUBLIC :: NSEA_GPU = 0
 
-PUBLIC :: WAM_GPU_SETUP, WAM_GPU_PREPARE_PROPAGATION, PROPAGS_GPU, IMPLSCH_GPU, OUTPUT_PARAMETERS_GPU, &
-&         WAM_GPU_END
+PUBLIC :: WAM_GPU_SETUP, WAM_GPU_PREPARE_PROPAGATION, WAM_GPU_PREPARE_PROPAGATION_DEV, PROPAGS_GPU, &
+&         IMPLSCH_GPU, OUTPUT_PARAMETERS_GPU, WAM_GPU_END
 
 CONTAINS
 
@@ -116,6 +121,14 @@ SUBROUTINE WAM_GPU_PREPARE_PROPAGATION (DEPTH, U, V)
    IF (PRESENT(V)) PV = C_LOC(V)
    CALL CHECK(c_wam_gpu_prepare_propagation(CTX, C_LOC(DEPTH), PU, PV), 'WAM_GPU_PREPARE_PROPAGATION')
 END SUBROUTINE WAM_GPU_PREPARE_PROPAGATION
+
+!  PREPARE_PROPAGATION from depth and currents already in device memory (NSEA
+!  each; D_U, D_V may be C_NULL_PTR without current refraction), e.g. from a
+!  coupled model that keeps them on the GPU.
+SUBROUTINE WAM_GPU_PREPARE_PROPAGATION_DEV (D_DEPTH, D_U, D_V)
+   TYPE(C_PTR), INTENT(IN) :: D_DEPTH, D_U, D_V
+   CALL CHECK(c_wam_gpu_prepare_propagation_dev(CTX, D_DEPTH, D_U, D_V), 'WAM_GPU_PREPARE_PROPAGATION_DEV')
+END SUBROUTINE WAM_GPU_PREPARE_PROPAGATION_DEV
 
 !  Drop-in for PROPAGS (F3) -- F3(NIJS:NIJL,KL,ML).
 SUBROUTINE PROPAGS_GPU (F3)

@@ -81,6 +81,18 @@ int wam_gpu_init_host(wam_ctx **ctx, const wam_config *cfg, const uint8_t *sea_m
    depth only).  Must precede IMPLSCH and PROPAGS. */
 int wam_gpu_prepare_propagation(wam_ctx *ctx, const float *depth, const float *u, const float *v);
 
+/* Device-resident PREPARE_PROPAGATION (wam_propagation_module.f90:279-598,
+   GRADIENT :182-275, PREPARE_PROPAGATION_COEF and CHECK_CFL): depth and
+   currents are HBM arrays of NSEA (d_u, d_v may be NULL without current
+   refraction), as when they come from a coupled model that lives on the
+   device; the reference re-runs PREPARE_PROPAGATION whenever they change
+   (wamodel.f90:317-326).  INDEP, INDEP_G, the depth and current gradients,
+   DELLA0 and DRGM are computed on the device (bit-identical to
+   wam_gpu_prepare_propagation) and copied back for wam_gpu_get_table; the
+   fields are copied, so the caller's arrays may change afterwards.  Returns
+   after the sub-step count is known (one synchronisation). */
+int wam_gpu_prepare_propagation_dev(wam_ctx *ctx, const float *d_depth, const float *d_u, const float *d_v);
+
 /* Where PROPAGS's weights live.  No counterpart in the reference, which
    always stores its COEF_* arrays (wam_propagation_module.f90:75-110):
    stored = 1 keeps the weight planes in HBM (5 or 9 planes of NSEA*KL*ML

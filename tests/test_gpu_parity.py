@@ -71,7 +71,7 @@ def case(request):
     cg = g.prepare_propagation(f["depth"], f["u"], f["v"])
     cs = gs.prepare_propagation(f["depth"], f["u"], f["v"])
     fl3 = W.jonswap_spectrum(cfg, f["udir"]).ravel()
-    return dict(name=request.param, cfg=cfg, f=f, o=o, g=g, gs=gs, co=co, cg=cg, cs=cs, fl3=fl3)
+    return dict(name=request.param, cfg=cfg, mask=mask, f=f, o=o, g=g, gs=gs, co=co, cg=cg, cs=cs, fl3=fl3)
 
 
 def test_propagation_weights_and_cfl(case):
@@ -82,6 +82,39 @@ def test_propagation_weights_and_cfl(case):
     assert bits_equal(gs.table("COEF"), o.get("COEF"))
     if case["cfg"]["shallow"]:
         assert np.array_equal(case["g"].table("INDEP"), o.get("INDEP"))
+
+
+def test_prepare_propagation_on_device(case):
+    """PREPARE_PROPAGATION from fields in HBM: the per-point tables (INDEP,
+    INDEP_G, gradients, DELLA0, DRGM), COUNTER and the CFL maximum equal the
+    host-prepared context's bit for bit, and PROPAGS and IMPLSCH then match
+    the oracle bit for bit."""
+    torch = pytest.importorskip("torch")
+    cfg, f, o = case["cfg"], case["f"], case["o"]
+    gd = WamGpu(cfg, case["mask"])
+    gd.set_propagation_weights(False)
+    dev = torch.device("cuda:0")
+    t = lambda x: None if x is None else torch.from_numpy(np.ascontiguousarray(x, np.float32)).to(dev)
+    cd = gd.prepare_propagation_dev(t(f["depth"]), t(f["u"]), t(f["v"]))
+    assert cd[0] == case["cg"][0]
+    assert np.float32(cd[1]) == np.float32(case["cg"][1])
+    for name in ("INDEP", "INDEP_G"):
+        assert np.array_equal(gd.table(name), case["g"].table(name)), name
+    for name in ("DDPHI", "DDLAM", "DUPHI", "DULAM", "DVPHI", "DVLAM", "DELLA0", "DRGM"):
+        assert bits_equal(gd.table(name), case["g"].table(name)), name
+    assert bits_equal(gd.propags(case["fl3"]), o.propags(case["fl3"]))
+    n = gd.nsea
+    fl3 = o.propags(case["fl3"])
+    z = np.zeros(n, np.float32)
+    ro = o.implsch(fl3, f["u10"], f["udir"], z, z, f["depth"])
+    rg = gd.implsch(fl3, f["u10"], f["udir"], z, z)
+    for k in ("fl3", "ustar", "tauw", "z0"):
+        assert bits_equal(rg[k], ro[k]), (k, maxrel(rg[k], ro[k]))
+    # a non-positive depth is refused, as on the host path
+    bad = np.array(f["depth"], np.float32)
+    bad[n // 2] = 0.0
+    with pytest.raises(WamError):
+        gd.prepare_propagation_dev(t(bad), t(f["u"]), t(f["v"]))
 
 
 @pytest.mark.parametrize("mode", ["computed", "stored"])

@@ -33,6 +33,7 @@ SIGNATURES = {
     "wam_gpu_init": (_I, [ctypes.POINTER(_P), ctypes.POINTER(WamConfig), _P]),
     "wam_gpu_init_host": (_I, [ctypes.POINTER(_P), ctypes.POINTER(WamConfig), _P]),
     "wam_gpu_prepare_propagation": (_I, [_P, _P, _P, _P]),
+    "wam_gpu_prepare_propagation_dev": (_I, [_P, _P, _P, _P]),
     "wam_gpu_propags": (_I, [_P, _I, _I, _I, _I, _P]),
     "wam_gpu_implsch": (_I, [_P, _I, _I, _I] + [_P] * 10 + [_I, _P, _P]),
     "wam_gpu_propags_dev": (_I, [_P, _P, _P]),

@@ -10,10 +10,12 @@
 
 #include "../../include/wam_gpu.h"
 #include "wam_dev.hpp"
+#include "wam_math.h"
 #include "wam_setup.hpp"
 
 namespace wam {
 __global__ void prep_coef_kernel(DevTables t, float *coef);
+__global__ void prep_points_kernel(DevTables t, PrepPoints p);
 __global__ void minmax_kernel(const float *x, size_t n, float *out);
 __global__ void rescale_kernel(float *coef, size_t nb, int nplanes, float rc);
 __global__ void propags_nocurr_kernel(DevTables t, const float *coef, const float *fin, float *fout);
@@ -180,6 +182,8 @@ static int upload_static(wam_ctx *ctx)
     UP(o, t.DPSN_S, DPSN_S); UP(o, t.DPSN_N, DPSN_N);
     UP(o, t.JXO1, JXO1); UP(o, t.JXO2, JXO2); UP(o, t.JYO1, JYO1); UP(o, t.JYO2, JYO2);
     UP(o, t.KPM_p, KPM_P); UP(o, t.KPM_m, KPM_M);
+    UP(o, t.DCO, DCO_PT); UP(o, t.DELLAM, DELLAM_ROW); UP(o, t.SINPH, SINPH_ROW); UP(o, t.KXLT, KXLT_PT);
+    d.DELPHI = t.DELPHI;
     std::vector<float> kdir((size_t)8 * t.KL, 0.f);
     for (int k = 0; k < t.KL; k++) {
         const int kp = k + 1 < t.KL ? k + 1 : 0, km = k > 0 ? k - 1 : t.KL - 1;
@@ -226,29 +230,18 @@ int wam_gpu_init(wam_ctx **out, const wam_config *cfg, const uint8_t *sea_mask)
     return rc;
 }
 
-int wam_gpu_prepare_propagation(wam_ctx *ctx, const float *depth, const float *u, const float *v)
+// PREPARE_PROPAGATION after the per-point fields are in ctx->dt: the
+// weights (stored planes or CHECK_CFL over computed ones), COUNTER, WAVNU2.
+static int finish_prepare(wam_ctx *ctx)
 {
-    if (!ctx || !depth) return WAM_E_ARG;
     Tables &t = ctx->tab;
-    int rc = prepare_points(t, depth, u, v, ctx->err);
-    if (rc) return rc;
-    free_list(ctx->point_allocs);
-    ctx->coef = nullptr; ctx->wn = nullptr; ctx->cgg = nullptr;
     const int ns = t.NSEA;
     auto &o = ctx->point_allocs;
-    std::vector<float> d(depth, depth + ns), uu(ns, 0.f), vv(ns, 0.f);
-    if (u) uu.assign(u, u + ns);
-    if (v) vv.assign(v, v + ns);
-    UP(o, d, DEPTH); UP(o, uu, U); UP(o, vv, V);
-    UP(o, t.INDEP, INDEP); UP(o, t.INDEP_G, INDEP_G); UP(o, t.DELLA0, DELLA0); UP(o, t.DRGM, DRGM);
-    UP(o, t.DDPHI, DDPHI); UP(o, t.DDLAM, DDLAM); UP(o, t.DUPHI, DUPHI); UP(o, t.DULAM, DULAM);
-    UP(o, t.DVPHI, DVPHI); UP(o, t.DVLAM, DVLAM);
-
+    void *p = nullptr;
     const size_t nb = (size_t)t.KL * t.ML * ns;
     ctx->ncoef = t.cfg.refr_c ? 9 : 5;
     hipStream_t st = ctx->stream;
     std::vector<float> lo(ctx->ncoef, 3.4e38f), hi(ctx->ncoef, -3.4e38f);
-    void *p = nullptr;
     // auto: computed in the stencil where the hoisted shallow-water + current
     // kernel applies (it is faster than streaming nine planes), else stored
     const bool stored = ctx->weights_mode == 1 ||
@@ -332,6 +325,82 @@ int wam_gpu_prepare_propagation(wam_ctx *ctx, const float *depth, const float *u
     HIPCHK(hipStreamSynchronize(st));
     ctx->prepared = true;
     return WAM_OK;
+}
+
+int wam_gpu_prepare_propagation(wam_ctx *ctx, const float *depth, const float *u, const float *v)
+{
+    if (!ctx || !depth) return WAM_E_ARG;
+    Tables &t = ctx->tab;
+    ctx->prepared = false;
+    int rc = prepare_points(t, depth, u, v, ctx->err);
+    if (rc) return rc;
+    free_list(ctx->point_allocs);
+    ctx->coef = nullptr; ctx->wn = nullptr; ctx->cgg = nullptr;
+    const int ns = t.NSEA;
+    auto &o = ctx->point_allocs;
+    std::vector<float> d(depth, depth + ns), uu(ns, 0.f), vv(ns, 0.f);
+    if (u) uu.assign(u, u + ns);
+    if (v) vv.assign(v, v + ns);
+    UP(o, d, DEPTH); UP(o, uu, U); UP(o, vv, V);
+    UP(o, t.INDEP, INDEP); UP(o, t.INDEP_G, INDEP_G); UP(o, t.DELLA0, DELLA0); UP(o, t.DRGM, DRGM);
+    UP(o, t.DDPHI, DDPHI); UP(o, t.DDLAM, DDLAM); UP(o, t.DUPHI, DUPHI); UP(o, t.DULAM, DULAM);
+    UP(o, t.DVPHI, DVPHI); UP(o, t.DVLAM, DVLAM);
+    return finish_prepare(ctx);
+}
+
+int wam_gpu_prepare_propagation_dev(wam_ctx *ctx, const float *d_depth, const float *d_u, const float *d_v)
+{
+    if (!ctx || !d_depth) return WAM_E_ARG;
+    Tables &t = ctx->tab;
+    ctx->prepared = false;
+    if (t.cfg.refr_c && (!d_u || !d_v)) { ctx->err = "current refraction needs u and v"; return WAM_E_ARG; }
+    free_list(ctx->point_allocs);
+    ctx->coef = nullptr; ctx->wn = nullptr; ctx->cgg = nullptr;
+    const int ns = t.NSEA;
+    auto &o = ctx->point_allocs;
+    hipStream_t st = ctx->stream;
+    // the context's own copies of the fields, then the per-point tables
+    auto dev = [&](size_t bytes, void **q) -> int {
+        HIPCHK(hipMalloc(q, bytes ? bytes : 4));
+        o.push_back(*q);
+        return WAM_OK;
+    };
+    float *dd = nullptr, *du = nullptr, *dv = nullptr, *f[8] = {};
+    int *ind = nullptr, *indg = nullptr;
+    unsigned *bad = nullptr;
+    int r = dev(ns * 4, (void **)&dd) | dev(ns * 4, (void **)&du) | dev(ns * 4, (void **)&dv) |
+            dev(ns * 4, (void **)&ind) | dev(ns * 4, (void **)&indg) | dev(4, (void **)&bad);
+    for (int i = 0; i < 8; i++) r |= dev(ns * 4, (void **)&f[i]);
+    if (r) return r;
+    HIPCHK(hipMemcpyAsync(dd, d_depth, ns * 4, hipMemcpyDeviceToDevice, st));
+    if (d_u) HIPCHK(hipMemcpyAsync(du, d_u, ns * 4, hipMemcpyDeviceToDevice, st));
+    else HIPCHK(hipMemsetAsync(du, 0, ns * 4, st));
+    if (d_v) HIPCHK(hipMemcpyAsync(dv, d_v, ns * 4, hipMemcpyDeviceToDevice, st));
+    else HIPCHK(hipMemsetAsync(dv, 0, ns * 4, st));
+    HIPCHK(hipMemsetAsync(bad, 0, 4, st));
+    PrepPoints pp{dd, du, dv, ind, indg, f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], bad,
+                  (float)t.cfg.deptha, wam_logf((float)t.cfg.depthd), (float)t.cfg.idelpro};
+    DevTables &d = ctx->dt;
+    d.DEPTH = dd; d.U = du; d.V = dv; d.INDEP = ind; d.INDEP_G = indg;
+    d.DDPHI = f[0]; d.DDLAM = f[1]; d.DUPHI = f[2]; d.DULAM = f[3]; d.DVPHI = f[4]; d.DVLAM = f[5];
+    d.DELLA0 = f[6]; d.DRGM = f[7];
+    hipLaunchKernelGGL(prep_points_kernel, dim3((ns + 255) / 256), dim3(256), 0, st, d, pp);
+    HIPCHK(hipGetLastError());
+    // the host copies of the per-point tables (INDEP is IMPLSCH's argument on
+    // the host path), and the depth check
+    unsigned hbad = 0;
+    t.INDEP.resize(ns); t.INDEP_G.resize(ns);
+    std::vector<float> *hf[8] = {&t.DDPHI, &t.DDLAM, &t.DUPHI, &t.DULAM, &t.DVPHI, &t.DVLAM, &t.DELLA0, &t.DRGM};
+    HIPCHK(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(t.INDEP.data(), ind, ns * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(t.INDEP_G.data(), indg, ns * 4, hipMemcpyDeviceToHost, st));
+    for (int i = 0; i < 8; i++) {
+        hf[i]->resize(ns);
+        HIPCHK(hipMemcpyAsync(hf[i]->data(), f[i], ns * 4, hipMemcpyDeviceToHost, st));
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    if (hbad) { ctx->err = "depth must be positive at every sea point"; return WAM_E_ARG; }
+    return finish_prepare(ctx);
 }
 
 int wam_gpu_propags_dev(wam_ctx *ctx, const float *d_in, float *d_out)
@@ -638,6 +707,15 @@ int64_t wam_gpu_get_table(wam_ctx *ctx, const char *name, void *out, int64_t n)
     if (s == "KXLT") return put(t.KXLT);
     if (s == "DCO") return put(t.DCO);
     if (s == "INDEP") return put(t.INDEP);
+    if (s == "INDEP_G") return put(t.INDEP_G);
+    if (s == "DDPHI") return put(t.DDPHI);
+    if (s == "DDLAM") return put(t.DDLAM);
+    if (s == "DUPHI") return put(t.DUPHI);
+    if (s == "DULAM") return put(t.DULAM);
+    if (s == "DVPHI") return put(t.DVPHI);
+    if (s == "DVLAM") return put(t.DVLAM);
+    if (s == "DELLA0") return put(t.DELLA0);
+    if (s == "DRGM") return put(t.DRGM);
     if (s == "NCONTRIB") return put(t.NCONTRIB);
     if (s == "CONTRIB") return put(t.CONTRIB);
     if (s == "SN_CH") return put(t.SN_CH);

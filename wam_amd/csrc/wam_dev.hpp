@@ -65,6 +65,12 @@ struct DevTables {
     // per direction K (cyclic K+1, K-1): SINTH, COSTH, DELTH0*(SINTH(K)+SINTH(K+1)),
     // DELTH0*(SINTH(K)+SINTH(K-1)), SINTH**2, SINTH*COSTH, COSTH**2, 0
     const float *KDIR;
+    // grid geometry for the per-point part of PREPARE_PROPAGATION on the
+    // device: DCO and KXLT (1-based latitude row) per sea point, DELLAM and
+    // SINPH per latitude row, the latitude increment [rad]
+    const float *DCO_PT, *DELLAM_ROW, *SINPH_ROW;
+    const int *KXLT_PT;
+    float DELPHI;
 };
 
 struct ImplArgs {
@@ -80,6 +86,16 @@ struct ImplArgs {
 // row, each point's 4 bins contiguous (16 bytes), so the per-lane kernels
 // move 1 KB per wavefront access and the per-point kernel reads a point's
 // bins as float4 (KL % 4 == 0).
+// Inputs and outputs of prep_points_kernel (the per-point part of
+// PREPARE_PROPAGATION); u, v NULL = no current.
+struct PrepPoints {
+    const float *depth, *u, *v;
+    int *indep, *indep_g;
+    float *ddphi, *ddlam, *duphi, *dulam, *dvphi, *dvlam, *della0, *drgm;
+    unsigned *bad;                 // set where DEPTH is not positive
+    float deptha, lndepthd, delpro;
+};
+
 struct ImplScratch {
     int NB4;                       // KL*ML/4
     float *ft, *slt, *flt, *smt;   // F, SL, FL, SL - SPOS of the second SINPUT (LCFLX only)

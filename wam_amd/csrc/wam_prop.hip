@@ -167,6 +167,57 @@ __device__ __forceinline__ void coef_bin(const DevTables &t, int ij, int m, int 
     }
 }
 
+// The per-point part of PREPARE_PROPAGATION (wam_propagation_module.f90:
+// 279-598): INDEP (depth table index), GRADIENT's (:182-275) depth and
+// current gradients (centred differences over the N/S and E/W neighbours,
+// zero where one is land; the longitude derivative times DCO), INDEP_G (the
+// index of the depth averaged with the four neighbours), DELLA0 and DRGM.
+// The same operations, in the same order, as the host's prepare_points.
+__global__ void __launch_bounds__(256) prep_points_kernel(DevTables t, PrepPoints p)
+{
+    const int ij = blockIdx.x * 256 + threadIdx.x;
+    if (ij >= t.NSEA) return;
+    const int ND = t.NDEPTH;
+    const float d0 = p.depth[ij];
+    if (!(d0 > 0.f)) p.bad[0] = 1u;
+    auto index = [&](float dep) {
+        const int id = wam_nint(wam_logf(dep / p.deptha) / p.lndepthd + 1.f);
+        return id < 1 ? 1 : (id > ND ? ND : id);
+    };
+    p.indep[ij] = t.shallow ? index(d0) : 1;
+    const int n = t.KLAT_N[ij], s = t.KLAT_S[ij], e = t.KLON_E[ij], w = t.KLON_W[ij];   // 0-based, -1 = land
+    const int kx = t.KXLT_PT[ij] - 1;
+    const float dco = t.DCO_PT[ij], dlam = t.DELLAM_ROW[kx];
+    auto grad = [&](const float *f, float &dlat, float &dlon) {
+        dlat = (n >= 0 && s >= 0) ? (f[n] - f[s]) / (2.f * t.DELPHI) : 0.f;
+        dlon = (e >= 0 && w >= 0) ? (f[e] - f[w]) / (2.f * dlam) : 0.f;
+    };
+    float ddphi = 0.f, ddlam = 0.f, duphi = 0.f, dulam = 0.f, dvphi = 0.f, dvlam = 0.f;
+    int indep_g = 1;
+    if ((t.refr_d || t.refr_c) && t.shallow) {
+        grad(p.depth, ddphi, ddlam);
+        ddlam = ddlam * dco;
+        float dm = 4.f * d0;
+        int iz = 4;
+        if (n >= 0 && s >= 0) { dm = dm + p.depth[n] + p.depth[s]; iz += 2; }
+        if (e >= 0 && w >= 0) { dm = dm + p.depth[e] + p.depth[w]; iz += 2; }
+        dm = dm / (float)iz;
+        indep_g = index(dm);
+    }
+    if (t.refr_c) {
+        grad(p.u, duphi, dulam);
+        grad(p.v, dvphi, dvlam);
+        dulam = dulam * dco;
+        dvlam = dvlam * dco;
+    }
+    p.indep_g[ij] = indep_g;
+    p.ddphi[ij] = ddphi; p.ddlam[ij] = ddlam;
+    p.duphi[ij] = duphi; p.dulam[ij] = dulam; p.dvphi[ij] = dvphi; p.dvlam[ij] = dvlam;
+    const bool deep_nocur = !t.refr_c && !t.shallow;
+    p.della0[ij] = deep_nocur ? (p.delpro * dco) / dlam : ((0.5f * p.delpro) * dco) / dlam;
+    p.drgm[ij] = t.SINPH_ROW[kx] * dco;
+}
+
 __global__ void prep_coef_kernel(DevTables t, float *__restrict__ coef)
 {
     const int ij = blockIdx.x * blockDim.x + threadIdx.x;

@@ -84,12 +84,26 @@ class WamGpu:
         self._check(self.lib.wam_gpu_prepare_propagation(self.ctx, _ptr(d), _ptr(uu), _ptr(vv)))
         return int(self.scalar("COUNTER")), float(self.scalar("CFL_MAX"))
 
+    def prepare_propagation_dev(self, depth, u=None, v=None):
+        """PREPARE_PROPAGATION from fields already in HBM (torch tensors of
+        NSEA float32 on the context's device, or device pointers as ints)."""
+        def dptr(x):
+            if x is None:
+                return None
+            if isinstance(x, int):
+                return x
+            if x.dtype.itemsize != 4 or x.numel() != self.nsea or not x.is_contiguous():
+                raise ValueError("per-point fields must be contiguous float32 tensors of NSEA elements")
+            return x.data_ptr()
+        self._check(self.lib.wam_gpu_prepare_propagation_dev(self.ctx, dptr(depth), dptr(u), dptr(v)))
+        return int(self.scalar("COUNTER")), float(self.scalar("CFL_MAX"))
+
     def table(self, name):
         n = self.lib.wam_gpu_get_table(self.ctx, name.encode(), None, 0)
         if n < 0:
             raise KeyError(name)
         ints = name in ("IKP", "IKP1", "IKM", "IKM1", "K1W", "K2W", "K11W", "K21W", "INLCOEF", "IXLG", "KXLT",
-                        "INDEP", "KLAT", "KLON", "NCONTRIB")
+                        "INDEP", "INDEP_G", "KLAT", "KLON", "NCONTRIB")
         out = np.zeros(n, np.int32 if ints else np.float32)
         self.lib.wam_gpu_get_table(self.ctx, name.encode(), _ptr(out), n)
         return out
