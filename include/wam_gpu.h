@@ -35,7 +35,8 @@ enum {
     WAM_E_HIP = 2,        /* HIP runtime error (allocation, launch) */
     WAM_E_CFL = 3,        /* negative propagation weight (CHECK_CFL abort) */
     WAM_E_STATE = 4,      /* call order violated (e.g. IMPLSCH before PREPARE_PROPAGATION) */
-    WAM_E_GRID = 5        /* empty or invalid sea mask */
+    WAM_E_GRID = 5,       /* empty or invalid sea mask */
+    WAM_E_IO = 6          /* restart file: cannot be opened, read or written, or inconsistent with the grid */
 };
 
 /* Model options and grid definition.  Mirrors the module variables the
@@ -170,6 +171,35 @@ int wam_gpu_output_parameters_dev(wam_ctx *ctx, const float *d_fl3, const float 
 int wam_gpu_output_parameters(wam_ctx *ctx, int32_t nsea, int32_t kl, int32_t ml, const float *fl3, const float *u10,
                               const float *udir, const float *ustar, const float *tauw, const float *z0,
                               int32_t orient, float *out);
+
+/* The restart file, binary form: SAVE_RESTART_FILE
+   (wam_restart_module.f90:431-600) writes and CONNECT_RESTART (:109-327)
+   reads Fortran sequential unformatted records, 4-byte little-endian counts
+   around each record, records longer than 2 147 483 639 bytes split into
+   subrecords as by the Intel and GNU Fortran runtimes:
+     NSEA, CDTPRO, CDTSOU, CDA, CDTA, CDCA (INTEGER*4, 5 x CHARACTER*14);
+     U10; UDIR; TAUW (NSEA each); FL3 (NSEA, KL, ML);
+     DEPTH (NSEA) if CDTA is not blank; U, V (one record) if CDCA is not blank;
+   sea points in the order of one undivided domain.  dates holds the five
+   dates, 70 characters, blank-padded, no terminator.  A file whose NSEA
+   differs from the grid's is refused with WAM_E_IO (the reference aborts).
+   On read, *present gets bit 0 for a depth record, bit 1 for currents;
+   depth, u, v may be NULL to skip those records.
+
+   Device-resident forms: FL3, U10, UDIR, TAUW in HBM, staged through pinned
+   host buffers; the depth and currents written are the context's fields of
+   PREPARE_PROPAGATION. */
+int wam_gpu_save_restart(wam_ctx *ctx, const char *path, const char *dates, const float *d_fl3, const float *d_u10,
+                         const float *d_udir, const float *d_tauw);
+int wam_gpu_connect_restart(wam_ctx *ctx, const char *path, char *dates, float *d_fl3, float *d_u10, float *d_udir,
+                            float *d_tauw, float *d_depth, float *d_u, float *d_v, int32_t *present);
+
+/* Host-memory forms (no context, no device). */
+int wam_restart_write(const char *path, int32_t nsea, int32_t kl, int32_t ml, const char *dates, const float *fl3,
+                      const float *u10, const float *udir, const float *tauw, const float *depth, const float *u,
+                      const float *v);
+int wam_restart_read(const char *path, int32_t nsea, int32_t kl, int32_t ml, char *dates, float *fl3, float *u10,
+                     float *udir, float *tauw, float *depth, float *u, float *v, int32_t *present);
 
 /* Per-kernel timing on the context stream.  No counterpart in the
    reference (its timing is host wall-clock around whole calls).  on = 1

@@ -4,5 +4,5 @@ The compute path is libwamgpu.so (hand-written HIP for gfx950 behind the
 C-ABI in include/wam_gpu.h); this package holds its ctypes binding, the host
 mirror of the reference's call sequence, and seeded synthetic workloads.
 """
-from .model import WamError, WamGpu  # noqa: F401
+from .model import WamError, WamGpu, restart_read, restart_write  # noqa: F401
 from . import workloads  # noqa: F401

@@ -42,6 +42,10 @@ SIGNATURES = {
     "wam_gpu_set_stream": (_I, [_P, _P]),
     "wam_gpu_output_parameters_dev": (_I, [_P] * 7 + [_I, _P]),
     "wam_gpu_output_parameters": (_I, [_P, _I, _I, _I] + [_P] * 6 + [_I, _P]),
+    "wam_gpu_save_restart": (_I, [_P, ctypes.c_char_p, ctypes.c_char_p] + [_P] * 4),
+    "wam_gpu_connect_restart": (_I, [_P, ctypes.c_char_p, _P] + [_P] * 7 + [_P]),
+    "wam_restart_write": (_I, [ctypes.c_char_p, _I, _I, _I, ctypes.c_char_p] + [_P] * 7),
+    "wam_restart_read": (_I, [ctypes.c_char_p, _I, _I, _I, _P] + [_P] * 7 + [_P]),
     "wam_gpu_set_timing": (_I, [_P, _I]),
     "wam_gpu_get_timing": (_I, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                 ctypes.POINTER(ctypes.c_int64)]),

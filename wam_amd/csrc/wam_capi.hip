@@ -2,6 +2,7 @@
 // include/wam_gpu.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -11,6 +12,7 @@
 #include "../../include/wam_gpu.h"
 #include "wam_dev.hpp"
 #include "wam_math.h"
+#include "wam_restart.hpp"
 #include "wam_setup.hpp"
 
 namespace wam {
@@ -401,6 +403,172 @@ int wam_gpu_prepare_propagation_dev(wam_ctx *ctx, const float *d_depth, const fl
     HIPCHK(hipStreamSynchronize(st));
     if (hbad) { ctx->err = "depth must be positive at every sea point"; return WAM_E_ARG; }
     return finish_prepare(ctx);
+}
+
+// Restart records from and to device memory, through two pinned host buffers:
+// the copy of one piece overlaps the file transfer of the other.
+namespace {
+struct Staging {
+    static constexpr size_t CHUNK = (size_t)64 << 20;
+    void *buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~Staging()
+    {
+        for (int i = 0; i < 2; i++) {
+            if (ev[i]) (void)hipEventDestroy(ev[i]);
+            if (buf[i]) (void)hipHostFree(buf[i]);
+        }
+    }
+    int init()
+    {
+        for (int i = 0; i < 2; i++) {
+            if (hipHostMalloc(&buf[i], CHUNK) != hipSuccess) return WAM_E_HIP;
+            if (hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) != hipSuccess) return WAM_E_HIP;
+        }
+        return WAM_OK;
+    }
+};
+
+int put_dev(Staging &sg, hipStream_t st, wam::RecordWriter &w, const void *src, uint64_t n)
+{
+    const char *s = static_cast<const char *>(src);
+    const size_t C = Staging::CHUNK;
+    const uint64_t nch = (n + C - 1) / C;
+    if (nch == 0) return WAM_OK;
+    auto issue = [&](uint64_t i) {
+        const size_t k = (size_t)std::min<uint64_t>(C, n - i * C);
+        if (hipMemcpyAsync(sg.buf[i % 2], s + i * C, k, hipMemcpyDefault, st) != hipSuccess) return false;
+        return hipEventRecord(sg.ev[i % 2], st) == hipSuccess;
+    };
+    if (!issue(0)) return WAM_E_HIP;
+    for (uint64_t i = 0; i < nch; i++) {
+        if (i + 1 < nch && !issue(i + 1)) return WAM_E_HIP;
+        if (hipEventSynchronize(sg.ev[i % 2]) != hipSuccess) return WAM_E_HIP;
+        const size_t k = (size_t)std::min<uint64_t>(C, n - i * C);
+        if (!w.put(sg.buf[i % 2], k)) return WAM_E_IO;
+    }
+    return WAM_OK;
+}
+
+int get_dev(Staging &sg, hipStream_t st, wam::RecordReader &r, void *dst, uint64_t n)
+{
+    char *d = static_cast<char *>(dst);
+    const size_t C = Staging::CHUNK;
+    for (uint64_t i = 0; i * C < n; i++) {
+        const size_t k = (size_t)std::min<uint64_t>(C, n - i * C);
+        // the buffer's previous copy must be done before it is refilled
+        if (i >= 2 && hipEventSynchronize(sg.ev[i % 2]) != hipSuccess) return WAM_E_HIP;
+        if (!r.get(sg.buf[i % 2], k)) return WAM_E_IO;
+        if (hipMemcpyAsync(d + i * C, sg.buf[i % 2], k, hipMemcpyDefault, st) != hipSuccess) return WAM_E_HIP;
+        if (hipEventRecord(sg.ev[i % 2], st) != hipSuccess) return WAM_E_HIP;
+    }
+    return hipStreamSynchronize(st) == hipSuccess ? WAM_OK : WAM_E_HIP;
+}
+
+struct FileCloser {
+    FILE *f = nullptr;
+    ~FileCloser() { if (f) std::fclose(f); }
+};
+}  // namespace
+
+int wam_gpu_save_restart(wam_ctx *ctx, const char *path, const char *dates, const float *d_fl3, const float *d_u10,
+                         const float *d_udir, const float *d_tauw)
+{
+    if (!ctx || !path || !dates || !d_fl3 || !d_u10 || !d_udir || !d_tauw) return WAM_E_ARG;
+    const Tables &t = ctx->tab;
+    const bool with_depth = !wam::date_blank(dates + 42), with_cur = !wam::date_blank(dates + 56);
+    if ((with_depth || with_cur) && !ctx->prepared) {
+        ctx->err = "depth or currents requested before PREPARE_PROPAGATION";
+        return WAM_E_STATE;
+    }
+    Staging sg;
+    int rc = sg.init();
+    if (rc) { ctx->err = "pinned staging buffers"; return rc; }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    FileCloser fc;
+    fc.f = std::fopen(path, "wb");
+    if (!fc.f) { ctx->err = std::string("cannot open ") + path; return WAM_E_IO; }
+    wam::RecordWriter w(fc.f);
+    const int32_t nsea = t.NSEA;
+    char head[74];
+    std::memcpy(head, &nsea, 4);
+    std::memcpy(head + 4, dates, 70);
+    const uint64_t n = (uint64_t)nsea * 4, nb = n * (uint64_t)t.KL * (uint64_t)t.ML;
+    hipStream_t st = ctx->stream;
+    auto rec = [&](const void *p, uint64_t bytes) {
+        if (!w.begin(bytes)) return (int)WAM_E_IO;
+        int r = put_dev(sg, st, w, p, bytes);
+        if (r) return r;
+        return w.end() ? (int)WAM_OK : (int)WAM_E_IO;
+    };
+    if (!(w.begin(74) && w.put(head, 74) && w.end())) rc = WAM_E_IO;
+    if (!rc) rc = rec(d_u10, n);
+    if (!rc) rc = rec(d_udir, n);
+    if (!rc) rc = rec(d_tauw, n);
+    if (!rc) rc = rec(d_fl3, nb);
+    if (!rc && with_depth) rc = rec(ctx->dt.DEPTH, n);
+    if (!rc && with_cur) {
+        if (!w.begin(2 * n)) rc = WAM_E_IO;
+        if (!rc) rc = put_dev(sg, st, w, ctx->dt.U, n);
+        if (!rc) rc = put_dev(sg, st, w, ctx->dt.V, n);
+        if (!rc && !w.end()) rc = WAM_E_IO;
+    }
+    if (!rc) {
+        const int c = std::fclose(fc.f);
+        fc.f = nullptr;
+        if (c != 0) rc = WAM_E_IO;
+    }
+    if (rc) ctx->err = rc == WAM_E_IO ? std::string("cannot write ") + path : "HIP copy to the staging buffers";
+    return rc;
+}
+
+int wam_gpu_connect_restart(wam_ctx *ctx, const char *path, char *dates, float *d_fl3, float *d_u10, float *d_udir,
+                            float *d_tauw, float *d_depth, float *d_u, float *d_v, int32_t *present)
+{
+    if (!ctx || !path || !dates || !d_fl3 || !d_u10 || !d_udir || !d_tauw) return WAM_E_ARG;
+    const Tables &t = ctx->tab;
+    Staging sg;
+    int rc = sg.init();
+    if (rc) { ctx->err = "pinned staging buffers"; return rc; }
+    FileCloser fc;
+    fc.f = std::fopen(path, "rb");
+    if (!fc.f) { ctx->err = std::string("cannot open ") + path; return WAM_E_IO; }
+    wam::RecordReader r(fc.f);
+    char head[74];
+    if (!(r.begin(74) && r.get(head, 74) && r.end())) { ctx->err = "restart header record"; return WAM_E_IO; }
+    int32_t nsea_r;
+    std::memcpy(&nsea_r, head, 4);
+    if (nsea_r != t.NSEA) {
+        ctx->err = "restart fields are inconsistent with the model grid (NSEA)";
+        return WAM_E_IO;
+    }
+    std::memcpy(dates, head + 4, 70);
+    const bool with_depth = !wam::date_blank(dates + 42), with_cur = !wam::date_blank(dates + 56);
+    const uint64_t n = (uint64_t)t.NSEA * 4, nb = n * (uint64_t)t.KL * (uint64_t)t.ML;
+    hipStream_t st = ctx->stream;
+    auto rec = [&](void *p, uint64_t bytes) {
+        if (!r.begin(bytes)) return (int)WAM_E_IO;
+        int e = p ? get_dev(sg, st, r, p, bytes) : (r.skip(bytes) ? (int)WAM_OK : (int)WAM_E_IO);
+        if (e) return e;
+        return r.end() ? (int)WAM_OK : (int)WAM_E_IO;
+    };
+    rc = rec(d_u10, n);
+    if (!rc) rc = rec(d_udir, n);
+    if (!rc) rc = rec(d_tauw, n);
+    if (!rc) rc = rec(d_fl3, nb);
+    if (!rc && with_depth) rc = rec(d_depth, n);
+    if (!rc && with_cur) {
+        if (!r.begin(2 * n)) rc = WAM_E_IO;
+        if (!rc) rc = d_u && d_v ? get_dev(sg, st, r, d_u, n) : (r.skip(n) ? WAM_OK : WAM_E_IO);
+        if (!rc) rc = d_u && d_v ? get_dev(sg, st, r, d_v, n) : (r.skip(n) ? WAM_OK : WAM_E_IO);
+        if (!rc && !r.end()) rc = WAM_E_IO;
+    }
+    if (rc) {
+        ctx->err = rc == WAM_E_IO ? std::string("cannot read ") + path : "HIP copy from the staging buffers";
+        return rc;
+    }
+    if (present) *present = (with_depth ? 1 : 0) | (with_cur ? 2 : 0);
+    return WAM_OK;
 }
 
 int wam_gpu_propags_dev(wam_ctx *ctx, const float *d_in, float *d_out)

@@ -10,6 +10,7 @@ Errors raise ``WamError`` carrying the library's code and message, where the
 reference calls ABORT1.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -28,6 +29,64 @@ def _ptr(a):
     if isinstance(a, np.ndarray):
         return a.ctypes.data_as(ctypes.c_void_p)
     return ctypes.c_void_p(a.data_ptr())      # torch tensor
+
+
+def _dates(dates):
+    """The five restart dates (CDTPRO, CDTSOU, CDA, CDTA, CDCA) as the file's
+    70 blank-padded characters; a blank CDTA or CDCA leaves out the depth or
+    current record."""
+    if len(dates) != 5:
+        raise ValueError("five dates: CDTPRO, CDTSOU, CDA, CDTA, CDCA")
+    out = b""
+    for d in dates:
+        d = (d or "").encode("ascii")
+        if len(d) > 14:
+            raise ValueError("a date has at most 14 characters")
+        out += d.ljust(14, b" ")
+    return out
+
+
+def _undates(raw):
+    return [raw[14 * i:14 * i + 14].decode("ascii").rstrip(" ") for i in range(5)]
+
+
+def restart_write(path, dates, fl3, u10, udir, tauw, depth=None, u=None, v=None):
+    """SAVE_RESTART_FILE's binary form from host arrays (wam_restart_write);
+    fl3 is (NSEA, KL, ML) in Fortran order, i.e. a numpy array [ML][KL][NSEA]."""
+    from . import _lib
+    lib = _lib.load()
+    f = np.ascontiguousarray(fl3, np.float32)
+    if f.ndim != 3:
+        raise ValueError("fl3 must be [ML][KL][NSEA]")
+    ml, kl, n = f.shape
+    a = [np.ascontiguousarray(x, np.float32) if x is not None else None for x in (u10, udir, tauw, depth, u, v)]
+    rc = lib.wam_restart_write(os.fsencode(path), n, kl, ml, _dates(dates), _ptr(f), *[_ptr(x) for x in a])
+    if rc:
+        raise WamError(rc, f"wam_restart_write({path})")
+
+
+def restart_read(path, nsea, kl, ml, depth=True, current=True):
+    """CONNECT_RESTART's binary form into host arrays (wam_restart_read):
+    returns the dates and a dict of fl3 [ML][KL][NSEA], u10, udir, tauw and,
+    when the file holds them, depth, u, v."""
+    from . import _lib
+    lib = _lib.load()
+    out = dict(fl3=np.empty((ml, kl, nsea), np.float32), u10=np.empty(nsea, np.float32),
+               udir=np.empty(nsea, np.float32), tauw=np.empty(nsea, np.float32),
+               depth=np.empty(nsea, np.float32), u=np.empty(nsea, np.float32), v=np.empty(nsea, np.float32))
+    raw = ctypes.create_string_buffer(70)
+    present = ctypes.c_int32(0)
+    rc = lib.wam_restart_read(os.fsencode(path), nsea, kl, ml, raw, _ptr(out["fl3"]), _ptr(out["u10"]),
+                              _ptr(out["udir"]), _ptr(out["tauw"]), _ptr(out["depth"]) if depth else None,
+                              _ptr(out["u"]) if current else None, _ptr(out["v"]) if current else None,
+                              ctypes.byref(present))
+    if rc:
+        raise WamError(rc, f"wam_restart_read({path})")
+    if not (present.value & 1) or not depth:
+        out.pop("depth")
+    if not (present.value & 2) or not current:
+        out.pop("u"), out.pop("v")
+    return _undates(raw.raw), out
 
 
 class WamGpu:
@@ -130,6 +189,22 @@ class WamGpu:
         p = lambda x: ctypes.c_void_p(x.data_ptr())
         self._check(self.lib.wam_gpu_output_parameters_dev(self.ctx, p(fl3), p(u10), p(udir), p(ustar), p(tauw), p(z0),
                                                            int(orient), p(out)))
+
+    def save_restart(self, path, dates, fl3, u10, udir, tauw):
+        """SAVE_RESTART_FILE from device tensors (wam_gpu_save_restart); the
+        depth and current records are the context's prepared fields."""
+        self._check(self.lib.wam_gpu_save_restart(self.ctx, os.fsencode(path), _dates(dates), _ptr(fl3), _ptr(u10),
+                                                  _ptr(udir), _ptr(tauw)))
+
+    def connect_restart(self, path, fl3, u10, udir, tauw, depth=None, u=None, v=None):
+        """CONNECT_RESTART into device tensors (wam_gpu_connect_restart);
+        returns the dates and which optional records the file held."""
+        raw = ctypes.create_string_buffer(70)
+        present = ctypes.c_int32(0)
+        self._check(self.lib.wam_gpu_connect_restart(self.ctx, os.fsencode(path), raw, _ptr(fl3), _ptr(u10),
+                                                     _ptr(udir), _ptr(tauw), _ptr(depth), _ptr(u), _ptr(v),
+                                                     ctypes.byref(present)))
+        return _undates(raw.raw), dict(depth=bool(present.value & 1), current=bool(present.value & 2))
 
     def set_timing(self, on):
         """Records a HIP event pair around every PROPAGS / IMPLSCH kernel
