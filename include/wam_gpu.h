@@ -36,7 +36,7 @@ enum {
     WAM_E_CFL = 3,        /* negative propagation weight (CHECK_CFL abort) */
     WAM_E_STATE = 4,      /* call order violated (e.g. IMPLSCH before PREPARE_PROPAGATION) */
     WAM_E_GRID = 5,       /* empty or invalid sea mask */
-    WAM_E_IO = 6          /* restart file: cannot be opened, read or written, or inconsistent with the grid */
+    WAM_E_IO = 6          /* restart or boundary file: cannot be opened, read or written, or inconsistent with the grid */
 };
 
 /* Model options and grid definition.  Mirrors the module variables the
@@ -200,6 +200,53 @@ int wam_restart_write(const char *path, int32_t nsea, int32_t kl, int32_t ml, co
                       const float *v);
 int wam_restart_read(const char *path, int32_t nsea, int32_t kl, int32_t ml, char *dates, float *fl3, float *u10,
                      float *udir, float *tauw, float *depth, float *u, float *v, int32_t *present);
+
+/* Nest boundary exchange.  A spectrum is F(1:KL, 1:ML), KL fastest; mean
+   values are [3][n]: EMEAN, THQ [rad], FMEAN [Hz], the order of the boundary
+   file.  Indices are 1-based, as in the reference.  KL, ML <= 64.
+
+   BOUNDARY_OUTPUT (wam_boundary_module.f90:291-397, one undivided domain):
+   fbc[nb][ML][KL] = FL3(IJARC(i), :, :) and mean[3][nb] from TOTAL_ENERGY,
+   MEAN_DIRECTION (THQ) and FEMEAN (FM) of each, into host memory.  IJARC
+   outside 1..NSEA is refused (WAM_E_ARG). */
+int wam_gpu_boundary_output_dev(wam_ctx *ctx, int32_t nb, const int32_t *ijarc, const float *d_fl3, float *fbc,
+                                float *mean);
+int wam_gpu_boundary_output(wam_ctx *ctx, int32_t nsea, int32_t kl, int32_t ml, const float *fl3, int32_t nb,
+                            const int32_t *ijarc, float *fbc, float *mean);
+
+/* BOUNDARY_INPUT (wam_boundary_module.f90:168-287) after its reads: the time
+   step takes the first set f1, m1 (which = 1: CDTPRO = CDATE1), the second
+   f2, m2 (which = 2: CDTPRO = CDATE2 or CDATE1 blank) or INTSPEC
+   (wam_interface_module.f90:290-408) between them at DEL1L seconds of DEL12
+   (which = 0); then FL3(IJARF(i), :, :) = INTSPEC(1, BFW(i), FI(IBFL(i)),
+   FI(IBFR(i))) for i = 1..NBOUNF in FL3 in HBM (the last entry of a sea point
+   listed twice stays).  f1, f2 are [nbinp][ML][KL], m1, m2 [3][nbinp]; the
+   set not taken may be NULL.  fi [nbinp][ML][KL] and mi [3][nbinp], the set at
+   the model time, are returned when not NULL.  Indices outside 1..NSEA or
+   1..NBINP are refused (WAM_E_ARG).  Synchronous. */
+int wam_gpu_boundary_input_dev(wam_ctx *ctx, int32_t nbinp, const float *f1, const float *m1, const float *f2,
+                               const float *m2, int32_t which, float del12, float del1l, int32_t nbounf,
+                               const int32_t *ijarf, const int32_t *ibfl, const int32_t *ibfr, const float *bfw,
+                               float *d_fl3, float *fi, float *mi);
+int wam_gpu_boundary_input(wam_ctx *ctx, int32_t nsea, int32_t kl, int32_t ml, float *fl3, int32_t nbinp,
+                           const float *f1, const float *m1, const float *f2, const float *m2, int32_t which,
+                           float del12, float del1l, int32_t nbounf, const int32_t *ijarf, const int32_t *ibfl,
+                           const int32_t *ibfr, const float *bfw, float *fi, float *mi);
+
+/* The boundary value file, binary form (no context, no device).
+   SAVE_BOUNDARY_FILE (wam_boundary_module.f90:709-808) writes the header
+   record hdr[8] = KL, ML, TH(1), FR(1), CO, NBOUNC, IDEL_B_OUT, IDEL_BO_FILE;
+   BOUNDARY_OUTPUT (:364-378) appends per point the record BLNGC, BLATC
+   (INTEGER, M_SEC), CDTPRO (CHARACTER*14), EMEAN, THQ, FMEAN, DEPTH and the
+   record FBC(1:KL, 1:ML); READ_BOUNDARY_INPUT (chief/read_boundary_input.f90
+   :66-195) reads them.  Reading carries the byte offset *pos from one call
+   to the next; dates gets 14 characters per point. */
+int wam_boundary_write_header(const char *path, const float *hdr);
+int wam_boundary_append(const char *path, int32_t nb, int32_t kl, int32_t ml, const int32_t *lon, const int32_t *lat,
+                        const char *cdtpro, const float *mean, const float *depth, const float *fbc);
+int wam_boundary_read_header(const char *path, float *hdr, int64_t *pos);
+int wam_boundary_read(const char *path, int64_t *pos, int32_t nb, int32_t kl, int32_t ml, int32_t *lon, int32_t *lat,
+                      char *dates, float *mean, float *depth, float *fbc);
 
 /* Per-kernel timing on the context stream.  No counterpart in the
    reference (its timing is host wall-clock around whole calls).  on = 1

@@ -66,6 +66,13 @@ def _lib(fp64, opcount=False):
         lib.orc_api_wavnu2.argtypes = [ctypes.c_void_p] * 4
         lib.orc_api_output_parameters.argtypes = [ctypes.c_void_p] * 10 + [ctypes.c_int, ctypes.c_void_p]
         lib.orc_api_snonlin.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_double, ctypes.c_double]
+        lib.orc_api_rotspec.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_double]
+        lib.orc_api_strspec.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_double]
+        lib.orc_api_intspec.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double] + [ctypes.c_void_p] * 6
+        lib.orc_api_boundary_output.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 3
+        lib.orc_api_boundary_input.argtypes = ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 4 +
+                                               [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int] +
+                                               [ctypes.c_void_p] * 6)
         lib.orc_api_math.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long]
         lib.orc_api_aki.restype = ctypes.c_double
         lib.orc_api_aki.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
@@ -193,6 +200,57 @@ class Oracle:
         out = np.zeros((17, n), self.dtype)
         self.lib.orc_api_output_parameters(self.ctx, *[_p(x) for x in args], int(orient), _p(out))
         return out
+
+    def rotspec(self, f, rthet):
+        """ROTSPEC of F[ML][KL] by rthet radians."""
+        f = np.ascontiguousarray(f, dtype=self.dtype)
+        out = np.zeros_like(f)
+        self.lib.orc_api_rotspec(self.ctx, _p(f), _p(out), float(rthet))
+        return out
+
+    def strspec(self, f, gamma):
+        """STRSPEC of F[ML][KL] by the factor gamma."""
+        f = np.ascontiguousarray(f, dtype=self.dtype)
+        out = np.zeros_like(f)
+        self.lib.orc_api_strspec(self.ctx, _p(f), _p(out), float(gamma))
+        return out
+
+    def intspec(self, del12, del1l, f1, m1, f2, m2):
+        """INTSPEC (orc_nest.c) of two spectra F[ML][KL] with their (EMEAN,
+        THQ, FMEAN); returns the spectrum and its (EMEAN, THQ, FMEAN)."""
+        a = lambda x: np.ascontiguousarray(x, dtype=self.dtype)
+        f1, m1, f2, m2 = a(f1), a(m1), a(f2), a(m2)
+        fl = np.zeros_like(f1)
+        mo = np.zeros(3, self.dtype)
+        self.lib.orc_api_intspec(self.ctx, float(del12), float(del1l), _p(f1), _p(m1), _p(f2), _p(m2), _p(fl), _p(mo))
+        return fl, mo
+
+    def boundary_output(self, fl3, ijarc):
+        """BOUNDARY_OUTPUT's spectra and mean parameters at the 1-based sea
+        points ijarc: FBC[nb][ML][KL] and [3][nb] (EMEAN, THQ, FMEAN)."""
+        fl3 = np.ascontiguousarray(fl3, dtype=self.dtype)
+        ij = np.ascontiguousarray(ijarc, dtype=np.int32)
+        nb = ij.size
+        fbc = np.zeros((nb, self.ml, self.kl), self.dtype)
+        mean = np.zeros((3, nb), self.dtype)
+        self.lib.orc_api_boundary_output(self.ctx, _p(fl3), nb, _p(ij), _p(fbc), _p(mean))
+        return fbc, mean
+
+    def boundary_input(self, fl3, f1, m1, f2, m2, which, del12, del1l, ijarf, ibfl, ibfr, bfw):
+        """BOUNDARY_INPUT's time then space interpolation into a copy of FL3
+        [ML][KL][NSEA]; returns (FL3, FI[nbinp][ML][KL], MI[3][nbinp])."""
+        a = lambda x: np.ascontiguousarray(x, dtype=self.dtype)
+        i = lambda x: np.ascontiguousarray(x, dtype=np.int32)
+        out = np.array(fl3, dtype=self.dtype, copy=True, order="C")
+        f1, m1, f2, m2, bfw = a(f1), a(m1), a(f2), a(m2), a(bfw)
+        ijarf, ibfl, ibfr = i(ijarf), i(ibfl), i(ibfr)
+        nbinp = m1.shape[-1]
+        fi = np.zeros((nbinp, self.ml, self.kl), self.dtype)
+        mi = np.zeros((3, nbinp), self.dtype)
+        self.lib.orc_api_boundary_input(self.ctx, _p(out), nbinp, _p(f1), _p(m1), _p(f2), _p(m2), int(which),
+                                        float(del12), float(del1l), ijarf.size, _p(ijarf), _p(ibfl), _p(ibfr),
+                                        _p(bfw), _p(fi), _p(mi))
+        return out, fi, mi
 
     def wavnu2(self, depth):
         d = np.ascontiguousarray(depth, dtype=self.dtype)

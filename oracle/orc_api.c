@@ -68,7 +68,7 @@ double orc_api_scalar(const orc_ctx *c, const char *name)
         {"ISP", c->ISP}, {"ISM", c->ISM}, {"MFRSTLW", c->MFRSTLW}, {"MLSTHG", c->MLSTHG}, {"KFRH", c->KFRH},
         {"X0TAUHF", c->X0TAUHF}, {"DELPHI", c->DELPHI}, {"NSEA", c->NSEA}, {"DELU", c->DELU},
         {"COUNTER", c->counter}, {"CFL_MAX", c->cfl_max}, {"CFL_MIN", c->cfl_min}, {"NCOEF", c->ncoef},
-        {"ALPHA", c->ALPHA}, {"JUMAX", c->JUMAX},
+        {"ALPHA", c->ALPHA}, {"JUMAX", c->JUMAX}, {"CO", c->CO}, {"EMIN", c->EMIN},
     };
     for (size_t i = 0; i < sizeof(tab) / sizeof(tab[0]); i++)
         if (!strcmp(tab[i].nm, name)) return tab[i].v;

@@ -46,6 +46,15 @@ SIGNATURES = {
     "wam_gpu_connect_restart": (_I, [_P, ctypes.c_char_p, _P] + [_P] * 7 + [_P]),
     "wam_restart_write": (_I, [ctypes.c_char_p, _I, _I, _I, ctypes.c_char_p] + [_P] * 7),
     "wam_restart_read": (_I, [ctypes.c_char_p, _I, _I, _I, _P] + [_P] * 7 + [_P]),
+    "wam_gpu_boundary_output_dev": (_I, [_P, _I, _P, _P, _P, _P]),
+    "wam_gpu_boundary_output": (_I, [_P, _I, _I, _I, _P, _I, _P, _P, _P]),
+    "wam_gpu_boundary_input_dev": (_I, [_P, _I] + [_P] * 4 + [_I, ctypes.c_float, ctypes.c_float, _I] + [_P] * 7),
+    "wam_gpu_boundary_input": (_I, [_P, _I, _I, _I, _P, _I] + [_P] * 4 + [_I, ctypes.c_float, ctypes.c_float, _I] +
+                               [_P] * 6),
+    "wam_boundary_write_header": (_I, [ctypes.c_char_p, _P]),
+    "wam_boundary_append": (_I, [ctypes.c_char_p, _I, _I, _I, _P, _P, ctypes.c_char_p, _P, _P, _P]),
+    "wam_boundary_read_header": (_I, [ctypes.c_char_p, _P, ctypes.POINTER(ctypes.c_int64)]),
+    "wam_boundary_read": (_I, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64), _I, _I, _I] + [_P] * 6),
     "wam_gpu_set_timing": (_I, [_P, _I]),
     "wam_gpu_get_timing": (_I, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                 ctypes.POINTER(ctypes.c_int64)]),

@@ -38,6 +38,12 @@ bool implsch_split_supported(const DevTables &t);
 hipError_t launch_output_params(const DevTables &t, const float *fl3, const float *u10, const float *udir,
                                 const float *ustar, const float *tauw, const float *z0, int orient, float *out,
                                 hipStream_t st);
+hipError_t launch_boundary_gather(const DevTables &t, const float *fl3, const int *ijarc, int nb, float *fbc,
+                                  float *mean, hipStream_t st);
+hipError_t launch_boundary_input(const DevTables &t, int nbinp, const float *f1, const float *m1, const float *f2,
+                                 const float *m2, int which, float del12, float del1l, float *fi, float *mi,
+                                 int nbounf, const int *ijarf, const int *ibfl, const int *ibfr, const float *bfw,
+                                 float *fl3, hipStream_t st);
 }  // namespace wam
 
 using namespace wam;
@@ -63,6 +69,8 @@ struct wam_ctx {
     std::vector<Timed> timed;
     float *stage = nullptr;               // host drop-in staging: FL3 + per-point fields
     size_t stage_n = 0;
+    float *nest = nullptr;                // nest boundary spectra, mean values and indices
+    size_t nest_n = 0;
     bool prepared = false;
     int ncu = 256;                 // compute units of the device
     std::string err;
@@ -775,6 +783,135 @@ int wam_gpu_output_parameters(wam_ctx *ctx, int32_t nsea, int32_t kl, int32_t ml
     return WAM_OK;
 }
 
+static int ensure_nest(wam_ctx *ctx, size_t n)
+{
+    if (ctx->nest_n >= n) return WAM_OK;
+    if (ctx->nest) {
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        (void)hipFree(ctx->nest);
+    }
+    ctx->nest = nullptr;
+    ctx->nest_n = 0;
+    HIPCHK(hipMalloc(&ctx->nest, n * sizeof(float)));
+    ctx->nest_n = n;
+    return WAM_OK;
+}
+
+int wam_gpu_boundary_output_dev(wam_ctx *ctx, int32_t nb, const int32_t *ijarc, const float *d_fl3, float *fbc,
+                                float *mean)
+{
+    if (!ctx || nb < 0 || (nb > 0 && (!ijarc || !d_fl3 || !fbc || !mean))) return WAM_E_ARG;
+    const Tables &t = ctx->tab;
+    if (t.KL > 64 || t.ML > 64) { ctx->err = "nest boundary exchange needs KL, ML <= 64"; return WAM_E_ARG; }
+    for (int32_t i = 0; i < nb; i++)
+        if (ijarc[i] < 1 || ijarc[i] > t.NSEA) { ctx->err = "IJARC outside 1..NSEA"; return WAM_E_ARG; }
+    if (nb == 0) return WAM_OK;
+    const size_t NB = (size_t)t.KL * t.ML, n = (size_t)nb;
+    int rc = ensure_nest(ctx, n * NB + 4 * n);
+    if (rc) return rc;
+    float *dfbc = ctx->nest, *dmean = dfbc + n * NB;
+    int *dij = reinterpret_cast<int *>(dmean + 3 * n);
+    hipStream_t st = ctx->stream;
+    HIPCHK(hipMemcpyAsync(dij, ijarc, n * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(launch_boundary_gather(ctx->dt, d_fl3, dij, nb, dfbc, dmean, st));
+    HIPCHK(hipMemcpyAsync(fbc, dfbc, n * NB * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(mean, dmean, 3 * n * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return WAM_OK;
+}
+
+int wam_gpu_boundary_input_dev(wam_ctx *ctx, int32_t nbinp, const float *f1, const float *m1, const float *f2,
+                               const float *m2, int32_t which, float del12, float del1l, int32_t nbounf,
+                               const int32_t *ijarf, const int32_t *ibfl, const int32_t *ibfr, const float *bfw,
+                               float *d_fl3, float *fi, float *mi)
+{
+    if (!ctx || nbinp <= 0 || nbounf < 0 || which < 0 || which > 2) return WAM_E_ARG;
+    if (which != 2 && (!f1 || !m1)) return WAM_E_ARG;
+    if (which != 1 && (!f2 || !m2)) return WAM_E_ARG;
+    if (nbounf > 0 && (!ijarf || !ibfl || !ibfr || !bfw || !d_fl3)) return WAM_E_ARG;
+    const Tables &t = ctx->tab;
+    for (int32_t i = 0; i < nbounf; i++) {
+        if (ijarf[i] < 1 || ijarf[i] > t.NSEA) { ctx->err = "IJARF outside 1..NSEA"; return WAM_E_ARG; }
+        if (ibfl[i] < 1 || ibfl[i] > nbinp || ibfr[i] < 1 || ibfr[i] > nbinp) {
+            ctx->err = "IBFL or IBFR outside 1..NBINP";
+            return WAM_E_ARG;
+        }
+    }
+    // the reference stores the points one after another: of a sea point
+    // listed twice the last entry stays; the kernel writes each point once
+    std::vector<int32_t> last(nbounf > 0 ? t.NSEA : 0, -1);
+    for (int32_t i = 0; i < nbounf; i++) last[ijarf[i] - 1] = i;
+    std::vector<int32_t> ij, il, ir;
+    std::vector<float> w;
+    for (int32_t i = 0; i < nbounf; i++)
+        if (last[ijarf[i] - 1] == i) {
+            ij.push_back(ijarf[i]);
+            il.push_back(ibfl[i]);
+            ir.push_back(ibfr[i]);
+            w.push_back(bfw[i]);
+        }
+    const size_t NB = (size_t)t.KL * t.ML, np = (size_t)nbinp, nf = ij.size();
+    int rc = ensure_nest(ctx, 3 * np * NB + 9 * np + 4 * nf);
+    if (rc) return rc;
+    float *df1 = ctx->nest, *df2 = df1 + np * NB, *dfi = df2 + np * NB, *dm1 = dfi + np * NB, *dm2 = dm1 + 3 * np,
+          *dmi = dm2 + 3 * np, *dw = dmi + 3 * np;
+    int *dij = reinterpret_cast<int *>(dw + nf), *dl = dij + nf, *dr = dl + nf;
+    hipStream_t st = ctx->stream;
+    if (which != 2) {
+        HIPCHK(hipMemcpyAsync(df1, f1, np * NB * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dm1, m1, 3 * np * 4, hipMemcpyHostToDevice, st));
+    }
+    if (which != 1) {
+        HIPCHK(hipMemcpyAsync(df2, f2, np * NB * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dm2, m2, 3 * np * 4, hipMemcpyHostToDevice, st));
+    }
+    if (nf > 0) {
+        HIPCHK(hipMemcpyAsync(dw, w.data(), nf * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dij, ij.data(), nf * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dl, il.data(), nf * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dr, ir.data(), nf * 4, hipMemcpyHostToDevice, st));
+    }
+    HIPCHK(launch_boundary_input(ctx->dt, nbinp, df1, dm1, df2, dm2, which, del12, del1l, dfi, dmi, (int)nf, dij, dl,
+                                 dr, dw, d_fl3, st));
+    if (fi) HIPCHK(hipMemcpyAsync(fi, dfi, np * NB * 4, hipMemcpyDeviceToHost, st));
+    if (mi) HIPCHK(hipMemcpyAsync(mi, dmi, 3 * np * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));     // the host vectors above are read by the copies
+    return WAM_OK;
+}
+
+int wam_gpu_boundary_output(wam_ctx *ctx, int32_t nsea, int32_t kl, int32_t ml, const float *fl3, int32_t nb,
+                            const int32_t *ijarc, float *fbc, float *mean)
+{
+    if (!ctx || !fl3) return WAM_E_ARG;
+    const Tables &t = ctx->tab;
+    if (nsea != t.NSEA || kl != t.KL || ml != t.ML) { ctx->err = "boundary output: size mismatch"; return WAM_E_ARG; }
+    const size_t nbyte = (size_t)t.KL * t.ML * nsea * 4;
+    int rc = ensure_stage(ctx, nbyte / 4);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(ctx->stage, fl3, nbyte, hipMemcpyHostToDevice, ctx->stream));
+    return wam_gpu_boundary_output_dev(ctx, nb, ijarc, ctx->stage, fbc, mean);
+}
+
+int wam_gpu_boundary_input(wam_ctx *ctx, int32_t nsea, int32_t kl, int32_t ml, float *fl3, int32_t nbinp,
+                           const float *f1, const float *m1, const float *f2, const float *m2, int32_t which,
+                           float del12, float del1l, int32_t nbounf, const int32_t *ijarf, const int32_t *ibfl,
+                           const int32_t *ibfr, const float *bfw, float *fi, float *mi)
+{
+    if (!ctx || !fl3) return WAM_E_ARG;
+    const Tables &t = ctx->tab;
+    if (nsea != t.NSEA || kl != t.KL || ml != t.ML) { ctx->err = "boundary input: size mismatch"; return WAM_E_ARG; }
+    const size_t nbyte = (size_t)t.KL * t.ML * nsea * 4;
+    int rc = ensure_stage(ctx, nbyte / 4);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(ctx->stage, fl3, nbyte, hipMemcpyHostToDevice, ctx->stream));
+    rc = wam_gpu_boundary_input_dev(ctx, nbinp, f1, m1, f2, m2, which, del12, del1l, nbounf, ijarf, ibfl, ibfr, bfw,
+                                    ctx->stage, fi, mi);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(fl3, ctx->stage, nbyte, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return WAM_OK;
+}
+
 int wam_gpu_set_propagation_weights(wam_ctx *ctx, int32_t stored)
 {
     if (!ctx) return WAM_E_ARG;
@@ -973,6 +1110,7 @@ void wam_gpu_finalize(wam_ctx *ctx)
     free_list(ctx->dev_allocs);
     if (ctx->fscratch) (void)hipFree(ctx->fscratch);
     if (ctx->stage) (void)hipFree(ctx->stage);
+    if (ctx->nest) (void)hipFree(ctx->nest);
     timing_clear(ctx);
     for (void *p : {(void *)ctx->isx.ft, (void *)ctx->isx.slt, (void *)ctx->isx.flt, (void *)ctx->isx.smt,
                     (void *)ctx->isx.axy, (void *)ctx->isx.pp, (void *)ctx->isx.llws, (void *)ctx->isx.tick})

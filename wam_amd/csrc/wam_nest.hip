@@ -1,0 +1,230 @@
+// wam_nest.hip -- the nest boundary exchange on gfx950.
+//
+//  * BOUNDARY_OUTPUT (wam_boundary_module.f90:291-397): the spectra of the
+//    coarse grid at the nest's boundary points (gathered from FL3 in HBM)
+//    with TOTAL_ENERGY, MEAN_DIRECTION's THQ and FEMEAN's FM of each;
+//  * BOUNDARY_INPUT (:168-287): the time interpolation of two sets of
+//    boundary spectra and the space interpolation into FL3 at the fine grid's
+//    boundary points, both by INTSPEC (wam_interface_module.f90:290-408) with
+//    ROTSPEC (:891-941) and STRSPEC (:945-1006).
+//
+// One workgroup per spectrum.  INTSPEC's scalars (weights, mean values, the
+// shift and stretch of either spectrum) are the same in every lane; each
+// output value is then an independent function of at most eight input values
+// (two directions x two frequencies x two spectra), so the lanes stride over
+// the KL*ML values with no exchange.  Every product and sum is rounded as the
+// reference writes it (hipcc -ffp-contract=off).  The sums of the mean
+// parameters run serially in the reference's order, one lane per frequency
+// or direction.  A spectrum is F(K, M) at f[M*KL + K].
+#include "wam_dev.hpp"
+#include "wam_math.h"
+
+namespace wam {
+
+constexpr int NEST_WG = 256;
+constexpr float EPSILON_1 = 1.1920929e-7f;       // EPSILON(1.)
+
+// ROTSPEC's rotation and STRSPEC's stretch of one spectrum, and its weight
+struct Adjust {
+    int sh_r, inc_s, mode;       // sh_r: ROTSPEC's INC modulo KL; mode 0: GAMMA = 1, 1: shifted out of range, 2: stretch
+    float adif_r, bdif_r, adif_s, bdif_s, w;
+};
+
+__device__ Adjust make_adjust(const DevTables &t, float rthet, float gamma, float gw, float em, float emk)
+{
+    Adjust a;
+    float ad = rthet * (float)t.KL / t.ZPI;
+    const int inc = -(int)wam_floorf(ad);
+    ad = ad + (float)inc;
+    a.sh_r = inc % t.KL;
+    if (a.sh_r < 0) a.sh_r += t.KL;
+    a.adif_r = ad;
+    a.bdif_r = 1.f - ad;
+    a.inc_s = 0;
+    a.adif_s = a.bdif_s = 0.f;
+    if (gamma == 1.0f) {
+        a.mode = 0;
+    } else {
+        a.inc_s = (int)wam_floorf(wam_log10f(gamma) / wam_log10f(t.CO));
+        if (abs(a.inc_s) >= t.ML - 1) {
+            a.mode = 1;
+        } else {
+            a.mode = 2;
+            // CO**(-INC); a negative power as the reciprocal of the positive one
+            const float p = a.inc_s <= 0 ? wam_ipowf(t.CO, -a.inc_s) : 1.f / wam_ipowf(t.CO, a.inc_s);
+            a.adif_s = (t.CO - gamma * p) / (t.CO - 1.f);
+            a.bdif_s = 1.f - a.adif_s;
+        }
+    }
+    a.w = (gw * em) / emk;
+    return a;
+}
+
+// BDIF*CSHIFT(F, INC) + ADIF*CSHIFT(F, INC-1) at (k, m)
+__device__ __forceinline__ float rot_at(const float *f, const Adjust &a, int KL, int k, int m)
+{
+    int k0 = k + a.sh_r;
+    if (k0 >= KL) k0 -= KL;
+    const int k1 = k0 == 0 ? KL - 1 : k0 - 1;
+    return a.bdif_r * f[m * KL + k0] + a.adif_r * f[m * KL + k1];
+}
+
+// STRSPEC(ROTSPEC(F)) at (k, m)
+__device__ float adjusted(const float *f, const Adjust &a, int KL, int ML, int k, int m)
+{
+    if (a.mode == 0) return rot_at(f, a, KL, k, m);
+    if (a.mode == 1) return 0.f;
+    const int s = a.inc_s;
+    if (s >= 0) {
+        if (m < ML - s - 1) return a.adif_s * rot_at(f, a, KL, k, m + s) + a.bdif_s * rot_at(f, a, KL, k, m + s + 1);
+        if (m == ML - s - 1) return a.adif_s * rot_at(f, a, KL, k, ML - 1);
+        return 0.f;
+    }
+    if (m >= -s) return a.adif_s * rot_at(f, a, KL, k, m + s) + a.bdif_s * rot_at(f, a, KL, k, m + s + 1);
+    if (m == -s - 1) return a.bdif_s * rot_at(f, a, KL, k, 0);
+    return 0.f;
+}
+
+// INTSPEC by one workgroup: out[i*os] for i = M*KL + K; the mean values
+// (EMEAN, THQ, FMEAN) to mo[0], mo[ms], mo[2*ms] when mo is given.
+__device__ void intspec_wg(const DevTables &t, float del12, float del1l, const float *f1, float em1, float th1,
+                           float fm1, const float *f2, float em2, float th2, float fm2, float *out, size_t os,
+                           float *mo, int ms)
+{
+    const int KL = t.KL, ML = t.ML, NB = KL * ML, l = threadIdx.x;
+    const float gw2 = del1l / del12, gw1 = 1.f - gw2;
+    if (wam_fabsf(gw1) < EPSILON_1 || em1 < EPSILON_1) {
+        for (int i = l; i < NB; i += NEST_WG) out[i * os] = gw2 * f2[i];
+        if (mo && l == 0) { mo[0] = gw2 * em2; mo[ms] = gw2 * th2; mo[2 * ms] = gw2 * fm2; }
+        return;
+    }
+    if (wam_fabsf(gw2) < EPSILON_1 || em2 < EPSILON_1) {
+        for (int i = l; i < NB; i += NEST_WG) out[i * os] = gw1 * f1[i];
+        if (mo && l == 0) { mo[0] = gw1 * em1; mo[ms] = gw1 * th1; mo[2 * ms] = gw1 * fm1; }
+        return;
+    }
+    const float em = gw1 * em1 + gw2 * em2;
+    const float fm = gw1 * fm1 + gw2 * fm2;
+    const float th = wam_atan2f(gw1 * wam_sinf(th1) + gw2 * wam_sinf(th2), gw1 * wam_cosf(th1) + gw2 * wam_cosf(th2));
+    const Adjust a = make_adjust(t, th - th1, fm1 / fm, gw1, em, em1);
+    const Adjust b = make_adjust(t, th - th2, fm2 / fm, gw2, em, em2);
+    for (int i = l; i < NB; i += NEST_WG) {
+        const int m = i / KL, k = i - m * KL;
+        out[i * os] = a.w * adjusted(f1, a, KL, ML, k, m) + b.w * adjusted(f2, b, KL, ML, k, m);
+    }
+    if (mo && l == 0) { mo[0] = em; mo[ms] = th; mo[2 * ms] = fm; }
+}
+
+// BOUNDARY_OUTPUT: FBC(ib) = FL3(IJARC(ib), :, :) and its mean parameters
+// mean[ib], mean[nb + ib], mean[2 nb + ib] = EMEAN, THQ, FMEAN.
+__global__ void __launch_bounds__(NEST_WG) boundary_gather_kernel(DevTables t, const float *__restrict__ fl3,
+                                                                  const int *__restrict__ ijarc, int nb,
+                                                                  float *__restrict__ fbc, float *__restrict__ mean)
+{
+    extern __shared__ float sf[];                 // [NB] the spectrum, [ML] direction sums, [KL] frequency sums
+    const int KL = t.KL, ML = t.ML, NB = KL * ML, ib = blockIdx.x, l = threadIdx.x;
+    const size_t NS = t.NSEA;
+    const int ij = ijarc[ib] - 1;
+    for (int i = l; i < NB; i += NEST_WG) {
+        const float v = fl3[(size_t)i * NS + ij];
+        sf[i] = v;
+        fbc[(size_t)ib * NB + i] = v;
+    }
+    __syncthreads();
+    float *T = sf + NB, *te = T + ML;
+    if (l < ML) {
+        // SUM(F, DIM=2) of TOTAL_ENERGY and FEMEAN
+        float s = 0.f;
+        for (int k = 0; k < KL; k++) s = k == 0 ? sf[l * KL] : s + sf[l * KL + k];
+        T[l] = s;
+    } else if (l >= 64 && l < 64 + KL) {
+        // MEAN_DIRECTION_B's TEMP(K): the frequency integral of one direction
+        const int k = l - 64;
+        float s = sf[k] * t.DFIM[0];
+        for (int m = 1; m < ML; m++) s = s + sf[m * KL + k] * t.DFIM[m];
+        te[k] = s;
+    }
+    __syncthreads();
+    if (l != 0) return;
+    float em = t.MO_TAIL * T[ML - 1];
+    for (int m = 0; m < ML; m++) em = em + T[m] * t.DFIM[m];
+    em = wam_maxf(em, t.EMIN);
+    float fm = t.MM1_TAIL * T[ML - 1];
+    for (int m = 0; m < ML; m++) fm = fm + T[m] * t.DFIMOFR[m];
+    fm = em / wam_maxf(fm, t.EMIN);
+    float si = 0.f, ci = 0.f;
+    for (int k = 0; k < KL; k++) {
+        si = si + t.SINTH[k] * te[k];
+        ci = ci + t.COSTH[k] * te[k];
+    }
+    if (ci == 0.f) ci = 0.1E-30f;
+    float thq = wam_atan2f(si, ci);
+    if (thq < 0.f) thq = thq + t.ZPI;
+    if (thq > t.ZPI - 0.001f) thq = 0.f;
+    mean[ib] = em;
+    mean[nb + ib] = thq;
+    mean[2 * nb + ib] = fm;
+}
+
+// BOUNDARY_INPUT's time step: FI(p), MI(p) from the first set (which = 1),
+// the second (2) or INTSPEC between them at DEL1L of DEL12 (0).
+__global__ void __launch_bounds__(NEST_WG) boundary_time_kernel(DevTables t, int nbinp, const float *__restrict__ f1,
+                                                                const float *__restrict__ m1,
+                                                                const float *__restrict__ f2,
+                                                                const float *__restrict__ m2, int which, float del12,
+                                                                float del1l, float *__restrict__ fi,
+                                                                float *__restrict__ mi)
+{
+    const int NB = t.KL * t.ML, p = blockIdx.x, l = threadIdx.x;
+    if (which != 0) {
+        const float *s = (which == 1 ? f1 : f2) + (size_t)p * NB, *sm = which == 1 ? m1 : m2;
+        for (int i = l; i < NB; i += NEST_WG) fi[(size_t)p * NB + i] = s[i];
+        if (l < 3) mi[l * nbinp + p] = sm[l * nbinp + p];
+        return;
+    }
+    intspec_wg(t, del12, del1l, f1 + (size_t)p * NB, m1[p], m1[nbinp + p], m1[2 * nbinp + p], f2 + (size_t)p * NB,
+               m2[p], m2[nbinp + p], m2[2 * nbinp + p], fi + (size_t)p * NB, 1, mi + p, nbinp);
+}
+
+// BOUNDARY_INPUT's space step: FL3(IJARF(i), :, :) = INTSPEC(1, BFW(i),
+// FI(IBFL(i)), FI(IBFR(i))).  The host passes each IJARF once.
+__global__ void __launch_bounds__(NEST_WG) boundary_space_kernel(DevTables t, int nbinp, const float *__restrict__ fi,
+                                                                 const float *__restrict__ mi,
+                                                                 const int *__restrict__ ijarf,
+                                                                 const int *__restrict__ ibfl,
+                                                                 const int *__restrict__ ibfr,
+                                                                 const float *__restrict__ bfw, float *fl3)
+{
+    const int NB = t.KL * t.ML, i = blockIdx.x;
+    const int lft = ibfl[i] - 1, rgt = ibfr[i] - 1;
+    const size_t ij = (size_t)(ijarf[i] - 1);
+    intspec_wg(t, 1.f, bfw[i], fi + (size_t)lft * NB, mi[lft], mi[nbinp + lft], mi[2 * nbinp + lft],
+               fi + (size_t)rgt * NB, mi[rgt], mi[nbinp + rgt], mi[2 * nbinp + rgt], fl3 + ij, (size_t)t.NSEA,
+               nullptr, 0);
+}
+
+hipError_t launch_boundary_gather(const DevTables &t, const float *fl3, const int *ijarc, int nb, float *fbc,
+                                  float *mean, hipStream_t st)
+{
+    if (nb <= 0) return hipSuccess;
+    const size_t lds = (size_t)(t.KL * t.ML + t.ML + t.KL) * sizeof(float);
+    hipLaunchKernelGGL(boundary_gather_kernel, dim3(nb), dim3(NEST_WG), lds, st, t, fl3, ijarc, nb, fbc, mean);
+    return hipGetLastError();
+}
+
+hipError_t launch_boundary_input(const DevTables &t, int nbinp, const float *f1, const float *m1, const float *f2,
+                                 const float *m2, int which, float del12, float del1l, float *fi, float *mi,
+                                 int nbounf, const int *ijarf, const int *ibfl, const int *ibfr, const float *bfw,
+                                 float *fl3, hipStream_t st)
+{
+    if (nbinp <= 0) return hipSuccess;
+    hipLaunchKernelGGL(boundary_time_kernel, dim3(nbinp), dim3(NEST_WG), 0, st, t, nbinp, f1, m1, f2, m2, which,
+                       del12, del1l, fi, mi);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || nbounf <= 0) return e;
+    hipLaunchKernelGGL(boundary_space_kernel, dim3(nbounf), dim3(NEST_WG), 0, st, t, nbinp, fi, mi, ijarf, ibfl, ibfr,
+                       bfw, fl3);
+    return hipGetLastError();
+}
+
+}  // namespace wam

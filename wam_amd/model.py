@@ -89,6 +89,64 @@ def restart_read(path, nsea, kl, ml, depth=True, current=True):
     return _undates(raw.raw), out
 
 
+def boundary_write_header(path, kl, ml, th1, fr1, co, nbounc, idel_b_out, idel_bo_file):
+    """SAVE_BOUNDARY_FILE's header record (wam_boundary_write_header): a new
+    file with the eight REALs KL, ML, TH(1), FR(1), CO, NBOUNC, IDEL_B_OUT,
+    IDEL_BO_FILE."""
+    lib = _lib.load()
+    hdr = np.array([kl, ml, th1, fr1, co, nbounc, idel_b_out, idel_bo_file], np.float32)
+    rc = lib.wam_boundary_write_header(os.fsencode(path), _ptr(hdr))
+    if rc:
+        raise WamError(rc, f"wam_boundary_write_header({path})")
+
+
+def boundary_append(path, lon, lat, cdtpro, mean, depth, fbc):
+    """BOUNDARY_OUTPUT's records for one output time (wam_boundary_append):
+    lon, lat [nb] (INTEGER, M_SEC), the date CDTPRO, mean [3][nb] (EMEAN, THQ,
+    FMEAN), depth [nb] and the spectra fbc [nb][ML][KL]."""
+    lib = _lib.load()
+    f = np.ascontiguousarray(fbc, np.float32)
+    nb, ml, kl = f.shape
+    lo, la = (np.ascontiguousarray(x, np.int32) for x in (lon, lat))
+    m = np.ascontiguousarray(mean, np.float32)
+    d = np.ascontiguousarray(depth, np.float32)
+    if lo.size != nb or la.size != nb or m.shape != (3, nb) or d.size != nb:
+        raise ValueError("lon, lat, depth [nb] and mean [3][nb] for fbc [nb][ML][KL]")
+    c = (cdtpro or "").encode("ascii").ljust(14, b" ")[:14]
+    rc = lib.wam_boundary_append(os.fsencode(path), nb, kl, ml, _ptr(lo), _ptr(la), c, _ptr(m), _ptr(d), _ptr(f))
+    if rc:
+        raise WamError(rc, f"wam_boundary_append({path})")
+
+
+def boundary_read_header(path):
+    """READ_BOUNDARY_INPUT's header (wam_boundary_read_header): the eight
+    header values and the byte offset of the first point record."""
+    lib = _lib.load()
+    hdr = np.zeros(8, np.float32)
+    pos = ctypes.c_int64(0)
+    rc = lib.wam_boundary_read_header(os.fsencode(path), _ptr(hdr), ctypes.byref(pos))
+    if rc:
+        raise WamError(rc, f"wam_boundary_read_header({path})")
+    return hdr, pos.value
+
+
+def boundary_read(path, pos, nb, kl, ml):
+    """One set of boundary spectra from byte offset pos
+    (wam_boundary_read): returns a dict of lon, lat, dates, mean [3][nb],
+    depth, fbc [nb][ML][KL] and the offset of the next set."""
+    lib = _lib.load()
+    out = dict(lon=np.empty(nb, np.int32), lat=np.empty(nb, np.int32), mean=np.empty((3, nb), np.float32),
+               depth=np.empty(nb, np.float32), fbc=np.empty((nb, ml, kl), np.float32))
+    raw = ctypes.create_string_buffer(14 * nb)
+    p = ctypes.c_int64(pos)
+    rc = lib.wam_boundary_read(os.fsencode(path), ctypes.byref(p), nb, kl, ml, _ptr(out["lon"]), _ptr(out["lat"]),
+                               raw, _ptr(out["mean"]), _ptr(out["depth"]), _ptr(out["fbc"]))
+    if rc:
+        raise WamError(rc, f"wam_boundary_read({path})")
+    out["dates"] = [raw.raw[14 * i:14 * i + 14].decode("ascii") for i in range(nb)]
+    return out, p.value
+
+
 class WamGpu:
     def __init__(self, cfg, mask, host_only=False):
         self.lib = _lib.load()
@@ -205,6 +263,47 @@ class WamGpu:
                                                      _ptr(udir), _ptr(tauw), _ptr(depth), _ptr(u), _ptr(v),
                                                      ctypes.byref(present)))
         return _undates(raw.raw), dict(depth=bool(present.value & 1), current=bool(present.value & 2))
+
+    def boundary_output(self, fl3, ijarc):
+        """BOUNDARY_OUTPUT's gather and mean parameters at the 1-based sea
+        points ijarc: FL3 as a host array [ML][KL][NSEA] or a device tensor;
+        returns fbc [nb][ML][KL] and mean [3][nb] (EMEAN, THQ, FMEAN)."""
+        ij = np.ascontiguousarray(ijarc, np.int32)
+        nb = ij.size
+        fbc = np.empty((nb, self.ml, self.kl), np.float32)
+        mean = np.empty((3, nb), np.float32)
+        if isinstance(fl3, np.ndarray):
+            f = np.ascontiguousarray(fl3, np.float32)
+            self._check(self.lib.wam_gpu_boundary_output(self.ctx, self.nsea, self.kl, self.ml, _ptr(f), nb, _ptr(ij),
+                                                         _ptr(fbc), _ptr(mean)))
+        else:
+            self._check(self.lib.wam_gpu_boundary_output_dev(self.ctx, nb, _ptr(ij), _ptr(fl3), _ptr(fbc),
+                                                             _ptr(mean)))
+        return fbc, mean
+
+    def boundary_input(self, fl3, f1, m1, f2, m2, which, del12, del1l, ijarf, ibfl, ibfr, bfw):
+        """BOUNDARY_INPUT's time and space interpolation into FL3 (a host
+        array [ML][KL][NSEA], updated in place, or a device tensor): which = 1
+        takes (f1, m1), 2 takes (f2, m2), 0 interpolates at del1l of del12;
+        f* are [nbinp][ML][KL], m* [3][nbinp] (EMEAN, THQ, FMEAN).  Returns
+        the set at the model time (FI, MI)."""
+        a = lambda x: None if x is None else np.ascontiguousarray(x, np.float32)
+        i = lambda x: np.ascontiguousarray(x, np.int32)
+        f1, m1, f2, m2, w = a(f1), a(m1), a(f2), a(m2), a(bfw)
+        ij, il, ir = i(ijarf), i(ibfl), i(ibfr)
+        nbinp = (m1 if m1 is not None else m2).shape[-1]
+        fi = np.empty((nbinp, self.ml, self.kl), np.float32)
+        mi = np.empty((3, nbinp), np.float32)
+        args = [_ptr(f1), _ptr(m1), _ptr(f2), _ptr(m2), int(which), float(del12), float(del1l), ij.size, _ptr(ij),
+                _ptr(il), _ptr(ir), _ptr(w)]
+        if isinstance(fl3, np.ndarray):
+            if fl3.dtype != np.float32 or not fl3.flags.c_contiguous:
+                raise ValueError("fl3 must be a C-contiguous float32 array [ML][KL][NSEA]")
+            self._check(self.lib.wam_gpu_boundary_input(self.ctx, self.nsea, self.kl, self.ml, _ptr(fl3), nbinp,
+                                                        *args, _ptr(fi), _ptr(mi)))
+        else:
+            self._check(self.lib.wam_gpu_boundary_input_dev(self.ctx, nbinp, *args, _ptr(fl3), _ptr(fi), _ptr(mi)))
+        return fi, mi
 
     def set_timing(self, on):
         """Records a HIP event pair around every PROPAGS / IMPLSCH kernel
