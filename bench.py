@@ -84,6 +84,113 @@ def cpu_baseline(args):
                 seconds_per_step=dt)
 
 
+def nest_rings(mask):
+    """The nest boundary point lists on one grid, in the sea-point order:
+    BOUNDARY_OUTPUT's points on the edge of an inner rectangle (rows ny/8 and
+    7ny/8, columns nx/8 to 7nx/8) and BOUNDARY_INPUT's points on the grid's
+    outer edge, each interpolated between two of NBINP = NBOUNF/5 input
+    spectra (a 5:1 nest) with weights 0, 0.2, ..., 0.8."""
+    ny, nx = mask.shape
+    sea = mask.astype(bool)
+    idx = np.cumsum(sea.ravel()).reshape(sea.shape)          # 1-based sea-point number of each cell
+
+    def ring(r0, r1, c0, c1):
+        sel = np.zeros(sea.shape, bool)
+        sel[[r0, r1], c0:c1 + 1] = True
+        sel[r0:r1 + 1, [c0, c1]] = True
+        return idx[sel & sea].astype(np.int32)
+
+    ijarc = ring(ny // 8, 7 * ny // 8, nx // 8, 7 * nx // 8)
+    ijarf = ring(0, ny - 1, 0, nx - 1)
+    nbinp = max(2, ijarf.size // 5)
+    j = np.arange(ijarf.size)
+    ibfl = (np.minimum(j // 5, nbinp - 2) + 1).astype(np.int32)
+    bfw = ((j % 5) / 5.0).astype(np.float32)
+    return ijarc, nbinp, ijarf, ibfl, ibfl + 1, bfw
+
+
+def bench_nest(args, torch, g, cfg, mask, fl3):
+    """The nest boundary exchange (BOUNDARY_OUTPUT, then BOUNDARY_INPUT's time
+    and space interpolation) on FL3 resident in HBM, the grid acting as the
+    coarse grid for the output and as the fine grid for the input (point
+    lists of nest_rings).  One exchange = wam_gpu_boundary_output_dev +
+    wam_gpu_boundary_input_dev (host arrays for the boundary spectra, as the
+    file I/O between them)."""
+    n, kl, ml = g.nsea, cfg["kl"], cfg["ml"]
+    nbs = kl * ml
+    ijarc, NB_INP, ijarf, ibfl, ibfr, bfw = nest_rings(mask)
+    NB_OUT, NB_FINE = ijarc.size, ijarf.size
+    f1, m1 = g.boundary_output(fl3, ijarc[:NB_INP])
+    f2, m2 = g.boundary_output(fl3, ijarc[::-1][:NB_INP].copy())
+
+    def exchange():
+        g.boundary_output(fl3, ijarc)
+        g.boundary_input(fl3, f1, m1, f2, m2, 0, 3600.0, 1800.0, ijarf, ibfl, ibfr, bfw)
+
+    for _ in range(args.warmup):
+        exchange()
+    g.set_timing(True)
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    for _ in range(args.steps):
+        exchange()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - w0) / args.steps
+    kt = {}
+    for name in ("boundary_gather", "boundary_time", "boundary_space"):
+        ms, calls = g.kernel_time(name)
+        kt[name] = ms / calls
+    g.set_timing(False)
+    # algorithmic bytes: every input value read once and every output value
+    # written once (spectra of KL*ML floats, three mean values and the indices)
+    nbytes = {"boundary_gather": NB_OUT * (nbs * 8 + 4 + 12),
+              "boundary_time": NB_INP * (nbs * 12 + 36),
+              "boundary_space": NB_INP * (nbs * 4 + 12) + NB_FINE * (nbs * 4 + 16)}
+    roofs = {}
+    for name, ms in kt.items():
+        gbs = nbytes[name] / (ms * 1e-3) / 1e9
+        roofs[name] = dict(kernel=f"{name}_kernel", bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS,
+                           unit="GB/s", frac=round(gbs / HBM_PEAK_GBS, 4), traffic=None,
+                           algorithmic_bytes_per_launch=nbytes[name], avg_launch_ms=round(ms, 4))
+    dominant = max(kt, key=kt.get)
+    out = {
+        "metric": "nest boundary exchanges/s (BOUNDARY_OUTPUT + BOUNDARY_INPUT)",
+        "value": 1.0 / wall, "unit": "exchanges/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (spectra of the resident c2 state; seeded boundary point lists)",
+        "config": {"workload": f"nest exchange on c2 FL3: {NB_OUT} output points (inner rectangle), {NB_INP} "
+                               f"input spectra, {NB_FINE} fine boundary points (outer edge, 5:1), {kl}x{ml}",
+                   "nsea": n},
+        "kernel_ms": {k: round(v, 4) for k, v in kt.items()},
+        "gpu_ms_per_exchange": round(sum(kt.values()), 4),
+        "roofline": roofs[dominant], "roofline_all": roofs,
+    }
+    if not args.no_cpu_baseline:
+        from oracle.oracle import Oracle
+        from wam_amd import workloads as W
+        log("[bench] CPU baseline (oracle, one exchange) ...")
+        cfg_, cmask, _ = W.c2_case(ny=args.cpu_ny, nx=args.cpu_nx, iphys=args.iphys)
+        o = Oracle(cfg_, cmask)
+        ns = o.nsea
+        h = np.broadcast_to(f1[:1].reshape(ml, kl, 1), (ml, kl, ns)).copy()
+        cij, cnb, cif, cl, cr, cw = nest_rings(cmask)
+        c0 = time.perf_counter()
+        o.boundary_output(h, cij)
+        o.boundary_input(h, f1[:cnb], m1[:, :cnb], f2[:cnb], m2[:, :cnb], 0, 3600.0, 1800.0, cif, cl, cr, cw,
+                         inplace=True)
+        ct = time.perf_counter() - c0
+        out["cpu_baseline"] = {"value": 1.0 / ct, "unit": "exchanges/s", "cores": 1, "kind": "port",
+                               "sample": f"one exchange by the oracle on the {args.cpu_ny}x{args.cpu_nx} "
+                                         f"sub-box ({ns} sea points; the same rings there: {cij.size} output, "
+                                         f"{cnb} input, {cif.size} fine points; single thread, FL3 updated in "
+                                         "place); rate scaled to the GPU's point counts",
+                               "raw_exchanges_per_s": 1.0 / ct}
+        # the oracle's time is linear in the point counts: scale to this workload's
+        scale = (cij.size + 2 * cnb + cif.size) / (NB_OUT + 2 * NB_INP + NB_FINE)
+        out["cpu_baseline"]["value"] = scale / ct
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +203,8 @@ def main():
     ap.add_argument("--cpu-ny", type=int, default=360)
     ap.add_argument("--cpu-nx", type=int, default=480)
     ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--nest", action="store_true",
+                    help="measure the nest boundary exchange on the resident state instead (one JSON line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one plain 1 GiB device copy before the timed steps (reference for HBM counters)")
     args = ap.parse_args()
@@ -124,6 +233,10 @@ def main():
     fb = torch.empty_like(fa)
     torch.cuda.synchronize()
     log(f"[bench] state resident in HBM ({fa.numel() * 4 / 1e9:.2f} GB spectrum), {time.time() - t0:.1f}s")
+
+    if args.nest:
+        bench_nest(args, torch, g, cfg, mask, fa)
+        return
 
     def step(ev=None):
         nonlocal fa, fb

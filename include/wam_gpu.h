@@ -250,15 +250,17 @@ int wam_boundary_read(const char *path, int64_t *pos, int32_t nb, int32_t kl, in
 
 /* Per-kernel timing on the context stream.  No counterpart in the
    reference (its timing is host wall-clock around whole calls).  on = 1
-   records a HIP event pair around every kernel launch of PROPAGS and IMPLSCH
-   from now on and drops earlier records; on = 0 stops recording. */
+   records a HIP event pair around every kernel launch of PROPAGS, IMPLSCH and
+   the nest boundary exchange from now on and drops earlier records; on = 0
+   stops recording. */
 int wam_gpu_set_timing(wam_ctx *ctx, int32_t on);
 
 /* Total GPU milliseconds and number of launches of one kernel since timing
    was switched on: name is "propags" (one PROPAGS sub-step), "implsch_pre",
-   "implsch_snl", "implsch_post" (the three IMPLSCH kernels of IPHYS = 0) or
-   "implsch" (the single IMPLSCH kernel of the other configurations).  Waits
-   for the context stream. */
+   "implsch_snl", "implsch_post" (the three IMPLSCH kernels of IPHYS = 0),
+   "implsch" (the single IMPLSCH kernel of the other configurations),
+   "boundary_gather" (BOUNDARY_OUTPUT), "boundary_time" or "boundary_space"
+   (BOUNDARY_INPUT's two steps).  Waits for the context stream. */
 int wam_gpu_get_timing(wam_ctx *ctx, const char *name, double *total_ms, int64_t *launches);
 int wam_gpu_sync(wam_ctx *ctx);
 

@@ -236,12 +236,18 @@ class Oracle:
         self.lib.orc_api_boundary_output(self.ctx, _p(fl3), nb, _p(ij), _p(fbc), _p(mean))
         return fbc, mean
 
-    def boundary_input(self, fl3, f1, m1, f2, m2, which, del12, del1l, ijarf, ibfl, ibfr, bfw):
+    def boundary_input(self, fl3, f1, m1, f2, m2, which, del12, del1l, ijarf, ibfl, ibfr, bfw, inplace=False):
         """BOUNDARY_INPUT's time then space interpolation into a copy of FL3
-        [ML][KL][NSEA]; returns (FL3, FI[nbinp][ML][KL], MI[3][nbinp])."""
+        [ML][KL][NSEA] (into FL3 itself with inplace=True); returns (FL3,
+        FI[nbinp][ML][KL], MI[3][nbinp])."""
         a = lambda x: np.ascontiguousarray(x, dtype=self.dtype)
         i = lambda x: np.ascontiguousarray(x, dtype=np.int32)
-        out = np.array(fl3, dtype=self.dtype, copy=True, order="C")
+        if inplace:
+            if fl3.dtype != self.dtype or not fl3.flags.c_contiguous:
+                raise ValueError("in place: FL3 must be C-contiguous in the oracle's precision")
+            out = fl3
+        else:
+            out = np.array(fl3, dtype=self.dtype, copy=True, order="C")
         f1, m1, f2, m2, bfw = a(f1), a(m1), a(f2), a(m2), a(bfw)
         ijarf, ibfl, ibfr = i(ijarf), i(ibfl), i(ibfr)
         nbinp = m1.shape[-1]

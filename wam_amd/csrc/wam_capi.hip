@@ -40,10 +40,12 @@ hipError_t launch_output_params(const DevTables &t, const float *fl3, const floa
                                 hipStream_t st);
 hipError_t launch_boundary_gather(const DevTables &t, const float *fl3, const int *ijarc, int nb, float *fbc,
                                   float *mean, hipStream_t st);
-hipError_t launch_boundary_input(const DevTables &t, int nbinp, const float *f1, const float *m1, const float *f2,
-                                 const float *m2, int which, float del12, float del1l, float *fi, float *mi,
-                                 int nbounf, const int *ijarf, const int *ibfl, const int *ibfr, const float *bfw,
-                                 float *fl3, hipStream_t st);
+hipError_t launch_boundary_time(const DevTables &t, int nbinp, const float *f1, const float *m1, const float *f2,
+                                const float *m2, int which, float del12, float del1l, float *fi, float *mi,
+                                hipStream_t st);
+hipError_t launch_boundary_space(const DevTables &t, int nbinp, const float *fi, const float *mi, int nbounf,
+                                 const int *ijarf, const int *ibfl, const int *ibfr, const float *bfw, float *fl3,
+                                 hipStream_t st);
 }  // namespace wam
 
 using namespace wam;
@@ -813,7 +815,13 @@ int wam_gpu_boundary_output_dev(wam_ctx *ctx, int32_t nb, const int32_t *ijarc, 
     int *dij = reinterpret_cast<int *>(dmean + 3 * n);
     hipStream_t st = ctx->stream;
     HIPCHK(hipMemcpyAsync(dij, ijarc, n * 4, hipMemcpyHostToDevice, st));
+    hipEvent_t ev[2];
+    rc = timing_events(ctx, ev, 2);
+    if (rc) return rc;
+    timing_record(ctx, ev[0]);
     HIPCHK(launch_boundary_gather(ctx->dt, d_fl3, dij, nb, dfbc, dmean, st));
+    timing_record(ctx, ev[1]);
+    timing_keep(ctx, "boundary_gather", ev[0], ev[1]);
     HIPCHK(hipMemcpyAsync(fbc, dfbc, n * NB * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(mean, dmean, 3 * n * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -871,8 +879,16 @@ int wam_gpu_boundary_input_dev(wam_ctx *ctx, int32_t nbinp, const float *f1, con
         HIPCHK(hipMemcpyAsync(dl, il.data(), nf * 4, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(dr, ir.data(), nf * 4, hipMemcpyHostToDevice, st));
     }
-    HIPCHK(launch_boundary_input(ctx->dt, nbinp, df1, dm1, df2, dm2, which, del12, del1l, dfi, dmi, (int)nf, dij, dl,
-                                 dr, dw, d_fl3, st));
+    hipEvent_t ev[3];
+    rc = timing_events(ctx, ev, 3);
+    if (rc) return rc;
+    timing_record(ctx, ev[0]);
+    HIPCHK(launch_boundary_time(ctx->dt, nbinp, df1, dm1, df2, dm2, which, del12, del1l, dfi, dmi, st));
+    timing_record(ctx, ev[1]);
+    HIPCHK(launch_boundary_space(ctx->dt, nbinp, dfi, dmi, (int)nf, dij, dl, dr, dw, d_fl3, st));
+    timing_record(ctx, ev[2]);
+    timing_keep(ctx, "boundary_time", ev[0], ev[1]);
+    timing_keep(ctx, "boundary_space", ev[1], ev[2]);
     if (fi) HIPCHK(hipMemcpyAsync(fi, dfi, np * NB * 4, hipMemcpyDeviceToHost, st));
     if (mi) HIPCHK(hipMemcpyAsync(mi, dmi, 3 * np * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));     // the host vectors above are read by the copies

@@ -312,7 +312,8 @@ class WamGpu:
 
     def kernel_time(self, name):
         """(total GPU ms, launches) of one kernel since set_timing(True):
-        "propags", "implsch_pre", "implsch_snl", "implsch_post" or "implsch"."""
+        "propags", "implsch_pre", "implsch_snl", "implsch_post", "implsch",
+        "boundary_gather", "boundary_time" or "boundary_space"."""
         ms = ctypes.c_double(0.0)
         n = ctypes.c_int64(0)
         self._check(self.lib.wam_gpu_get_timing(self.ctx, name.encode(), ctypes.byref(ms), ctypes.byref(n)))
