@@ -62,6 +62,19 @@ INTERFACE
       TYPE(C_PTR), VALUE :: ctx, fl3, u10, udir, ustar, tauw, z0, out
       INTEGER(C_INT32_T), VALUE :: nsea, kl, ml, orient
    END FUNCTION
+   INTEGER(C_INT) FUNCTION c_wam_gpu_boundary_output(ctx, nsea, kl, ml, fl3, nb, ijarc, fbc, mean) &
+&     BIND(C, NAME='wam_gpu_boundary_output')
+      IMPORT :: C_INT, C_INT32_T, C_PTR
+      TYPE(C_PTR), VALUE :: ctx, fl3, ijarc, fbc, mean
+      INTEGER(C_INT32_T), VALUE :: nsea, kl, ml, nb
+   END FUNCTION
+   INTEGER(C_INT) FUNCTION c_wam_gpu_boundary_input(ctx, nsea, kl, ml, fl3, nbinp, f1, m1, f2, m2, which, &
+&     del12, del1l, nbounf, ijarf, ibfl, ibfr, bfw, fi, mi) BIND(C, NAME='wam_gpu_boundary_input')
+      IMPORT :: C_INT, C_INT32_T, C_PTR, C_FLOAT
+      TYPE(C_PTR), VALUE :: ctx, fl3, f1, m1, f2, m2, ijarf, ibfl, ibfr, bfw, fi, mi
+      INTEGER(C_INT32_T), VALUE :: nsea, kl, ml, nbinp, which, nbounf
+      REAL(C_FLOAT), VALUE :: del12, del1l
+   END FUNCTION
    FUNCTION c_wam_gpu_last_error(ctx) BIND(C, NAME='wam_gpu_last_error')
       IMPORT :: C_PTR
       TYPE(C_PTR), VALUE :: ctx
@@ -87,7 +100,7 @@ TYPE(C_PTR), SAVE :: CTX = C_NULL_PTR
 INTEGER, SAVE, PUBLIC :: NSEA_GPU = 0
 
 PUBLIC :: WAM_GPU_SETUP, WAM_GPU_PREPARE_PROPAGATION, WAM_GPU_PREPARE_PROPAGATION_DEV, PROPAGS_GPU, &
-&         IMPLSCH_GPU, OUTPUT_PARAMETERS_GPU, WAM_GPU_END
+&         IMPLSCH_GPU, OUTPUT_PARAMETERS_GPU, BOUNDARY_OUTPUT_GPU, BOUNDARY_INPUT_GPU, WAM_GPU_END
 
 CONTAINS
 
@@ -167,6 +180,70 @@ SUBROUTINE OUTPUT_PARAMETERS_GPU (FL3, U10, UDIR, USTAR, TAUW, Z0, ORIENT, BOUT)
 &             INT(SIZE(FL3,3), C_INT32_T), C_LOC(FL3), C_LOC(U10), C_LOC(UDIR), C_LOC(USTAR), C_LOC(TAUW), &
 &             C_LOC(Z0), INT(ORIENT, C_INT32_T), C_LOC(BOUT)), 'OUTPUT_PARAMETERS_GPU')
 END SUBROUTINE OUTPUT_PARAMETERS_GPU
+
+!  BOUNDARY_OUTPUT's gather and mean values (wam_boundary_module.f90:350-362) at
+!  the sea points IJARC(1:NB): FBC(:,:,I) = FL3(IJARC(I),:,:), one spectrum per
+!  point as the boundary file's record holds it (WRITE (IU) FBC(:,:,I)), and
+!  EMEANC, THQC, FMEANC of TOTAL_ENERGY, MEAN_DIRECTION and FEMEAN.
+SUBROUTINE BOUNDARY_OUTPUT_GPU (FL3, IJARC, FBC, EMEANC, THQC, FMEANC)
+   REAL(C_FLOAT),      INTENT(IN),  TARGET, CONTIGUOUS :: FL3(:,:,:)
+   INTEGER(C_INT32_T), INTENT(IN),  TARGET, CONTIGUOUS :: IJARC(:)
+   REAL(C_FLOAT),      INTENT(OUT), TARGET, CONTIGUOUS :: FBC(:,:,:)
+   REAL(C_FLOAT),      INTENT(OUT)                     :: EMEANC(:), THQC(:), FMEANC(:)
+   REAL(C_FLOAT), ALLOCATABLE, TARGET :: MEAN(:,:)
+   INTEGER :: NB
+   NB = SIZE(IJARC)
+   IF (SIZE(FBC,1) .NE. SIZE(FL3,2) .OR. SIZE(FBC,2) .NE. SIZE(FL3,3) .OR. SIZE(FBC,3) .LT. NB .OR.        &
+&      SIZE(EMEANC) .LT. NB .OR. SIZE(THQC) .LT. NB .OR. SIZE(FMEANC) .LT. NB) THEN
+      WRITE (*,*) ' *** WAM GPU ERROR in BOUNDARY_OUTPUT_GPU: FBC must be (KL, ML, NB), the means (NB)'
+      ERROR STOP 1
+   END IF
+   ALLOCATE (MEAN(NB,3))
+   CALL CHECK(c_wam_gpu_boundary_output(CTX, INT(SIZE(FL3,1), C_INT32_T), INT(SIZE(FL3,2), C_INT32_T),    &
+&             INT(SIZE(FL3,3), C_INT32_T), C_LOC(FL3), INT(NB, C_INT32_T), C_LOC(IJARC), C_LOC(FBC),        &
+&             C_LOC(MEAN)), 'BOUNDARY_OUTPUT_GPU')
+   EMEANC(1:NB) = MEAN(:,1)
+   THQC(1:NB) = MEAN(:,2)
+   FMEANC(1:NB) = MEAN(:,3)
+END SUBROUTINE BOUNDARY_OUTPUT_GPU
+
+!  BOUNDARY_INPUT after its reads (wam_boundary_module.f90:226-285): the set at
+!  the model time (WHICH = 1: F1, 2: F2, 0: INTSPEC at DEL1L of DEL12) into FI
+!  and its means, then FL3(IJARF(I),:,:) = INTSPEC(1, BFW(I), FI(:,:,IBFL(I)),
+!  FI(:,:,IBFR(I))).  F1, F2, FI are (KL, ML, NBINP) as in the reference.
+SUBROUTINE BOUNDARY_INPUT_GPU (FL3, F1, EMEAN1, THQ1, FMEAN1, F2, EMEAN2, THQ2, FMEAN2, WHICH, DEL12, DEL1L, &
+&                              IJARF, IBFL, IBFR, BFW, FI, EMEANI, THQI, FMEANI)
+   REAL(C_FLOAT),      INTENT(INOUT), TARGET, CONTIGUOUS :: FL3(:,:,:)
+   REAL(C_FLOAT),      INTENT(IN),    TARGET, CONTIGUOUS :: F1(:,:,:), F2(:,:,:)
+   REAL(C_FLOAT),      INTENT(IN)                       :: EMEAN1(:), THQ1(:), FMEAN1(:)
+   REAL(C_FLOAT),      INTENT(IN)                       :: EMEAN2(:), THQ2(:), FMEAN2(:)
+   INTEGER,            INTENT(IN)                       :: WHICH
+   REAL,               INTENT(IN)                       :: DEL12, DEL1L
+   INTEGER(C_INT32_T), INTENT(IN),    TARGET, CONTIGUOUS :: IJARF(:), IBFL(:), IBFR(:)
+   REAL(C_FLOAT),      INTENT(IN),    TARGET, CONTIGUOUS :: BFW(:)
+   REAL(C_FLOAT),      INTENT(OUT),   TARGET, CONTIGUOUS :: FI(:,:,:)
+   REAL(C_FLOAT),      INTENT(OUT)                      :: EMEANI(:), THQI(:), FMEANI(:)
+   REAL(C_FLOAT), ALLOCATABLE, TARGET :: M1(:,:), M2(:,:), MI(:,:)
+   INTEGER :: NBINP
+   NBINP = SIZE(F1,3)
+   IF (SIZE(F2,3) .NE. NBINP .OR. SIZE(FI,3) .NE. NBINP .OR. SIZE(F1,1) .NE. SIZE(FL3,2) .OR.              &
+&      SIZE(F1,2) .NE. SIZE(FL3,3) .OR. SIZE(IBFL) .NE. SIZE(IJARF) .OR. SIZE(IBFR) .NE. SIZE(IJARF) .OR.   &
+&      SIZE(BFW) .NE. SIZE(IJARF)) THEN
+      WRITE (*,*) ' *** WAM GPU ERROR in BOUNDARY_INPUT_GPU: F1, F2, FI must be (KL, ML, NBINP)'
+      ERROR STOP 1
+   END IF
+   ALLOCATE (M1(NBINP,3), M2(NBINP,3), MI(NBINP,3))
+   M1(:,1) = EMEAN1(1:NBINP) ; M1(:,2) = THQ1(1:NBINP) ; M1(:,3) = FMEAN1(1:NBINP)
+   M2(:,1) = EMEAN2(1:NBINP) ; M2(:,2) = THQ2(1:NBINP) ; M2(:,3) = FMEAN2(1:NBINP)
+   CALL CHECK(c_wam_gpu_boundary_input(CTX, INT(SIZE(FL3,1), C_INT32_T), INT(SIZE(FL3,2), C_INT32_T),     &
+&             INT(SIZE(FL3,3), C_INT32_T), C_LOC(FL3), INT(NBINP, C_INT32_T), C_LOC(F1), C_LOC(M1),         &
+&             C_LOC(F2), C_LOC(M2), INT(WHICH, C_INT32_T), REAL(DEL12, C_FLOAT), REAL(DEL1L, C_FLOAT),      &
+&             INT(SIZE(IJARF), C_INT32_T), C_LOC(IJARF), C_LOC(IBFL), C_LOC(IBFR), C_LOC(BFW), C_LOC(FI),   &
+&             C_LOC(MI)), 'BOUNDARY_INPUT_GPU')
+   EMEANI(1:NBINP) = MI(:,1)
+   THQI(1:NBINP) = MI(:,2)
+   FMEANI(1:NBINP) = MI(:,3)
+END SUBROUTINE BOUNDARY_INPUT_GPU
 
 SUBROUTINE WAM_GPU_END
    IF (C_ASSOCIATED(CTX)) CALL c_wam_gpu_finalize(CTX)

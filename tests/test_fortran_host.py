@@ -81,3 +81,20 @@ def test_fortran_host_matches_oracle(tmp_path, which):
     ro = o.output_parameters(fo, f["u10"], f["udir"], r["ustar"], tw, z0, f["depth"],
                              f["u"] if cur else None, f["v"] if cur else None, orient=1)
     assert np.array_equal(bout.view(np.int32), ro.view(np.int32))
+    # the nest exchange of BOUNDARY_OUTPUT_GPU and BOUNDARY_INPUT_GPU on the result
+    kl, ml = cfg["kl"], cfg["ml"]
+    nb, nbf = min(n, 50), min(n, 80)
+    ijarc = 1 + (np.arange(nb) * n) // nb
+    fbc = np.fromfile(os.path.join(d, "fbc.f32"), np.float32).reshape(nb, ml, kl)
+    bmean = np.fromfile(os.path.join(d, "bmean.f32"), np.float32).reshape(3, nb)
+    rf, rm = o.boundary_output(fo, ijarc)
+    assert np.array_equal(fbc.view(np.int32), rf.view(np.int32))
+    assert np.array_equal(bmean.view(np.int32), rm.view(np.int32))
+    i = np.arange(1, nbf + 1)
+    ijarf = n - ((i - 1) * n) // nbf
+    ibfl = 1 + (i - 1) % (nb - 1)
+    bfw = (np.mod(i, 7).astype(np.float32) / np.float32(7.0)).astype(np.float32)
+    want, _, _ = o.boundary_input(fo.reshape(ml, kl, n), rf, rm, rf[::-1], rm[:, ::-1], 0, 3600.0, 1200.0, ijarf,
+                                  ibfl, ibfl + 1, bfw)
+    got = np.fromfile(os.path.join(d, "fl3_nest.f32"), np.float32)
+    assert np.array_equal(got.view(np.int32), want.ravel().view(np.int32))
