@@ -172,6 +172,34 @@ int wam_gpu_output_parameters(wam_ctx *ctx, int32_t nsea, int32_t kl, int32_t ml
                               const float *udir, const float *ustar, const float *tauw, const float *z0,
                               int32_t orient, float *out);
 
+/* Gridded integrated parameters: WRITE_INT_PAR_OUTPUT
+   (wam_output_module.f90:786-911), the MAP file.  fflag[WAM_MAP_NOUT] is
+   FFLAG_P, non-zero for a parameter IP = index + 1 to be written; only the
+   computed parameters 1-16 and 32 may be set (else WAM_E_ARG).
+
+   wam_gpu_output_grids_dev: from d_out (WAM_NOUT * NSEA, the layout of
+   wam_gpu_output_parameters_dev) to d_grids, one GRID(NX, NY) per flagged
+   parameter in the order of IP: GRID(IXLG(I), KXLT(I)) = the point's value,
+   ZMISS = -9999999 on land.  Asynchronous on the context stream. */
+#define WAM_MAP_NOUT 70
+int wam_gpu_output_grids_dev(wam_ctx *ctx, const int32_t *fflag, const float *d_out, float *d_grids);
+
+/* One output time of the MAP file: the grids of d_out as above, then the
+   records of wam_map_write with the context's grid (NX, NY, corners, NLON_RG =
+   NX and the longitude increment of every row).  append = 0 starts a new file.
+   cdtpro, cdatea: 14 characters each (CDTPRO, and CDATEA, the run's start). */
+int wam_gpu_write_map(wam_ctx *ctx, const char *path, int32_t append, const char *cdtpro, const char *cdatea,
+                      const int32_t *fflag, const float *d_out);
+
+/* Host form of the MAP file's records for one output time (no context):
+     CDTPRO, REAL(NX), REAL(NY), AMOWEP, AMOSOP, AMOEAP, AMONOP, CDATEA;
+     NLON_RG(1:NY), ZDELLO(1:NY); FFLAG_P(1:70) (LOGICAL*4, .TRUE. written as 1);
+     GRID(1:NX, 1:NY) per flagged parameter (grids[k][NY][NX] in order of IP);
+   corners = AMOWEP, AMOSOP, AMOEAP, AMONOP [M_SEC]. */
+int wam_map_write(const char *path, int32_t append, const char *cdtpro, const char *cdatea, int32_t nx, int32_t ny,
+                  const int32_t *corners, const int32_t *nlon_rg, const int32_t *zdello, const int32_t *fflag,
+                  const float *grids);
+
 /* The restart file, binary form: SAVE_RESTART_FILE
    (wam_restart_module.f90:431-600) writes and CONNECT_RESTART (:109-327)
    reads Fortran sequential unformatted records, 4-byte little-endian counts

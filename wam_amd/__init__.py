@@ -5,5 +5,5 @@ C-ABI in include/wam_gpu.h); this package holds its ctypes binding, the host
 mirror of the reference's call sequence, and seeded synthetic workloads.
 """
 from .model import (WamError, WamGpu, boundary_append, boundary_read, boundary_read_header,  # noqa: F401
-                    boundary_write_header, restart_read, restart_write)
+                    boundary_write_header, map_write, restart_read, restart_write)
 from . import workloads  # noqa: F401

@@ -55,6 +55,9 @@ SIGNATURES = {
     "wam_boundary_append": (_I, [ctypes.c_char_p, _I, _I, _I, _P, _P, ctypes.c_char_p, _P, _P, _P]),
     "wam_boundary_read_header": (_I, [ctypes.c_char_p, _P, ctypes.POINTER(ctypes.c_int64)]),
     "wam_boundary_read": (_I, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64), _I, _I, _I] + [_P] * 6),
+    "wam_gpu_output_grids_dev": (_I, [_P, _P, _P, _P]),
+    "wam_gpu_write_map": (_I, [_P, ctypes.c_char_p, _I, ctypes.c_char_p, ctypes.c_char_p, _P, _P]),
+    "wam_map_write": (_I, [ctypes.c_char_p, _I, ctypes.c_char_p, ctypes.c_char_p, _I, _I] + [_P] * 5),
     "wam_gpu_set_timing": (_I, [_P, _I]),
     "wam_gpu_get_timing": (_I, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                 ctypes.POINTER(ctypes.c_int64)]),

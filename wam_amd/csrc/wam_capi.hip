@@ -38,6 +38,8 @@ bool implsch_split_supported(const DevTables &t);
 hipError_t launch_output_params(const DevTables &t, const float *fl3, const float *u10, const float *udir,
                                 const float *ustar, const float *tauw, const float *z0, int orient, float *out,
                                 hipStream_t st);
+hipError_t launch_map_grids(const float *out, int nsea, const int *cell_pt, int ncell, const int *rows, int nrows,
+                            float *grids, hipStream_t st);
 hipError_t launch_boundary_gather(const DevTables &t, const float *fl3, const int *ijarc, int nb, float *fbc,
                                   float *mean, hipStream_t st);
 hipError_t launch_boundary_time(const DevTables &t, int nbinp, const float *f1, const float *m1, const float *f2,
@@ -73,6 +75,9 @@ struct wam_ctx {
     size_t stage_n = 0;
     float *nest = nullptr;                // nest boundary spectra, mean values and indices
     size_t nest_n = 0;
+    int *cell_pt = nullptr;               // grid cell -> sea point (0-based, -1 land), for gridded output
+    float *grids = nullptr;               // gridded parameter fields
+    size_t grids_n = 0;
     bool prepared = false;
     int ncu = 256;                 // compute units of the device
     std::string err;
@@ -928,6 +933,65 @@ int wam_gpu_boundary_input(wam_ctx *ctx, int32_t nsea, int32_t kl, int32_t ml, f
     return WAM_OK;
 }
 
+// BOUT row of parameter IP (1-based): parameters 1-16 and 32 are computed
+static int map_row(int ip) { return ip >= 1 && ip <= 16 ? ip - 1 : (ip == 32 ? 16 : -1); }
+
+int wam_gpu_output_grids_dev(wam_ctx *ctx, const int32_t *fflag, const float *d_out, float *d_grids)
+{
+    if (!ctx || !fflag || !d_out || !d_grids) return WAM_E_ARG;
+    const Tables &t = ctx->tab;
+    int rows[WAM_NOUT], n = 0;
+    for (int ip = 1; ip <= WAM_MAP_NOUT; ip++) {
+        if (!fflag[ip - 1]) continue;
+        const int r = map_row(ip);
+        if (r < 0) { ctx->err = "gridded output: only parameters 1-16 and 32 are computed"; return WAM_E_ARG; }
+        rows[n++] = r;
+    }
+    const int nx = t.cfg.nx, ny = t.cfg.ny, ncell = nx * ny;
+    if (!ctx->cell_pt) {
+        std::vector<int> cp((size_t)ncell, -1);
+        for (int ij = 0; ij < t.NSEA; ij++) cp[(size_t)(t.KXLT[ij] - 1) * nx + (t.IXLG[ij] - 1)] = ij;
+        HIPCHK(hipMalloc(&ctx->cell_pt, (size_t)ncell * sizeof(int)));
+        HIPCHK(hipMemcpy(ctx->cell_pt, cp.data(), (size_t)ncell * sizeof(int), hipMemcpyHostToDevice));
+    }
+    HIPCHK(launch_map_grids(d_out, t.NSEA, ctx->cell_pt, ncell, rows, n, d_grids, ctx->stream));
+    return WAM_OK;
+}
+
+int wam_gpu_write_map(wam_ctx *ctx, const char *path, int32_t append, const char *cdtpro, const char *cdatea,
+                      const int32_t *fflag, const float *d_out)
+{
+    if (!ctx || !path || !cdtpro || !cdatea || !fflag || !d_out) return WAM_E_ARG;
+    const Tables &t = ctx->tab;
+    int n = 0;
+    for (int i = 0; i < WAM_MAP_NOUT; i++) n += fflag[i] != 0;
+    const int nx = t.cfg.nx, ny = t.cfg.ny;
+    const size_t ncell = (size_t)nx * ny;
+    if (ctx->grids_n < n * ncell) {
+        if (ctx->grids) {
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            (void)hipFree(ctx->grids);
+        }
+        ctx->grids = nullptr;
+        ctx->grids_n = 0;
+        HIPCHK(hipMalloc(&ctx->grids, n * ncell * sizeof(float)));
+        ctx->grids_n = n * ncell;
+    }
+    int rc = n ? wam_gpu_output_grids_dev(ctx, fflag, d_out, ctx->grids) : WAM_OK;
+    if (rc) return rc;
+    std::vector<float> host(n * ncell);
+    if (n) HIPCHK(hipMemcpyAsync(host.data(), ctx->grids, n * ncell * sizeof(float), hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const int32_t corners[4] = {(int32_t)t.cfg.amowep, (int32_t)t.cfg.amosop, (int32_t)t.cfg.amoeap,
+                                (int32_t)t.cfg.amonop};
+    std::vector<int32_t> nlon((size_t)ny, nx), zdello((size_t)ny, t.ZDELLO);
+    rc = wam_map_write(path, append, cdtpro, cdatea, nx, ny, corners, nlon.data(), zdello.data(), fflag,
+                       host.data());
+    if (rc) ctx->err = "gridded output: cannot write the file";
+    return rc;
+}
+
 int wam_gpu_set_propagation_weights(wam_ctx *ctx, int32_t stored)
 {
     if (!ctx) return WAM_E_ARG;
@@ -1127,6 +1191,8 @@ void wam_gpu_finalize(wam_ctx *ctx)
     if (ctx->fscratch) (void)hipFree(ctx->fscratch);
     if (ctx->stage) (void)hipFree(ctx->stage);
     if (ctx->nest) (void)hipFree(ctx->nest);
+    if (ctx->cell_pt) (void)hipFree(ctx->cell_pt);
+    if (ctx->grids) (void)hipFree(ctx->grids);
     timing_clear(ctx);
     for (void *p : {(void *)ctx->isx.ft, (void *)ctx->isx.slt, (void *)ctx->isx.flt, (void *)ctx->isx.smt,
                     (void *)ctx->isx.axy, (void *)ctx->isx.pp, (void *)ctx->isx.llws, (void *)ctx->isx.tick})

@@ -129,4 +129,33 @@ hipError_t launch_output_params(const DevTables &t, const float *fl3, const floa
     return hipGetLastError();
 }
 
+// WRITE_INT_PAR_OUTPUT's GRID(IXLG(I), KXLT(I)) = BLOCK(I), ZMISS on land
+// (wam_output_module.f90:866-868) for the rows picked by `rows`: one lane per
+// grid cell, reading its sea point through the cell -> point map (-1 land);
+// the sea-point order is the cell order, so the reads are near-coalesced.
+struct MapRows {
+    int p[NOUT];
+};
+
+__global__ void __launch_bounds__(256) map_grid_kernel(const float *__restrict__ out, int nsea,
+                                                       const int *__restrict__ cell_pt, int ncell, MapRows rows,
+                                                       float *__restrict__ grids)
+{
+    const int c = blockIdx.x * 256 + threadIdx.x, r = blockIdx.y;
+    if (c >= ncell) return;
+    const int ij = cell_pt[c];
+    grids[(size_t)r * ncell + c] = ij >= 0 ? out[(size_t)rows.p[r] * nsea + ij] : -9999999.f;
+}
+
+hipError_t launch_map_grids(const float *out, int nsea, const int *cell_pt, int ncell, const int *rows, int nrows,
+                            float *grids, hipStream_t st)
+{
+    if (nrows <= 0) return hipSuccess;
+    MapRows r{};
+    for (int i = 0; i < nrows && i < NOUT; i++) r.p[i] = rows[i];
+    hipLaunchKernelGGL(map_grid_kernel, dim3((ncell + 255) / 256, nrows), dim3(256), 0, st, out, nsea, cell_pt,
+                       ncell, r, grids);
+    return hipGetLastError();
+}
+
 }  // namespace wam

@@ -507,6 +507,7 @@ static int grid(Tables &t, const uint8_t *mask, std::string &err)
     int zd = g.iper ? wam_nint((float)(g.amoeap - g.amowep + g.xdello) / (float)NX)
                     : wam_nint((float)(g.amoeap - g.amowep) / (float)(NX - 1));
     for (int k = 0; k < NY; k++) t.DELLAM[k] = (((float)zd / (float)M_DEGREE) * t.k.CIRC) / 360.f;
+    t.ZDELLO = zd;
     // sea-point numbering and a cell -> sea point map
     std::vector<int> num((size_t)NX * NY, 0);
     int ns = 0;

@@ -60,6 +60,7 @@ struct Tables {
     float DELPHI = 0;
     std::vector<float> SINPH, COSPH, DELLAM;             // [NY]
     std::vector<int> IXLG, KXLT;                         // 1-based
+    int ZDELLO = 0;                                      // longitude increment of every row [M_SEC]
     std::vector<int> KLAT_S, KLAT_N, KLON_W, KLON_E;     // 1-based sea point, 0 = land
     std::vector<float> DCO, DPSN_S, DPSN_N;              // DCO[NSEA]
     // propagation per-point data (PREPARE_PROPAGATION)

@@ -147,6 +147,37 @@ def boundary_read(path, pos, nb, kl, ml):
     return out, p.value
 
 
+MAP_NOUT = 70
+
+
+def _fflag(params):
+    """FFLAG_P(1:70) as int32 from the 1-based parameter numbers to write."""
+    f = np.zeros(MAP_NOUT, np.int32)
+    for ip in params:
+        f[int(ip) - 1] = 1
+    return f
+
+
+def _date14(d):
+    return (d or "").encode("ascii").ljust(14, b" ")[:14]
+
+
+def map_write(path, cdtpro, cdatea, nx, ny, corners, nlon_rg, zdello, params, grids, append=False):
+    """WRITE_INT_PAR_OUTPUT's records for one output time (wam_map_write):
+    corners = AMOWEP, AMOSOP, AMOEAP, AMONOP [M_SEC]; grids [len(params)][NY][NX]
+    for the parameter numbers params in increasing order."""
+    lib = _lib.load()
+    c = np.ascontiguousarray(corners, np.int32)
+    nl, zd = np.ascontiguousarray(nlon_rg, np.int32), np.ascontiguousarray(zdello, np.int32)
+    g = np.ascontiguousarray(grids, np.float32)
+    if list(params) != sorted(params) or g.shape != (len(params), ny, nx) or nl.size != ny or zd.size != ny:
+        raise ValueError("params increasing, grids [len(params)][NY][NX], nlon_rg and zdello [NY]")
+    rc = lib.wam_map_write(os.fsencode(path), int(bool(append)), _date14(cdtpro), _date14(cdatea), nx, ny, _ptr(c),
+                           _ptr(nl), _ptr(zd), _ptr(_fflag(params)), _ptr(g))
+    if rc:
+        raise WamError(rc, f"wam_map_write({path})")
+
+
 class WamGpu:
     def __init__(self, cfg, mask, host_only=False):
         self.lib = _lib.load()
@@ -304,6 +335,17 @@ class WamGpu:
         else:
             self._check(self.lib.wam_gpu_boundary_input_dev(self.ctx, nbinp, *args, _ptr(fl3), _ptr(fi), _ptr(mi)))
         return fi, mi
+
+    def output_grids_dev(self, params, out, grids):
+        """The gridded fields of the parameter numbers params (1-16, 32) from
+        out (device tensor, [NOUT][NSEA]) into grids ([len(params)][NY][NX])."""
+        self._check(self.lib.wam_gpu_output_grids_dev(self.ctx, _ptr(_fflag(params)), _ptr(out), _ptr(grids)))
+
+    def write_map(self, path, cdtpro, cdatea, params, out, append=False):
+        """One output time of the MAP file from the device parameters out
+        ([NOUT][NSEA], output_parameters_dev) for the parameter numbers params."""
+        self._check(self.lib.wam_gpu_write_map(self.ctx, os.fsencode(path), int(bool(append)), _date14(cdtpro),
+                                               _date14(cdatea), _ptr(_fflag(params)), _ptr(out)))
 
     def set_timing(self, on):
         """Records a HIP event pair around every PROPAGS / IMPLSCH kernel
