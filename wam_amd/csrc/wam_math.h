@@ -9,7 +9,9 @@
  * (FRCUTINDEX :3639).  Every routine here uses only IEEE-754 single-precision
  * +, -, *, / and exact bit manipulation, evaluated in a fixed order, so the
  * same source gives bit-identical results on the host (gcc, SSE, no FMA
- * contraction) and on gfx950 (hipcc with -ffp-contract=off).
+ * contraction) and on gfx950 (hipcc with -ffp-contract=off).  Truncation,
+ * floor and ceiling are exact operations: the device takes them from the
+ * hardware instructions, the host from bit manipulation.
  *
  * Algorithms: Cody-Waite range reduction followed by the minimax polynomials
  * published with the Cephes library (S. L. Moshier, "Methods and Programs
@@ -55,22 +57,34 @@ WAM_HD float wam_minf(float a, float b) { return a < b ? a : b; }
 /* floor/ceil/round on floats without libm; exact. */
 WAM_HD float wam_truncf(float x)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_truncf(x);                  /* v_trunc_f32: exact, the same value */
+#else
     int32_t i = wam_f2i(x);
     int e = ((i >> 23) & 0xff) - 127;
     if (e >= 23) return x;                       /* integral, inf or nan */
     if (e < 0) return wam_i2f(i & (int32_t)0x80000000);   /* |x| < 1 -> +-0 */
     int32_t m = (int32_t)0x007fffff >> e;
     return wam_i2f(i & ~m);
+#endif
 }
 WAM_HD float wam_floorf(float x)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_floorf(x);                  /* v_floor_f32: exact, the same value */
+#else
     float t = wam_truncf(x);
     return (t > x) ? t - 1.0f : t;
+#endif
 }
 WAM_HD float wam_ceilf(float x)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_ceilf(x);                   /* v_ceil_f32: exact, the same value */
+#else
     float t = wam_truncf(x);
     return (t < x) ? t + 1.0f : t;
+#endif
 }
 /* Fortran NINT: nearest integer, halves away from zero. */
 WAM_HD int wam_nint(float x)
