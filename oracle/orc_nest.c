@@ -11,7 +11,11 @@
  *    (:891-941) and STRSPEC (:945-1006).
  * A spectrum here is F(KL, ML), f[m*KL + k].
  *
- * What pins it (tests/test_nest.py): ROTSPEC by whole direction steps is a
+ * What pins it: golden vectors of the reference's own TOTAL_ENERGY, FEMEAN,
+ * MEAN_DIRECTION, INTSPEC, ROTSPEC and STRSPEC built with flang
+ * (tests/test_ref_golden_nest.py: EMEAN, FMEAN and ROTSPEC bit-identical,
+ * THQ, STRSPEC and INTSPEC within the libm boundary); and (tests/test_nest.py)
+ * ROTSPEC by whole direction steps is a
  * cyclic shift and by zero the identity; STRSPEC by CO**n is a frequency
  * shift, by one the identity, by a factor outside the range zero; INTSPEC at
  * the end points returns either spectrum, is symmetric under exchanging the
