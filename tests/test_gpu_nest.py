@@ -109,3 +109,38 @@ def test_boundary_input(pair, which, del1l):
     assert bits_equal(d.cpu().numpy(), want)
     with pytest.raises(WamError):
         g.boundary_input(fl3.copy(), f1, m1, f2, m2, which, 3600.0, del1l, ijarf, ibfl, ibfr + nbinp, bfw)
+
+
+@pytest.mark.parametrize("name", ["kl24_ml25", "kl36_ml36"])
+def test_against_reference_golden(name):
+    """The HIP path against the reference's own BOUNDARY_OUTPUT means and
+    INTSPEC (tests/golden/ref_nest_*.npz, the flang-built reference): the
+    bars of the oracle's golden test."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", f"ref_nest_{name}.npz"))
+    cfg = W.base_config(kl=int(g["kl"]), ml=int(g["ml"]))
+    mask = np.ones((cfg["ny"], cfg["nx"]), np.uint8)
+    gpu = WamGpu(cfg, mask)
+    f, mean = g["f"], g["mean"]
+    nb = f.shape[0]
+    fl3 = np.zeros((gpu.ml, gpu.kl, gpu.nsea), np.float32)
+    fl3[:, :, :nb] = f.transpose(1, 2, 0)
+    print(f"reference golden {name}", flush=True)
+    _, m = gpu.boundary_output(fl3, np.arange(1, nb + 1, dtype=np.int32))
+    assert np.array_equal(m[0], mean[0]) and np.array_equal(m[2], mean[2])
+    np.testing.assert_allclose(m[1], mean[1], rtol=1e-6)
+    for (i1, i2, d12, d1l), want in zip(g["pairs"], g["intspec"]):
+        a, b = int(i1) - 1, int(i2) - 1
+        out = np.zeros_like(fl3)
+        if d12 == 1.0:            # the space step: INTSPEC(1, BFW, FI(IBFL), FI(IBFR))
+            gpu.boundary_input(out, f, mean, None, None, 1, 3600.0, 0.0, np.array([1], np.int32),
+                               np.array([a + 1], np.int32), np.array([b + 1], np.int32), np.array([d1l], np.float32))
+            got = out[:, :, 0]
+        else:                     # the time step: INTSPEC(DEL12, DEL1L, F1, F2)
+            fi, _ = gpu.boundary_input(out, f[a:a + 1], mean[:, a:a + 1], f[b:b + 1], mean[:, b:b + 1], 0, d12, d1l,
+                                       np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.int32),
+                                       np.zeros(0, np.float32))
+            got = fi[0]
+        err = np.abs(got.astype(np.float64) - want) - 2e-7 * np.abs(want).max()
+        assert np.all(err <= 5e-5 * np.abs(want)), (i1, i2, d12, d1l)
+    gpu.close()
