@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/wam_gpu.h"
@@ -852,17 +853,21 @@ int wam_gpu_boundary_input_dev(wam_ctx *ctx, int32_t nbinp, const float *f1, con
     }
     // the reference stores the points one after another: of a sea point
     // listed twice the last entry stays; the kernel writes each point once
-    std::vector<int32_t> last(nbounf > 0 ? t.NSEA : 0, -1);
-    for (int32_t i = 0; i < nbounf; i++) last[ijarf[i] - 1] = i;
+    std::vector<std::pair<int32_t, int32_t>> byp((size_t)nbounf);    // (IJARF, entry), sorted
+    for (int32_t i = 0; i < nbounf; i++) byp[i] = {ijarf[i], i};
+    std::sort(byp.begin(), byp.end());
+    std::vector<int32_t> keep;                                       // the last entry of each point
+    for (size_t j = 0; j < byp.size(); j++)
+        if (j + 1 == byp.size() || byp[j + 1].first != byp[j].first) keep.push_back(byp[j].second);
+    std::sort(keep.begin(), keep.end());
     std::vector<int32_t> ij, il, ir;
     std::vector<float> w;
-    for (int32_t i = 0; i < nbounf; i++)
-        if (last[ijarf[i] - 1] == i) {
-            ij.push_back(ijarf[i]);
-            il.push_back(ibfl[i]);
-            ir.push_back(ibfr[i]);
-            w.push_back(bfw[i]);
-        }
+    for (const int32_t i : keep) {
+        ij.push_back(ijarf[i]);
+        il.push_back(ibfl[i]);
+        ir.push_back(ibfr[i]);
+        w.push_back(bfw[i]);
+    }
     const size_t NB = (size_t)t.KL * t.ML, np = (size_t)nbinp, nf = ij.size();
     int rc = ensure_nest(ctx, 3 * np * NB + 9 * np + 4 * nf);
     if (rc) return rc;
