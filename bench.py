@@ -125,7 +125,7 @@ def bench_nest(args, torch, g, cfg, mask, fl3):
 
     def exchange():
         g.boundary_output(fl3, ijarc)
-        g.boundary_input(fl3, f1, m1, f2, m2, 0, 3600.0, 1800.0, ijarf, ibfl, ibfr, bfw)
+        g.boundary_input(fl3, f1, m1, f2, m2, 0, 3600.0, 1800.0, ijarf, ibfl, ibfr, bfw, return_set=False)
 
     for _ in range(args.warmup):
         exchange()

@@ -312,19 +312,20 @@ class WamGpu:
                                                              _ptr(mean)))
         return fbc, mean
 
-    def boundary_input(self, fl3, f1, m1, f2, m2, which, del12, del1l, ijarf, ibfl, ibfr, bfw):
+    def boundary_input(self, fl3, f1, m1, f2, m2, which, del12, del1l, ijarf, ibfl, ibfr, bfw, return_set=True):
         """BOUNDARY_INPUT's time and space interpolation into FL3 (a host
         array [ML][KL][NSEA], updated in place, or a device tensor): which = 1
         takes (f1, m1), 2 takes (f2, m2), 0 interpolates at del1l of del12;
         f* are [nbinp][ML][KL], m* [3][nbinp] (EMEAN, THQ, FMEAN).  Returns
-        the set at the model time (FI, MI)."""
+        the set at the model time (FI, MI), or nothing with return_set=False
+        (no copy back)."""
         a = lambda x: None if x is None else np.ascontiguousarray(x, np.float32)
         i = lambda x: np.ascontiguousarray(x, np.int32)
         f1, m1, f2, m2, w = a(f1), a(m1), a(f2), a(m2), a(bfw)
         ij, il, ir = i(ijarf), i(ibfl), i(ibfr)
         nbinp = (m1 if m1 is not None else m2).shape[-1]
-        fi = np.empty((nbinp, self.ml, self.kl), np.float32)
-        mi = np.empty((3, nbinp), np.float32)
+        fi = np.empty((nbinp, self.ml, self.kl), np.float32) if return_set else None
+        mi = np.empty((3, nbinp), np.float32) if return_set else None
         args = [_ptr(f1), _ptr(m1), _ptr(f2), _ptr(m2), int(which), float(del12), float(del1l), ij.size, _ptr(ij),
                 _ptr(il), _ptr(ir), _ptr(w)]
         if isinstance(fl3, np.ndarray):
@@ -334,7 +335,7 @@ class WamGpu:
                                                         *args, _ptr(fi), _ptr(mi)))
         else:
             self._check(self.lib.wam_gpu_boundary_input_dev(self.ctx, nbinp, *args, _ptr(fl3), _ptr(fi), _ptr(mi)))
-        return fi, mi
+        return (fi, mi) if return_set else None
 
     def output_grids_dev(self, params, out, grids):
         """The gridded fields of the parameter numbers params (1-16, 32) from
