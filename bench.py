@@ -55,12 +55,28 @@ def device_spectrum(torch, cfg, udir_d, fp=0.3, alpha=0.018, gamma=3.3):
     return (e[:, None, None] * st[None, :, :]).contiguous()
 
 
+def host_cpu():
+    """nproc and the CPU model of the host."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return os.cpu_count() or 1, model
+
+
 def cpu_baseline(args):
     """The oracle (C restatement, REAL*4) on the host cores, on a bounded
-    sub-box of the same workload (same statistics, 36 x 36, shallow, currents)."""
+    sub-box of the same workload (same statistics, 36 x 36, shallow, currents).
+    Threads: OMP_NUM_THREADS when set (the host's CPU share per GPU), else
+    every core."""
     from oracle.oracle import Oracle
     from wam_amd import workloads as W
-    threads = max(1, min(16, os.cpu_count() or 1))
+    nproc, model = host_cpu()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or nproc
     os.environ["OMP_NUM_THREADS"] = str(threads)
     cfg, mask, f = W.c2_case(ny=args.cpu_ny, nx=args.cpu_nx)
     o = Oracle(cfg, mask)
@@ -78,7 +94,7 @@ def cpu_baseline(args):
         r = o.implsch(fl3, f["u10"], f["udir"], tauw, z0, f["depth"], nthreads=threads)
         fl3, tauw, z0 = r["fl3"], r["tauw"], r["z0"]
     dt = (time.perf_counter() - t0) / steps
-    return dict(value=n / dt, unit="sea-point updates/s", cores=threads, kind="port",
+    return dict(value=n / dt, unit="sea-point updates/s", cores=threads, nproc=nproc, cpu_model=model, kind="port",
                 sample=f"oracle REAL*4 on a {args.cpu_ny}x{args.cpu_nx} sub-box of the same workload "
                        f"({n} sea points, 36x36, COUNTER={counter}), {steps} steps, OpenMP over points",
                 seconds_per_step=dt)
@@ -319,13 +335,14 @@ def main():
             st = opc["stages"][stage_of[name]]
             fpp, tpp, kname = st["flops_per_point"], st["trans_per_point"], f"{name}_kernel<1>"
         tflops = fpp * n / (ms * 1e-3) / 1e12
-        roofs[name] = dict(kernel=kname, bound="mfma", achieved=round(tflops, 3), peak=FP32_PEAK_TFLOPS,
+        roofs[name] = dict(kernel=kname, bound="valu", achieved=round(tflops, 3), peak=FP32_PEAK_TFLOPS,
                            unit="TFLOP/s", frac=round(tflops / FP32_PEAK_TFLOPS, 4),
                            traffic=traffic.get(kname, {}).get("bytes_per_dispatch"),
                            algorithmic_flops_per_launch=round(fpp * n), flops_per_point=round(fpp, 1),
                            transcendentals_per_point=round(tpp, 1), avg_launch_ms=round(ms, 3),
-                           note="FP32 VALU work (no GEMM-shaped arithmetic); peak is MI355X's FP32 157.3 "
-                                "TFLOP/s; unfused add/mul/div/sqrt as counted by the oracle's op counter")
+                           note="FP32 vector (VALU) work, no GEMM-shaped arithmetic, so no MFMA; peak is "
+                                "MI355X's FP32 vector 157.3 TFLOP/s; unfused add/mul/div/sqrt as counted by the "
+                                "oracle's op counter")
     # the dominant kernel: the largest share of the step's GPU time
     share = {k: v * (counter if k == "propags" else 1) for k, v in ktime.items()}
     dominant = max(share, key=share.get)

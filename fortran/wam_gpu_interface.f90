@@ -80,6 +80,12 @@ INTERFACE
       TYPE(C_PTR), VALUE :: ctx
       TYPE(C_PTR) :: c_wam_gpu_last_error
    END FUNCTION
+   INTEGER(C_INT64_T) FUNCTION c_wam_gpu_get_table(ctx, name, out, n) BIND(C, NAME='wam_gpu_get_table')
+      IMPORT :: C_INT64_T, C_PTR, C_CHAR
+      TYPE(C_PTR), VALUE :: ctx, out
+      CHARACTER(KIND=C_CHAR), DIMENSION(*) :: name
+      INTEGER(C_INT64_T), VALUE :: n
+   END FUNCTION
    REAL(C_DOUBLE) FUNCTION c_wam_gpu_get_scalar(ctx, name) BIND(C, NAME='wam_gpu_get_scalar')
       IMPORT :: C_DOUBLE, C_PTR, C_CHAR
       TYPE(C_PTR), VALUE :: ctx
@@ -125,14 +131,26 @@ SUBROUTINE WAM_GPU_SETUP (CFG, MASK)
    NSEA_GPU = NINT(c_wam_gpu_get_scalar(CTX, 'NSEA'//C_NULL_CHAR))
 END SUBROUTINE WAM_GPU_SETUP
 
-SUBROUTINE WAM_GPU_PREPARE_PROPAGATION (DEPTH, U, V)
+SUBROUTINE WAM_GPU_PREPARE_PROPAGATION (DEPTH, U, V, INDEP)
    REAL(C_FLOAT), INTENT(IN), TARGET, CONTIGUOUS :: DEPTH(:)
    REAL(C_FLOAT), INTENT(IN), TARGET, CONTIGUOUS, OPTIONAL :: U(:), V(:)
+   !! the depth table index of every sea point, which the reference's
+   !! PREPARE_PROPAGATION stores in the module array INDEP
+   !! (wam_propagation_module.f90:334-336) and WAMODEL passes to IMPLSCH
+   INTEGER(C_INT32_T), INTENT(OUT), TARGET, CONTIGUOUS, OPTIONAL :: INDEP(:)
    TYPE(C_PTR) :: PU, PV
+   INTEGER(C_INT64_T) :: N
    PU = C_NULL_PTR; PV = C_NULL_PTR
    IF (PRESENT(U)) PU = C_LOC(U)
    IF (PRESENT(V)) PV = C_LOC(V)
    CALL CHECK(c_wam_gpu_prepare_propagation(CTX, C_LOC(DEPTH), PU, PV), 'WAM_GPU_PREPARE_PROPAGATION')
+   IF (PRESENT(INDEP)) THEN
+      N = c_wam_gpu_get_table(CTX, 'INDEP'//C_NULL_CHAR, C_LOC(INDEP), INT(SIZE(INDEP), C_INT64_T))
+      IF (N .NE. SIZE(INDEP)) THEN
+         WRITE (*,*) ' *** WAM GPU ERROR in WAM_GPU_PREPARE_PROPAGATION: INDEP must have NSEA elements'
+         ERROR STOP 1
+      END IF
+   END IF
 END SUBROUTINE WAM_GPU_PREPARE_PROPAGATION
 
 !  PREPARE_PROPAGATION from depth and currents already in device memory (NSEA
@@ -151,17 +169,50 @@ SUBROUTINE PROPAGS_GPU (F3)
 END SUBROUTINE PROPAGS_GPU
 
 !  Drop-in for IMPLSCH (FL3, U10, UDIR, TAUW, USTAR, Z0, ROAIRN, WSTAR, DEPTH, INDEP).
-SUBROUTINE IMPLSCH_GPU (FL3, U10, UDIR, TAUW, USTAR, Z0, ROAIRN, WSTAR, DEPTH, INDEP)
+!  DEPTH and INDEP must be those of the last WAM_GPU_PREPARE_PROPAGATION, as in
+!  the reference.  The reference's side outputs live in module variables: PHIAW
+!  and, when LCFLX is set (wamodel.f90:342, output times), PHIOC, TAUOC_X,
+!  TAUOC_Y, PHIBOT, TAUBOT_X, TAUBOT_Y of wam_flux_module.f90:15-21 (WNFLUXES).
+!  Here they are optional arguments, so a host passes its own module arrays:
+!     CALL IMPLSCH_GPU (..., DEPTH, INDEP, LCFLX=LCFLX, PHIAW=PHIAW, PHIOC=PHIOC, ...)
+SUBROUTINE IMPLSCH_GPU (FL3, U10, UDIR, TAUW, USTAR, Z0, ROAIRN, WSTAR, DEPTH, INDEP, LCFLX, PHIAW, PHIOC, &
+&                       TAUOC_X, TAUOC_Y, PHIBOT, TAUBOT_X, TAUBOT_Y)
    REAL(C_FLOAT),      INTENT(INOUT), TARGET, CONTIGUOUS :: FL3(:,:,:)
    REAL(C_FLOAT),      INTENT(IN),    TARGET, CONTIGUOUS :: U10(:), UDIR(:), ROAIRN(:), WSTAR(:)
    REAL(C_FLOAT),      INTENT(INOUT), TARGET, CONTIGUOUS :: TAUW(:), Z0(:)
    REAL(C_FLOAT),      INTENT(OUT),   TARGET, CONTIGUOUS :: USTAR(:)
    REAL(C_FLOAT),      INTENT(IN),    TARGET, CONTIGUOUS :: DEPTH(:)
    INTEGER(C_INT32_T), INTENT(IN),    TARGET, CONTIGUOUS :: INDEP(:)
-   CALL CHECK(c_wam_gpu_implsch(CTX, INT(SIZE(FL3,1), C_INT32_T), INT(SIZE(FL3,2), C_INT32_T), &
+   LOGICAL,            INTENT(IN),                      OPTIONAL :: LCFLX
+   REAL(C_FLOAT),      INTENT(OUT),   TARGET, CONTIGUOUS, OPTIONAL :: PHIAW(:)
+   REAL(C_FLOAT),      INTENT(OUT),                     OPTIONAL :: PHIOC(:), TAUOC_X(:), TAUOC_Y(:), PHIBOT(:), &
+&                                                                   TAUBOT_X(:), TAUBOT_Y(:)
+   REAL(C_FLOAT), ALLOCATABLE, TARGET :: FLUX(:,:)
+   TYPE(C_PTR) :: PPH, PFX
+   INTEGER(C_INT32_T) :: LC
+   INTEGER :: N
+   N = SIZE(FL3,1)
+   LC = 0_C_INT32_T
+   IF (PRESENT(LCFLX)) THEN
+      IF (LCFLX) LC = 1_C_INT32_T
+   END IF
+   PPH = C_NULL_PTR; PFX = C_NULL_PTR
+   IF (PRESENT(PHIAW)) PPH = C_LOC(PHIAW)
+   IF (LC .NE. 0) THEN
+      ALLOCATE (FLUX(N, 6))
+      PFX = C_LOC(FLUX)
+   END IF
+   CALL CHECK(c_wam_gpu_implsch(CTX, INT(N, C_INT32_T), INT(SIZE(FL3,2), C_INT32_T), &
 &             INT(SIZE(FL3,3), C_INT32_T), C_LOC(FL3), C_LOC(U10), C_LOC(UDIR), C_LOC(TAUW), C_LOC(USTAR), &
-&             C_LOC(Z0), C_LOC(ROAIRN), C_LOC(WSTAR), C_LOC(DEPTH), C_LOC(INDEP), 0_C_INT32_T, C_NULL_PTR, &
-&             C_NULL_PTR), 'IMPLSCH_GPU')
+&             C_LOC(Z0), C_LOC(ROAIRN), C_LOC(WSTAR), C_LOC(DEPTH), C_LOC(INDEP), LC, PPH, PFX), 'IMPLSCH_GPU')
+   IF (LC .NE. 0) THEN
+      IF (PRESENT(PHIOC))    PHIOC    = FLUX(:, 1)
+      IF (PRESENT(TAUOC_X))  TAUOC_X  = FLUX(:, 2)
+      IF (PRESENT(TAUOC_Y))  TAUOC_Y  = FLUX(:, 3)
+      IF (PRESENT(PHIBOT))   PHIBOT   = FLUX(:, 4)
+      IF (PRESENT(TAUBOT_X)) TAUBOT_X = FLUX(:, 5)
+      IF (PRESENT(TAUBOT_Y)) TAUBOT_Y = FLUX(:, 6)
+   END IF
 END SUBROUTINE IMPLSCH_GPU
 
 !  Integrated parameters of COMPUTE_OUTPUT_PARAMETER (wam_output_module.f90:481-601):

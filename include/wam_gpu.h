@@ -108,7 +108,10 @@ int wam_gpu_set_propagation_weights(wam_ctx *ctx, int32_t stored);
 int wam_gpu_propags(wam_ctx *ctx, int32_t nijs, int32_t nijl, int32_t kl, int32_t ml, float *f3);
 
 /* Drop-in IMPLSCH with the reference's argument list (host memory, NSEA
-   points).  lcflx != 0 also computes the WNFLUXES side outputs.  phiaw (may be
+   points).  DEPTH and INDEP (either may be NULL) must be the fields of the
+   last PREPARE_PROPAGATION, as the reference's module arrays are; a DEPTH
+   that differs, or an INDEP that differs in shallow water (it is undefined
+   in deep water), returns WAM_E_STATE.  lcflx != 0 also computes the WNFLUXES side outputs.  phiaw (may be
    NULL) receives PHIAW; fluxes (may be NULL) receives [6][NSEA] = PHIOC,
    TAUOC_X, TAUOC_Y, PHIBOT, TAUBOT_X, TAUBOT_Y (wam_flux_module.f90:15-21). */
 int wam_gpu_implsch(wam_ctx *ctx, int32_t nsea, int32_t kl, int32_t ml, float *fl3,
@@ -298,6 +301,14 @@ int wam_gpu_sync(wam_ctx *ctx);
 int64_t wam_gpu_get_table(wam_ctx *ctx, const char *name, void *out, int64_t n);
 double wam_gpu_get_scalar(wam_ctx *ctx, const char *name);
 const char *wam_gpu_last_error(wam_ctx *ctx);
+
+/* The shared elementary functions evaluated on the device, for parity
+   checks of the device build against the host build (no counterpart in the
+   reference, which calls its runtime's intrinsics): out[i] = f(x[i], y[i])
+   for fn 0 EXP, 1 LOG, 2 LOG10, 3 TANH, 4 COSH, 5 SINH, 6 COS, 7 SIN,
+   8 x**y (real y), 9 NINT, 10 x**INT(y), 11 ATAN2(x, y); host arrays of n
+   (y may be NULL for one-argument functions), device 0, synchronous. */
+int wam_gpu_eval_math(int32_t fn, const float *x, const float *y, float *out, int64_t n);
 
 void wam_gpu_finalize(wam_ctx *ctx);
 

@@ -1,8 +1,9 @@
 """The Fortran host (fortran/): ISO_C_BINDING shims with the reference's
 IMPLSCH/PROPAGS argument lists driving the HIP library, as WAMODEL does
 (wamodel.f90:286-360).  The CPU test checks that flang builds and links it;
-the GPU test runs a few steps, then the integrated output parameters, and
-compares both with the oracle bit-for-bit."""
+the GPU test runs a few steps (the last with LCFLX, returning PHIAW and the
+WNFLUXES fields), then the integrated output parameters, and compares all of
+them with the oracle bit-for-bit."""
 import os
 import subprocess
 
@@ -69,12 +70,19 @@ def test_fortran_host_matches_oracle(tmp_path, which):
     o.prepare_propagation(f["depth"], f["u"], f["v"])
     n = o.nsea
     fo, tw, z0 = fl3, np.zeros(n, np.float32), np.zeros(n, np.float32)
-    for _ in range(nstep):
+    for istep in range(nstep):
         fo = o.propags(fo)
-        r = o.implsch(fo, f["u10"], f["udir"], tw, z0, f["depth"])
+        # the last step is at an output time: LCFLX (wamodel.f90:342)
+        r = o.implsch(fo, f["u10"], f["udir"], tw, z0, f["depth"], lcflx=istep == nstep - 1)
         fo, tw, z0 = r["fl3"], r["tauw"], r["z0"]
     assert np.array_equal(out.view(np.int32), fo.view(np.int32))
     assert np.array_equal(ust.view(np.int32), r["ustar"].view(np.int32))
+    # PHIAW and the six WNFLUXES fields returned through IMPLSCH_GPU's optional arguments
+    phiaw = np.fromfile(os.path.join(d, "phiaw_out.f32"), np.float32)
+    flx = np.fromfile(os.path.join(d, "flux_out.f32"), np.float32).reshape(6, n)
+    assert np.array_equal(phiaw.view(np.int32), np.asarray(r["phiaw"], np.float32).view(np.int32))
+    assert np.array_equal(flx.view(np.int32), np.asarray(r["flux"], np.float32).reshape(6, n).view(np.int32))
+    assert np.abs(flx).max() > 0
     # BOUT(NSEA,17) of OUTPUT_PARAMETERS_GPU (directions "coming from")
     bout = np.fromfile(os.path.join(d, "bout.f32"), np.float32).reshape(17, n)
     cur = cfg["refr_c"] != 0

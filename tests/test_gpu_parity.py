@@ -217,3 +217,25 @@ def test_call_order_is_checked():
     with pytest.raises(WamError) as e:
         g.propags(fl3)
     assert e.value.code == 4
+
+
+def test_depth_and_indep_must_match_the_prepared_fields(case):
+    """IMPLSCH's DEPTH and INDEP arguments (wamodel.f90:354 passes the module
+    arrays PREPARE_PROPAGATION used): the prepared ones are accepted and give
+    the same result as none; a changed depth, or a changed INDEP in shallow
+    water, reports WAM_E_STATE instead of silently using the old fields."""
+    g, o, cfg, f = case["g"], case["o"], case["cfg"], case["f"]
+    n = g.nsea
+    z = np.zeros(n, np.float32)
+    depth = np.asarray(f["depth"], np.float32)
+    indep = g.table("INDEP").astype(np.int32)
+    a = g.implsch(case["fl3"], f["u10"], f["udir"], z, z)
+    b = g.implsch(case["fl3"], f["u10"], f["udir"], z, z, depth=depth, indep=indep)
+    assert bits_equal(a["fl3"], b["fl3"])
+    with pytest.raises(WamError) as e:
+        g.implsch(case["fl3"], f["u10"], f["udir"], z, z, depth=depth * np.float32(1.5), indep=indep)
+    assert e.value.code == 4
+    if cfg["shallow"]:
+        with pytest.raises(WamError) as e:
+            g.implsch(case["fl3"], f["u10"], f["udir"], z, z, depth=depth, indep=indep + 1)
+        assert e.value.code == 4

@@ -2,17 +2,21 @@
 tests/golden/make_ref_implsch.py, which runs wam_source_module.f90:374-700
 built from the reference sources (oracle/Makefile.ref).  CPU only.
 
-Tolerances.  The oracle evaluates the Fortran intrinsics with wam_math.h
-(deterministic single precision, <= 2 ulp for exp/log/tanh); the reference
-used flang's runtime and glibc libm, so individual elementary-function
-values can differ by an ulp or two, which the implicit scheme carries into
-the spectrum.  Bars:
-  * spectrum after one step: |F - F_ref| <= 2e-5 * max(|F_ref|, 1e-6 max|F_ref|)
-    per bin, and the 99.9th percentile of that ratio <= 1e-5;
+Tolerances.  The oracle evaluates the Fortran intrinsics with wam_math.h,
+whose results are the correctly rounded ones (tests/test_math_cr.py); the
+reference used glibc's float functions behind flang, which differ from the
+correctly rounded value by an ulp for a few per cent of the arguments of
+tanhf, sinhf and atan2f and far fewer of the others.  Bars (north_star's):
+  * spectrum after one step: |F - F_ref| <= 1e-5 * max(|F_ref|, 1e-6 max|F_ref|)
+    per bin;
   * per-point outputs (USTAR, TAUW, Z0, PHIAW, energy fluxes, and the stress
     vectors measured against their magnitude) after one step <= 1e-5 rel;
-  * Hs, mean period and USTAR after 24 h of source integration (288 steps of
-    300 s) <= 1e-4 rel, spectrum <= 1e-4."""
+  * Hs, mean period and USTAR after many steps (288 steps of 300 s, or 36 of
+    600 s for ST6) <= 1e-4 rel.  The spectrum itself is checked there on
+    the bins holding at least a tenth of their point's peak (<= 1e-4) and on
+    every bin (<= 1e-3): from the reference's own state each ST6 step
+    agrees within 1e-6 per bin, but over 36 steps the sub-ulp differences
+    grow to a few 1e-4 in bins a thousandth of the peak."""
 import glob
 import json
 import os
@@ -69,7 +73,7 @@ def check(z, cfg, get, dfim, fr):
     hs, tm = integrated(cfg, dfim, fr, get("fl3"), n)
     hs_r, tm_r = integrated(cfg, dfim, fr, z["out_fl3"], n)
     if nstep == 1:
-        assert err.max() <= 2e-5 and np.quantile(err, 0.999) <= 1e-5, (err.max(), np.quantile(err, 0.999))
+        assert err.max() <= 1e-5, err.max()
         for k, floor in (("ustar", 1e-6), ("tauw", 1e-9), ("z0", 1e-12), ("phiaw", 1e-9)):
             assert rel(get(k), z["out_" + k], floor) <= 1e-5, k
         if int(z["lcflx"]):
@@ -83,7 +87,10 @@ def check(z, cfg, get, dfim, fr):
                 mag = np.hypot(z["out_" + kx].astype(np.float64), z["out_" + ky].astype(np.float64))
                 assert np.max(np.hypot(dx, dy) / np.maximum(mag, 1e-30)) <= 1e-5, kx
     else:
-        assert err.max() <= 1e-4, err.max()
+        assert err.max() <= 1e-3, err.max()
+        ref = np.asarray(z["out_fl3"], np.float64).reshape(cfg["ml"] * cfg["kl"], n)
+        big = ref >= 0.1 * ref.max(axis=0)
+        assert err.reshape(ref.shape)[big].max() <= 1e-4
         assert rel(hs, hs_r) <= 1e-4 and rel(tm, tm_r) <= 1e-4
         assert rel(get("ustar"), z["out_ustar"], 1e-6) <= 1e-4
     assert rel(hs, hs_r) <= 1e-4
@@ -108,10 +115,11 @@ def test_fixtures_present():
 # wam_propagation_module.f90 built from its sources).  Bars: COUNTER and
 # INDEP exact; the max CFL number the reference prints (after its division by
 # COUNTER) within 2e-6 relative (it prints eight digits); the spectrum after
-# the PROPAGS calls within 5e-5 per bin (same normalisation as above).  The
-# remaining differences are a few ulp in the shallow-water tables (TCGOND,
-# TSIHKD, TFAK come from the shared elementary functions, the reference's from
-# libm), carried through the nine-term sums of the upwind stencil.
+# the PROPAGS calls within 1e-5 per bin (same normalisation as above).  The
+# remaining differences are single ulps in the shallow-water tables (TCGOND,
+# TSIHKD, TFAK come from SINH, COSH and TANH, where glibc's float functions
+# are not always correctly rounded), carried through the nine-term sums of
+# the upwind stencil.
 PROP_FIXTURES = sorted(glob.glob(os.path.join(HERE, "golden", "ref_propags_*.npz")))
 
 
@@ -120,7 +128,7 @@ def check_propags(z, counter, cfl, indep, fl3):
     assert abs(cfl / counter - float(z["out_cfl"])) <= 2e-6 * float(z["out_cfl"])
     assert np.array_equal(np.asarray(indep), z["out_indep"]) or not json.loads(str(z["cfg"]))["shallow"]
     err = spec_err(fl3, z["out_fl3"])
-    assert err.max() <= 5e-5, err.max()
+    assert err.max() <= 1e-5, err.max()
 
 
 @pytest.mark.parametrize("path", PROP_FIXTURES, ids=[os.path.basename(p)[12:-4] for p in PROP_FIXTURES])

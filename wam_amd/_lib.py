@@ -66,6 +66,7 @@ SIGNATURES = {
     "wam_gpu_get_scalar": (ctypes.c_double, [_P, ctypes.c_char_p]),
     "wam_gpu_last_error": (ctypes.c_char_p, [_P]),
     "wam_gpu_finalize": (None, [_P]),
+    "wam_gpu_eval_math": (_I, [_I, _P, _P, _P, ctypes.c_int64]),
 }
 
 _lib = None

@@ -3,7 +3,9 @@
 The HIP sources are compiled with hipcc; -ffp-contract=off keeps every
 multiply and add separately rounded (the kernels reproduce the reference's
 REAL*4 operation order), division and square root are correctly rounded,
-and packed FP32 instructions are disabled.
+and packed FP32 instructions are disabled.  Host code is built with -mfma so
+that the double-precision fused multiply-adds of wam_math.h are single
+instructions (the same correctly rounded operation as the library call).
 """
 import os
 import subprocess
@@ -19,8 +21,8 @@ HIP_SOURCES = ["wam_prop.hip", "wam_implsch.hip", "wam_output.hip", "wam_nest.hi
 CPP_SOURCES = ["wam_setup.cpp", "wam_restart.cpp", "wam_boundary_file.cpp", "wam_map_file.cpp"]
 HIP_FLAGS = ["-O3", f"--offload-arch={ARCH}", "-fPIC", "-std=c++17", "-ffp-contract=off",
              "-fhip-fp32-correctly-rounded-divide-sqrt", "-Xclang", "-target-feature", "-Xclang",
-             "-packed-fp32-ops", "-I" + CSRC]
-CPP_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-I" + CSRC]
+             "-packed-fp32-ops", "-Xarch_host", "-mfma", "-I" + CSRC]
+CPP_FLAGS = ["-O2", "-mfma", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-I" + CSRC]
 
 
 def _run(cmd):

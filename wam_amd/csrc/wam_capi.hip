@@ -32,6 +32,7 @@ __global__ void propags_fused_kernel(DevTables t, int counter, const float *fin,
 template <int RESCALE>
 __global__ void propags_fused_sc_kernel(DevTables t, int counter, const float *fin, float *fout);
 __global__ void wavnu2_kernel(DevTables t, float *wn, float *cgg);
+__global__ void sbottom_factor_kernel(DevTables t, float *sbot);
 hipError_t launch_implsch(const DevTables &t, const ImplArgs &a, int ncu, hipStream_t st);
 hipError_t launch_implsch_split(const DevTables &t, const ImplArgs &a, const ImplScratch &x, int ncu, hipStream_t st,
                                 const hipEvent_t *ev);
@@ -52,6 +53,31 @@ hipError_t launch_boundary_space(const DevTables &t, int nbinp, const float *fi,
 }  // namespace wam
 
 using namespace wam;
+
+namespace {
+__global__ void eval_math_kernel(int fn, const float *x, const float *y, float *out, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float a = x[i], b = y ? y[i] : 0.f;
+    float r = 0.f;
+    switch (fn) {
+    case 0: r = wam_expf(a); break;
+    case 1: r = wam_logf(a); break;
+    case 2: r = wam_log10f(a); break;
+    case 3: r = wam_tanhf(a); break;
+    case 4: r = wam_coshf(a); break;
+    case 5: r = wam_sinhf(a); break;
+    case 6: r = wam_cosf(a); break;
+    case 7: r = wam_sinf(a); break;
+    case 8: r = wam_powf(a, b); break;
+    case 9: r = (float)wam_nint(a); break;
+    case 10: r = wam_ipowf(a, (int)b); break;
+    case 11: r = wam_atan2f(a, b); break;
+    }
+    out[i] = r;
+}
+}  // namespace
 
 struct wam_ctx {
     Tables tab;
@@ -79,6 +105,7 @@ struct wam_ctx {
     int *cell_pt = nullptr;               // grid cell -> sea point (0-based, -1 land), for gridded output
     float *grids = nullptr;               // gridded parameter fields
     size_t grids_n = 0;
+    std::vector<float> depth_h;           // the DEPTH of the last PREPARE_PROPAGATION (host copy)
     bool prepared = false;
     int ncu = 256;                 // compute units of the device
     std::string err;
@@ -332,6 +359,13 @@ static int finish_prepare(wam_ctx *ctx)
             HIPCHK(hipGetLastError());
         }
     }
+    if (t.cfg.shallow) {
+        HIPCHK(hipMalloc(&p, (size_t)t.ML * ns * sizeof(float)));
+        o.push_back(p);
+        ctx->dt.SBOT = (const float *)p;
+        hipLaunchKernelGGL(sbottom_factor_kernel, dim3((ns + 255) / 256, t.ML), dim3(256), 0, st, ctx->dt, (float *)p);
+        HIPCHK(hipGetLastError());
+    }
     if (t.cfg.iphys == 2) {
         HIPCHK(hipMalloc(&p, 2 * (size_t)t.ML * ns * sizeof(float)));
         o.push_back(p);
@@ -353,10 +387,11 @@ int wam_gpu_prepare_propagation(wam_ctx *ctx, const float *depth, const float *u
     int rc = prepare_points(t, depth, u, v, ctx->err);
     if (rc) return rc;
     free_list(ctx->point_allocs);
-    ctx->coef = nullptr; ctx->wn = nullptr; ctx->cgg = nullptr;
+    ctx->coef = nullptr; ctx->wn = nullptr; ctx->cgg = nullptr; ctx->dt.SBOT = nullptr;
     const int ns = t.NSEA;
     auto &o = ctx->point_allocs;
     std::vector<float> d(depth, depth + ns), uu(ns, 0.f), vv(ns, 0.f);
+    ctx->depth_h = d;
     if (u) uu.assign(u, u + ns);
     if (v) vv.assign(v, v + ns);
     UP(o, d, DEPTH); UP(o, uu, U); UP(o, vv, V);
@@ -373,7 +408,7 @@ int wam_gpu_prepare_propagation_dev(wam_ctx *ctx, const float *d_depth, const fl
     ctx->prepared = false;
     if (t.cfg.refr_c && (!d_u || !d_v)) { ctx->err = "current refraction needs u and v"; return WAM_E_ARG; }
     free_list(ctx->point_allocs);
-    ctx->coef = nullptr; ctx->wn = nullptr; ctx->cgg = nullptr;
+    ctx->coef = nullptr; ctx->wn = nullptr; ctx->cgg = nullptr; ctx->dt.SBOT = nullptr;
     const int ns = t.NSEA;
     auto &o = ctx->point_allocs;
     hipStream_t st = ctx->stream;
@@ -409,6 +444,8 @@ int wam_gpu_prepare_propagation_dev(wam_ctx *ctx, const float *d_depth, const fl
     unsigned hbad = 0;
     t.INDEP.resize(ns); t.INDEP_G.resize(ns);
     std::vector<float> *hf[8] = {&t.DDPHI, &t.DDLAM, &t.DUPHI, &t.DULAM, &t.DVPHI, &t.DVLAM, &t.DELLA0, &t.DRGM};
+    ctx->depth_h.resize(ns);
+    HIPCHK(hipMemcpyAsync(ctx->depth_h.data(), dd, ns * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(t.INDEP.data(), ind, ns * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(t.INDEP_G.data(), indg, ns * 4, hipMemcpyDeviceToHost, st));
@@ -647,7 +684,8 @@ static int ensure_impl_scratch(wam_ctx *ctx, int lcflx)
         HIPCHK(hipMalloc(&x.slt, tiled * sizeof(float)));
         HIPCHK(hipMalloc(&x.flt, tiled * sizeof(float)));
         HIPCHK(hipMalloc(&x.axy, 3 * (size_t)t.ML * ns * sizeof(float)));
-        HIPCHK(hipMalloc(&x.pp, 4 * ns * sizeof(float)));
+        HIPCHK(hipMalloc(&x.pp, 6 * ns * sizeof(float)));
+        HIPCHK(hipMalloc(&x.sprd, (size_t)t.KL * ns * sizeof(float)));
         HIPCHK(hipMalloc(&x.llws, nw * ns * sizeof(unsigned)));
         HIPCHK(hipMalloc(&x.tick, 8 * 32 * sizeof(unsigned)));
     }
@@ -721,10 +759,19 @@ int wam_gpu_implsch(wam_ctx *ctx, int32_t nsea, int32_t kl, int32_t ml, float *f
     const Tables &t = ctx->tab;
     if (nsea != t.NSEA || kl != t.KL || ml != t.ML) { ctx->err = "IMPLSCH size mismatch"; return WAM_E_ARG; }
     if (!ctx->prepared) { ctx->err = "PREPARE_PROPAGATION not done"; return WAM_E_STATE; }
-    // DEPTH and INDEP are the fields given to wam_gpu_prepare_propagation
-    // (the reference passes DEPTH(NIJS:NIJL), INDEP(NIJS:NIJL) of the same module arrays)
-    (void)depth;
-    (void)indep;
+    // The reference passes DEPTH(NIJS:NIJL) and INDEP(NIJS:NIJL) of the same
+    // module arrays PREPARE_PROPAGATION used (wamodel.f90:354); the kernels
+    // read the context's prepared fields, so arguments that differ from them
+    // mean the host changed the depth without preparing again.  INDEP is only
+    // defined in shallow water (wam_propagation_module.f90:334-336).
+    if (depth && std::memcmp(depth, ctx->depth_h.data(), (size_t)nsea * sizeof(float)) != 0) {
+        ctx->err = "DEPTH differs from the field of PREPARE_PROPAGATION (call wam_gpu_prepare_propagation again)";
+        return WAM_E_STATE;
+    }
+    if (indep && t.cfg.shallow && std::memcmp(indep, t.INDEP.data(), (size_t)nsea * sizeof(int32_t)) != 0) {
+        ctx->err = "INDEP differs from the table of PREPARE_PROPAGATION (call wam_gpu_prepare_propagation again)";
+        return WAM_E_STATE;
+    }
     const size_t nb = (size_t)t.KL * t.ML * nsea, ns = nsea;
     // FL3, then 8 per-point arrays (U10 UDIR TAUW USTAR Z0 ROAIRN WSTAR PHIAW) and 6 flux rows
     int rc = ensure_stage(ctx, nb + 14 * ns);
@@ -1186,6 +1233,25 @@ double wam_gpu_get_scalar(wam_ctx *ctx, const char *name)
     return NAN;
 }
 
+int wam_gpu_eval_math(int32_t fn, const float *x, const float *y, float *out, int64_t n)
+{
+    if (fn < 0 || fn > 11 || !x || !out || n < 0 || ((fn == 8 || fn == 10 || fn == 11) && !y)) return WAM_E_ARG;
+    if (n == 0) return WAM_OK;
+    float *d = nullptr;
+    const size_t bytes = (size_t)n * sizeof(float);
+    if (hipMalloc(&d, 3 * bytes) != hipSuccess) return WAM_E_HIP;
+    hipError_t e = hipMemcpy(d, x, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess && y) e = hipMemcpy(d + n, y, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(eval_math_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, nullptr, fn, d,
+                           y ? d + n : nullptr, d + 2 * n, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(out, d + 2 * n, bytes, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    return e == hipSuccess ? WAM_OK : WAM_E_HIP;
+}
+
 const char *wam_gpu_last_error(wam_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 void wam_gpu_finalize(wam_ctx *ctx)
@@ -1200,7 +1266,8 @@ void wam_gpu_finalize(wam_ctx *ctx)
     if (ctx->grids) (void)hipFree(ctx->grids);
     timing_clear(ctx);
     for (void *p : {(void *)ctx->isx.ft, (void *)ctx->isx.slt, (void *)ctx->isx.flt, (void *)ctx->isx.smt,
-                    (void *)ctx->isx.axy, (void *)ctx->isx.pp, (void *)ctx->isx.llws, (void *)ctx->isx.tick})
+                    (void *)ctx->isx.axy, (void *)ctx->isx.pp, (void *)ctx->isx.sprd, (void *)ctx->isx.llws,
+                    (void *)ctx->isx.tick})
         if (p) (void)hipFree(p);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;

@@ -60,6 +60,10 @@ struct DevTables {
     const float *DEPTH, *DPSN_S, *DPSN_N, *DELLA0, *DRGM, *U, *V;
     const float *DDPHI, *DDLAM, *DUPHI, *DULAM, *DVPHI, *DVLAM;
     const float *WN, *CGG;                // [ML][NSEA] (IPHYS = 2)
+    // SBOTTOM's factor (-2*0.038/G * k)/SINH(MIN(2 k D, 50)) with k = TFAK(INDEP, M)
+    // (wam_source_module.f90:1527-1589): [ML][NSEA], shallow water only; it
+    // depends on the depth field alone, so PREPARE_PROPAGATION tabulates it
+    const float *SBOT;
     const int *JXO1, *JYO1, *JXO2, *JYO2, *KPM_P, *KPM_M;   // 1-based
     float DELTH0, DELPH0, DELTHR, DELFR0;
     // per direction K (cyclic K+1, K-1): SINTH, COSTH, DELTH0*(SINTH(K)+SINTH(K+1)),
@@ -100,7 +104,8 @@ struct ImplScratch {
     int NB4;                       // KL*ML/4
     float *ft, *slt, *flt, *smt;   // F, SL, FL, SL - SPOS of the second SINPUT (LCFLX only)
     float *axy;                    // [3][ML][NSEA]: STRESSO's direction sums of the first SINPUT
-    float *pp;                     // [4][NSEA]: ENHFR, MAX(FMEANWS, FMEAN), MIJ, PHIAW
+    float *pp;                     // [6][NSEA]: ENHFR, MAX(FMEANWS, FMEAN), MIJ, PHIAW, SIN(UDIR), COS(UDIR)
+    float *sprd;                   // [KL][NSEA]: SPRD(K) = MAX(0, COS(TH(K) - UDIR))**2 of the call
     unsigned *llws;                // [(KL*ML+31)/32][NSEA]: LLWS bits of the second SINPUT
     unsigned *tick;                // [8][32]: point counters of implsch_snl_kernel, one per XCD (own 128-byte line)
 };

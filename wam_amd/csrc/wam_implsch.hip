@@ -663,13 +663,13 @@ __device__ __forceinline__ void snonlin(const DevTables &t, const Smem &s, float
 // WNFLUXES (:3970-4054) on SSOURCE = SL - SMIN (in s.SMIN) and SL_BOT, one
 // thread: the six flux components out[c * ns], c = 0..5.
 template <int SHALLOW>
-__device__ __forceinline__ void wnfluxes(const DevTables &t, const Smem &s, float phiaw, float ust, float udir, int mij,
-                                         const float *tfk, float *out, int ns)
+__device__ __forceinline__ void wnfluxes(const DevTables &t, const Smem &s, float phiaw, float ust, float sinu,
+                                         float cosu, int mij, const float *tfk, float *out, int ns)
 {
     const int KL = t.KL;
     const float rog = t.ROWATER * t.G;
     float phioc = phiaw, tau = t.ROAIR * (ust * ust);
-    float tx = tau * wam_sinf(udir), ty = tau * wam_cosf(udir);
+    float tx = tau * sinu, ty = tau * cosu;      // SIN(UDIR), COS(UDIR)
     float pb = 0.f, bx = 0.f, by = 0.f;
     const float scdfm = (0.5f * t.DELTH) * (1.f - 1.f / t.CO);
     for (int m = 0; m < mij; m++) {
@@ -1146,7 +1146,9 @@ __global__ void __launch_bounds__(NT) implsch_kernel(DevTables t, ImplArgs a)
             // WNFLUXES (:3970-4054) on SSOURCE = SL - SMIN
             for (int b = tid; b < NB; b += NT) s.SMIN[b] = s.SL[b] - s.SMIN[b];
             __syncthreads();
-            if (tid == 0 && a.fluxes) wnfluxes<SHALLOW>(t, s, P.phiaw, P.ust, P.udir, P.mij, tfk, a.fluxes + pt, NS);
+            if (tid == 0 && a.fluxes)
+                wnfluxes<SHALLOW>(t, s, P.phiaw, P.ust, wam_sinf(P.udir), wam_cosf(P.udir), P.mij, tfk, a.fluxes + pt,
+                                  NS);
         }
         // 4. limiter and implicit update
         if (tid < ML) {
@@ -1259,7 +1261,7 @@ __device__ __forceinline__ float stresso_cm(const DevTables &t, int m, float tfk
 }
 
 template <int SHALLOW>
-__global__ void __launch_bounds__(PW) implsch_pre_kernel(DevTables t, ImplArgs a, ImplScratch x)
+__global__ void __launch_bounds__(PW) __attribute__((amdgpu_waves_per_eu(4))) implsch_pre_kernel(DevTables t, ImplArgs a, ImplScratch x)
 {
     extern __shared__ float cu_lds[];            // [KL][PW]: COS(TH(K) - UDIR) of each lane's point
     const int KL = t.KL, ML = t.ML, NB = KL * ML, NS = t.NSEA, NB4 = x.NB4, lane = threadIdx.x;
@@ -1271,7 +1273,12 @@ __global__ void __launch_bounds__(PW) implsch_pre_kernel(DevTables t, ImplArgs a
     const float u10 = a.u10[pt], udir = a.udir[pt], roairn = a.roairn[pt], wstar = a.wstar[pt];
     const int indep = SHALLOW ? t.INDEP[pt] : 1;
     float tauw = a.tauw[pt], ust = 0.f, z0 = a.z0[pt];
-    for (int k = 0; k < KL; k++) cu[k * PW] = wam_cosf(t.TH[k] - udir);
+    for (int k = 0; k < KL; k++) {
+        const float c = wam_cosf(t.TH[k] - udir);
+        cu[k * PW] = c;
+        // the limiter's floor FLMINFR(JU, M) * SPRD(K) (:494-498) is applied by implsch_snl_kernel
+        if (on) x.sprd[(size_t)k * NS + pt] = wam_ipowf(wam_maxf(0.f, c), 2);
+    }
     airsea(t, u10, tauw, ust, z0);
 
     const float const3 = ((2.f * t.XKAPPA) / t.BETAMAX) * (t.XKAPPA * t.XKAPPA);
@@ -1467,6 +1474,10 @@ __global__ void __launch_bounds__(PW) implsch_pre_kernel(DevTables t, ImplArgs a
         x.pp[(size_t)NS + pt] = wam_maxf(fmeanws, fmean);
         x.pp[(size_t)2 * NS + pt] = (float)mij;
         x.pp[(size_t)3 * NS + pt] = phiaw;
+        if (a.lcflx) {   // for WNFLUXES in implsch_snl_kernel
+            x.pp[(size_t)4 * NS + pt] = wam_sinf(udir);
+            x.pp[(size_t)5 * NS + pt] = wam_cosf(udir);
+        }
     }
     // implsch_snl_kernel's point tickets start from zero
     if (blockIdx.x == 0 && lane < 8) x.tick[32 * lane] = 0u;
@@ -1482,7 +1493,7 @@ __global__ void __launch_bounds__(SNL_NT) __attribute__((amdgpu_waves_per_eu(6))
 implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x)
 {
     extern __shared__ __align__(16) float lds_raw[];
-    __shared__ float cu[MAXF], sprd[MAXF], flm[MAXF], tfk[MAXF], tlim[MAXF], tbot[MAXF];
+    __shared__ float sprd[MAXF], flm[MAXF], tfk[MAXF], tlim[MAXF], tbot[MAXF];
     __shared__ float Ps[5];
     __shared__ int Pmij, Pnext;
     constexpr int S = 256 * SP_QPT_DEV;
@@ -1562,33 +1573,28 @@ implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x)
                 fl[j] = x.flt[o];
             }
         }
-        const float udir = a.udir[pt];
         if (tid == 0) {
             Ps[0] = a.ustar[pt]; Ps[1] = x.pp[pt]; Ps[2] = x.pp[(size_t)NS + pt]; Ps[3] = x.pp[(size_t)3 * NS + pt];
-            Ps[4] = t.DEPTH[pt];
             Pmij = (int)x.pp[(size_t)2 * NS + pt];
         }
-        if (tid < KL) {
-            const float c = wam_cosf(s.th[tid] - udir);
-            cu[tid] = c;
-            sprd[tid] = wam_ipowf(wam_maxf(0.f, c), 2);
-        }
+        // no elementary functions here: SPRD comes from implsch_pre_kernel and
+        // SBOTTOM's factor from PREPARE_PROPAGATION's table (their double-
+        // precision evaluation would make this kernel's SNONLIN loop spill)
+        if (tid < KL) sprd[tid] = x.sprd[(size_t)tid * NS + pt];
         if (tid >= 64 && tid - 64 < ML) {
             const int m = tid - 64;
             const int ju = min(t.JUMAX, max(wam_nint(a.u10[pt] / t.DELU), 1));
             flm[m] = t.FLMINFR[(size_t)m * t.JUMAX + ju - 1];
-            if (SHALLOW) tfk[m] = t.TFAK[tb(t, m, t.INDEP[pt])];
+            if (SHALLOW) {
+                tfk[m] = t.TFAK[tb(t, m, t.INDEP[pt])];
+                tbot[m] = t.SBOT[(size_t)m * NS + pt];
+            }
         }
         __syncthreads();
         // SNONLIN, with the ISNONLIN = 0 shallow-water enhancement
         snonlin_products<SNL_NT, SNL_NJ>(t, s, U, SHALLOW ? Ps[1] : 1.f, qslot, qdir, qdk, sl, fl);
-        // SBOTTOM's factor (:1527-1589) and the limiter's (:652-690)
+        // the limiter's factor (:652-690)
         if (tid < ML) {
-            if (SHALLOW) {
-                const float wav = tfk[tid];
-                const float sbo = wam_minf((2.f * Ps[4]) * wav, 50.f);
-                tbot[tid] = (((-2.0f * 0.038f) / t.G) * wav) / wam_sinhf(sbo);
-            }
             const float delfl = (t.C5G / wam_ipowf(s.fr[tid], 4)) * delt;
             tlim[tid] = (Ps[0] * delfl) * Ps[2];
         }
@@ -1604,7 +1610,9 @@ implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x)
                 }
             }
             __syncthreads();
-            if (tid == 0 && a.fluxes) wnfluxes<SHALLOW>(t, s, Ps[3], Ps[0], udir, Pmij, tfk, a.fluxes + pt, NS);
+            if (tid == 0 && a.fluxes)
+                wnfluxes<SHALLOW>(t, s, Ps[3], Ps[0], x.pp[(size_t)4 * NS + pt], x.pp[(size_t)5 * NS + pt], Pmij, tfk,
+                                  a.fluxes + pt, NS);
         }
         // SBOTTOM (SL_BOT = 0 + factor * F, FL + factor), the limiter and the
         // implicit update, into F in place (nothing reads F after SNONLIN's

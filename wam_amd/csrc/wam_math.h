@@ -2,28 +2,54 @@
  * wam_math.h -- deterministic single-precision elementary functions.
  *
  * The WAM source terms call the Fortran runtime's EXP, LOG, LOG10, TANH,
- * COSH, SINH, COS, SIN and ** (wam_source_module.f90, e.g. SINPUT :2279,
- * TAU_PHI_HF :3946-3958, WAVNU2 :4420-4448, SINPUT_ST6 :4846-4866).  Those
- * results differ by an ulp or two between vendor libraries, and an ulp in
- * FMEAN or USTAR can move the integer cut-off MIJ = NINT(LOG10(.)/LOG10(CO))+1
- * (FRCUTINDEX :3639).  Every routine here uses only IEEE-754 single-precision
- * +, -, *, / and exact bit manipulation, evaluated in a fixed order, so the
- * same source gives bit-identical results on the host (gcc, SSE, no FMA
- * contraction) and on gfx950 (hipcc with -ffp-contract=off).  Truncation,
- * floor and ceiling are exact operations: the device takes them from the
- * hardware instructions, the host from bit manipulation.
+ * COSH, SINH, COS, SIN, ATAN2 and ** (wam_source_module.f90, e.g. SINPUT
+ * :2279, TAU_PHI_HF :3946-3958, WAVNU2 :4420-4448, SINPUT_ST6 :4846-4866;
+ * FETCH_LAW's ** in wam_jonswap_module.f90 feeds the MIN_ENERGY floor).  An
+ * ulp of difference in one of them is carried forward by the implicit
+ * scheme, and an ulp in FMEAN or USTAR can move the integer cut-off
+ * MIJ = NINT(LOG10(.)/LOG10(CO))+1 (FRCUTINDEX :3639).
  *
- * Algorithms: Cody-Waite range reduction followed by the minimax polynomials
- * published with the Cephes library (S. L. Moshier, "Methods and Programs
- * for Mathematical Functions", 1989): expf, logf, sinf/cosf, tanhf, sinhf,
- * atanf (and atan2f from it).
- * Accuracy (checked against double-precision libm in the CPU test-suite):
- * <= 2 ulp over the argument ranges WAM uses.
+ * Each float function here is the correctly rounded float of a double
+ * evaluation whose relative error is bounded well below 2^-44 (exp, log,
+ * log10, sin, cos, tanh, sinh, cosh, atan2: below 2^-49; pow: below
+ * |y log x| * 2^-51 + 2^-51), so it returns the correctly rounded result
+ * except where the exact value lies within that bound of a rounding
+ * boundary between two floats.  A correctly rounded result is what any
+ * accurate libm returns for most arguments, so the reference's values
+ * (glibc's float functions behind flang) are met far more often than by a
+ * float-only evaluation.
+ *
+ * Only IEEE-754 double +, -, *, /, fused multiply-add (one rounding), exact
+ * conversions, the final double -> float rounding and exact bit
+ * manipulation are used, in a fixed order, so the same source gives
+ * bit-identical results on the host (gcc; fma is the correctly rounded
+ * fused operation) and on gfx950 (hipcc; v_fma_f64, IEEE division), as
+ * long as no multiply and add are contracted (-ffp-contract=off).
+ *
+ * Methods (standard; constants derived in exact rational arithmetic by
+ * gen_math_consts.py): Cody-Waite reduction x = k ln2 + r (exp), x = k pi/2
+ * + r (sin, cos) with a split constant whose leading part times k is
+ * exact; Taylor polynomials on the reduced interval (exp, expm1, sin, cos);
+ * log m = 2 atanh((m-1)/(m+1)) on m in [sqrt(1/2), sqrt(2)); atan by the
+ * addition formula about j/8 with a table of atan(j/8); tanh, sinh and cosh
+ * from expm1 and exp without cancellation.
  */
 #ifndef WAM_MATH_H
 #define WAM_MATH_H
 
 #include <stdint.h>
+
+/* split and rounded constants (gen_math_consts.py) */
+#define WAM_PIO2_1 0x1.921fb54440000p+0      /* pi/2, leading 36 bits */
+#define WAM_PIO2_2 0x1.68c234c4c0000p-39     /* next 36 bits */
+#define WAM_PIO2_3 0x1.98a2e03707345p-77     /* rest, rounded */
+#define WAM_2_OVER_PI 0x1.45f306dc9c883p-1
+#define WAM_PI_D 0x1.921fb54442d18p+1
+#define WAM_PIO2_D 0x1.921fb54442d18p+0
+#define WAM_LN2_HI 0x1.62e42fefa3800p-1      /* ln 2, leading 44 bits */
+#define WAM_LN2_LO 0x1.ef35793c76730p-45     /* rest, rounded */
+#define WAM_INV_LN2 0x1.71547652b82fep+0
+#define WAM_INV_LN10 0x1.bcb7b1526e50ep-2
 
 #if defined(__HIPCC__)
 #define WAM_HD __host__ __device__ __attribute__((always_inline)) inline
@@ -33,17 +59,6 @@
 
 WAM_HD int32_t wam_f2i(float f) { int32_t i; __builtin_memcpy(&i, &f, 4); return i; }
 WAM_HD float wam_i2f(int32_t i) { float f; __builtin_memcpy(&f, &i, 4); return f; }
-
-/* 2**k as a float, exact for -126 <= k <= 127. */
-WAM_HD float wam_exp2i(int k) { return wam_i2f((int32_t)(k + 127) << 23); }
-
-/* x * 2**k with a single final IEEE rounding (only in the subnormal range). */
-WAM_HD float wam_ldexpf(float x, int k)
-{
-    if (k > 127) { x = x * wam_exp2i(127); k -= 127; if (k > 127) k = 127; }
-    if (k < -126) { x = x * wam_exp2i(-126); k += 126; if (k < -126) k = -126; }
-    return x * wam_exp2i(k);
-}
 
 WAM_HD float wam_fabsf(float x) { return wam_i2f(wam_f2i(x) & 0x7fffffff); }
 WAM_HD float wam_copysignf(float a, float b)
@@ -86,17 +101,25 @@ WAM_HD float wam_ceilf(float x)
     return (t < x) ? t + 1.0f : t;
 #endif
 }
+/* an integral float to int, saturating (NaN -> 0), the same on host and
+   device (a plain conversion out of range is undefined) */
+WAM_HD int wam_f2int(float t)
+{
+    if (!(t == t)) return 0;
+    if (t >= 2147483648.0f) return 2147483647;
+    if (t < -2147483648.0f) return (int)0x80000000;
+    return (int)t;
+}
 /* Fortran NINT: nearest integer, halves away from zero. */
 WAM_HD int wam_nint(float x)
 {
     float a = wam_fabsf(x);
     float t = wam_truncf(a);
     if (a - t >= 0.5f) t = t + 1.0f;     /* a - t is exact for |a| < 2**23 */
-    int r = (int)t;
-    return x < 0.0f ? -r : r;
+    return wam_f2int(x < 0.0f ? -t : t);
 }
 /* Fortran INT: truncation toward zero. */
-WAM_HD int wam_int(float x) { return (int)wam_truncf(x); }
+WAM_HD int wam_int(float x) { return wam_f2int(wam_truncf(x)); }
 /* Fortran SIGN(a,b). */
 WAM_HD float wam_signf(float a, float b) { return b >= 0.0f ? wam_fabsf(a) : -wam_fabsf(a); }
 
@@ -113,25 +136,171 @@ WAM_HD float wam_ipowf(float x, int n)
     return r;
 }
 
+
+/* ---- double-precision cores ------------------------------------------- */
+
+WAM_HD int64_t wam_d2i(double d) { int64_t i; __builtin_memcpy(&i, &d, 8); return i; }
+WAM_HD double wam_i2d(int64_t i) { double d; __builtin_memcpy(&d, &i, 8); return d; }
+WAM_HD double wam_fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
+/* 2**k as a double, -1022 <= k <= 1023 */
+WAM_HD double wam_pow2d(int k) { return wam_i2d((int64_t)(k + 1023) << 52); }
+/* nearest integer, ties to even, exactly, for |x| < 2**51 */
+WAM_HD double wam_rintd(double x) { return (x + 0x1.8p52) - 0x1.8p52; }
+
+/* exp(x) for -740 < x < 709: x = k ln2 + r with |r| <= ln2/2 (k * LN2_HI is
+   exact and so, by Sterbenz's lemma, is x - k * LN2_HI); exp(r) by its
+   Taylor polynomial of degree 13 (truncation below 2^-57).  Relative error
+   below 2^-51. */
+WAM_HD double wam_exp_d(double x)
+{
+    const double kd = wam_rintd(x * WAM_INV_LN2);
+    const double r = (x - kd * WAM_LN2_HI) - kd * WAM_LN2_LO;
+    double p = 0x1.6124613a86d09p-33;                 /* 1/13! */
+    p = wam_fmad(p, r, 0x1.1eed8eff8d898p-29);        /* 1/12! */
+    p = wam_fmad(p, r, 0x1.ae64567f544e4p-26);        /* 1/11! */
+    p = wam_fmad(p, r, 0x1.27e4fb7789f5cp-22);        /* 1/10! */
+    p = wam_fmad(p, r, 0x1.71de3a556c734p-19);        /* 1/9! */
+    p = wam_fmad(p, r, 0x1.a01a01a01a01ap-16);        /* 1/8! */
+    p = wam_fmad(p, r, 0x1.a01a01a01a01ap-13);        /* 1/7! */
+    p = wam_fmad(p, r, 0x1.6c16c16c16c17p-10);        /* 1/6! */
+    p = wam_fmad(p, r, 0x1.1111111111111p-7);         /* 1/5! */
+    p = wam_fmad(p, r, 0x1.5555555555555p-5);         /* 1/4! */
+    p = wam_fmad(p, r, 0x1.5555555555555p-3);         /* 1/3! */
+    p = wam_fmad(p, r, 0.5);
+    p = wam_fmad(p, r, 1.0);
+    p = wam_fmad(p, r, 1.0);
+    const int k = (int)kd;
+    if (k < -1000) return (p * wam_pow2d(k + 200)) * wam_pow2d(-200);
+    return p * wam_pow2d(k);
+}
+
+/* exp(x) - 1 for 0 <= x < 709: below ln2/2 the Taylor polynomial of degree
+   14 (no cancellation), above exp(x) - 1 (cancellation at most a factor
+   3.5).  Relative error below 2^-49. */
+WAM_HD double wam_expm1_d(double x)
+{
+    if (x > 0.34657359027997264) return wam_exp_d(x) - 1.0;
+    double p = 0x1.93974a8c07c9dp-37;                 /* 1/14! */
+    p = wam_fmad(p, x, 0x1.6124613a86d09p-33);
+    p = wam_fmad(p, x, 0x1.1eed8eff8d898p-29);
+    p = wam_fmad(p, x, 0x1.ae64567f544e4p-26);
+    p = wam_fmad(p, x, 0x1.27e4fb7789f5cp-22);
+    p = wam_fmad(p, x, 0x1.71de3a556c734p-19);
+    p = wam_fmad(p, x, 0x1.a01a01a01a01ap-16);
+    p = wam_fmad(p, x, 0x1.a01a01a01a01ap-13);
+    p = wam_fmad(p, x, 0x1.6c16c16c16c17p-10);
+    p = wam_fmad(p, x, 0x1.1111111111111p-7);
+    p = wam_fmad(p, x, 0x1.5555555555555p-5);
+    p = wam_fmad(p, x, 0x1.5555555555555p-3);
+    p = wam_fmad(p, x, 0.5);
+    p = wam_fmad(p, x, 1.0);
+    return p * x;
+}
+
+/* log(x) for a positive finite float x: x = 2^e m with m in [sqrt(1/2),
+   sqrt(2)), log m = 2 atanh(s), s = (m-1)/(m+1) (m-1 and m+1 exact, one
+   correctly rounded division), |s| < 0.1716, series to s^19 (truncation
+   below 2^-55); log x = e LN2_HI + (e LN2_LO + log m).  Relative error
+   below 2^-51. */
+WAM_HD double wam_log_d(float x)
+{
+    int32_t i = wam_f2i(x);
+    int e = 0;
+    if (i < 0x00800000) {                 /* subnormal: scale up by 2**25 */
+        i = wam_f2i(x * 33554432.0f);
+        e = -25;
+    }
+    e += ((i >> 23) & 0xff) - 127;
+    int32_t mi = (i & 0x007fffff) | 0x3f800000;        /* m in [1, 2) */
+    if (mi > 0x3fb504f3) { mi -= 0x00800000; e += 1; } /* m > sqrt(2): m/2 */
+    const double m = (double)wam_i2f(mi);
+    const double s = (m - 1.0) / (m + 1.0);
+    const double z = s * s;
+    double p = 0x1.af286bca1af28p-5;                  /* 1/19 */
+    p = wam_fmad(p, z, 0x1.e1e1e1e1e1e1ep-5);         /* 1/17 */
+    p = wam_fmad(p, z, 0x1.1111111111111p-4);         /* 1/15 */
+    p = wam_fmad(p, z, 0x1.3b13b13b13b14p-4);         /* 1/13 */
+    p = wam_fmad(p, z, 0x1.745d1745d1746p-4);         /* 1/11 */
+    p = wam_fmad(p, z, 0x1.c71c71c71c71cp-4);         /* 1/9 */
+    p = wam_fmad(p, z, 0x1.2492492492492p-3);         /* 1/7 */
+    p = wam_fmad(p, z, 0x1.999999999999ap-3);         /* 1/5 */
+    p = wam_fmad(p, z, 0x1.5555555555555p-2);         /* 1/3 */
+    const double s2 = s + s;
+    const double lm = wam_fmad(s2 * z, p, s2);
+    const double ed = (double)e;
+    return ed * WAM_LN2_HI + (ed * WAM_LN2_LO + lm);
+}
+
+/* sin(r) and cos(r) for |r| <= pi/4 (+ rounding): Taylor polynomials to r^15
+   and r^16 (truncation below 2^-54). */
+WAM_HD double wam_sin_poly(double r)
+{
+    const double z = r * r;
+    double p = -0x1.ae7f3e733b81fp-41;                /* -1/15! */
+    p = wam_fmad(p, z, 0x1.6124613a86d09p-33);        /* 1/13! */
+    p = wam_fmad(p, z, -0x1.ae64567f544e4p-26);       /* -1/11! */
+    p = wam_fmad(p, z, 0x1.71de3a556c734p-19);        /* 1/9! */
+    p = wam_fmad(p, z, -0x1.a01a01a01a01ap-13);       /* -1/7! */
+    p = wam_fmad(p, z, 0x1.1111111111111p-7);         /* 1/5! */
+    p = wam_fmad(p, z, -0x1.5555555555555p-3);        /* -1/3! */
+    return wam_fmad(r * z, p, r);
+}
+WAM_HD double wam_cos_poly(double r)
+{
+    const double z = r * r;
+    double p = 0x1.ae7f3e733b81fp-45;                 /* 1/16! */
+    p = wam_fmad(p, z, -0x1.93974a8c07c9dp-37);       /* -1/14! */
+    p = wam_fmad(p, z, 0x1.1eed8eff8d898p-29);        /* 1/12! */
+    p = wam_fmad(p, z, -0x1.27e4fb7789f5cp-22);       /* -1/10! */
+    p = wam_fmad(p, z, 0x1.a01a01a01a01ap-16);        /* 1/8! */
+    p = wam_fmad(p, z, -0x1.6c16c16c16c17p-10);       /* -1/6! */
+    p = wam_fmad(p, z, 0x1.5555555555555p-5);         /* 1/4! */
+    p = wam_fmad(p, z, -0.5);
+    return wam_fmad(z, p, 1.0);
+}
+
+/* x = q pi/2 + r for a float x, |x| < 2^16: r with relative error below
+   2^-51 (the nearest multiple of pi/2 to a float of that range is no
+   closer than about 2^-26, and x - q PIO2_1 is exact); *q gets q mod 4. */
+WAM_HD double wam_reduce_pio2(float x, int *q)
+{
+    const double xd = (double)x;
+    const double qd = wam_rintd(xd * WAM_2_OVER_PI);
+    *q = (int)qd & 3;
+    return ((xd - qd * WAM_PIO2_1) - qd * WAM_PIO2_2) - qd * WAM_PIO2_3;
+}
+
+/* atan(t) for 0 <= t <= 1: atan(t) = atan(c) + atan((t - c)/(1 + t c)) with
+   c = j/8 the nearest eighth (t - c exact by Sterbenz's lemma for j >= 1),
+   |u| <= 1/16, series to u^13 (truncation below 2^-60). */
+WAM_HD double wam_atan01_d(double t)
+{
+    const int j = (int)wam_rintd(t * 8.0);
+    const double c = (double)j * 0.125;
+    const double u = (t - c) / wam_fmad(t, c, 1.0);
+    const double z = u * u;
+    double p = 0x1.3b13b13b13b14p-4;                  /* 1/13 */
+    p = wam_fmad(p, z, -0x1.745d1745d1746p-4);        /* -1/11 */
+    p = wam_fmad(p, z, 0x1.c71c71c71c71cp-4);         /* 1/9 */
+    p = wam_fmad(p, z, -0x1.2492492492492p-3);        /* -1/7 */
+    p = wam_fmad(p, z, 0x1.999999999999ap-3);         /* 1/5 */
+    p = wam_fmad(p, z, -0x1.5555555555555p-2);        /* -1/3 */
+    const double a = wam_fmad(u * z, p, u);
+    /* atan(j/8), j = 0..8 */
+    const double A = j == 0 ? 0.0 : j == 1 ? 0x1.fd5ba9aac2f6ep-4 : j == 2 ? 0x1.f5b75f92c80ddp-3
+                   : j == 3 ? 0x1.6f61941e4def1p-2 : j == 4 ? 0x1.dac670561bb4fp-2 : j == 5 ? 0x1.1e00babdefeb4p-1
+                   : j == 6 ? 0x1.4978fa3269ee1p-1 : j == 7 ? 0x1.700a7c5784634p-1 : 0x1.921fb54442d18p-1;
+    return A + a;
+}
+
+/* ---- the float functions ---------------------------------------------- */
+
 WAM_HD float wam_expf(float x)
 {
     if (wam_isnan(x)) return x;
-    if (x > 88.72283935546875f) return wam_i2f(0x7f800000);
-    if (x < -103.972084045410f) return 0.0f;
-    float fk = wam_floorf(x * 1.44269504088896341f + 0.5f);
-    int k = (int)fk;
-    float r = x - fk * 0.693359375f;
-    r = r - fk * -2.12194440e-4f;
-    float z = r * r;
-    float y = 1.9875691500E-4f;
-    y = y * r + 1.3981999507E-3f;
-    y = y * r + 8.3334519073E-3f;
-    y = y * r + 4.1665795894E-2f;
-    y = y * r + 1.6666665459E-1f;
-    y = y * r + 5.0000001201E-1f;
-    y = y * z + r;
-    y = y + 1.0f;
-    return wam_ldexpf(y, k);
+    if (x > 89.0f) return wam_i2f(0x7f800000);
+    if (x < -104.0f) return 0.0f;
+    return (float)wam_exp_d((double)x);
 }
 
 WAM_HD float wam_logf(float x)
@@ -140,170 +309,107 @@ WAM_HD float wam_logf(float x)
     if (x < 0.0f) return wam_i2f(0x7fc00000);
     if (x == 0.0f) return wam_i2f((int32_t)0xff800000);
     if (wam_f2i(x) == 0x7f800000) return x;
-    int32_t i = wam_f2i(x);
-    int e = 0;
-    if (i < 0x00800000) {                 /* subnormal: scale up by 2**25 */
-        x = x * 33554432.0f;
-        i = wam_f2i(x);
-        e = -25;
-    }
-    /* frexp: x = m * 2**e with m in [0.5, 1) */
-    e += ((i >> 23) & 0xff) - 126;
-    float m = wam_i2f((i & 0x007fffff) | 0x3f000000);
-    if (m < 0.707106781186547524f) {
-        e -= 1;
-        m = m + m - 1.0f;
-    } else {
-        m = m - 1.0f;
-    }
-    float z = m * m;
-    float y = 7.0376836292E-2f;
-    y = y * m - 1.1514610310E-1f;
-    y = y * m + 1.1676998740E-1f;
-    y = y * m - 1.2420140846E-1f;
-    y = y * m + 1.4249322787E-1f;
-    y = y * m - 1.6668057665E-1f;
-    y = y * m + 2.0000714765E-1f;
-    y = y * m - 2.4999993993E-1f;
-    y = y * m + 3.3333331174E-1f;
-    y = y * m * z;
-    float fe = (float)e;
-    y = y + fe * -2.12194440e-4f;
-    y = y - 0.5f * z;
-    float r = m + y;
-    r = r + fe * 0.693359375f;
-    return r;
+    return (float)wam_log_d(x);
 }
 
 WAM_HD float wam_log10f(float x)
 {
-    /* log10(x) = log(x) * log10(e), with log10(e) split hi+lo */
-    float l = wam_logf(x);
-    return l * 4.3359375E-1f + l * 7.00731903251827651129E-4f;
+    if (wam_isnan(x)) return x;
+    if (x < 0.0f) return wam_i2f(0x7fc00000);
+    if (x == 0.0f) return wam_i2f((int32_t)0xff800000);
+    if (wam_f2i(x) == 0x7f800000) return x;
+    return (float)(wam_log_d(x) * WAM_INV_LN10);
 }
 
+/* tanh x = t / (t + 2), t = expm1(2|x|); |x| > 10 rounds to +-1. */
 WAM_HD float wam_tanhf(float x)
 {
-    float a = wam_fabsf(x);
-    if (wam_isnan(x)) return x;
-    if (a > 9.01f) return x > 0.0f ? 1.0f : -1.0f;
-    if (a >= 0.625f) {
-        float s = wam_expf(a + a);
-        float y = 1.0f - 2.0f / (s + 1.0f);
-        return x < 0.0f ? -y : y;
-    }
-    if (x == 0.0f) return x;
-    float z = x * x;
-    float y = -5.70498872745E-3f;
-    y = y * z + 2.06390887954E-2f;
-    y = y * z - 5.37397155531E-2f;
-    y = y * z + 1.33314422036E-1f;
-    y = y * z - 3.33332819422E-1f;
-    y = y * z * x + x;
-    return y;
+    if (wam_isnan(x) || x == 0.0f) return x;
+    const float a = wam_fabsf(x);
+    if (a > 10.0f) return wam_copysignf(1.0f, x);
+    const double t = wam_expm1_d(2.0 * (double)a);
+    return wam_copysignf((float)(t / (t + 2.0)), x);
 }
 
+/* cosh x = (e + 1/e)/2, e = exp|x|. */
 WAM_HD float wam_coshf(float x)
 {
-    float a = wam_fabsf(x);
-    if (a > 89.41598629223294f) return wam_i2f(0x7f800000);
-    float e = wam_expf(a);
-    return 0.5f * e + 0.5f / e;
+    if (wam_isnan(x)) return x;
+    const float a = wam_fabsf(x);
+    if (a > 90.0f) return wam_i2f(0x7f800000);
+    const double e = wam_exp_d((double)a);
+    return (float)(0.5 * (e + 1.0 / e));
 }
 
+/* sinh x = (t + t/(t + 1))/2, t = expm1|x|. */
 WAM_HD float wam_sinhf(float x)
 {
-    float a = wam_fabsf(x);
-    if (a > 89.41598629223294f) return x > 0.0f ? wam_i2f(0x7f800000) : wam_i2f((int32_t)0xff800000);
-    float y;
-    if (a > 1.0f) {
-        float e = wam_expf(a);
-        y = 0.5f * e - 0.5f / e;
-    } else {
-        float z = a * a;
-        y = 2.03721912945E-4f;
-        y = y * z + 8.33028376239E-3f;
-        y = y * z + 1.66667160211E-1f;
-        y = y * z * a + a;
-    }
-    return x < 0.0f ? -y : y;
+    if (wam_isnan(x) || x == 0.0f) return x;
+    const float a = wam_fabsf(x);
+    if (a > 90.0f) return wam_copysignf(wam_i2f(0x7f800000), x);
+    const double t = wam_expm1_d((double)a);
+    return wam_copysignf((float)(0.5 * (t + t / (t + 1.0))), x);
 }
 
-/* sin and cos: reduction by pi/4 with a three-part constant; fine for |x| < 8192. */
-WAM_HD float wam_sincos_core(float x, int want_cos)
+/* sin and cos for |x| < 2^16 (the angles WAM evaluates are a few radians);
+   beyond, and for infinities, NaN. */
+WAM_HD float wam_sinf(float x)
 {
-    int sign = 0;
-    float a = x;
-    if (a < 0.0f) { a = -a; if (!want_cos) sign = 1; }
-    int j = (int)(a * 1.27323954473516f);   /* 4/pi */
-    float y = (float)j;
-    if (j & 1) { j += 1; y += 1.0f; }
-    j &= 7;
-    if (j > 3) { j -= 4; sign = !sign; }
-    if (want_cos && j > 1) sign = !sign;
-    float r = ((a - y * 0.78515625f) - y * 2.4187564849853515625e-4f) - y * 3.77489497744594108e-8f;
-    float z = r * r;
-    int use_cos_poly = want_cos ? (j == 0 || j == 3) : (j == 1 || j == 2);
-    float p;
-    if (use_cos_poly) {
-        p = 2.443315711809948E-5f;
-        p = p * z - 1.388731625493765E-3f;
-        p = p * z + 4.166664568298827E-2f;
-        p = p * z * z;
-        p = p - 0.5f * z;
-        p = p + 1.0f;
-    } else {
-        p = -1.9515295891E-4f;
-        p = p * z + 8.3321608736E-3f;
-        p = p * z - 1.6666654611E-1f;
-        p = p * z * r;
-        p = p + r;
-    }
-    return sign ? -p : p;
+    if (wam_isnan(x) || x == 0.0f) return x;
+    if (!(wam_fabsf(x) < 65536.0f)) return wam_i2f(0x7fc00000);
+    int q;
+    const double r = wam_reduce_pio2(x, &q);
+    double v = (q & 1) ? wam_cos_poly(r) : wam_sin_poly(r);
+    if (q & 2) v = -v;
+    return (float)v;
 }
-WAM_HD float wam_sinf(float x) { return wam_sincos_core(x, 0); }
-WAM_HD float wam_cosf(float x) { return wam_sincos_core(x, 1); }
+WAM_HD float wam_cosf(float x)
+{
+    if (wam_isnan(x)) return x;
+    if (!(wam_fabsf(x) < 65536.0f)) return wam_i2f(0x7fc00000);
+    int q;
+    const double r = wam_reduce_pio2(x, &q);
+    double v = (q & 1) ? wam_sin_poly(r) : wam_cos_poly(r);
+    if (q == 1 || q == 2) v = -v;
+    return (float)v;
+}
 
-/* x**y for x >= 0 (real exponent), as exp(y*log(x)). */
+/* x**y for a real exponent (x >= 0; a negative base gives NaN), as
+   exp(y log x) in double. */
 WAM_HD float wam_powf(float x, float y)
 {
-    if (x == 0.0f) return y > 0.0f ? 0.0f : (y == 0.0f ? 1.0f : wam_i2f(0x7f800000));
-    return wam_expf(y * wam_logf(x));
+    if (wam_isnan(x) || wam_isnan(y)) return wam_i2f(0x7fc00000);
+    if (y == 0.0f || x == 1.0f) return 1.0f;
+    if (x < 0.0f) return wam_i2f(0x7fc00000);
+    if (x == 0.0f) return y > 0.0f ? 0.0f : wam_i2f(0x7f800000);
+    if (wam_f2i(x) == 0x7f800000) return y > 0.0f ? x : 0.0f;
+    const double t = (double)y * wam_log_d(x);
+    if (t > 90.0) return wam_i2f(0x7f800000);
+    if (t < -105.0) return 0.0f;
+    return (float)wam_exp_d(t);
 }
 
-/* atan(x): reduction to |x| <= tan(pi/8) by atan(x) = pi/2 - atan(1/x) and
-   pi/4 + atan((x-1)/(x+1)), then Cephes' odd minimax polynomial. */
-WAM_HD float wam_atanf(float x)
-{
-    const float ax = wam_fabsf(x);
-    float y = 0.0f, r = ax;
-    if (ax > 2.414213562373095f) {           /* tan(3 pi/8) */
-        y = 1.5707963267948966f;
-        r = -(1.0f / ax);
-    } else if (ax > 0.4142135623730950f) {  /* tan(pi/8) */
-        y = 0.7853981633974483f;
-        r = (ax - 1.0f) / (ax + 1.0f);
-    }
-    const float z = r * r;
-    y = y + ((((8.05374449538e-2f * z - 1.38776856032e-1f) * z + 1.99777106478e-1f) * z - 3.33329491539e-1f) * z *
-                 r +
-             r);
-    return x < 0.0f ? -y : y;
-}
-
-/* atan2(y, x) in (-pi, pi], by quadrant from wam_atanf(y/x). */
+/* atan2(y, x) in [-pi, pi] with the IEEE signed-zero conventions: the ratio
+   of the smaller to the larger magnitude (correctly rounded in double),
+   atan of it, then the octant. */
 WAM_HD float wam_atan2f(float y, float x)
 {
-    const float pi = 3.14159265358979323846f;
-    if (x == 0.0f) {
-        if (y > 0.0f) return 1.5707963267948966f;
-        if (y < 0.0f) return -1.5707963267948966f;
-        return 0.0f;
+    if (wam_isnan(x) || wam_isnan(y)) return wam_i2f(0x7fc00000);
+    const double ay = (double)wam_fabsf(y), ax = (double)wam_fabsf(x);
+    const int xneg = wam_f2i(x) < 0;
+    double a;
+    if (ay == 0.0) a = xneg ? WAM_PI_D : 0.0;
+    else if (ax == 0.0) a = WAM_PIO2_D;
+    else if (ay > ax) {
+        a = WAM_PIO2_D - wam_atan01_d(ax / ay);
+        if (xneg) a = WAM_PI_D - a;
+    } else {
+        a = wam_atan01_d(ay / ax);
+        if (xneg) a = WAM_PI_D - a;
     }
-    if (y == 0.0f) return x > 0.0f ? 0.0f : pi;
-    const float w = wam_atanf(y / x);
-    if (x > 0.0f) return w;
-    return y < 0.0f ? w - pi : w + pi;
+    return wam_copysignf((float)a, y);
 }
+
+WAM_HD float wam_atanf(float x) { return wam_atan2f(x, 1.0f); }
 
 #endif

@@ -688,4 +688,18 @@ __global__ void wavnu2_kernel(DevTables t, float *__restrict__ wn, float *__rest
     cgg[(size_t)m * t.NSEA + ij] = cg;
 }
 
+// SBOTTOM's factor of every (sea point, frequency), [ML][NSEA]:
+// (-2*0.038/G * XK) / SINH(MIN(2 XK DEPTH, 50)) with XK = TFAK(INDEP, M)
+// (wam_source_module.f90:1527-1589), the operations of IMPLSCH's own
+// evaluation; it depends on the depth field only.
+__global__ void sbottom_factor_kernel(DevTables t, float *__restrict__ sbot)
+{
+    const int ij = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ij >= t.NSEA) return;
+    const int m = blockIdx.y;
+    const float wav = t.TFAK[m * t.NDEPTH + t.INDEP[ij] - 1];
+    const float sbo = wam_minf((2.f * t.DEPTH[ij]) * wav, 50.f);
+    sbot[(size_t)m * t.NSEA + ij] = (((-2.0f * 0.038f) / t.G) * wav) / wam_sinhf(sbo);
+}
+
 }  // namespace wam

@@ -388,8 +388,12 @@ class WamGpu:
         d = np.ascontiguousarray(udir, np.float32)
         if f.size != n * self.kl * self.ml:
             raise ValueError("FL3 must have NSEA*KL*ML elements")
+        # DEPTH and INDEP, when given, must be the fields of the last
+        # prepare_propagation (the library checks them, WAM_E_STATE otherwise)
+        dp = None if depth is None else np.ascontiguousarray(depth, np.float32)
+        ip = None if indep is None else np.ascontiguousarray(indep, np.int32)
         self._check(self.lib.wam_gpu_implsch(self.ctx, n, self.kl, self.ml, _ptr(f), _ptr(u), _ptr(d), _ptr(tw),
-                                             _ptr(us), _ptr(z), _ptr(ro), _ptr(ws), None, None,
+                                             _ptr(us), _ptr(z), _ptr(ro), _ptr(ws), _ptr(dp), _ptr(ip),
                                              int(bool(lcflx)), _ptr(ph), _ptr(fx)))
         return dict(fl3=f, tauw=tw, ustar=us, z0=z, phiaw=ph, flux=fx)
 
