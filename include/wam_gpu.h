@@ -129,6 +129,21 @@ int wam_gpu_implsch_dev(wam_ctx *ctx, float *d_fl3, const float *d_u10, const fl
                         const float *d_wstar, int32_t lcflx, float *d_phiaw, float *d_fluxes,
                         int32_t *d_mij);
 
+/* IMPLSCH on device memory without allocating anything: the arguments and
+   results of wam_gpu_implsch_dev, bit-identical to it.  No counterpart in the
+   reference; it is for callers that cannot allocate device memory (code under
+   HIP graph capture, tight serving loops, hosts that forbid it).  Its work
+   arrays are a static device array of 256 MiB per GPU (held until the process
+   ends), so the sea points are advanced in chunks of the points whose work
+   arrays fit (about 12 000 at 36 x 36); spectra whose 64-point work arrays
+   exceed 256 MiB are refused with WAM_E_ARG.  Every call on one GPU uses that
+   GPU's array: calls that overlap on one GPU (other streams, other contexts)
+   race and must be ordered by the caller.  Calls no hipMalloc or hipFree;
+   asynchronous on the context stream; no per-kernel timing. */
+int wam_gpu_implsch_dev_noalloc(wam_ctx *ctx, float *d_fl3, const float *d_u10, const float *d_udir, float *d_tauw,
+                                float *d_ustar, float *d_z0, const float *d_roairn, const float *d_wstar,
+                                int32_t lcflx, float *d_phiaw, float *d_fluxes, int32_t *d_mij);
+
 /* Stream used by the context (a hipStream_t; NULL = the context's own). */
 int wam_gpu_set_stream(wam_ctx *ctx, void *stream);
 

@@ -38,6 +38,7 @@ SIGNATURES = {
     "wam_gpu_implsch": (_I, [_P, _I, _I, _I] + [_P] * 10 + [_I, _P, _P]),
     "wam_gpu_propags_dev": (_I, [_P, _P, _P]),
     "wam_gpu_implsch_dev": (_I, [_P] + [_P] * 8 + [_I, _P, _P, _P]),
+    "wam_gpu_implsch_dev_noalloc": (_I, [_P] + [_P] * 8 + [_I, _P, _P, _P]),
     "wam_gpu_set_propagation_weights": (_I, [_P, _I]),
     "wam_gpu_set_stream": (_I, [_P, _P]),
     "wam_gpu_output_parameters_dev": (_I, [_P] * 7 + [_I, _P]),

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -53,6 +54,15 @@ hipError_t launch_boundary_space(const DevTables &t, int nbinp, const float *fi,
 }  // namespace wam
 
 using namespace wam;
+
+// The work arrays of wam_gpu_implsch_dev_noalloc: a file-scope device array,
+// one copy per GPU (held from the library's first use on that GPU until the
+// process ends).  Host code never passes its name to a kernel: the device
+// address comes from hipGetSymbolAddress, per device, at wam_gpu_init.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)256 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+static void *scratch_dev[64];             // device address of scratch on each device (nullptr: not yet looked up)
+static std::mutex scratch_mu;
 
 namespace {
 __global__ void eval_math_kernel(int fn, const float *x, const float *y, float *out, int64_t n)
@@ -107,6 +117,7 @@ struct wam_ctx {
     size_t grids_n = 0;
     std::vector<float> depth_h;           // the DEPTH of the last PREPARE_PROPAGATION (host copy)
     bool prepared = false;
+    void *noalloc = nullptr;              // this device's copy of scratch (wam_gpu_implsch_dev_noalloc)
     int ncu = 256;                 // compute units of the device
     std::string err;
 };
@@ -264,6 +275,19 @@ int wam_gpu_init(wam_ctx **out, const wam_config *cfg, const uint8_t *sea_mask)
     if (hipGetDevice(&dev) != hipSuccess) { ctx->err = "no HIP device"; *out = ctx; return WAM_E_HIP; }
     if (hipStreamCreate(&ctx->own_stream) != hipSuccess) { ctx->err = "hipStreamCreate failed"; *out = ctx; return WAM_E_HIP; }
     ctx->stream = ctx->own_stream;
+    // the device address of the no-allocation scratch array, once per device
+    // (with that device current, as it is here)
+    if (dev >= 0 && dev < 64) {
+        std::lock_guard<std::mutex> lock(scratch_mu);
+        if (!scratch_dev[dev]) {
+            // a failure leaves the no-allocation entry point unusable
+            // (it reports WAM_E_STATE) and every other one as it is
+            void *ptr = nullptr;
+            if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) == hipSuccess && ptr) scratch_dev[dev] = ptr;
+            else (void)hipGetLastError();
+        }
+        ctx->noalloc = scratch_dev[dev];
+    }
     rc = upload_static(ctx);
     if (!rc) {
         int ncu = 256;
@@ -680,6 +704,7 @@ static int ensure_impl_scratch(wam_ctx *ctx, int lcflx)
     const size_t tiled = (ns + 63) / 64 * 64 * nb;      // whole 64-point tiles
     if (!x.ft) {
         x.NB4 = (int)(nb / 4);
+        x.P0 = 0; x.NP = (int)ns; x.NX = (int)ns;      // the whole grid in one chunk
         HIPCHK(hipMalloc(&x.ft, tiled * sizeof(float)));
         HIPCHK(hipMalloc(&x.slt, tiled * sizeof(float)));
         HIPCHK(hipMalloc(&x.flt, tiled * sizeof(float)));
@@ -720,6 +745,66 @@ int wam_gpu_implsch_dev(wam_ctx *ctx, float *d_fl3, const float *d_u10, const fl
         timing_keep(ctx, "implsch", ev[0], ev[1]);
         for (int i = 2; i < 4; i++)
             if (ev[i]) (void)hipEventDestroy(ev[i]);
+    }
+    return WAM_OK;
+}
+
+// The no-allocation form of wam_gpu_implsch_dev: the three-kernel IMPLSCH on
+// chunks of sea points whose work arrays fit the static scratch array, carved
+// as F, SL, FL, SMIN tiles, STRESSO's sums, the per-point scalars, SPRD, the
+// LLWS bits and the ticket counters (each 256-byte aligned).  Returns the
+// points per chunk (a multiple of 64), 0 if not even 64 fit.
+static int noalloc_layout(const Tables &t, void *base, ImplScratch &x)
+{
+    const size_t nb = (size_t)t.KL * t.ML, nw = (nb + 31) / 32;
+    const size_t per_pt = (4 * nb + 3 * (size_t)t.ML + 6 + t.KL + nw) * 4;
+    const size_t fixed = 1024 + 10 * 256;                       // tickets, alignment
+    if (NOALLOC_SCRATCH_BYTES < fixed + 64 * per_pt) return 0;
+    const size_t c = (NOALLOC_SCRATCH_BYTES - fixed) / per_pt / 64 * 64;
+    unsigned char *p = (unsigned char *)base;
+    auto take = [&](size_t bytes) { void *q = p; p += (bytes + 255) / 256 * 256; return q; };
+    x.NB4 = (int)(nb / 4);
+    x.NX = (int)c;
+    x.ft = (float *)take(c * nb * 4);
+    x.slt = (float *)take(c * nb * 4);
+    x.flt = (float *)take(c * nb * 4);
+    x.smt = (float *)take(c * nb * 4);
+    x.axy = (float *)take(3 * (size_t)t.ML * c * 4);
+    x.pp = (float *)take(6 * c * 4);
+    x.sprd = (float *)take((size_t)t.KL * c * 4);
+    x.llws = (unsigned *)take(nw * c * 4);
+    x.tick = (unsigned *)take(8 * 32 * 4);
+    if ((size_t)(p - (unsigned char *)base) > NOALLOC_SCRATCH_BYTES) return 0;
+    return (int)c;
+}
+
+int wam_gpu_implsch_dev_noalloc(wam_ctx *ctx, float *d_fl3, const float *d_u10, const float *d_udir, float *d_tauw,
+                                float *d_ustar, float *d_z0, const float *d_roairn, const float *d_wstar, int32_t lcflx,
+                                float *d_phiaw, float *d_fluxes, int32_t *d_mij)
+{
+    if (!ctx || !d_fl3 || !d_u10 || !d_udir || !d_tauw || !d_ustar || !d_z0 || !d_roairn || !d_wstar) return WAM_E_ARG;
+    if (!ctx->prepared) { ctx->err = "PREPARE_PROPAGATION not done"; return WAM_E_STATE; }
+    ImplArgs a{};
+    a.fl3 = d_fl3; a.u10 = d_u10; a.udir = d_udir; a.tauw = d_tauw; a.ustar = d_ustar; a.z0 = d_z0;
+    a.roairn = d_roairn; a.wstar = d_wstar; a.lcflx = lcflx ? 1 : 0; a.phiaw = d_phiaw; a.fluxes = d_fluxes;
+    a.mij = d_mij;
+    if (!implsch_split_supported(ctx->dt)) {       // the single kernel uses no work arrays
+        HIPCHK(launch_implsch(ctx->dt, a, ctx->ncu, ctx->stream));
+        return WAM_OK;
+    }
+    if (!ctx->noalloc) { ctx->err = "the device address of the static scratch array is unknown"; return WAM_E_STATE; }
+    ImplScratch x{};
+    const int c = noalloc_layout(ctx->tab, ctx->noalloc, x);
+    if (c < 64) {
+        ctx->err = "KL*ML too large for the no-allocation scratch (256 MiB holds the work arrays of fewer than 64 "
+                   "sea points)";
+        return WAM_E_ARG;
+    }
+    const int ns = ctx->tab.NSEA;
+    for (int p0 = 0; p0 < ns; p0 += c) {
+        x.P0 = p0;
+        x.NP = std::min(c, ns - p0);
+        HIPCHK(launch_implsch_split(ctx->dt, a, x, ctx->ncu, ctx->stream, nullptr));
     }
     return WAM_OK;
 }

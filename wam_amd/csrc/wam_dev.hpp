@@ -102,11 +102,16 @@ struct PrepPoints {
 
 struct ImplScratch {
     int NB4;                       // KL*ML/4
+    // the chunk of sea points one launch sequence advances: points P0 ..
+    // P0 + NP - 1, in rows 0 .. NP - 1 of the work arrays below, whose
+    // per-point rows are NX apart (NX >= NP; one chunk of NSEA points with
+    // NX = NSEA when the arrays are sized for the whole grid)
+    int P0, NP, NX;
     float *ft, *slt, *flt, *smt;   // F, SL, FL, SL - SPOS of the second SINPUT (LCFLX only)
-    float *axy;                    // [3][ML][NSEA]: STRESSO's direction sums of the first SINPUT
-    float *pp;                     // [6][NSEA]: ENHFR, MAX(FMEANWS, FMEAN), MIJ, PHIAW, SIN(UDIR), COS(UDIR)
-    float *sprd;                   // [KL][NSEA]: SPRD(K) = MAX(0, COS(TH(K) - UDIR))**2 of the call
-    unsigned *llws;                // [(KL*ML+31)/32][NSEA]: LLWS bits of the second SINPUT
+    float *axy;                    // [3][ML][NX]: STRESSO's direction sums of the first SINPUT
+    float *pp;                     // [6][NX]: ENHFR, MAX(FMEANWS, FMEAN), MIJ, PHIAW, SIN(UDIR), COS(UDIR)
+    float *sprd;                   // [KL][NX]: SPRD(K) = MAX(0, COS(TH(K) - UDIR))**2 of the call
+    unsigned *llws;                // [(KL*ML+31)/32][NX]: LLWS bits of the second SINPUT
     unsigned *tick;                // [8][32]: point counters of implsch_snl_kernel, one per XCD (own 128-byte line)
 };
 

@@ -1264,10 +1264,11 @@ template <int SHALLOW>
 __global__ void __launch_bounds__(PW) __attribute__((amdgpu_waves_per_eu(4))) implsch_pre_kernel(DevTables t, ImplArgs a, ImplScratch x)
 {
     extern __shared__ float cu_lds[];            // [KL][PW]: COS(TH(K) - UDIR) of each lane's point
-    const int KL = t.KL, ML = t.ML, NB = KL * ML, NS = t.NSEA, NB4 = x.NB4, lane = threadIdx.x;
-    const int pt0 = blockIdx.x * PW + lane;
-    const bool on = pt0 < NS;
-    const int pt = on ? pt0 : NS - 1;
+    const int KL = t.KL, ML = t.ML, NB = KL * ML, NS = t.NSEA, NB4 = x.NB4, XS = x.NX, lane = threadIdx.x;
+    // lp: the point's row in the work arrays (its place in the chunk), pt: the sea point
+    const int lp0 = blockIdx.x * PW + lane;
+    const bool on = lp0 < x.NP;
+    const int lp = on ? lp0 : x.NP - 1, pt = x.P0 + lp;
     float *cu = cu_lds + lane;
     const float *F = a.fl3 + pt;                  // F(b) = F[b * NS]
     const float u10 = a.u10[pt], udir = a.udir[pt], roairn = a.roairn[pt], wstar = a.wstar[pt];
@@ -1277,7 +1278,7 @@ __global__ void __launch_bounds__(PW) __attribute__((amdgpu_waves_per_eu(4))) im
         const float c = wam_cosf(t.TH[k] - udir);
         cu[k * PW] = c;
         // the limiter's floor FLMINFR(JU, M) * SPRD(K) (:494-498) is applied by implsch_snl_kernel
-        if (on) x.sprd[(size_t)k * NS + pt] = wam_ipowf(wam_maxf(0.f, c), 2);
+        if (on) x.sprd[(size_t)k * XS + lp] = wam_ipowf(wam_maxf(0.f, c), 2);
     }
     airsea(t, u10, tauw, ust, z0);
 
@@ -1319,10 +1320,10 @@ __global__ void __launch_bounds__(PW) __attribute__((amdgpu_waves_per_eu(4))) im
                 }
             }
             if (on) {
-                float *axy = x.axy + (size_t)m * NS + pt;
+                float *axy = x.axy + (size_t)m * XS + lp;
                 axy[0] = sa;
-                axy[(size_t)ML * NS] = sx;
-                axy[(size_t)2 * ML * NS] = sy;
+                axy[(size_t)ML * XS] = sx;
+                axy[(size_t)2 * ML * XS] = sy;
             }
             if (i == 0) {
                 Tl = T; Ql = Q;
@@ -1372,12 +1373,12 @@ __global__ void __launch_bounds__(PW) __attribute__((amdgpu_waves_per_eu(4))) im
     {
         float phi = 0.f, xs = 0.f, ys = 0.f;
         for (int m = 0; m < mij; m++) {
-            const float *axy = x.axy + (size_t)m * NS + pt;
+            const float *axy = x.axy + (size_t)m * XS + lp;
             const float rw = stresso_rw(t, m, mij);
             const float cmrw = stresso_cm<SHALLOW>(t, m, SHALLOW ? t.TFAK[tb(t, m, indep)] : 0.f) * rw;
             phi = phi + axy[0] * rw;
-            xs = xs + axy[(size_t)ML * NS] * cmrw;
-            ys = ys + axy[(size_t)2 * ML * NS] * cmrw;
+            xs = xs + axy[(size_t)ML * XS] * cmrw;
+            ys = ys + axy[(size_t)2 * ML * XS] * cmrw;
         }
         float phiaw;
         tauw = stresso_close(t, mij, ust, z0, udir, phi, xs, ys, h1, h2, phiaw);
@@ -1399,8 +1400,8 @@ __global__ void __launch_bounds__(PW) __attribute__((amdgpu_waves_per_eu(4))) im
     {
         const float sig_n = wsigstar(t, ust, z0, wstar);
         const float usn1 = ust * (1.0f - sig_n), usn2 = ust * (1.0f + sig_n);
-        const size_t q0 = quad_offset(pt, 0, NB4);
-        unsigned *LLW = x.llws + pt;
+        const size_t q0 = quad_offset(lp, 0, NB4);
+        unsigned *LLW = x.llws + lp;
         unsigned word = 0;
         for (int m = 0; m < ML; m++) {
             const float tfk = SHALLOW ? t.TFAK[tb(t, m, indep)] : 0.f;
@@ -1433,7 +1434,7 @@ __global__ void __launch_bounds__(PW) __attribute__((amdgpu_waves_per_eu(4))) im
                     sm4[u] = sl - spos;
                     word |= (unsigned)ll << (b & 31);
                     if ((b & 31) == 31 || b == NB - 1) {
-                        if (on) LLW[(size_t)(b >> 5) * NS] = word;
+                        if (on) LLW[(size_t)(b >> 5) * XS] = word;
                         word = 0;
                     }
                     const float spp = wam_maxf(spos, 0.f);
@@ -1470,13 +1471,13 @@ __global__ void __launch_bounds__(PW) __attribute__((amdgpu_waves_per_eu(4))) im
             const float enh = wam_maxf((0.75f * t.DEPTH[pt]) * akmean, .5f);
             enhfr = 1.f + ((5.5f / enh) * (1.f - .833f * enh)) * wam_expf(-1.25f * enh);
         }
-        x.pp[pt] = enhfr;
-        x.pp[(size_t)NS + pt] = wam_maxf(fmeanws, fmean);
-        x.pp[(size_t)2 * NS + pt] = (float)mij;
-        x.pp[(size_t)3 * NS + pt] = phiaw;
+        x.pp[lp] = enhfr;
+        x.pp[(size_t)XS + lp] = wam_maxf(fmeanws, fmean);
+        x.pp[(size_t)2 * XS + lp] = (float)mij;
+        x.pp[(size_t)3 * XS + lp] = phiaw;
         if (a.lcflx) {   // for WNFLUXES in implsch_snl_kernel
-            x.pp[(size_t)4 * NS + pt] = wam_sinf(udir);
-            x.pp[(size_t)5 * NS + pt] = wam_cosf(udir);
+            x.pp[(size_t)4 * XS + lp] = wam_sinf(udir);
+            x.pp[(size_t)5 * XS + lp] = wam_cosf(udir);
         }
     }
     // implsch_snl_kernel's point tickets start from zero
@@ -1497,7 +1498,7 @@ implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x)
     __shared__ float Ps[5];
     __shared__ int Pmij, Pnext;
     constexpr int S = 256 * SP_QPT_DEV;
-    const int KL = t.KL, ML = t.ML, NB = KL * ML, NS = t.NSEA, NB4 = x.NB4, tid = threadIdx.x;
+    const int KL = t.KL, ML = t.ML, NB = KL * ML, NS = t.NSEA, NB4 = x.NB4, XS = x.NX, tid = threadIdx.x;
     const int MH = t.MLSTHG;
 
     // LDS: the products U [9][S] float2, F, (SMIN, SL_BOT with LCFLX), tables
@@ -1542,14 +1543,15 @@ implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x)
     // share the 128-byte lines of the tiled arrays in its L2.  The ticket for
     // the next point is taken at the start of the current one.
     const int xcd = blockIdx.x % 8;
-    const int per = (NS + 7) / 8;
-    const int p0 = xcd * per, p1 = min(NS, p0 + per);
+    const int per = (x.NP + 7) / 8;                // points of the chunk, rows of the work arrays
+    const int p0 = xcd * per, p1 = min(x.NP, p0 + per);
     unsigned *tick = x.tick + 32 * xcd;
     const float delt = t.DELT;
     if (tid == 0) Pnext = p0 + (int)atomicAdd(tick, 1u);
     __syncthreads();
-    for (int pt = Pnext; pt < p1; pt = Pnext) {
-        const size_t q0 = quad_offset(pt, 0, NB4);
+    for (int lp = Pnext; lp < p1; lp = Pnext) {
+        const int pt = x.P0 + lp;                  // the sea point
+        const size_t q0 = quad_offset(lp, 0, NB4);
         int tnext = 0;
         if (tid == 0) tnext = (int)atomicAdd(tick, 1u);
         {
@@ -1574,13 +1576,13 @@ implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x)
             }
         }
         if (tid == 0) {
-            Ps[0] = a.ustar[pt]; Ps[1] = x.pp[pt]; Ps[2] = x.pp[(size_t)NS + pt]; Ps[3] = x.pp[(size_t)3 * NS + pt];
-            Pmij = (int)x.pp[(size_t)2 * NS + pt];
+            Ps[0] = a.ustar[pt]; Ps[1] = x.pp[lp]; Ps[2] = x.pp[(size_t)XS + lp]; Ps[3] = x.pp[(size_t)3 * XS + lp];
+            Pmij = (int)x.pp[(size_t)2 * XS + lp];
         }
         // no elementary functions here: SPRD comes from implsch_pre_kernel and
         // SBOTTOM's factor from PREPARE_PROPAGATION's table (their double-
         // precision evaluation would make this kernel's SNONLIN loop spill)
-        if (tid < KL) sprd[tid] = x.sprd[(size_t)tid * NS + pt];
+        if (tid < KL) sprd[tid] = x.sprd[(size_t)tid * XS + lp];
         if (tid >= 64 && tid - 64 < ML) {
             const int m = tid - 64;
             const int ju = min(t.JUMAX, max(wam_nint(a.u10[pt] / t.DELU), 1));
@@ -1611,7 +1613,7 @@ implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x)
             }
             __syncthreads();
             if (tid == 0 && a.fluxes)
-                wnfluxes<SHALLOW>(t, s, Ps[3], Ps[0], x.pp[(size_t)4 * NS + pt], x.pp[(size_t)5 * NS + pt], Pmij, tfk,
+                wnfluxes<SHALLOW>(t, s, Ps[3], Ps[0], x.pp[(size_t)4 * XS + lp], x.pp[(size_t)5 * XS + lp], Pmij, tfk,
                                   a.fluxes + pt, NS);
         }
         // SBOTTOM (SL_BOT = 0 + factor * F, FL + factor), the limiter and the
@@ -1647,12 +1649,12 @@ implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x)
 template <int SHALLOW>
 __global__ void __launch_bounds__(PW) implsch_post_kernel(DevTables t, ImplArgs a, ImplScratch x)
 {
-    const int KL = t.KL, ML = t.ML, NS = t.NSEA, NB4 = x.NB4, lane = threadIdx.x;
-    const int pt0 = blockIdx.x * PW + lane;
-    const bool on = pt0 < NS;
-    const int pt = on ? pt0 : NS - 1;
-    const float *FQ = x.ft + quad_offset(pt, 0, NB4);   // bins 4q..4q+3 at FQ[q * 256]
-    const unsigned *LLW = x.llws + pt;
+    const int KL = t.KL, ML = t.ML, NS = t.NSEA, NB4 = x.NB4, XS = x.NX, lane = threadIdx.x;
+    const int lp0 = blockIdx.x * PW + lane;        // the point's row in the work arrays
+    const bool on = lp0 < x.NP;
+    const int lp = on ? lp0 : x.NP - 1, pt = x.P0 + lp;
+    const float *FQ = x.ft + quad_offset(lp, 0, NB4);   // bins 4q..4q+3 at FQ[q * 256]
+    const unsigned *LLW = x.llws + lp;
     const float ust = a.ustar[pt];
     // direction sums (plain and LLWS-masked) and their integrals; frequency ML first
     float s_e = 0.f, s_fm = 0.f, s_w1 = 0.f, s_w2 = 0.f, Tl = 0.f, Ql = 0.f;
@@ -1663,7 +1665,7 @@ __global__ void __launch_bounds__(PW) implsch_post_kernel(DevTables t, ImplArgs 
         for (int k0 = 0; k0 < KL; k0 += 4) {
             const int b0 = k0 + KL * m;
             const float4 f4 = *reinterpret_cast<const float4 *>(FQ + (size_t)(b0 >> 2) * 256);
-            const unsigned w = LLW[(size_t)(b0 >> 5) * NS] >> (b0 & 31);   // 4 | 32 and b0 % 4 == 0
+            const unsigned w = LLW[(size_t)(b0 >> 5) * XS] >> (b0 & 31);   // 4 | 32 and b0 % 4 == 0
             const float fv[4] = {f4.x, f4.y, f4.z, f4.w};
 #pragma unroll
             for (int u = 0; u < 4; u++) {
@@ -1741,12 +1743,13 @@ bool implsch_split_supported(const DevTables &t)
     return t.iphys == 0 && t.KL % 4 == 0 && t.KL * t.ML <= SNL_NT * SNL_NJ && 2 * t.KL < 256 * SP_QPT_DEV;
 }
 
-// The three kernels of IPHYS = 0 on one stream; the middle one persistent,
-// one workgroup per resident slot.
+// The three kernels of IPHYS = 0 on one stream for the chunk of sea points
+// x.P0 .. x.P0 + x.NP - 1 (rows 0 .. NP - 1 of the work arrays); the middle
+// one persistent, one workgroup per resident slot.
 hipError_t launch_implsch_split(const DevTables &t, const ImplArgs &a, const ImplScratch &x, int ncu, hipStream_t st,
                                 const hipEvent_t *ev)
 {
-    const int nwg = (t.NSEA + PW - 1) / PW;
+    const int nwg = (x.NP + PW - 1) / PW;
     auto mark = [&](int i) { if (ev && ev[i]) (void)hipEventRecord(ev[i], st); };
     mark(0);
     const size_t lds1 = (size_t)t.KL * PW * sizeof(float);
@@ -1764,7 +1767,7 @@ hipError_t launch_implsch_split(const DevTables &t, const ImplArgs &a, const Imp
     if (e != hipSuccess) return e;
     if (per_cu < 1) return hipErrorInvalidConfiguration;
     const long want = (long)per_cu * ncu;
-    const int nblocks = (int)(want < t.NSEA ? want : t.NSEA);
+    const int nblocks = (int)(want < x.NP ? want : x.NP);
     hipLaunchKernelGGL(k, dim3(nblocks), dim3(SNL_NT), lds, st, t, a, x);
     e = hipGetLastError();
     if (e != hipSuccess) return e;

@@ -406,3 +406,10 @@ class WamGpu:
         self._check(self.lib.wam_gpu_implsch_dev(self.ctx, _ptr(fl3), _ptr(u10), _ptr(udir), _ptr(tauw),
                                                  _ptr(ustar), _ptr(z0), _ptr(roairn), _ptr(wstar),
                                                  int(bool(lcflx)), _ptr(phiaw), _ptr(fluxes), _ptr(mij)))
+
+    def implsch_dev_noalloc(self, fl3, u10, udir, tauw, ustar, z0, roairn, wstar, lcflx=False, phiaw=None,
+                            fluxes=None, mij=None):
+        """implsch_dev on the library's static device scratch array: allocates nothing."""
+        self._check(self.lib.wam_gpu_implsch_dev_noalloc(self.ctx, _ptr(fl3), _ptr(u10), _ptr(udir), _ptr(tauw),
+                                                         _ptr(ustar), _ptr(z0), _ptr(roairn), _ptr(wstar),
+                                                         int(bool(lcflx)), _ptr(phiaw), _ptr(fluxes), _ptr(mij)))
