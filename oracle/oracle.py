@@ -61,6 +61,7 @@ def _lib(fp64, opcount=False):
         lib.orc_api_prepare_propagation.restype = ctypes.c_int
         lib.orc_api_prepare_propagation.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 3
         lib.orc_api_propags.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        lib.orc_api_force_counter.argtypes = [ctypes.c_void_p, ctypes.c_int]
         lib.orc_api_implsch.argtypes = ([ctypes.c_void_p] + [ctypes.c_void_p] * 9 +
                                         [ctypes.c_int] + [ctypes.c_void_p] * 3 + [ctypes.c_int])
         lib.orc_api_wavnu2.argtypes = [ctypes.c_void_p] * 4
@@ -131,6 +132,11 @@ class Oracle:
         if st != 0:
             raise ValueError(f"oracle prepare_propagation status {st}")
         return int(self.scalar("COUNTER")), float(self.scalar("CFL_MAX"))
+
+    def force_counter(self, counter):
+        """For a window of a larger grid: CHECK_CFL's sub-step count is that
+        grid's (the weights rescaled by it), from the next prepare_propagation."""
+        self.lib.orc_api_force_counter(self.ctx, int(counter))
 
     def propags(self, fl3):
         f = np.array(fl3, dtype=self.dtype, copy=True).ravel()

@@ -80,6 +80,9 @@ int orc_api_prepare_propagation(orc_ctx *c, const real *depth, const real *u, co
     return orc_prepare_propagation(c, depth, u, v);
 }
 void orc_api_propags(const orc_ctx *c, real *fl3) { orc_propags(c, fl3); }
+/* CHECK_CFL's sub-step count of the grid this context is a window of (0:
+   its own); set before orc_api_prepare_propagation */
+void orc_api_force_counter(orc_ctx *c, int counter) { c->force_counter = counter; }
 void orc_api_implsch(const orc_ctx *c, real *fl3, const real *u10, const real *udir, real *tauw,
                      real *ustar, real *z0, const real *roairn, const real *wstar, const real *depth,
                      int lcflx, real *phiaw, real *out_flux, int *mij, int nthreads)

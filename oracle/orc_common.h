@@ -172,6 +172,7 @@ typedef struct {
     real *DPSN;                             /* [2][NSEA] */
     /* propagation */
     int counter; real cfl_max, cfl_min;
+    int force_counter;   /* > 0: COUNTER of the grid this one is a window of (CHECK_CFL's rescale by it) */
     int ncoef;                              /* coefficient planes present */
     real *COEF;                             /* [ncoef][ML][KL][NSEA] */
     int *INDEP;                             /* [NSEA] */

@@ -782,8 +782,9 @@ int orc_prepare_propagation(orc_ctx *c, const real *depth, const real *u, const 
     (void)lon1mn;
     c->cfl_max = cmax; c->cfl_min = cmin;
     if (cmin < R(0.)) return 3;                      /* ABORT1 in the reference */
-    if (cmax > R(0.999)) {
-        c->counter = (int)CEILR(cmax);
+    /* a window of a larger grid takes that grid's COUNTER (force_counter) */
+    if (c->force_counter > 0 ? c->force_counter > 1 : cmax > R(0.999)) {
+        c->counter = c->force_counter > 0 ? c->force_counter : (int)CEILR(cmax);
         real rc = (real)c->counter;
         for (int p = 1; p < C_NPLANES; p++)
             for (size_t q = 0; q < NB; q++) c->COEF[(size_t)p * NB + q] = c->COEF[(size_t)p * NB + q] / rc;

@@ -1,0 +1,129 @@
+"""The coupled time step against the reference's own: PROPAGS then IMPLSCH per
+step (wamodel.f90:332,354) for 6 to 24 hours, golden vectors written by
+tests/golden/make_ref_step.py from the reference built from its sources
+(oracle/ref_step_driver.f90).  The oracle (CPU) and the HIP path (through
+the C-ABI, spectra resident in HBM) are both held to north_star's bars:
+
+  * after the first step, the spectrum within 1e-5 per bin
+    (|F - F_ref| <= 1e-5 max(|F_ref|, 1e-6 max|F_ref|)) and USTAR, TAUW, Z0
+    within 1e-5;
+  * after the last step, Hs, the mean period and USTAR within 1e-4 (relative,
+    every sea point), and the spectrum within 1e-4 on the bins holding at
+    least a tenth of their point's peak.
+
+The HIP path is also required to be bit-identical to the oracle after the
+last step, so its agreement with the reference is the oracle's."""
+import glob
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIXTURES = sorted(glob.glob(os.path.join(HERE, "golden", "ref_step_*.npz")))
+IDS = [os.path.basename(p)[9:-4] for p in FIXTURES]
+
+
+def load(path):
+    z = np.load(path)
+    return dict(z), json.loads(str(z["cfg"]))
+
+
+def spec_err(f, ref):
+    ref = np.asarray(ref, np.float64)
+    den = np.maximum(np.abs(ref), 1e-6 * np.abs(ref).max())
+    return np.abs(np.asarray(f, np.float64) - ref) / den
+
+
+def rel(a, b, floor=1e-30):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor)))
+
+
+def hs_tm(cfg, dfim, fr, f, n):
+    e = np.asarray(f, np.float64).reshape(cfg["ml"], cfg["kl"], n).sum(1)
+    tot = (e * dfim[:, None]).sum(0)
+    return 4.0 * np.sqrt(tot), tot / (e * (dfim / fr)[:, None]).sum(0)
+
+
+def check(z, cfg, first, last, dfim, fr):
+    """first/last: dicts of fl3, tauw, ustar, z0 after the first and the last step."""
+    n = z["in_u10"].size
+    assert spec_err(first["fl3"], z["out_fl3_1"]).max() <= 1e-5
+    for k, floor in (("ustar", 1e-6), ("tauw", 1e-9), ("z0", 1e-12)):
+        assert rel(first[k], z[f"out_{k}_1"], floor) <= 1e-5, k
+    hs, tm = hs_tm(cfg, dfim, fr, last["fl3"], n)
+    hs_r, tm_r = hs_tm(cfg, dfim, fr, z["out_fl3_n"], n)
+    assert rel(hs, hs_r) <= 1e-4, rel(hs, hs_r)
+    assert rel(tm, tm_r) <= 1e-4, rel(tm, tm_r)
+    assert rel(last["ustar"], z["out_ustar_n"], 1e-6) <= 1e-4
+    ref = np.asarray(z["out_fl3_n"], np.float64).reshape(-1, n)
+    big = ref >= 0.1 * ref.max(axis=0)
+    assert spec_err(last["fl3"], z["out_fl3_n"]).reshape(ref.shape)[big].max() <= 1e-4
+
+
+def oracle_run(z, cfg):
+    o = Oracle(cfg, z["mask"])
+    cur = (z["in_u"], z["in_v"]) if cfg["refr_c"] else (None, None)
+    o.prepare_propagation(z["in_depth"], *cur)
+    f, tw, z0 = z["in_fl3"].copy(), z["in_tauw"].copy(), z["in_z0"].copy()
+    first = None
+    for i in range(int(z["nstep"])):
+        f = o.propags(f)
+        r = o.implsch(f, z["in_u10"], z["in_udir"], tw, z0, z["in_depth"], roairn=z["in_roairn"],
+                      wstar=z["in_wstar"])
+        f, tw, z0 = r["fl3"], r["tauw"], r["z0"]
+        if i == 0:
+            first = dict(r)
+    return o, first, r
+
+
+@pytest.mark.parametrize("path", FIXTURES, ids=IDS)
+def test_oracle_matches_reference_time_loop(path):
+    z, cfg = load(path)
+    o, first, last = oracle_run(z, cfg)
+    check(z, cfg, first, last, o.get("DFIM"), o.get("FR"))
+
+
+def test_step_fixtures_present():
+    assert len(FIXTURES) >= 3 and any("24h" in p for p in FIXTURES)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", FIXTURES, ids=IDS)
+def test_gpu_matches_reference_time_loop(path):
+    import torch
+    from wam_amd import WamGpu
+    z, cfg = load(path)
+    print(f"[case] {os.path.basename(path)}", flush=True)
+    g = WamGpu(cfg, z["mask"])
+    n = g.nsea
+    cur = (z["in_u"], z["in_v"]) if cfg["refr_c"] else (None, None)
+    g.prepare_propagation(z["in_depth"], *cur)
+    dev = torch.device("cuda:0")
+    t = lambda x: torch.from_numpy(np.ascontiguousarray(x, np.float32)).to(dev)
+    fa, fb = t(z["in_fl3"]), torch.empty(z["in_fl3"].size, device=dev)
+    u10, udir, tauw, z0 = t(z["in_u10"]), t(z["in_udir"]), t(z["in_tauw"]), t(z["in_z0"])
+    roairn, wstar, ustar = t(z["in_roairn"]), t(z["in_wstar"]), torch.zeros(n, device=dev)
+    first = None
+    for i in range(int(z["nstep"])):
+        g.propags_dev(fa, fb)
+        fa, fb = fb, fa
+        g.implsch_dev(fa, u10, udir, tauw, ustar, z0, roairn, wstar)
+        if i == 0:
+            torch.cuda.synchronize()
+            first = dict(fl3=fa.cpu().numpy(), tauw=tauw.cpu().numpy(), ustar=ustar.cpu().numpy(),
+                         z0=z0.cpu().numpy())
+    torch.cuda.synchronize()
+    last = dict(fl3=fa.cpu().numpy(), tauw=tauw.cpu().numpy(), ustar=ustar.cpu().numpy(), z0=z0.cpu().numpy())
+    check(z, cfg, first, last, g.table("DFIM"), g.table("FR"))
+    # bit-identical to the oracle's own time loop
+    _, _, ro = oracle_run(z, cfg)
+    for k in ("fl3", "tauw", "ustar", "z0"):
+        assert np.array_equal(np.asarray(last[k], np.float32).view(np.int32),
+                              np.asarray(ro[k], np.float32).view(np.int32)), k
+    g.close()

@@ -8,7 +8,8 @@ The two single-GPU workloads of the hot path:
   * ``c2``: 0.1 deg spherical grid 901 x 1600 with a seeded fractal land mask
     (30 % land, about 1.0 M sea points), 36 x 36 spectra, log-normal fractal
     bathymetry 5-5000 m with depth and current refraction, Gaussian random
-    wind field (mean 10, sd 5 m/s), divergence-free currents up to 1 m/s,
+    wind field (mean 10, sd 5 m/s, k**-3 spectrum, 500 km decorrelation),
+    divergence-free currents up to 1 m/s,
     300 s steps, Janssen physics (IPHYS=0).
 Any grid can be cut to a smaller sub-box for parity runs.  Coordinates are in
 the reference's M_SEC units (1 degree = 360000, wam_coordinate_module.f90:41-43).
@@ -72,6 +73,21 @@ def _smooth_field(ny, nx, scale_cells, rng):
     return (f - f.mean()) / (f.std() + 1e-30)
 
 
+def _kminus3_field(ny, nx, scale_cells, rng):
+    """Gaussian random field, unit variance, with the isotropic power spectrum
+    P(kappa) ~ (kappa**2 + 1/L**2)**(-3/2): k**-3 above 1/L and flat below.  In
+    two dimensions its covariance is exp(-r/L), so L (cells) is the e-folding
+    decorrelation length."""
+    ky = np.fft.fftfreq(ny)[:, None]
+    kx = np.fft.rfftfreq(nx)[None, :]
+    kap2 = (2 * math.pi) ** 2 * (kx * kx + ky * ky)
+    amp = (kap2 + 1.0 / (scale_cells * scale_cells)) ** -0.75
+    amp[0, 0] = 0.0
+    noise = rng.standard_normal((ny, nx // 2 + 1)) + 1j * rng.standard_normal((ny, nx // 2 + 1))
+    f = np.fft.irfft2(noise * amp, s=(ny, nx))
+    return (f - f.mean()) / (f.std() + 1e-30)
+
+
 def c1_case():
     """3600-point deep-water ST6 case: (cfg, mask, fields)."""
     cfg = base_config(iphys=2, kl=24, ml=25, idelpro=600, idelt=600)
@@ -101,9 +117,10 @@ def c2_case(ny=901, nx=1600, seed=20261203, iphys=0, kl=36, ml=36, dlat=0.1, dlo
     logd = np.log(2500.0) + 0.5 * _smooth_field(ny, nx, 25.0, rng) + 0.05 * _fractal(ny, nx, 4.0, rng)
     depth2 = np.exp(np.log(5.0) + (logd - np.log(5.0)) * shelf)
     depth2 = np.clip(depth2, 5.0, 5000.0)
-    # wind speed: Gaussian random field, mean 10 and sd 5 m/s, ~500 km scale (0.1 deg ~ 11 km)
+    # wind speed: Gaussian random field, mean 10 and sd 5 m/s, k**-3 spectrum
+    # with a 500 km decorrelation length (0.1 deg ~ 11 km)
     cells = 500e3 / (dlat * 111e3)
-    wspd = np.clip(10.0 + 5.0 * _smooth_field(ny, nx, cells / 4.0, rng), 0.5, None)
+    wspd = np.clip(10.0 + 5.0 * _kminus3_field(ny, nx, cells, rng), 0.5, None)
     wdir = math.pi / 2 + 0.8 * _smooth_field(ny, nx, cells / 2.0, rng)
     # divergence-free current from a stream function, max 1 m/s
     psi = _smooth_field(ny, nx, cells / 8.0, rng)
