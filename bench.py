@@ -207,6 +207,109 @@ def bench_nest(args, torch, g, cfg, mask, fl3):
     print(json.dumps(out), flush=True)
 
 
+def bench_ensemble(args, torch):
+    """The batched ensemble (configuration 3): NMEM members on the 0.1 deg
+    450 x 800 sub-box (252 000 sea points, 36 x 36, shallow with currents),
+    members differing by their winds.  The launch shapes are chosen by the
+    library's autotuner on the state, then K steps (PROPAGS + IMPLSCH of every
+    member) run as one captured HIP graph and, for comparison, launch by
+    launch; a plain run of one member on the same grid gives the per-member
+    reference cost."""
+    from wam_amd import WamGpu, workloads as W
+    t0 = time.time()
+    nmem, K = args.ens_members, args.ens_steps
+    cfg, mask, f, u10m, udirm = W.c3_case(nmem=nmem)
+    g = WamGpu(cfg, mask)
+    n = g.nsea
+    counter, cfl = g.prepare_propagation(f["depth"], f["u"], f["v"])
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    g.set_stream(stream.cuda_stream)
+    t = lambda x: torch.from_numpy(np.ascontiguousarray(x, np.float32)).to(dev)
+    u10, udir = t(u10m.ravel()), t(udirm.ravel())
+    nmp = nmem * n
+    tauw, ustar, z0 = (torch.zeros(nmp, device=dev) for _ in range(3))
+    roairn, wstar = torch.full((nmp,), 1.225, device=dev), torch.zeros(nmp, device=dev)
+    base = device_spectrum(torch, cfg, udir[:n]).reshape(-1)
+    nb = base.numel()
+    fa = base.repeat(nmem)
+    fb = torch.empty_like(fa)
+    del base
+    torch.cuda.synchronize()
+    log(f"[ensemble] {nmem} members x {n} sea points, COUNTER={counter}, state {2 * fa.numel() * 4 / 1e9:.1f} GB "
+        f"({time.time() - t0:.1f}s)")
+    out = g.ensemble_run_dev(nmem, 1, fa, fb, u10, udir, tauw, ustar, z0, roairn, wstar, graph=False)
+    if out is fb:
+        fa, fb = fb, fa
+    torch.cuda.synchronize()
+    a0 = time.time()
+    shapes = g.autotune(nmem, fa, fb, u10, udir, tauw, z0, roairn, wstar, reps=1)
+    log(f"[ensemble] autotuned in {time.time() - a0:.1f}s: {shapes}")
+    # eager: K steps launch by launch, per-kernel HIP-event times
+    g.set_timing(True)
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    out = g.ensemble_run_dev(nmem, K, fa, fb, u10, udir, tauw, ustar, z0, roairn, wstar, graph=False)
+    torch.cuda.synchronize()
+    t_eager = (time.perf_counter() - w0) / K
+    kt = {}
+    for name in KERNELS:
+        ms, calls = g.kernel_time(name)
+        if calls:
+            kt[name] = ms / K
+    g.set_timing(False)
+    if out is fb:
+        fa, fb = fb, fa
+    # the captured loop: built and run once, then timed on a replay
+    g.ensemble_run_dev(nmem, K, fa, fb, u10, udir, tauw, ustar, z0, roairn, wstar, graph=True)
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    out = g.ensemble_run_dev(nmem, K, fa, fb, u10, udir, tauw, ustar, z0, roairn, wstar, graph=True)
+    torch.cuda.synchronize()
+    t_graph = (time.perf_counter() - w0) / K
+    finite = bool(torch.isfinite(out).all().item())
+    log(f"[ensemble] {K} steps: graph {t_graph * 1e3:.1f} ms/step, eager {t_eager * 1e3:.1f} ms/step")
+    # the plain run of member 0 on the same grid, K steps after one warm-up step
+    p_a, p_b = fa[:nb].clone(), torch.empty(nb, device=dev)
+    sl = slice(0, n)
+    pt_tw, pt_us, pt_z0 = tauw[sl].clone(), ustar[sl].clone(), z0[sl].clone()
+    pu, pd, pr, pw = u10[sl].contiguous(), udir[sl].contiguous(), roairn[sl].contiguous(), wstar[sl].contiguous()
+
+    def plain_step():
+        nonlocal p_a, p_b
+        g.propags_dev(p_a, p_b)
+        g.implsch_dev(p_b, pu, pd, pt_tw, pt_us, pt_z0, pr, pw)
+        p_a, p_b = p_b, p_a
+
+    plain_step()
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    for _ in range(K):
+        plain_step()
+    torch.cuda.synchronize()
+    t_plain = (time.perf_counter() - w0) / K
+    opc = json.load(open(OPCOUNT_FILE))["c2_sub_iphys0"]
+    snl = kt.get("implsch_snl")
+    res = {
+        "workload": f"c3: {nmem} members x {n} sea points (0.1deg 450x800 sub-box, 36x36, shallow + currents, "
+                    f"IPHYS=0, 300 s), members differ by 10% wind perturbations (200 km scale)",
+        "members": nmem, "nsea": n, "member_points": nmp, "counter": counter, "steps": K,
+        "value": nmp / t_graph, "unit": "member-point updates/s",
+        "ms_per_step_graph": round(t_graph * 1e3, 2), "ms_per_step_eager": round(t_eager * 1e3, 2),
+        "plain_member_ms_per_step": round(t_plain * 1e3, 3),
+        "cost_per_member_vs_plain": round(t_graph / nmem / t_plain, 4),
+        "kernel_ms_per_step": {k: round(v, 3) for k, v in kt.items()},
+        "launch_shapes": dict(zip(("prop_wg", "src_wg", "snl_per_cu", "chunk"), shapes)),
+        "state_finite": finite,
+    }
+    if snl:
+        tf = opc["stages"]["snl"]["flops_per_point"] * nmp / (snl * 1e-3) / 1e12
+        res["roofline_snl"] = dict(kernel="implsch_snl_kernel<1>", bound="valu", achieved=round(tf, 3),
+                                   peak=FP32_PEAK_TFLOPS, unit="TFLOP/s", frac=round(tf / FP32_PEAK_TFLOPS, 4))
+    g.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -221,6 +324,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--nest", action="store_true",
                     help="measure the nest boundary exchange on the resident state instead (one JSON line)")
+    ap.add_argument("--no-ensemble", action="store_true", help="leave out the batched ensemble measurement")
+    ap.add_argument("--ens-members", type=int, default=32)
+    ap.add_argument("--ens-steps", type=int, default=4)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one plain 1 GiB device copy before the timed steps (reference for HBM counters)")
     args = ap.parse_args()
@@ -366,6 +472,13 @@ def main():
         "roofline_all": others,
         "state_finite": finite, "mean_hs_m": round(float(hs.mean()), 4),
     }
+    if not args.no_ensemble:
+        # the single-grid state and its context are released first
+        del fa, fb, u10, udir, tauw, ustar, z0, roairn, wstar, dfim, energy, hs
+        g.close()
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        out["batched"] = bench_ensemble(args, torch)
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(args)

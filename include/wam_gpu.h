@@ -144,6 +144,55 @@ int wam_gpu_implsch_dev_noalloc(wam_ctx *ctx, float *d_fl3, const float *d_u10, 
                                 float *d_ustar, float *d_z0, const float *d_roairn, const float *d_wstar,
                                 int32_t lcflx, float *d_phiaw, float *d_fluxes, int32_t *d_mij);
 
+/* Ensembles: NMEM members on the context's grid, sharing its depth, currents
+   and propagation weights (wam_gpu_prepare_propagation), each with its own
+   spectrum, forcing and state.  No counterpart in the reference, where an
+   ensemble is NMEM separate runs of WAMODEL (wamodel.f90:332,354 per member).
+   Member arrays are concatenated: spectra [NMEM][ML][KL][NSEA] (each member a
+   plain FL3), per-point fields [NMEM][NSEA], fluxes [6][NMEM*NSEA].  Every
+   member is advanced exactly as a plain run of it (bit for bit); each launch
+   covers all members (PROPAGS with members on the grid's second dimension,
+   IMPLSCH over chunks of member-points, whose size wam_gpu_autotune picks).
+   NMEM * NSEA < 2^31.  HBM pointers, asynchronous on the context stream. */
+int wam_gpu_propags_ensemble_dev(wam_ctx *ctx, int32_t nmem, const float *d_in, float *d_out);
+int wam_gpu_implsch_ensemble_dev(wam_ctx *ctx, int32_t nmem, float *d_fl3, const float *d_u10, const float *d_udir,
+                                 float *d_tauw, float *d_ustar, float *d_z0, const float *d_roairn,
+                                 const float *d_wstar, int32_t lcflx, float *d_phiaw, float *d_fluxes,
+                                 int32_t *d_mij);
+
+/* NSTEPS model steps of an ensemble, as WAMODEL's time loop does them
+   (wamodel.f90:332,354): PROPAGS d_fa -> d_fb, IMPLSCH on d_fb, then the two
+   swap roles; the state after the last step is in d_fa if NSTEPS is even,
+   else in d_fb.  The forcing stays the arrays given.  graph != 0 runs the
+   whole loop as one HIP graph, captured at the first call and replayed while
+   the arguments, the launch shapes and the prepared fields stay the same
+   (graph = 0: the same launches issued one by one).  No LCFLX outputs. */
+int wam_gpu_ensemble_run_dev(wam_ctx *ctx, int32_t nmem, int32_t nsteps, float *d_fa, float *d_fb,
+                             const float *d_u10, const float *d_udir, float *d_tauw, float *d_ustar, float *d_z0,
+                             const float *d_roairn, const float *d_wstar, int32_t graph);
+
+/* Autotuner of the launch shapes (no counterpart in the reference): times, on
+   the given ensemble state with HIP events, the points per PROPAGS workgroup
+   (64, 128, 256), the member-points per IMPLSCH launch sequence (powers of
+   two from 2^18, and all; those whose work arrays fit half the free memory),
+   the sea points per workgroup of the per-lane IMPLSCH kernels (64, 128, 256)
+   and the workgroups per CU of the SNONLIN kernel, and keeps the fastest (a
+   coordinate search, best of reps runs each).  Every shape gives the same
+   results bit for bit.  d_fa is read; d_fb is overwritten (it is the
+   ensemble's second buffer); TAUW and Z0 are not changed.  The shapes apply to
+   every later call of the context, plain and ensemble. */
+int wam_gpu_autotune(wam_ctx *ctx, int32_t nmem, const float *d_fa, float *d_fb, const float *d_u10,
+                     const float *d_udir, const float *d_tauw, const float *d_z0, const float *d_roairn,
+                     const float *d_wstar, int32_t reps);
+/* The launch shapes: shapes[4] = PROPAGS workgroup width, per-lane IMPLSCH
+   workgroup width, SNONLIN workgroups per CU (0 = as many as fit),
+   member-points per IMPLSCH launch sequence (0 = the default, 2^20). */
+int wam_gpu_get_tuning(wam_ctx *ctx, int64_t *shapes);
+int wam_gpu_set_tuning(wam_ctx *ctx, const int64_t *shapes);
+/* The autotuner's measurements, one "parameter value milliseconds" line per
+   candidate, into buf (n bytes, NUL-terminated); returns the full length. */
+int64_t wam_gpu_get_tuning_log(wam_ctx *ctx, char *buf, int64_t n);
+
 /* Stream used by the context (a hipStream_t; NULL = the context's own). */
 int wam_gpu_set_stream(wam_ctx *ctx, void *stream);
 

@@ -39,6 +39,13 @@ SIGNATURES = {
     "wam_gpu_propags_dev": (_I, [_P, _P, _P]),
     "wam_gpu_implsch_dev": (_I, [_P] + [_P] * 8 + [_I, _P, _P, _P]),
     "wam_gpu_implsch_dev_noalloc": (_I, [_P] + [_P] * 8 + [_I, _P, _P, _P]),
+    "wam_gpu_propags_ensemble_dev": (_I, [_P, _I, _P, _P]),
+    "wam_gpu_implsch_ensemble_dev": (_I, [_P, _I] + [_P] * 8 + [_I, _P, _P, _P]),
+    "wam_gpu_ensemble_run_dev": (_I, [_P, _I, _I] + [_P] * 9 + [_I]),
+    "wam_gpu_autotune": (_I, [_P, _I] + [_P] * 8 + [_I]),
+    "wam_gpu_get_tuning": (_I, [_P, _P]),
+    "wam_gpu_set_tuning": (_I, [_P, _P]),
+    "wam_gpu_get_tuning_log": (ctypes.c_int64, [_P, _P, ctypes.c_int64]),
     "wam_gpu_set_propagation_weights": (_I, [_P, _I]),
     "wam_gpu_set_stream": (_I, [_P, _P]),
     "wam_gpu_output_parameters_dev": (_I, [_P] * 7 + [_I, _P]),

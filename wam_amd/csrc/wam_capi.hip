@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <mutex>
 #include <cstdio>
 #include <cstring>
@@ -36,7 +37,7 @@ __global__ void wavnu2_kernel(DevTables t, float *wn, float *cgg);
 __global__ void sbottom_factor_kernel(DevTables t, float *sbot);
 hipError_t launch_implsch(const DevTables &t, const ImplArgs &a, int ncu, hipStream_t st);
 hipError_t launch_implsch_split(const DevTables &t, const ImplArgs &a, const ImplScratch &x, int ncu, hipStream_t st,
-                                const hipEvent_t *ev);
+                                const hipEvent_t *ev, const LaunchTune &tu);
 bool implsch_split_supported(const DevTables &t);
 hipError_t launch_output_params(const DevTables &t, const float *fl3, const float *u10, const float *udir,
                                 const float *ustar, const float *tauw, const float *z0, int orient, float *out,
@@ -104,6 +105,19 @@ struct wam_ctx {
     float *wn = nullptr, *cgg = nullptr;
     float *fscratch = nullptr;            // COUNTER > 1 ping-pong
     ImplScratch isx{};                    // IMPLSCH work arrays (IPHYS = 0), allocated at the first call
+    ImplScratch esx{};                    // IMPLSCH work arrays of ensembles (a chunk of member-points)
+    float *fscratch_e = nullptr;          // ensembles: COUNTER > 1 ping-pong
+    size_t fscratch_e_n = 0;
+    LaunchTune tune;                      // launch shapes (wam_gpu_autotune, wam_gpu_set_tuning)
+    struct TuneEntry { std::string what; long value; float ms; };
+    std::vector<TuneEntry> tune_log;      // the last autotuner's measurements
+    // the captured ensemble loop (wam_gpu_ensemble_run_dev): its stream, the
+    // two ordering events, the graph and the arguments it was built for
+    hipStream_t gstream = nullptr;
+    hipEvent_t gev[2] = {nullptr, nullptr};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<uintptr_t> gkey;
     // per-kernel timing (wam_gpu_set_timing): an event pair per launch
     bool timing = false;
     struct Timed { std::string name; hipEvent_t a, b; };
@@ -648,44 +662,43 @@ int wam_gpu_connect_restart(wam_ctx *ctx, const char *path, char *dates, float *
     return WAM_OK;
 }
 
-int wam_gpu_propags_dev(wam_ctx *ctx, const float *d_in, float *d_out)
+// PROPAGS for nmem members (members on blockIdx.y, the weights and per-point
+// fields shared), COUNTER sub-steps alternating between d_out and scratch so
+// that the last one lands in d_out.
+static int propags_members(wam_ctx *ctx, int nmem, const float *d_in, float *d_out, float *scratch, bool timed)
 {
-    if (!ctx || !d_in || !d_out || d_in == d_out) return WAM_E_ARG;
-    if (!ctx->prepared) { ctx->err = "PREPARE_PROPAGATION not done"; return WAM_E_STATE; }
     const Tables &t = ctx->tab;
-    const size_t nb = (size_t)t.KL * t.ML * t.NSEA;
-    if (ctx->counter > 1 && !ctx->fscratch) HIPCHK(hipMalloc(&ctx->fscratch, nb * sizeof(float)));
-    const int nch = (t.NSEA + 255) / 256;
-    const int grid = ((nch + 7) / 8) * 8;
+    const int wg = ctx->tune.prop_wg;
+    const int nch = (t.NSEA + wg - 1) / wg;
+    const dim3 grid(((nch + 7) / 8) * 8, nmem), block(wg);
     // KPM is cyclic (K+1, K-1 modulo KL), so for the common direction counts
     // the register-row form of PROP_CURR applies
     auto kern = propags_nocurr_kernel;
     if (t.cfg.refr_c)
         kern = t.KL == 36 ? propags_curr_rows_kernel<36> : t.KL == 24 ? propags_curr_rows_kernel<24> : propags_curr_kernel;
-    // sub-steps alternate between out and scratch so that the last one lands in d_out
     const float *src = d_in;
     for (int i = 0; i < ctx->counter; i++) {
         const int left = ctx->counter - 1 - i;
-        float *dst = (left % 2 == 0) ? d_out : ctx->fscratch;
-        hipEvent_t ev[2];
-        const int rc = timing_events(ctx, ev, 2);
-        if (rc) return rc;
+        float *dst = (left % 2 == 0) ? d_out : scratch;
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        if (timed) {
+            const int rc = timing_events(ctx, ev, 2);
+            if (rc) return rc;
+        }
         timing_record(ctx, ev[0]);
         if (ctx->coef)
-            hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, ctx->stream, ctx->dt, ctx->coef, src, dst);
+            hipLaunchKernelGGL(kern, grid, block, 0, ctx->stream, ctx->dt, ctx->coef, src, dst);
         else if (t.cfg.refr_c && t.cfg.shallow && 3ull * 4 * t.KL * t.NSEA < (1ull << 32)) {
             // (32-bit buffer offsets over three frequency planes)
             const int c = ctx->counter;
             auto k = c == 1 ? propags_fused_sc_kernel<0>
                             : (c & (c - 1)) == 0 ? propags_fused_sc_kernel<1> : propags_fused_sc_kernel<2>;
-            hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, ctx->stream, ctx->dt, c, src, dst);
+            hipLaunchKernelGGL(k, grid, block, 0, ctx->stream, ctx->dt, c, src, dst);
         }
         else if (t.cfg.refr_c)
-            hipLaunchKernelGGL(propags_fused_kernel<true>, dim3(grid), dim3(256), 0, ctx->stream, ctx->dt, ctx->counter,
-                               src, dst);
+            hipLaunchKernelGGL(propags_fused_kernel<true>, grid, block, 0, ctx->stream, ctx->dt, ctx->counter, src, dst);
         else
-            hipLaunchKernelGGL(propags_fused_kernel<false>, dim3(grid), dim3(256), 0, ctx->stream, ctx->dt,
-                               ctx->counter, src, dst);
+            hipLaunchKernelGGL(propags_fused_kernel<false>, grid, block, 0, ctx->stream, ctx->dt, ctx->counter, src, dst);
         HIPCHK(hipGetLastError());
         timing_record(ctx, ev[1]);
         timing_keep(ctx, "propags", ev[0], ev[1]);
@@ -694,27 +707,120 @@ int wam_gpu_propags_dev(wam_ctx *ctx, const float *d_in, float *d_out)
     return WAM_OK;
 }
 
-// Work arrays of the three-kernel IMPLSCH: tiled F, SL, FL (and SL - SPOS
-// with LCFLX), STRESSO's direction sums, per-point scalars, LLWS bits.
-static int ensure_impl_scratch(wam_ctx *ctx, int lcflx)
+int wam_gpu_propags_dev(wam_ctx *ctx, const float *d_in, float *d_out)
+{
+    if (!ctx || !d_in || !d_out || d_in == d_out) return WAM_E_ARG;
+    if (!ctx->prepared) { ctx->err = "PREPARE_PROPAGATION not done"; return WAM_E_STATE; }
+    const size_t nb = (size_t)ctx->tab.KL * ctx->tab.ML * ctx->tab.NSEA;
+    if (ctx->counter > 1 && !ctx->fscratch) HIPCHK(hipMalloc(&ctx->fscratch, nb * sizeof(float)));
+    return propags_members(ctx, 1, d_in, d_out, ctx->fscratch, true);
+}
+
+// Work arrays of the three-kernel IMPLSCH for cap points (member-points) per
+// launch sequence: tiled F, SL, FL (and SL - SPOS with LCFLX), STRESSO's
+// direction sums, per-point scalars, SPRD, LLWS bits, ticket counters.
+// Reallocated when the capacity changes.
+static void free_impl_scratch(ImplScratch &x)
+{
+    for (void *p : {(void *)x.ft, (void *)x.slt, (void *)x.flt, (void *)x.smt, (void *)x.axy, (void *)x.pp,
+                    (void *)x.sprd, (void *)x.llws, (void *)x.tick})
+        if (p) (void)hipFree(p);
+    x = ImplScratch{};
+}
+
+static int ensure_impl_scratch(wam_ctx *ctx, ImplScratch &x, size_t cap, int lcflx)
 {
     const Tables &t = ctx->tab;
-    ImplScratch &x = ctx->isx;
-    const size_t ns = t.NSEA, nb = (size_t)t.KL * t.ML, nw = (nb + 31) / 32;
-    const size_t tiled = (ns + 63) / 64 * 64 * nb;      // whole 64-point tiles
+    const size_t nb = (size_t)t.KL * t.ML, nw = (nb + 31) / 32;
+    if (x.ft && (size_t)x.NX != cap) free_impl_scratch(x);
+    const size_t tiled = (cap + 63) / 64 * 64 * nb;      // whole 64-point tiles
     if (!x.ft) {
         x.NB4 = (int)(nb / 4);
-        x.P0 = 0; x.NP = (int)ns; x.NX = (int)ns;      // the whole grid in one chunk
+        x.P0 = 0; x.NP = (int)cap; x.NX = (int)cap;
         HIPCHK(hipMalloc(&x.ft, tiled * sizeof(float)));
         HIPCHK(hipMalloc(&x.slt, tiled * sizeof(float)));
         HIPCHK(hipMalloc(&x.flt, tiled * sizeof(float)));
-        HIPCHK(hipMalloc(&x.axy, 3 * (size_t)t.ML * ns * sizeof(float)));
-        HIPCHK(hipMalloc(&x.pp, 6 * ns * sizeof(float)));
-        HIPCHK(hipMalloc(&x.sprd, (size_t)t.KL * ns * sizeof(float)));
-        HIPCHK(hipMalloc(&x.llws, nw * ns * sizeof(unsigned)));
+        HIPCHK(hipMalloc(&x.axy, 3 * (size_t)t.ML * cap * sizeof(float)));
+        HIPCHK(hipMalloc(&x.pp, 6 * cap * sizeof(float)));
+        HIPCHK(hipMalloc(&x.sprd, (size_t)t.KL * cap * sizeof(float)));
+        HIPCHK(hipMalloc(&x.llws, nw * cap * sizeof(unsigned)));
         HIPCHK(hipMalloc(&x.tick, 8 * 32 * sizeof(unsigned)));
     }
     if (lcflx && !x.smt) HIPCHK(hipMalloc(&x.smt, tiled * sizeof(float)));
+    return WAM_OK;
+}
+
+// Bytes of those work arrays per point.
+static size_t impl_scratch_per_point(const Tables &t, int lcflx)
+{
+    const size_t nb = (size_t)t.KL * t.ML, nw = (nb + 31) / 32;
+    return ((3 + (lcflx ? 1 : 0)) * nb + 3 * (size_t)t.ML + 6 + t.KL + nw) * 4;
+}
+
+// The member-points per IMPLSCH launch sequence of an ensemble: the tuned
+// value, else all of them up to 2^20.
+static size_t ensemble_chunk(const wam_ctx *ctx, size_t nmp)
+{
+    size_t c = ctx->tune.chunk > 0 ? (size_t)ctx->tune.chunk : ((size_t)1 << 20);
+    return std::min(c, nmp);
+}
+
+static ImplArgs impl_args(float *d_fl3, const float *d_u10, const float *d_udir, float *d_tauw, float *d_ustar,
+                          float *d_z0, const float *d_roairn, const float *d_wstar, int32_t lcflx, float *d_phiaw,
+                          float *d_fluxes, int32_t *d_mij, int nmp)
+{
+    ImplArgs a{};
+    a.fl3 = d_fl3; a.u10 = d_u10; a.udir = d_udir; a.tauw = d_tauw; a.ustar = d_ustar; a.z0 = d_z0;
+    a.roairn = d_roairn; a.wstar = d_wstar; a.lcflx = lcflx ? 1 : 0; a.phiaw = d_phiaw; a.fluxes = d_fluxes;
+    a.mij = d_mij; a.NMP = nmp;
+    return a;
+}
+
+// IMPLSCH for nmem members (a.NMP = nmem * NSEA member-points): the three
+// kernels over chunks of member-points whose work arrays x holds (allocated
+// beforehand), or, for the configurations of the single kernel, one launch
+// per member.
+static int implsch_members(wam_ctx *ctx, int nmem, const ImplArgs &a, ImplScratch &x, bool timed)
+{
+    const Tables &t = ctx->tab;
+    if (implsch_split_supported(ctx->dt)) {
+        const int nmp = a.NMP, cap = x.NX;
+        for (int p0 = 0; p0 < nmp; p0 += cap) {
+            x.P0 = p0;
+            x.NP = std::min(cap, nmp - p0);
+            hipEvent_t ev[4] = {};
+            if (timed) {
+                const int rc = timing_events(ctx, ev, 4);
+                if (rc) return rc;
+            }
+            HIPCHK(launch_implsch_split(ctx->dt, a, x, ctx->ncu, ctx->stream, ev, ctx->tune));
+            timing_keep(ctx, "implsch_pre", ev[0], ev[1]);
+            timing_keep(ctx, "implsch_snl", ev[1], ev[2]);
+            timing_keep(ctx, "implsch_post", ev[2], ev[3]);
+        }
+        x.P0 = 0;
+        x.NP = x.NX;
+        return WAM_OK;
+    }
+    const size_t nb = (size_t)t.KL * t.ML * t.NSEA, ns = t.NSEA;
+    for (int m = 0; m < nmem; m++) {
+        ImplArgs am = a;                 // member m: its own arrays; the flux rows stay NMP long
+        am.fl3 = a.fl3 + m * nb;
+        am.u10 = a.u10 + m * ns; am.udir = a.udir + m * ns; am.roairn = a.roairn + m * ns;
+        am.wstar = a.wstar + m * ns; am.tauw = a.tauw + m * ns; am.ustar = a.ustar + m * ns; am.z0 = a.z0 + m * ns;
+        if (a.phiaw) am.phiaw = a.phiaw + m * ns;
+        if (a.fluxes) am.fluxes = a.fluxes + m * ns;
+        if (a.mij) am.mij = a.mij + m * ns;
+        hipEvent_t ev[2] = {};
+        if (timed) {
+            const int rc = timing_events(ctx, ev, 2);
+            if (rc) return rc;
+        }
+        timing_record(ctx, ev[0]);
+        HIPCHK(launch_implsch(ctx->dt, am, ctx->ncu, ctx->stream));
+        timing_record(ctx, ev[1]);
+        timing_keep(ctx, "implsch", ev[0], ev[1]);
+    }
     return WAM_OK;
 }
 
@@ -724,29 +830,318 @@ int wam_gpu_implsch_dev(wam_ctx *ctx, float *d_fl3, const float *d_u10, const fl
 {
     if (!ctx || !d_fl3 || !d_u10 || !d_udir || !d_tauw || !d_ustar || !d_z0 || !d_roairn || !d_wstar) return WAM_E_ARG;
     if (!ctx->prepared) { ctx->err = "PREPARE_PROPAGATION not done"; return WAM_E_STATE; }
-    ImplArgs a{};
-    a.fl3 = d_fl3; a.u10 = d_u10; a.udir = d_udir; a.tauw = d_tauw; a.ustar = d_ustar; a.z0 = d_z0;
-    a.roairn = d_roairn; a.wstar = d_wstar; a.lcflx = lcflx ? 1 : 0; a.phiaw = d_phiaw; a.fluxes = d_fluxes;
-    a.mij = d_mij;
-    hipEvent_t ev[4];
-    int rc = timing_events(ctx, ev, 4);
-    if (rc) return rc;
+    const ImplArgs a = impl_args(d_fl3, d_u10, d_udir, d_tauw, d_ustar, d_z0, d_roairn, d_wstar, lcflx, d_phiaw,
+                                 d_fluxes, d_mij, ctx->tab.NSEA);
     if (implsch_split_supported(ctx->dt)) {
-        rc = ensure_impl_scratch(ctx, a.lcflx);
+        const int rc = ensure_impl_scratch(ctx, ctx->isx, ctx->tab.NSEA, a.lcflx);
         if (rc) return rc;
-        HIPCHK(launch_implsch_split(ctx->dt, a, ctx->isx, ctx->ncu, ctx->stream, ev));
-        timing_keep(ctx, "implsch_pre", ev[0], ev[1]);
-        timing_keep(ctx, "implsch_snl", ev[1], ev[2]);
-        timing_keep(ctx, "implsch_post", ev[2], ev[3]);
-    } else {
-        timing_record(ctx, ev[0]);
-        HIPCHK(launch_implsch(ctx->dt, a, ctx->ncu, ctx->stream));
-        timing_record(ctx, ev[1]);
-        timing_keep(ctx, "implsch", ev[0], ev[1]);
-        for (int i = 2; i < 4; i++)
-            if (ev[i]) (void)hipEventDestroy(ev[i]);
+    }
+    return implsch_members(ctx, 1, a, ctx->isx, true);
+}
+
+// ---- ensembles ------------------------------------------------------------
+
+static int ensemble_args_ok(wam_ctx *ctx, int32_t nmem)
+{
+    if (!ctx) return WAM_E_ARG;
+    if (!ctx->prepared) { ctx->err = "PREPARE_PROPAGATION not done"; return WAM_E_STATE; }
+    if (nmem < 1 || (int64_t)nmem * ctx->tab.NSEA > 0x7fffffff || nmem > 65535) {
+        ctx->err = "nmem must be >= 1 with nmem * NSEA < 2^31";
+        return WAM_E_ARG;
     }
     return WAM_OK;
+}
+
+// the COUNTER > 1 ping-pong spectra of nmem members
+static int ensure_ensemble_fscratch(wam_ctx *ctx, int nmem)
+{
+    if (ctx->counter <= 1) return WAM_OK;
+    const size_t n = (size_t)nmem * ctx->tab.KL * ctx->tab.ML * ctx->tab.NSEA;
+    if (ctx->fscratch_e_n >= n) return WAM_OK;
+    if (ctx->fscratch_e) (void)hipFree(ctx->fscratch_e);
+    ctx->fscratch_e = nullptr; ctx->fscratch_e_n = 0;
+    HIPCHK(hipMalloc(&ctx->fscratch_e, n * sizeof(float)));
+    ctx->fscratch_e_n = n;
+    return WAM_OK;
+}
+
+int wam_gpu_propags_ensemble_dev(wam_ctx *ctx, int32_t nmem, const float *d_in, float *d_out)
+{
+    int rc = ensemble_args_ok(ctx, nmem);
+    if (rc) return rc;
+    if (!d_in || !d_out || d_in == d_out) return WAM_E_ARG;
+    rc = ensure_ensemble_fscratch(ctx, nmem);
+    if (rc) return rc;
+    return propags_members(ctx, nmem, d_in, d_out, ctx->fscratch_e, true);
+}
+
+int wam_gpu_implsch_ensemble_dev(wam_ctx *ctx, int32_t nmem, float *d_fl3, const float *d_u10, const float *d_udir,
+                                 float *d_tauw, float *d_ustar, float *d_z0, const float *d_roairn,
+                                 const float *d_wstar, int32_t lcflx, float *d_phiaw, float *d_fluxes,
+                                 int32_t *d_mij)
+{
+    int rc = ensemble_args_ok(ctx, nmem);
+    if (rc) return rc;
+    if (!d_fl3 || !d_u10 || !d_udir || !d_tauw || !d_ustar || !d_z0 || !d_roairn || !d_wstar) return WAM_E_ARG;
+    const int nmp = nmem * ctx->tab.NSEA;
+    const ImplArgs a = impl_args(d_fl3, d_u10, d_udir, d_tauw, d_ustar, d_z0, d_roairn, d_wstar, lcflx, d_phiaw,
+                                 d_fluxes, d_mij, nmp);
+    if (implsch_split_supported(ctx->dt)) {
+        rc = ensure_impl_scratch(ctx, ctx->esx, ensemble_chunk(ctx, nmp), a.lcflx);
+        if (rc) return rc;
+    }
+    return implsch_members(ctx, nmem, a, ctx->esx, true);
+}
+
+// nsteps model steps (PROPAGS d_fa -> d_fb, IMPLSCH on d_fb, then the roles
+// swapped) on the context stream.
+static int ensemble_steps(wam_ctx *ctx, int nmem, int nsteps, float *d_fa, float *d_fb, const ImplArgs &a0,
+                          bool timed)
+{
+    float *cur = d_fa, *nxt = d_fb;
+    for (int i = 0; i < nsteps; i++) {
+        int rc = propags_members(ctx, nmem, cur, nxt, ctx->fscratch_e, timed);
+        if (rc) return rc;
+        ImplArgs a = a0;
+        a.fl3 = nxt;
+        rc = implsch_members(ctx, nmem, a, ctx->esx, timed);
+        if (rc) return rc;
+        std::swap(cur, nxt);
+    }
+    return WAM_OK;
+}
+
+int wam_gpu_ensemble_run_dev(wam_ctx *ctx, int32_t nmem, int32_t nsteps, float *d_fa, float *d_fb,
+                             const float *d_u10, const float *d_udir, float *d_tauw, float *d_ustar, float *d_z0,
+                             const float *d_roairn, const float *d_wstar, int32_t graph)
+{
+    int rc = ensemble_args_ok(ctx, nmem);
+    if (rc) return rc;
+    if (nsteps < 0 || !d_fa || !d_fb || d_fa == d_fb || !d_u10 || !d_udir || !d_tauw || !d_ustar || !d_z0 ||
+        !d_roairn || !d_wstar)
+        return WAM_E_ARG;
+    if (nsteps == 0) return WAM_OK;
+    const int nmp = nmem * ctx->tab.NSEA;
+    const ImplArgs a = impl_args(d_fb, d_u10, d_udir, d_tauw, d_ustar, d_z0, d_roairn, d_wstar, 0, nullptr, nullptr,
+                                 nullptr, nmp);
+    // every allocation before a capture
+    rc = ensure_ensemble_fscratch(ctx, nmem);
+    if (rc) return rc;
+    if (implsch_split_supported(ctx->dt)) {
+        rc = ensure_impl_scratch(ctx, ctx->esx, ensemble_chunk(ctx, nmp), 0);
+        if (rc) return rc;
+    }
+    if (!graph) return ensemble_steps(ctx, nmem, nsteps, d_fa, d_fb, a, true);
+
+    // The captured loop: built for these arguments, launch shapes and
+    // sub-step count, replayed while they stay the same.
+    const std::vector<uintptr_t> key = {
+        (uintptr_t)nmem, (uintptr_t)nsteps, (uintptr_t)d_fa, (uintptr_t)d_fb, (uintptr_t)d_u10, (uintptr_t)d_udir,
+        (uintptr_t)d_tauw, (uintptr_t)d_ustar, (uintptr_t)d_z0, (uintptr_t)d_roairn, (uintptr_t)d_wstar,
+        (uintptr_t)ctx->tune.prop_wg, (uintptr_t)ctx->tune.src_wg, (uintptr_t)ctx->tune.snl_per_cu,
+        (uintptr_t)ctx->esx.NX, (uintptr_t)ctx->counter, (uintptr_t)ctx->dt.DEPTH, (uintptr_t)ctx->coef};
+    if (!ctx->gstream) {
+        HIPCHK(hipStreamCreateWithFlags(&ctx->gstream, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&ctx->gev[0], hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&ctx->gev[1], hipEventDisableTiming));
+    }
+    if (!ctx->gexec || ctx->gkey != key) {
+        if (ctx->gexec) { (void)hipGraphExecDestroy(ctx->gexec); ctx->gexec = nullptr; }
+        if (ctx->graph) { (void)hipGraphDestroy(ctx->graph); ctx->graph = nullptr; }
+        hipStream_t keep = ctx->stream;
+        ctx->stream = ctx->gstream;
+        HIPCHK(hipStreamBeginCapture(ctx->gstream, hipStreamCaptureModeRelaxed));
+        rc = ensemble_steps(ctx, nmem, nsteps, d_fa, d_fb, a, false);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(ctx->gstream, &g);
+        ctx->stream = keep;
+        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+        HIPCHK(e);
+        ctx->graph = g;
+        HIPCHK(hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0));
+        ctx->gkey = key;
+    }
+    // ordered after the work already on the context stream, and before what follows
+    HIPCHK(hipEventRecord(ctx->gev[0], ctx->stream));
+    HIPCHK(hipStreamWaitEvent(ctx->gstream, ctx->gev[0], 0));
+    HIPCHK(hipGraphLaunch(ctx->gexec, ctx->gstream));
+    HIPCHK(hipEventRecord(ctx->gev[1], ctx->gstream));
+    HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->gev[1], 0));
+    return WAM_OK;
+}
+
+// ---- autotuner ------------------------------------------------------------
+
+// Milliseconds of fn() on the context stream, the best of reps runs.
+static int time_best(wam_ctx *ctx, int reps, const std::function<int()> &fn, float &best)
+{
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    best = 3.4e38f;
+    int rc = WAM_OK;
+    for (int r = 0; r < reps && !rc; r++) {
+        (void)hipEventRecord(e0, ctx->stream);
+        rc = fn();
+        (void)hipEventRecord(e1, ctx->stream);
+        if (rc) break;
+        if (hipEventSynchronize(e1) != hipSuccess) { rc = WAM_E_HIP; break; }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        best = std::min(best, ms);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+int wam_gpu_autotune(wam_ctx *ctx, int32_t nmem, const float *d_fa, float *d_fb, const float *d_u10,
+                     const float *d_udir, const float *d_tauw, const float *d_z0, const float *d_roairn,
+                     const float *d_wstar, int32_t reps)
+{
+    int rc = ensemble_args_ok(ctx, nmem);
+    if (rc) return rc;
+    if (!d_fa || !d_fb || d_fa == d_fb || !d_u10 || !d_udir || !d_tauw || !d_z0 || !d_roairn || !d_wstar)
+        return WAM_E_ARG;
+    reps = std::max(1, reps);
+    const Tables &t = ctx->tab;
+    const size_t ns = t.NSEA, nmp = (size_t)nmem * ns;
+    rc = ensure_ensemble_fscratch(ctx, nmem);
+    if (rc) return rc;
+    ctx->tune_log.clear();
+    // PROPAGS: points per workgroup
+    {
+        int best_wg = ctx->tune.prop_wg;
+        float best = 3.4e38f;
+        for (int wg : {64, 128, 256}) {
+            ctx->tune.prop_wg = wg;
+            float ms = 0.f;
+            rc = time_best(ctx, reps, [&] { return propags_members(ctx, nmem, d_fa, d_fb, ctx->fscratch_e, false); },
+                           ms);
+            if (rc) return rc;
+            ctx->tune_log.push_back({"prop_wg", wg, ms});
+            if (ms < best) { best = ms; best_wg = wg; }
+        }
+        ctx->tune.prop_wg = best_wg;
+    }
+    if (!implsch_split_supported(ctx->dt)) return WAM_OK;     // the single kernel has no launch shapes to tune
+    // IMPLSCH on the propagated state in d_fb, from the given TAUW and Z0
+    // every time (copies; USTAR, TAUW, Z0 of the runs are discarded)
+    float *tmp = nullptr;
+    HIPCHK(hipMalloc(&tmp, 3 * nmp * sizeof(float)));
+    float *tw = tmp, *z0 = tmp + nmp, *us = tmp + 2 * nmp;
+    auto run = [&]() -> int {
+        int r = propags_members(ctx, nmem, d_fa, d_fb, ctx->fscratch_e, false);
+        if (r) return r;
+        HIPCHK(hipMemcpyAsync(tw, d_tauw, nmp * 4, hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(z0, d_z0, nmp * 4, hipMemcpyDeviceToDevice, ctx->stream));
+        return WAM_OK;
+    };
+    const ImplArgs a = impl_args(d_fb, d_u10, d_udir, tw, us, z0, d_roairn, d_wstar, 0, nullptr, nullptr, nullptr,
+                                 (int)nmp);
+    auto time_implsch = [&](float &best) -> int {
+        best = 3.4e38f;
+        for (int r = 0; r < reps; r++) {
+            int e = run();
+            if (e) return e;
+            float ms = 0.f;
+            e = time_best(ctx, 1, [&] { return implsch_members(ctx, nmem, a, ctx->esx, false); }, ms);
+            if (e) return e;
+            best = std::min(best, ms);
+        }
+        return WAM_OK;
+    };
+    // member-points per launch sequence: powers of two from 2^18 and all of
+    // them, whose work arrays fit half of the free memory
+    size_t freeb = 0, totalb = 0;
+    if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) freeb = (size_t)64 << 30;
+    const size_t ppp = impl_scratch_per_point(t, 0);
+    std::vector<size_t> chunks;
+    for (size_t c = (size_t)1 << 18; c < nmp; c <<= 1) chunks.push_back(c);
+    chunks.push_back(nmp);
+    std::vector<size_t> fit;
+    for (size_t c : chunks)
+        if (c * ppp <= freeb / 2 + (ctx->esx.ft ? (size_t)ctx->esx.NX * ppp : 0)) fit.push_back(c);
+    if (fit.empty()) fit.push_back(std::min(nmp, (size_t)1 << 18));
+    rc = WAM_OK;
+    // coordinate search: chunk, then the per-lane kernels' width, then the SNONLIN kernel's occupancy
+    float best = 3.4e38f;
+    size_t best_c = fit.back() < ((size_t)1 << 20) ? fit.back() : ((size_t)1 << 20);
+    for (size_t c : fit) {
+        ctx->tune.chunk = (long)c;
+        rc = ensure_impl_scratch(ctx, ctx->esx, c, 0);
+        float ms = 0.f;
+        if (!rc) rc = time_implsch(ms);
+        if (rc) break;
+        ctx->tune_log.push_back({"chunk", (long)c, ms});
+        if (ms < best) { best = ms; best_c = c; }
+    }
+    if (!rc) {
+        ctx->tune.chunk = (long)best_c;
+        rc = ensure_impl_scratch(ctx, ctx->esx, best_c, 0);
+    }
+    if (!rc) {
+        int best_w = ctx->tune.src_wg;
+        best = 3.4e38f;
+        for (int w : {64, 128, 256}) {
+            ctx->tune.src_wg = w;
+            float ms = 0.f;
+            rc = time_implsch(ms);
+            if (rc) break;
+            ctx->tune_log.push_back({"src_wg", w, ms});
+            if (ms < best) { best = ms; best_w = w; }
+        }
+        ctx->tune.src_wg = best_w;
+    }
+    if (!rc) {
+        int best_o = 0;
+        best = 3.4e38f;
+        for (int o : {0, 1, 2}) {
+            ctx->tune.snl_per_cu = o;
+            float ms = 0.f;
+            rc = time_implsch(ms);
+            if (rc) break;
+            ctx->tune_log.push_back({"snl_per_cu", o, ms});
+            if (ms < best) { best = ms; best_o = o; }
+        }
+        ctx->tune.snl_per_cu = best_o;
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(tmp);
+    return rc;
+}
+
+int wam_gpu_get_tuning(wam_ctx *ctx, int64_t *out)
+{
+    if (!ctx || !out) return WAM_E_ARG;
+    out[0] = ctx->tune.prop_wg; out[1] = ctx->tune.src_wg; out[2] = ctx->tune.snl_per_cu; out[3] = ctx->tune.chunk;
+    return WAM_OK;
+}
+
+int wam_gpu_set_tuning(wam_ctx *ctx, const int64_t *in)
+{
+    if (!ctx || !in) return WAM_E_ARG;
+    const int pw = (int)in[0], sw = (int)in[1];
+    if ((pw != 64 && pw != 128 && pw != 256) || (sw != 64 && sw != 128 && sw != 256) || in[2] < 0 || in[2] > 64 ||
+        in[3] < 0 || in[3] > 0x7fffffff) {
+        ctx->err = "launch shapes: PROPAGS and per-lane widths 64, 128 or 256; workgroups per CU 0-64; chunk >= 0";
+        return WAM_E_ARG;
+    }
+    ctx->tune.prop_wg = pw; ctx->tune.src_wg = sw; ctx->tune.snl_per_cu = (int)in[2]; ctx->tune.chunk = (long)in[3];
+    return WAM_OK;
+}
+
+int64_t wam_gpu_get_tuning_log(wam_ctx *ctx, char *buf, int64_t n)
+{
+    if (!ctx) return -1;
+    std::string s;
+    for (auto &e : ctx->tune_log) s += e.what + " " + std::to_string(e.value) + " " + std::to_string(e.ms) + "\n";
+    if (buf && n > 0) {
+        const size_t k = std::min((size_t)n - 1, s.size());
+        std::memcpy(buf, s.data(), k);
+        buf[k] = 0;
+    }
+    return (int64_t)s.size();
 }
 
 // The no-allocation form of wam_gpu_implsch_dev: the three-kernel IMPLSCH on
@@ -784,10 +1179,8 @@ int wam_gpu_implsch_dev_noalloc(wam_ctx *ctx, float *d_fl3, const float *d_u10, 
 {
     if (!ctx || !d_fl3 || !d_u10 || !d_udir || !d_tauw || !d_ustar || !d_z0 || !d_roairn || !d_wstar) return WAM_E_ARG;
     if (!ctx->prepared) { ctx->err = "PREPARE_PROPAGATION not done"; return WAM_E_STATE; }
-    ImplArgs a{};
-    a.fl3 = d_fl3; a.u10 = d_u10; a.udir = d_udir; a.tauw = d_tauw; a.ustar = d_ustar; a.z0 = d_z0;
-    a.roairn = d_roairn; a.wstar = d_wstar; a.lcflx = lcflx ? 1 : 0; a.phiaw = d_phiaw; a.fluxes = d_fluxes;
-    a.mij = d_mij;
+    const ImplArgs a = impl_args(d_fl3, d_u10, d_udir, d_tauw, d_ustar, d_z0, d_roairn, d_wstar, lcflx, d_phiaw,
+                                 d_fluxes, d_mij, ctx->tab.NSEA);
     if (!implsch_split_supported(ctx->dt)) {       // the single kernel uses no work arrays
         HIPCHK(launch_implsch(ctx->dt, a, ctx->ncu, ctx->stream));
         return WAM_OK;
@@ -804,7 +1197,7 @@ int wam_gpu_implsch_dev_noalloc(wam_ctx *ctx, float *d_fl3, const float *d_u10, 
     for (int p0 = 0; p0 < ns; p0 += c) {
         x.P0 = p0;
         x.NP = std::min(c, ns - p0);
-        HIPCHK(launch_implsch_split(ctx->dt, a, x, ctx->ncu, ctx->stream, nullptr));
+        HIPCHK(launch_implsch_split(ctx->dt, a, x, ctx->ncu, ctx->stream, nullptr, ctx->tune));
     }
     return WAM_OK;
 }
@@ -1350,10 +1743,14 @@ void wam_gpu_finalize(wam_ctx *ctx)
     if (ctx->cell_pt) (void)hipFree(ctx->cell_pt);
     if (ctx->grids) (void)hipFree(ctx->grids);
     timing_clear(ctx);
-    for (void *p : {(void *)ctx->isx.ft, (void *)ctx->isx.slt, (void *)ctx->isx.flt, (void *)ctx->isx.smt,
-                    (void *)ctx->isx.axy, (void *)ctx->isx.pp, (void *)ctx->isx.sprd, (void *)ctx->isx.llws,
-                    (void *)ctx->isx.tick})
-        if (p) (void)hipFree(p);
+    free_impl_scratch(ctx->isx);
+    free_impl_scratch(ctx->esx);
+    if (ctx->fscratch_e) (void)hipFree(ctx->fscratch_e);
+    if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    for (hipEvent_t e : ctx->gev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->gstream) (void)hipStreamDestroy(ctx->gstream);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
 }

@@ -77,12 +77,17 @@ struct DevTables {
     float DELPHI;
 };
 
+// IMPLSCH's arguments for NMP member-points: an ensemble of NMP / NSEA
+// members on one grid, member-point mp = member * NSEA + sea point.  fl3 is
+// [member][ML][KL][NSEA] (each member a plain FL3), the per-point arrays
+// [member][NSEA], fluxes [6][NMP]; a single run is one member.
 struct ImplArgs {
     float *fl3;
     const float *u10, *udir, *roairn, *wstar;
     float *tauw, *ustar, *z0, *phiaw, *fluxes;
     int *mij;
     int lcflx;
+    int NMP;
 };
 
 // Device work arrays of the three-kernel IMPLSCH (IPHYS = 0).  Spectra are
@@ -113,6 +118,15 @@ struct ImplScratch {
     float *sprd;                   // [KL][NX]: SPRD(K) = MAX(0, COS(TH(K) - UDIR))**2 of the call
     unsigned *llws;                // [(KL*ML+31)/32][NX]: LLWS bits of the second SINPUT
     unsigned *tick;                // [8][32]: point counters of implsch_snl_kernel, one per XCD (own 128-byte line)
+};
+
+// Launch shapes, chosen by the autotuner (wam_gpu_autotune); every choice
+// gives the same results, bit for bit.
+struct LaunchTune {
+    int prop_wg = 256;        // sea points per PROPAGS workgroup: 64, 128 or 256
+    int src_wg = 64;          // sea points per workgroup of implsch_pre/post_kernel: 64, 128 or 256
+    int snl_per_cu = 0;       // implsch_snl_kernel workgroups per CU (0: as many as fit)
+    long chunk = 0;           // member-points per IMPLSCH launch sequence of an ensemble (0: the default)
 };
 
 // offset of the quad of bins 4q..4q+3 of sea point pt in the tiled arrays

@@ -1148,7 +1148,7 @@ __global__ void __launch_bounds__(NT) implsch_kernel(DevTables t, ImplArgs a)
             __syncthreads();
             if (tid == 0 && a.fluxes)
                 wnfluxes<SHALLOW>(t, s, P.phiaw, P.ust, wam_sinf(P.udir), wam_cosf(P.udir), P.mij, tfk, a.fluxes + pt,
-                                  NS);
+                                  a.NMP);
         }
         // 4. limiter and implicit update
         if (tid < ML) {
@@ -1219,7 +1219,7 @@ __global__ void __launch_bounds__(NT) implsch_kernel(DevTables t, ImplArgs a)
 //   implsch_post_kernel: mean parameters, FRCUTINDEX, IMPHFTAIL, FL3 out
 // ---------------------------------------------------------------------------
 
-constexpr int PW = 64;    // sea points per workgroup of the per-lane kernels
+constexpr int PW = 256;   // most sea points per workgroup of the per-lane kernels (the launch's width, 64-256)
 
 // STRESSO's closing part (:3164-3356) for one point: the direction sums phi,
 // xs, ys over M < MIJ, the tail moments h1, h2 of F(., MIJ) and TAU_PHI_HF in
@@ -1262,21 +1262,24 @@ __device__ __forceinline__ float stresso_cm(const DevTables &t, int m, float tfk
 
 template <int SHALLOW>
 __global__ void __launch_bounds__(PW) __attribute__((amdgpu_waves_per_eu(4))) implsch_pre_kernel(DevTables t, ImplArgs a, ImplScratch x)
+// (workgroups of 64, 128 or 256 lanes, one sea point each)
 {
-    extern __shared__ float cu_lds[];            // [KL][PW]: COS(TH(K) - UDIR) of each lane's point
+    extern __shared__ float cu_lds[];            // [KL][W]: COS(TH(K) - UDIR) of each lane's point
     const int KL = t.KL, ML = t.ML, NB = KL * ML, NS = t.NSEA, NB4 = x.NB4, XS = x.NX, lane = threadIdx.x;
     // lp: the point's row in the work arrays (its place in the chunk), pt: the sea point
-    const int lp0 = blockIdx.x * PW + lane;
+    const int W = blockDim.x;
+    const int lp0 = blockIdx.x * W + lane;
     const bool on = lp0 < x.NP;
-    const int lp = on ? lp0 : x.NP - 1, pt = x.P0 + lp;
+    const int lp = on ? lp0 : x.NP - 1, mp = x.P0 + lp;      // member-point (mp = member * NSEA + sea point)
+    const int mem = mp / NS, pt = mp - mem * NS;            // ensemble member, sea point
     float *cu = cu_lds + lane;
-    const float *F = a.fl3 + pt;                  // F(b) = F[b * NS]
-    const float u10 = a.u10[pt], udir = a.udir[pt], roairn = a.roairn[pt], wstar = a.wstar[pt];
+    const float *F = a.fl3 + (size_t)mem * NB * NS + pt;   // F(b) = F[b * NS]
+    const float u10 = a.u10[mp], udir = a.udir[mp], roairn = a.roairn[mp], wstar = a.wstar[mp];
     const int indep = SHALLOW ? t.INDEP[pt] : 1;
-    float tauw = a.tauw[pt], ust = 0.f, z0 = a.z0[pt];
+    float tauw = a.tauw[mp], ust = 0.f, z0 = a.z0[mp];
     for (int k = 0; k < KL; k++) {
         const float c = wam_cosf(t.TH[k] - udir);
-        cu[k * PW] = c;
+        cu[k * W] = c;
         // the limiter's floor FLMINFR(JU, M) * SPRD(K) (:494-498) is applied by implsch_snl_kernel
         if (on) x.sprd[(size_t)k * XS + lp] = wam_ipowf(wam_maxf(0.f, c), 2);
     }
@@ -1309,7 +1312,7 @@ __global__ void __launch_bounds__(PW) __attribute__((amdgpu_waves_per_eu(4))) im
                     T = k == 0 ? f : T + f;
                     float sp, fl;
                     bool ll;
-                    sin_bin(t, r, cu[k * PW], const3, sp, fl, ll);
+                    sin_bin(t, r, cu[k * W], const3, sp, fl, ll);
                     if (ll) {
                         Q = qfirst ? f : Q + f;
                         qfirst = false;
@@ -1365,7 +1368,7 @@ __global__ void __launch_bounds__(PW) __attribute__((amdgpu_waves_per_eu(4))) im
     //    STRESSO calls: IMPHFTAIL changes the frequencies above MIJ only.
     float h1 = 0.f, h2 = 0.f;
     for (int k = 0; k < KL; k++) {
-        const float cosw = wam_maxf(cu[k * PW], 0.f);
+        const float cosw = wam_maxf(cu[k * W], 0.f);
         const float f = F[(size_t)(k + KL * (mij - 1)) * NS];
         if (k == 0) { h2 = f * wam_ipowf(cosw, 2); h1 = h2 * cosw; }
         else { h1 = h1 + f * wam_ipowf(cosw, 3); h2 = h2 + f * wam_ipowf(cosw, 2); }
@@ -1426,7 +1429,7 @@ __global__ void __launch_bounds__(PW) __attribute__((amdgpu_waves_per_eu(4))) im
                     const float f = tail ? ttail * f4[u] : f4[u];
                     float sp, fl;
                     bool ll;
-                    sin_bin(t, r, cu[k * PW], const3, sp, fl, ll);
+                    sin_bin(t, r, cu[k * W], const3, sp, fl, ll);
                     const float sl = fl * f, spos = sp * f;
                     f4[u] = f;
                     sl4[u] = sl + t2d * f;
@@ -1462,10 +1465,10 @@ __global__ void __launch_bounds__(PW) __attribute__((amdgpu_waves_per_eu(4))) im
     float phiaw;
     tauw = stresso_close(t, mij, ust, z0, udir, phi, xs, ys, h1, h2, phiaw);
     if (on) {
-        a.tauw[pt] = tauw;
-        a.ustar[pt] = ust;
-        a.z0[pt] = z0;
-        if (a.phiaw) a.phiaw[pt] = phiaw;
+        a.tauw[mp] = tauw;
+        a.ustar[mp] = ust;
+        a.z0[mp] = z0;
+        if (a.phiaw) a.phiaw[mp] = phiaw;
         float enhfr = 1.f;
         if (SHALLOW) {
             const float enh = wam_maxf((0.75f * t.DEPTH[pt]) * akmean, .5f);
@@ -1550,7 +1553,7 @@ implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x)
     if (tid == 0) Pnext = p0 + (int)atomicAdd(tick, 1u);
     __syncthreads();
     for (int lp = Pnext; lp < p1; lp = Pnext) {
-        const int pt = x.P0 + lp;                  // the sea point
+        const int mp = x.P0 + lp, mem = mp / NS, pt = mp - mem * NS;   // member-point, member, sea point
         const size_t q0 = quad_offset(lp, 0, NB4);
         int tnext = 0;
         if (tid == 0) tnext = (int)atomicAdd(tick, 1u);
@@ -1576,7 +1579,7 @@ implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x)
             }
         }
         if (tid == 0) {
-            Ps[0] = a.ustar[pt]; Ps[1] = x.pp[lp]; Ps[2] = x.pp[(size_t)XS + lp]; Ps[3] = x.pp[(size_t)3 * XS + lp];
+            Ps[0] = a.ustar[mp]; Ps[1] = x.pp[lp]; Ps[2] = x.pp[(size_t)XS + lp]; Ps[3] = x.pp[(size_t)3 * XS + lp];
             Pmij = (int)x.pp[(size_t)2 * XS + lp];
         }
         // no elementary functions here: SPRD comes from implsch_pre_kernel and
@@ -1585,7 +1588,7 @@ implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x)
         if (tid < KL) sprd[tid] = x.sprd[(size_t)tid * XS + lp];
         if (tid >= 64 && tid - 64 < ML) {
             const int m = tid - 64;
-            const int ju = min(t.JUMAX, max(wam_nint(a.u10[pt] / t.DELU), 1));
+            const int ju = min(t.JUMAX, max(wam_nint(a.u10[mp] / t.DELU), 1));
             flm[m] = t.FLMINFR[(size_t)m * t.JUMAX + ju - 1];
             if (SHALLOW) {
                 tfk[m] = t.TFAK[tb(t, m, t.INDEP[pt])];
@@ -1614,7 +1617,7 @@ implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x)
             __syncthreads();
             if (tid == 0 && a.fluxes)
                 wnfluxes<SHALLOW>(t, s, Ps[3], Ps[0], x.pp[(size_t)4 * XS + lp], x.pp[(size_t)5 * XS + lp], Pmij, tfk,
-                                  a.fluxes + pt, NS);
+                                  a.fluxes + mp, a.NMP);
         }
         // SBOTTOM (SL_BOT = 0 + factor * F, FL + factor), the limiter and the
         // implicit update, into F in place (nothing reads F after SNONLIN's
@@ -1650,12 +1653,13 @@ template <int SHALLOW>
 __global__ void __launch_bounds__(PW) implsch_post_kernel(DevTables t, ImplArgs a, ImplScratch x)
 {
     const int KL = t.KL, ML = t.ML, NS = t.NSEA, NB4 = x.NB4, XS = x.NX, lane = threadIdx.x;
-    const int lp0 = blockIdx.x * PW + lane;        // the point's row in the work arrays
+    const int lp0 = blockIdx.x * blockDim.x + lane;   // the point's row in the work arrays
     const bool on = lp0 < x.NP;
-    const int lp = on ? lp0 : x.NP - 1, pt = x.P0 + lp;
+    const int lp = on ? lp0 : x.NP - 1, mp = x.P0 + lp;      // member-point
+    const int mem = mp / NS, pt = mp - mem * NS;            // ensemble member, sea point
     const float *FQ = x.ft + quad_offset(lp, 0, NB4);   // bins 4q..4q+3 at FQ[q * 256]
     const unsigned *LLW = x.llws + lp;
-    const float ust = a.ustar[pt];
+    const float ust = a.ustar[mp];
     // direction sums (plain and LLWS-masked) and their integrals; frequency ML first
     float s_e = 0.f, s_fm = 0.f, s_w1 = 0.f, s_w2 = 0.f, Tl = 0.f, Ql = 0.f;
     for (int i = 0; i < ML; i++) {
@@ -1719,11 +1723,11 @@ __global__ void __launch_bounds__(PW) implsch_post_kernel(DevTables t, ImplArgs 
             if (on) {
 #pragma unroll
                 for (int u = 0; u < 4; u++)
-                    a.fl3[(size_t)(k0 + u + KL * m) * NS + pt] = tail ? ttail * fv[u] : fv[u];
+                    a.fl3[((size_t)mem * KL * ML + k0 + u + KL * m) * NS + pt] = tail ? ttail * fv[u] : fv[u];
             }
         }
     }
-    if (on && a.mij) a.mij[pt] = mij;
+    if (on && a.mij) a.mij[mp] = mij;
 }
 
 size_t implsch_snl_lds_bytes(const DevTables &t, int lcflx)
@@ -1747,14 +1751,15 @@ bool implsch_split_supported(const DevTables &t)
 // x.P0 .. x.P0 + x.NP - 1 (rows 0 .. NP - 1 of the work arrays); the middle
 // one persistent, one workgroup per resident slot.
 hipError_t launch_implsch_split(const DevTables &t, const ImplArgs &a, const ImplScratch &x, int ncu, hipStream_t st,
-                                const hipEvent_t *ev)
+                                const hipEvent_t *ev, const LaunchTune &tu)
 {
-    const int nwg = (x.NP + PW - 1) / PW;
+    const int W = tu.src_wg;
+    const int nwg = (x.NP + W - 1) / W;
     auto mark = [&](int i) { if (ev && ev[i]) (void)hipEventRecord(ev[i], st); };
     mark(0);
-    const size_t lds1 = (size_t)t.KL * PW * sizeof(float);
-    if (t.shallow) hipLaunchKernelGGL(implsch_pre_kernel<1>, dim3(nwg), dim3(PW), lds1, st, t, a, x);
-    else hipLaunchKernelGGL(implsch_pre_kernel<0>, dim3(nwg), dim3(PW), lds1, st, t, a, x);
+    const size_t lds1 = (size_t)t.KL * W * sizeof(float);
+    if (t.shallow) hipLaunchKernelGGL(implsch_pre_kernel<1>, dim3(nwg), dim3(W), lds1, st, t, a, x);
+    else hipLaunchKernelGGL(implsch_pre_kernel<0>, dim3(nwg), dim3(W), lds1, st, t, a, x);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     mark(1);
@@ -1766,14 +1771,15 @@ hipError_t launch_implsch_split(const DevTables &t, const ImplArgs &a, const Imp
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, SNL_NT, lds);
     if (e != hipSuccess) return e;
     if (per_cu < 1) return hipErrorInvalidConfiguration;
+    if (tu.snl_per_cu > 0 && tu.snl_per_cu < per_cu) per_cu = tu.snl_per_cu;
     const long want = (long)per_cu * ncu;
     const int nblocks = (int)(want < x.NP ? want : x.NP);
     hipLaunchKernelGGL(k, dim3(nblocks), dim3(SNL_NT), lds, st, t, a, x);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     mark(2);
-    if (t.shallow) hipLaunchKernelGGL(implsch_post_kernel<1>, dim3(nwg), dim3(PW), 0, st, t, a, x);
-    else hipLaunchKernelGGL(implsch_post_kernel<0>, dim3(nwg), dim3(PW), 0, st, t, a, x);
+    if (t.shallow) hipLaunchKernelGGL(implsch_post_kernel<1>, dim3(nwg), dim3(W), 0, st, t, a, x);
+    else hipLaunchKernelGGL(implsch_post_kernel<0>, dim3(nwg), dim3(W), 0, st, t, a, x);
     e = hipGetLastError();
     mark(3);
     return e;

@@ -269,8 +269,9 @@ __global__ void rescale_kernel(float *__restrict__ coef, size_t nb, int nplanes,
     }
 }
 
-// Chunk of 256 sea points handled by this block: each XCD (blocks are
-// dispatched round-robin over the 8 XCDs) owns one contiguous range.
+// Chunk of blockDim.x sea points handled by this block (the grid a multiple
+// of 8): each XCD (blocks are dispatched round-robin over the 8 XCDs,
+// blockIdx.x % 8) owns one contiguous eighth of the chunks.
 __device__ __forceinline__ int xcd_chunk(int nchunks)
 {
     const int per = (nchunks + 7) / 8;
@@ -289,15 +290,24 @@ __device__ __forceinline__ float ld_or0(const float *__restrict__ p, int j)
     return j >= 0 ? p[j] : 0.f;
 }
 
+// Ensemble members share the grid, its weights and per-point fields; member
+// blockIdx.y owns the spectra at this offset of the input and output.
+__device__ __forceinline__ size_t member_offset(const DevTables &t)
+{
+    return (size_t)blockIdx.y * t.KL * t.ML * t.NSEA;
+}
+
 // PROP_NO_CURR: F3 = SUM*F + LAT*F(KLAT(IC)) + LON*F(KLON(IS)) + TH1*F(K+1) + TH2*F(K-1)
 __global__ void __launch_bounds__(256) propags_nocurr_kernel(DevTables t, const float *__restrict__ coef,
                                                               const float *__restrict__ fin,
                                                               float *__restrict__ fout)
 {
-    const int nch = (t.NSEA + 255) / 256;
+    fin += member_offset(t);
+    fout += member_offset(t);
+    const int wg = blockDim.x, nch = (t.NSEA + wg - 1) / wg;
     const int ch = xcd_chunk(nch);
     if (ch >= nch) return;
-    const int ij = ch * 256 + threadIdx.x;
+    const int ij = ch * wg + threadIdx.x;
     if (ij >= t.NSEA) return;
     const int KL = t.KL, ML = t.ML, NS = t.NSEA;
     const size_t NB = (size_t)KL * ML * NS;
@@ -325,10 +335,12 @@ __global__ void __launch_bounds__(256) propags_curr_kernel(DevTables t, const fl
                                                             const float *__restrict__ fin,
                                                             float *__restrict__ fout)
 {
-    const int nch = (t.NSEA + 255) / 256;
+    fin += member_offset(t);
+    fout += member_offset(t);
+    const int wg = blockDim.x, nch = (t.NSEA + wg - 1) / wg;
     const int ch = xcd_chunk(nch);
     if (ch >= nch) return;
-    const int ij = ch * 256 + threadIdx.x;
+    const int ij = ch * wg + threadIdx.x;
     if (ij >= t.NSEA) return;
     const int KL = t.KL, ML = t.ML, NS = t.NSEA;
     const size_t NB = (size_t)KL * ML * NS;
@@ -367,10 +379,12 @@ __global__ void __launch_bounds__(256) propags_curr_rows_kernel(DevTables t, con
                                                                  const float *__restrict__ fin,
                                                                  float *__restrict__ fout)
 {
-    const int nch = (t.NSEA + 255) / 256;
+    fin += member_offset(t);
+    fout += member_offset(t);
+    const int wg = blockDim.x, nch = (t.NSEA + wg - 1) / wg;
     const int ch = xcd_chunk(nch);
     if (ch >= nch) return;
-    const int ij = ch * 256 + threadIdx.x;
+    const int ij = ch * wg + threadIdx.x;
     if (ij >= t.NSEA) return;
     const int ML = t.ML, NS = t.NSEA;
     const size_t NB = (size_t)KLT * ML * NS;
@@ -456,10 +470,12 @@ template <bool CURR>
 __global__ void __launch_bounds__(256) propags_fused_kernel(DevTables t, int counter, const float *__restrict__ fin,
                                                              float *__restrict__ fout)
 {
-    const int nch = (t.NSEA + 255) / 256;
+    fin += member_offset(t);
+    fout += member_offset(t);
+    const int wg = blockDim.x, nch = (t.NSEA + wg - 1) / wg;
     const int ch = xcd_chunk(nch);
     if (ch >= nch) return;
-    const int ij = ch * 256 + threadIdx.x;
+    const int ij = ch * wg + threadIdx.x;
     if (ij >= t.NSEA) return;
     const int KL = t.KL, ML = t.ML, NS = t.NSEA;
     const float rc = (float)counter;
@@ -521,16 +537,18 @@ __global__ void __launch_bounds__(256) propags_fused_sc_kernel(DevTables t, int 
                                                                 const float *__restrict__ fin,
                                                                 float *__restrict__ fout)
 {
+    fin += member_offset(t);
+    fout += member_offset(t);
     // KDIR in LDS (KL <= 64): uniform reads of it are broadcasts
     __shared__ float4 kdir4[2 * 64];
     const int KL = t.KL, ML = t.ML, NS = t.NSEA, ND = t.NDEPTH;
-    for (int i = threadIdx.x; i < 2 * KL; i += 256) kdir4[i] = reinterpret_cast<const float4 *>(t.KDIR)[i];
+    for (int i = threadIdx.x; i < 2 * KL; i += blockDim.x) kdir4[i] = reinterpret_cast<const float4 *>(t.KDIR)[i];
     __syncthreads();
     const float *kdir = reinterpret_cast<const float *>(kdir4);
-    const int nch = (NS + 255) / 256;
+    const int wg = blockDim.x, nch = (NS + wg - 1) / wg;
     const int ch = xcd_chunk(nch);
     if (ch >= nch) return;
-    const int ij = ch * 256 + threadIdx.x;
+    const int ij = ch * wg + threadIdx.x;
     if (ij >= NS) return;
     const int ns = t.KLAT_S[ij], nn = t.KLAT_N[ij], nw = t.KLON_W[ij], ne = t.KLON_E[ij];
     const bool hs = ns >= 0, hn = nn >= 0, hw = nw >= 0, he = ne >= 0;

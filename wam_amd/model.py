@@ -407,6 +407,46 @@ class WamGpu:
                                                  _ptr(ustar), _ptr(z0), _ptr(roairn), _ptr(wstar),
                                                  int(bool(lcflx)), _ptr(phiaw), _ptr(fluxes), _ptr(mij)))
 
+    # -- ensembles: NMEM members on this grid, member arrays concatenated --
+    def propags_ensemble_dev(self, nmem, d_in, d_out):
+        self._check(self.lib.wam_gpu_propags_ensemble_dev(self.ctx, int(nmem), _ptr(d_in), _ptr(d_out)))
+
+    def implsch_ensemble_dev(self, nmem, fl3, u10, udir, tauw, ustar, z0, roairn, wstar, lcflx=False, phiaw=None,
+                             fluxes=None, mij=None):
+        self._check(self.lib.wam_gpu_implsch_ensemble_dev(self.ctx, int(nmem), _ptr(fl3), _ptr(u10), _ptr(udir),
+                                                          _ptr(tauw), _ptr(ustar), _ptr(z0), _ptr(roairn),
+                                                          _ptr(wstar), int(bool(lcflx)), _ptr(phiaw), _ptr(fluxes),
+                                                          _ptr(mij)))
+
+    def ensemble_run_dev(self, nmem, nsteps, fa, fb, u10, udir, tauw, ustar, z0, roairn, wstar, graph=True):
+        """nsteps PROPAGS + IMPLSCH steps of the ensemble; returns the buffer
+        holding the result (fa after an even number of steps, else fb)."""
+        self._check(self.lib.wam_gpu_ensemble_run_dev(self.ctx, int(nmem), int(nsteps), _ptr(fa), _ptr(fb),
+                                                      _ptr(u10), _ptr(udir), _ptr(tauw), _ptr(ustar), _ptr(z0),
+                                                      _ptr(roairn), _ptr(wstar), int(bool(graph))))
+        return fa if nsteps % 2 == 0 else fb
+
+    def autotune(self, nmem, fa, fb, u10, udir, tauw, z0, roairn, wstar, reps=2):
+        self._check(self.lib.wam_gpu_autotune(self.ctx, int(nmem), _ptr(fa), _ptr(fb), _ptr(u10), _ptr(udir),
+                                              _ptr(tauw), _ptr(z0), _ptr(roairn), _ptr(wstar), int(reps)))
+        return self.tuning()
+
+    def tuning(self):
+        """(PROPAGS workgroup width, per-lane IMPLSCH width, SNONLIN workgroups per CU, member-points per chunk)."""
+        out = np.zeros(4, np.int64)
+        self._check(self.lib.wam_gpu_get_tuning(self.ctx, _ptr(out)))
+        return tuple(int(v) for v in out)
+
+    def set_tuning(self, prop_wg=256, src_wg=64, snl_per_cu=0, chunk=0):
+        self._check(self.lib.wam_gpu_set_tuning(self.ctx, _ptr(np.array([prop_wg, src_wg, snl_per_cu, chunk],
+                                                                         np.int64))))
+
+    def tuning_log(self):
+        n = self.lib.wam_gpu_get_tuning_log(self.ctx, None, 0)
+        buf = ctypes.create_string_buffer(int(n) + 1)
+        self.lib.wam_gpu_get_tuning_log(self.ctx, buf, n + 1)
+        return [(w, int(v), float(ms)) for w, v, ms in (l.split() for l in buf.value.decode().splitlines())]
+
     def implsch_dev_noalloc(self, fl3, u10, udir, tauw, ustar, z0, roairn, wstar, lcflx=False, phiaw=None,
                             fluxes=None, mij=None):
         """implsch_dev on the library's static device scratch array: allocates nothing."""

@@ -134,6 +134,24 @@ def c2_case(ny=901, nx=1600, seed=20261203, iphys=0, kl=36, ml=36, dlat=0.1, dlo
     return cfg, mask, fields
 
 
+def c3_case(nmem=32, ny=450, nx=800, seed=20261204, iphys=0, kl=36, ml=36, amp=0.10, scale_km=200.0):
+    """Ensemble of NMEM members on a 0.1 deg sub-box of the production-shaped
+    grid (450 x 800 cells, about 250 k sea points): the members share the
+    grid, depth and currents and differ by their winds, the base wind times
+    (1 + amp * GRF_m), GRF_m a unit Gaussian random field with a Gaussian
+    correlation of scale_km, seeded per member.  Returns (cfg, mask, fields,
+    u10[nmem][nsea], udir[nmem][nsea])."""
+    cfg, mask, f = c2_case(ny=ny, nx=nx, seed=seed, iphys=iphys, kl=kl, ml=ml)
+    sea = mask.astype(bool)
+    cells = scale_km * 1e3 / (0.1 * 111e3)
+    u10 = np.empty((nmem, int(sea.sum())), np.float32)
+    for m in range(nmem):
+        g = _smooth_field(ny, nx, cells, np.random.default_rng(seed + 1 + m))
+        u10[m] = np.clip(f["u10"] * (1.0 + amp * g[sea]), 0.5, None).astype(np.float32)
+    udir = np.broadcast_to(f["udir"], u10.shape).astype(np.float32)
+    return cfg, mask, f, u10, udir
+
+
 def frequencies(cfg):
     fr = np.empty(cfg["ml"], np.float64)
     fr[0] = cfg["fr1"] * 1.1 ** (1 - cfg["iref"])
