@@ -310,6 +310,55 @@ def bench_ensemble(args, torch):
     return res
 
 
+def bench_nested(args, torch):
+    """Configuration 4: three grids (0.5 deg 24 x 25, 0.2 deg and 0.05 deg
+    36 x 36, about 50 k, 200 k and 800 k sea points), the last two a nest with
+    the boundary exchange after every step, advanced K steps grid after grid
+    on one stream (sequential launches) and as one graph whose grids are
+    concurrent branches (fused)."""
+    from wam_amd import WamGpu, WamMultiGrid, workloads as W
+    t0 = time.time()
+    K = args.nest_steps
+    grids_in, (c, f, link) = W.c4_case()
+    dev = torch.device("cuda:0")
+    t = lambda x: torch.from_numpy(np.ascontiguousarray(x, np.float32)).to(dev)
+    grids = []
+    for cfg, mask, fl in grids_in:
+        g = WamGpu(cfg, mask)
+        g.prepare_propagation(fl["depth"], fl["u"], fl["v"])
+        n = g.nsea
+        st = dict(fa=device_spectrum(torch, cfg, t(fl["udir"])).reshape(-1), u10=t(fl["u10"]), udir=t(fl["udir"]),
+                  roairn=torch.full((n,), 1.225, device=dev), wstar=torch.zeros(n, device=dev),
+                  tauw=torch.zeros(n, device=dev), ustar=torch.zeros(n, device=dev), z0=torch.zeros(n, device=dev))
+        st["fb"] = torch.empty_like(st["fa"])
+        grids.append((g, st))
+    nsea = [g.nsea for g, _ in grids]
+    torch.cuda.synchronize()
+    log(f"[nested] grids {nsea}, nest {link[0].size} -> {link[1].size} points ({time.time() - t0:.1f}s)")
+    mg = WamMultiGrid(grids)
+    mg.add_nest(c, f, *link)
+    res = {"workload": "c4: 0.5deg 24x25, 0.2deg 36x36 and 0.05deg 36x36 grids (shallow + currents, IPHYS=0, "
+                       "300 s); 0.2deg -> 0.05deg nest exchanged every step",
+           "nsea": nsea, "nest_points": [int(link[0].size), int(link[1].size)], "steps": K,
+           "counter": [int(g.scalar("COUNTER")) for g, _ in grids]}
+    for name, fused in (("sequential", False), ("fused", True)):
+        mg.run(K, fused=fused)                 # warm-up (and the capture of the graph)
+        torch.cuda.synchronize()
+        w0 = time.perf_counter()
+        mg.run(K, fused=fused)
+        torch.cuda.synchronize()
+        res[f"ms_per_step_{name}"] = round((time.perf_counter() - w0) / K * 1e3, 3)
+    res["value"] = sum(nsea) / (res["ms_per_step_fused"] * 1e-3)
+    res["unit"] = "sea-point updates/s (all grids)"
+    res["fused_vs_sequential"] = round(res["ms_per_step_sequential"] / res["ms_per_step_fused"], 4)
+    res["state_finite"] = all(bool(torch.isfinite(st[mg.state(i)]).all()) for i, (_, st) in enumerate(grids))
+    log(f"[nested] {K} steps: sequential {res['ms_per_step_sequential']} ms/step, fused {res['ms_per_step_fused']}")
+    mg.close()
+    for g, _ in grids:
+        g.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -327,6 +376,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="leave out the batched ensemble measurement")
     ap.add_argument("--ens-members", type=int, default=32)
     ap.add_argument("--ens-steps", type=int, default=4)
+    ap.add_argument("--no-nested", action="store_true", help="leave out the nested-grid measurement")
+    ap.add_argument("--nest-steps", type=int, default=6)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one plain 1 GiB device copy before the timed steps (reference for HBM counters)")
     args = ap.parse_args()
@@ -472,13 +523,16 @@ def main():
         "roofline_all": others,
         "state_finite": finite, "mean_hs_m": round(float(hs.mean()), 4),
     }
+    # the single-grid state and its context are released before the other configurations
+    del fa, fb, u10, udir, tauw, ustar, z0, roairn, wstar, dfim, energy, hs
+    g.close()
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
     if not args.no_ensemble:
-        # the single-grid state and its context are released first
-        del fa, fb, u10, udir, tauw, ustar, z0, roairn, wstar, dfim, energy, hs
-        g.close()
-        torch.cuda.synchronize()
-        torch.cuda.empty_cache()
         out["batched"] = bench_ensemble(args, torch)
+    if not args.no_nested:
+        torch.cuda.empty_cache()
+        out["nested"] = bench_nested(args, torch)
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(args)

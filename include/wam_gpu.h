@@ -193,6 +193,39 @@ int wam_gpu_set_tuning(wam_ctx *ctx, const int64_t *shapes);
    candidate, into buf (n bytes, NUL-terminated); returns the full length. */
 int64_t wam_gpu_get_tuning_log(wam_ctx *ctx, char *buf, int64_t n);
 
+/* Several grids advanced together, with nests (no counterpart in the
+   reference, where every grid is its own run of WAMODEL coupled through
+   boundary value files): every step runs PROPAGS and IMPLSCH on each grid,
+   then each nest's exchange, BOUNDARY_OUTPUT's gather on the coarse grid
+   (wam_boundary_module.f90:350-362) and BOUNDARY_INPUT's space
+   interpolation into the fine grid (:226-285), as WAMODEL inserts boundary
+   values after the sources (wamodel.f90:332,354,370), all in HBM.  A grid's
+   state is d_fa/d_fb (the step writes the other one; wam_gpu_multigrid_state
+   tells which holds it), its forcing and TAUW, USTAR, Z0 (device, NSEA each).
+   wam_gpu_multigrid_run with fused = 0 issues the steps grid after grid on
+   one stream; fused = 1 runs them as one HIP graph in which the grids of a
+   step are concurrent branches (captured at the first call, replayed while
+   nsteps and the state parity stay the same); the results are the same bit
+   for bit, and equal to plain runs with the nest entry points in between.
+   A nest needs the same directions and frequencies on both grids
+   (READ_BOUNDARY_INPUT's check, read_boundary_input.f90:136-156), else
+   WAM_E_ARG.  Point lists are 1-based, host arrays, copied. */
+typedef struct wam_grid_state {
+    wam_ctx *ctx;
+    float *d_fa, *d_fb;
+    const float *d_u10, *d_udir, *d_roairn, *d_wstar;
+    float *d_tauw, *d_ustar, *d_z0;
+} wam_grid_state;
+typedef struct wam_multigrid wam_multigrid;
+int wam_gpu_multigrid_create(wam_multigrid **mg, int32_t ngrid, const wam_grid_state *grids);
+int wam_gpu_multigrid_add_nest(wam_multigrid *mg, int32_t coarse, int32_t fine, int32_t nb, const int32_t *ijarc,
+                               int32_t nbounf, const int32_t *ijarf, const int32_t *ibfl, const int32_t *ibfr,
+                               const float *bfw);
+int wam_gpu_multigrid_run(wam_multigrid *mg, int32_t nsteps, int32_t fused);
+int wam_gpu_multigrid_state(wam_multigrid *mg, int32_t grid, float **d_cur);
+const char *wam_gpu_multigrid_last_error(wam_multigrid *mg);
+void wam_gpu_multigrid_free(wam_multigrid *mg);
+
 /* Stream used by the context (a hipStream_t; NULL = the context's own). */
 int wam_gpu_set_stream(wam_ctx *ctx, void *stream);
 

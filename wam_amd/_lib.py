@@ -46,6 +46,12 @@ SIGNATURES = {
     "wam_gpu_get_tuning": (_I, [_P, _P]),
     "wam_gpu_set_tuning": (_I, [_P, _P]),
     "wam_gpu_get_tuning_log": (ctypes.c_int64, [_P, _P, ctypes.c_int64]),
+    "wam_gpu_multigrid_create": (_I, [ctypes.POINTER(_P), _I, _P]),
+    "wam_gpu_multigrid_add_nest": (_I, [_P, _I, _I, _I, _P, _I, _P, _P, _P, _P]),
+    "wam_gpu_multigrid_run": (_I, [_P, _I, _I]),
+    "wam_gpu_multigrid_state": (_I, [_P, _I, ctypes.POINTER(_P)]),
+    "wam_gpu_multigrid_last_error": (ctypes.c_char_p, [_P]),
+    "wam_gpu_multigrid_free": (None, [_P]),
     "wam_gpu_set_propagation_weights": (_I, [_P, _I]),
     "wam_gpu_set_stream": (_I, [_P, _P]),
     "wam_gpu_output_parameters_dev": (_I, [_P] * 7 + [_I, _P]),

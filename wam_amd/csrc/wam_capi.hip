@@ -1111,6 +1111,268 @@ int wam_gpu_autotune(wam_ctx *ctx, int32_t nmem, const float *d_fa, float *d_fb,
     return rc;
 }
 
+// ---- several grids, nested -----------------------------------------------
+
+}  // extern "C"
+
+struct wam_multigrid {
+    struct Grid { wam_grid_state st; ImplArgs a; };
+    struct Link {
+        int coarse, fine, nb, nf;
+        int *ijarc = nullptr;           // device: [nb] coarse output points, then [3][nf] IJARF, IBFL, IBFR
+        float *fbc = nullptr;           // device: [nb][ML][KL] gathered spectra, [3][nb] their mean values, [nf] BFW
+        float *mean = nullptr, *bfw = nullptr;
+        int *ijarf = nullptr, *ibfl = nullptr, *ibfr = nullptr;
+    };
+    std::vector<Grid> grids;
+    std::vector<Link> links;
+    hipStream_t origin = nullptr;
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> ev;         // [0] fork, [1..n] joins
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int gsteps = -1, gparity = -1;
+    int parity = 0;                     // swaps done: grid states are in d_fb when odd
+    std::string err;
+};
+
+static int mg_fail(wam_multigrid *mg, const std::string &m, int rc)
+{
+    mg->err = m;
+    return rc;
+}
+
+#define MGCHK(x)                                                                   \
+    do {                                                                           \
+        hipError_t e_ = (x);                                                       \
+        if (e_ != hipSuccess) return mg_fail(mg, std::string(#x) + ": " + hipGetErrorString(e_), WAM_E_HIP); \
+    } while (0)
+
+// One model step of grid g on stream st (the context's stream redirected).
+static int mg_grid_step(wam_multigrid *mg, wam_multigrid::Grid &g, float *cur, float *nxt, hipStream_t st)
+{
+    wam_ctx *ctx = g.st.ctx;
+    hipStream_t keep = ctx->stream;
+    ctx->stream = st;
+    int rc = propags_members(ctx, 1, cur, nxt, ctx->fscratch, false);
+    if (!rc) {
+        ImplArgs a = g.a;
+        a.fl3 = nxt;
+        rc = implsch_members(ctx, 1, a, ctx->isx, false);
+    }
+    ctx->stream = keep;
+    if (rc) mg->err = ctx->err;
+    return rc;
+}
+
+// The nest exchanges after every grid's step: BOUNDARY_OUTPUT's gather on the
+// coarse grid, BOUNDARY_INPUT's space interpolation into the fine grid.
+static int mg_links(wam_multigrid *mg, int parity, hipStream_t st)
+{
+    for (auto &l : mg->links) {
+        auto &c = mg->grids[l.coarse], &f = mg->grids[l.fine];
+        float *cf = parity ? c.st.d_fb : c.st.d_fa, *ff = parity ? f.st.d_fb : f.st.d_fa;
+        MGCHK(launch_boundary_gather(c.st.ctx->dt, cf, l.ijarc, l.nb, l.fbc, l.mean, st));
+        MGCHK(launch_boundary_space(f.st.ctx->dt, l.nb, l.fbc, l.mean, l.nf, l.ijarf, l.ibfl, l.ibfr, l.bfw, ff, st));
+    }
+    return WAM_OK;
+}
+
+// nsteps steps from the given parity, grids one after another on one stream
+// (fused = 0) or each on its own stream between a fork and a join per step.
+static int mg_steps(wam_multigrid *mg, int nsteps, int parity, bool concurrent)
+{
+    const int n = (int)mg->grids.size();
+    for (int s = 0; s < nsteps; s++) {
+        if (concurrent) MGCHK(hipEventRecord(mg->ev[0], mg->origin));
+        for (int i = 0; i < n; i++) {
+            auto &g = mg->grids[i];
+            float *cur = parity ? g.st.d_fb : g.st.d_fa, *nxt = parity ? g.st.d_fa : g.st.d_fb;
+            hipStream_t st = mg->origin;
+            if (concurrent) {
+                st = mg->side[i];
+                MGCHK(hipStreamWaitEvent(st, mg->ev[0], 0));
+            }
+            const int rc = mg_grid_step(mg, g, cur, nxt, st);
+            if (rc) return rc;
+            if (concurrent) {
+                MGCHK(hipEventRecord(mg->ev[1 + i], st));
+                MGCHK(hipStreamWaitEvent(mg->origin, mg->ev[1 + i], 0));
+            }
+        }
+        parity ^= 1;
+        const int rc = mg_links(mg, parity, mg->origin);
+        if (rc) return rc;
+    }
+    return WAM_OK;
+}
+
+extern "C" {
+
+int wam_gpu_multigrid_create(wam_multigrid **out, int32_t ngrid, const wam_grid_state *grids)
+{
+    if (!out || ngrid < 1 || ngrid > 64 || !grids) return WAM_E_ARG;
+    *out = nullptr;
+    auto *mg = new wam_multigrid();
+    for (int i = 0; i < ngrid; i++) {
+        const wam_grid_state &g = grids[i];
+        wam_ctx *ctx = g.ctx;
+        if (!ctx || !g.d_fa || !g.d_fb || g.d_fa == g.d_fb || !g.d_u10 || !g.d_udir || !g.d_tauw || !g.d_ustar ||
+            !g.d_z0 || !g.d_roairn || !g.d_wstar) {
+            delete mg;
+            return WAM_E_ARG;
+        }
+        if (!ctx->prepared) { ctx->err = "PREPARE_PROPAGATION not done"; delete mg; return WAM_E_STATE; }
+        for (int j = 0; j < i; j++)      // a context's work arrays serve one grid
+            if (grids[j].ctx == ctx) { delete mg; return WAM_E_ARG; }
+        // the work arrays and sub-step buffers, allocated before any capture
+        const size_t nb = (size_t)ctx->tab.KL * ctx->tab.ML * ctx->tab.NSEA;
+        int rc = WAM_OK;
+        if (ctx->counter > 1 && !ctx->fscratch && hipMalloc(&ctx->fscratch, nb * sizeof(float)) != hipSuccess)
+            rc = WAM_E_HIP;
+        if (!rc && implsch_split_supported(ctx->dt)) rc = ensure_impl_scratch(ctx, ctx->isx, ctx->tab.NSEA, 0);
+        if (rc) { delete mg; return rc; }
+        wam_multigrid::Grid gr;
+        gr.st = g;
+        gr.a = impl_args(g.d_fb, g.d_u10, g.d_udir, g.d_tauw, g.d_ustar, g.d_z0, g.d_roairn, g.d_wstar, 0, nullptr,
+                         nullptr, nullptr, ctx->tab.NSEA);
+        mg->grids.push_back(gr);
+    }
+    bool ok = hipStreamCreateWithFlags(&mg->origin, hipStreamNonBlocking) == hipSuccess;
+    mg->side.resize(ngrid);
+    mg->ev.resize(ngrid + 1);
+    for (int i = 0; ok && i < ngrid; i++) ok = hipStreamCreateWithFlags(&mg->side[i], hipStreamNonBlocking) == hipSuccess;
+    for (auto &e : mg->ev) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+    *out = mg;
+    return ok ? WAM_OK : mg_fail(mg, "stream or event creation failed", WAM_E_HIP);
+}
+
+int wam_gpu_multigrid_add_nest(wam_multigrid *mg, int32_t coarse, int32_t fine, int32_t nb, const int32_t *ijarc,
+                               int32_t nbounf, const int32_t *ijarf, const int32_t *ibfl, const int32_t *ibfr,
+                               const float *bfw)
+{
+    if (!mg) return WAM_E_ARG;
+    const int n = (int)mg->grids.size();
+    if (coarse < 0 || coarse >= n || fine < 0 || fine >= n || coarse == fine || nb < 1 || nbounf < 0 || !ijarc ||
+        (nbounf > 0 && (!ijarf || !ibfl || !ibfr || !bfw)))
+        return mg_fail(mg, "nest: bad grid index or point list", WAM_E_ARG);
+    const Tables &tc = mg->grids[coarse].st.ctx->tab, &tf = mg->grids[fine].st.ctx->tab;
+    // READ_BOUNDARY_INPUT aborts unless the spectra match (read_boundary_input.f90:136-156)
+    if (tc.KL != tf.KL || tc.ML != tf.ML || std::fabs(tc.FR[0] - tf.FR[0]) > 0.00001f ||
+        std::fabs(tc.TH[0] - tf.TH[0]) > 0.0001f)
+        return mg_fail(mg, "nest: the coarse and fine grids' directions and frequencies differ", WAM_E_ARG);
+    for (int32_t i = 0; i < nb; i++)
+        if (ijarc[i] < 1 || ijarc[i] > tc.NSEA) return mg_fail(mg, "nest: IJARC outside 1..NSEA", WAM_E_ARG);
+    // each fine point once, with its last entry (as the reference's loop leaves it)
+    std::vector<std::pair<int32_t, int32_t>> byp((size_t)nbounf);
+    for (int32_t i = 0; i < nbounf; i++) {
+        if (ijarf[i] < 1 || ijarf[i] > tf.NSEA || ibfl[i] < 1 || ibfl[i] > nb || ibfr[i] < 1 || ibfr[i] > nb)
+            return mg_fail(mg, "nest: IJARF outside 1..NSEA or IBFL, IBFR outside 1..NB", WAM_E_ARG);
+        byp[i] = {ijarf[i], i};
+    }
+    std::sort(byp.begin(), byp.end());
+    std::vector<int32_t> keep;
+    for (size_t j = 0; j < byp.size(); j++)
+        if (j + 1 == byp.size() || byp[j + 1].first != byp[j].first) keep.push_back(byp[j].second);
+    std::sort(keep.begin(), keep.end());
+    const int nf = (int)keep.size();
+    std::vector<int32_t> idx((size_t)nb + 3 * nf);
+    std::vector<float> w(nf);
+    std::copy(ijarc, ijarc + nb, idx.begin());
+    for (int j = 0; j < nf; j++) {
+        idx[nb + j] = ijarf[keep[j]];
+        idx[nb + nf + j] = ibfl[keep[j]];
+        idx[nb + 2 * nf + j] = ibfr[keep[j]];
+        w[j] = bfw[keep[j]];
+    }
+    wam_multigrid::Link l;
+    l.coarse = coarse; l.fine = fine; l.nb = nb; l.nf = nf;
+    const size_t NB = (size_t)tc.KL * tc.ML;
+    MGCHK(hipMalloc(&l.ijarc, idx.size() * sizeof(int32_t)));
+    MGCHK(hipMalloc(&l.fbc, ((size_t)nb * NB + 3 * (size_t)nb + nf) * sizeof(float)));
+    l.mean = l.fbc + (size_t)nb * NB;
+    l.bfw = l.mean + 3 * (size_t)nb;
+    l.ijarf = l.ijarc + nb; l.ibfl = l.ijarf + nf; l.ibfr = l.ibfl + nf;
+    MGCHK(hipMemcpy(l.ijarc, idx.data(), idx.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (nf) MGCHK(hipMemcpy(l.bfw, w.data(), nf * sizeof(float), hipMemcpyHostToDevice));
+    mg->links.push_back(l);
+    if (mg->gexec) { (void)hipGraphExecDestroy(mg->gexec); mg->gexec = nullptr; }
+    if (mg->graph) { (void)hipGraphDestroy(mg->graph); mg->graph = nullptr; }
+    return WAM_OK;
+}
+
+int wam_gpu_multigrid_run(wam_multigrid *mg, int32_t nsteps, int32_t fused)
+{
+    if (!mg || nsteps < 0) return WAM_E_ARG;
+    if (nsteps == 0) return WAM_OK;
+    const int n = (int)mg->grids.size();
+    // after the work already queued on every grid's context stream
+    for (int i = 0; i < n; i++) {
+        MGCHK(hipEventRecord(mg->ev[1 + i], mg->grids[i].st.ctx->stream));
+        MGCHK(hipStreamWaitEvent(mg->origin, mg->ev[1 + i], 0));
+    }
+    if (!fused) {
+        const int rc = mg_steps(mg, nsteps, mg->parity, false);
+        if (rc) return rc;
+    } else {
+        if (!mg->gexec || mg->gsteps != nsteps || mg->gparity != mg->parity) {
+            if (mg->gexec) { (void)hipGraphExecDestroy(mg->gexec); mg->gexec = nullptr; }
+            if (mg->graph) { (void)hipGraphDestroy(mg->graph); mg->graph = nullptr; }
+            hipStream_t cap = nullptr;
+            MGCHK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+            // capture on a stream of its own (origin is waiting on the grids' streams)
+            hipStream_t keep = mg->origin;
+            mg->origin = cap;
+            hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed);
+            int rc = e == hipSuccess ? mg_steps(mg, nsteps, mg->parity, true) : WAM_E_HIP;
+            hipGraph_t g = nullptr;
+            const hipError_t e2 = hipStreamEndCapture(cap, &g);
+            mg->origin = keep;
+            (void)hipStreamDestroy(cap);
+            if (e != hipSuccess) return mg_fail(mg, "hipStreamBeginCapture failed", WAM_E_HIP);
+            if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+            MGCHK(e2);
+            mg->graph = g;
+            MGCHK(hipGraphInstantiate(&mg->gexec, g, nullptr, nullptr, 0));
+            mg->gsteps = nsteps;
+            mg->gparity = mg->parity;
+        }
+        MGCHK(hipGraphLaunch(mg->gexec, mg->origin));
+    }
+    mg->parity ^= nsteps & 1;
+    // and before what the grids' streams do next
+    MGCHK(hipEventRecord(mg->ev[0], mg->origin));
+    for (int i = 0; i < n; i++) MGCHK(hipStreamWaitEvent(mg->grids[i].st.ctx->stream, mg->ev[0], 0));
+    return WAM_OK;
+}
+
+int wam_gpu_multigrid_state(wam_multigrid *mg, int32_t grid, float **d_cur)
+{
+    if (!mg || !d_cur || grid < 0 || grid >= (int)mg->grids.size()) return WAM_E_ARG;
+    *d_cur = mg->parity ? mg->grids[grid].st.d_fb : mg->grids[grid].st.d_fa;
+    return WAM_OK;
+}
+
+const char *wam_gpu_multigrid_last_error(wam_multigrid *mg) { return mg ? mg->err.c_str() : "null handle"; }
+
+void wam_gpu_multigrid_free(wam_multigrid *mg)
+{
+    if (!mg) return;
+    if (mg->origin) (void)hipStreamSynchronize(mg->origin);
+    for (auto &l : mg->links) {
+        if (l.ijarc) (void)hipFree(l.ijarc);
+        if (l.fbc) (void)hipFree(l.fbc);
+    }
+    if (mg->gexec) (void)hipGraphExecDestroy(mg->gexec);
+    if (mg->graph) (void)hipGraphDestroy(mg->graph);
+    for (auto e : mg->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto st : mg->side)
+        if (st) (void)hipStreamDestroy(st);
+    if (mg->origin) (void)hipStreamDestroy(mg->origin);
+    delete mg;
+}
+
 int wam_gpu_get_tuning(wam_ctx *ctx, int64_t *out)
 {
     if (!ctx || !out) return WAM_E_ARG;

@@ -453,3 +453,63 @@ class WamGpu:
         self._check(self.lib.wam_gpu_implsch_dev_noalloc(self.ctx, _ptr(fl3), _ptr(u10), _ptr(udir), _ptr(tauw),
                                                          _ptr(ustar), _ptr(z0), _ptr(roairn), _ptr(wstar),
                                                          int(bool(lcflx)), _ptr(phiaw), _ptr(fluxes), _ptr(mij)))
+
+
+class _GridState(ctypes.Structure):
+    """struct wam_grid_state of include/wam_gpu.h."""
+    _fields_ = [("ctx", ctypes.c_void_p), ("d_fa", ctypes.c_void_p), ("d_fb", ctypes.c_void_p),
+                ("d_u10", ctypes.c_void_p), ("d_udir", ctypes.c_void_p), ("d_roairn", ctypes.c_void_p),
+                ("d_wstar", ctypes.c_void_p), ("d_tauw", ctypes.c_void_p), ("d_ustar", ctypes.c_void_p),
+                ("d_z0", ctypes.c_void_p)]
+
+
+class WamMultiGrid:
+    """Several grids advanced together with nests (wam_gpu_multigrid_*).
+    grids: list of (WamGpu, dict of device tensors fa, fb, u10, udir, roairn,
+    wstar, tauw, ustar, z0)."""
+
+    def __init__(self, grids):
+        self.lib = _lib.load()
+        self._keep = grids
+        arr = (_GridState * len(grids))()
+        for i, (g, st) in enumerate(grids):
+            arr[i] = _GridState(g.ctx, *[st[k].data_ptr() for k in ("fa", "fb", "u10", "udir", "roairn", "wstar",
+                                                                      "tauw", "ustar", "z0")])
+        h = ctypes.c_void_p()
+        rc = self.lib.wam_gpu_multigrid_create(ctypes.byref(h), len(grids), arr)
+        self.mg = h.value
+        if rc:
+            msg = self.lib.wam_gpu_multigrid_last_error(self.mg) if self.mg else b""
+            self.close()
+            raise WamError(rc, (msg or b"").decode())
+
+    def _check(self, rc):
+        if rc:
+            raise WamError(rc, self.lib.wam_gpu_multigrid_last_error(self.mg).decode())
+
+    def add_nest(self, coarse, fine, ijarc, ijarf, ibfl, ibfr, bfw):
+        a = [np.ascontiguousarray(x, np.int32) for x in (ijarc, ijarf, ibfl, ibfr)]
+        w = np.ascontiguousarray(bfw, np.float32)
+        self._check(self.lib.wam_gpu_multigrid_add_nest(self.mg, int(coarse), int(fine), a[0].size, _ptr(a[0]),
+                                                        a[1].size, _ptr(a[1]), _ptr(a[2]), _ptr(a[3]), _ptr(w)))
+
+    def run(self, nsteps, fused=True):
+        self._check(self.lib.wam_gpu_multigrid_run(self.mg, int(nsteps), int(bool(fused))))
+
+    def state(self, grid):
+        """'fa' or 'fb': the tensor of grid i that holds its state."""
+        p = ctypes.c_void_p()
+        self._check(self.lib.wam_gpu_multigrid_state(self.mg, int(grid), ctypes.byref(p)))
+        st = self._keep[grid][1]
+        return "fa" if p.value == st["fa"].data_ptr() else "fb"
+
+    def close(self):
+        if getattr(self, "mg", None):
+            self.lib.wam_gpu_multigrid_free(self.mg)
+            self.mg = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

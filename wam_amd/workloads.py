@@ -152,6 +152,90 @@ def c3_case(nmem=32, ny=450, nx=800, seed=20261204, iphys=0, kl=36, ml=36, amp=0
     return cfg, mask, f, u10, udir
 
 
+def grid_lat_lon(cfg):
+    """Latitudes and longitudes [deg] of the rows and columns."""
+    lat = (cfg["amosop"] + np.arange(cfg["ny"]) * cfg["xdella"]) / M_DEGREE
+    lon = (cfg["amowep"] + np.arange(cfg["nx"]) * cfg["xdello"]) / M_DEGREE
+    return lat, lon
+
+
+def nest_points(ccfg, cmask, fcfg, fmask):
+    """The tables of a nest, as a pre-processor sets them up: the fine grid's
+    sea points on its outer edge (IJARF, 1-based) take the spectrum between two
+    coarse-grid sea points (IBFL, IBFR, 1-based into IJARC, the coarse output
+    points, themselves 1-based sea-point numbers of the coarse grid), at the
+    fraction BFW of the way from the first to the second: along the fine
+    grid's southern and northern edges the coarse columns either side on the
+    nearest coarse row, along its western and eastern edges the coarse rows
+    either side on the nearest column.  A coarse cell on land gives way to
+    the other one; where both are land the point is left out."""
+    clat, clon = grid_lat_lon(ccfg)
+    flat, flon = grid_lat_lon(fcfg)
+    cidx = np.full(cmask.shape, 0, np.int64)
+    cidx[cmask.astype(bool)] = np.arange(1, int(cmask.sum()) + 1)
+    fidx = np.full(fmask.shape, 0, np.int64)
+    fidx[fmask.astype(bool)] = np.arange(1, int(fmask.sum()) + 1)
+    dla, dlo = ccfg["xdella"] / M_DEGREE, ccfg["xdello"] / M_DEGREE
+    ny, nx = fmask.shape
+    out_pts, ijarf, left, right, w = {}, [], [], [], []
+
+    def coarse(r, c):
+        if 0 <= r < cmask.shape[0] and 0 <= c < cmask.shape[1] and cidx[r, c] > 0:
+            return int(cidx[r, c])
+        return 0
+
+    for r in range(ny):
+        for c in range(nx):
+            if fidx[r, c] == 0 or not (r in (0, ny - 1) or c in (0, nx - 1)):
+                continue
+            if r in (0, ny - 1):            # along a row: bracket in longitude
+                cr = int(round((flat[r] - clat[0]) / dla))
+                x = (flon[c] - clon[0]) / dlo
+                c0 = int(np.floor(x))
+                a, b, f = coarse(cr, c0), coarse(cr, c0 + 1), x - c0
+            else:                           # along a column: bracket in latitude
+                cc = int(round((flon[c] - clon[0]) / dlo))
+                y = (flat[r] - clat[0]) / dla
+                r0 = int(np.floor(y))
+                a, b, f = coarse(r0, cc), coarse(r0 + 1, cc), y - r0
+            if a == 0 and b == 0:
+                continue
+            if a == 0:
+                a, f = b, 0.0
+            if b == 0:
+                b, f = a, 0.0
+            for q in (a, b):
+                out_pts.setdefault(q, len(out_pts) + 1)
+            ijarf.append(int(fidx[r, c]))
+            left.append(out_pts[a])
+            right.append(out_pts[b])
+            w.append(f)
+    ijarc = np.array(sorted(out_pts, key=out_pts.get), np.int32)
+    return (ijarc, np.array(ijarf, np.int32), np.array(left, np.int32), np.array(right, np.int32),
+            np.array(w, np.float32))
+
+
+def c4_case(scale=1.0, seed=20261205):
+    """Three grids of configuration 4: 0.5 deg 24 x 25 (about 50 k sea points),
+    0.2 deg 36 x 36 (about 200 k) and 0.05 deg 36 x 36 (about 800 k) inside
+    the 0.2 deg one, generated as c2_case (bathymetry, winds, currents), 300 s
+    steps; the last two form a nest (coarse 0.2 deg -> fine 0.05 deg).  The
+    0.5 deg grid's spectra differ, which the reference's boundary input does
+    not accept (read_boundary_input.f90:136-156), so it runs beside them.
+    scale < 1 shrinks every grid's rows and columns by that factor.  Returns
+    ([(cfg, mask, fields)] * 3, (coarse, fine, nest_points(...)))."""
+    s = lambda v: max(8, int(round(v * scale)))
+    g1 = c2_case(ny=s(220), nx=s(325), seed=seed, kl=24, ml=25, dlat=0.5, dlon=0.5, south=-55.0, west=0.0)
+    g2 = c2_case(ny=s(400), nx=s(715), seed=seed + 1, dlat=0.2, dlon=0.2, south=-40.0, west=0.0)
+    # the fine grid: rows and columns of 0.05 deg inside the 0.2 deg grid's box
+    ny3, nx3 = s(900), s(1270)
+    south3 = -40.0 + 0.2 * s(400) / 2 - 0.05 * ny3 / 2
+    west3 = 0.2 * s(715) / 2 - 0.05 * nx3 / 2
+    g3 = c2_case(ny=ny3, nx=nx3, seed=seed + 2, dlat=0.05, dlon=0.05, south=round(south3, 2), west=round(west3, 2))
+    link = nest_points(g2[0], g2[1], g3[0], g3[1])
+    return [g1, g2, g3], (1, 2, link)
+
+
 def frequencies(cfg):
     fr = np.empty(cfg["ml"], np.float64)
     fr[0] = cfg["fr1"] * 1.1 ** (1 - cfg["iref"])
