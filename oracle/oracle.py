@@ -43,8 +43,8 @@ def build():
 _LIBS = {}
 
 
-def _lib(fp64, opcount=False):
-    key = "liborc_opcount.so" if opcount else ("liborc64.so" if fp64 else "liborc32.so")
+def _lib(fp64, opcount=False, ad=False):
+    key = "liborc_ad.so" if ad else ("liborc_opcount.so" if opcount else ("liborc64.so" if fp64 else "liborc32.so"))
     if key not in _LIBS:
         path = os.path.join(_HERE, key)
         if not os.path.exists(path):
@@ -77,6 +77,9 @@ def _lib(fp64, opcount=False):
         lib.orc_api_math.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long]
         lib.orc_api_aki.restype = ctypes.c_double
         lib.orc_api_aki.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        if ad:
+            lib.orc_ad_step.restype = ctypes.c_long
+            lib.orc_ad_step.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 11
         if opcount:
             lib.orc_opcount_implsch.argtypes = ([ctypes.c_void_p] + [ctypes.c_void_p] * 9 +
                                                 [ctypes.c_int] + [ctypes.c_void_p] * 5)
@@ -91,12 +94,15 @@ def _p(a):
 class Oracle:
     """One model configuration on one regular grid (sea mask nx*ny, row-major from the south-west)."""
 
-    def __init__(self, cfg, mask, fp64=False, opcount=False):
+    def __init__(self, cfg, mask, fp64=False, opcount=False, ad=False):
+        """ad=True: the FP64 restatement together with one model step's
+        tangent-linear model and adjoint (step_ad)."""
         self.cfg = dict(cfg)
-        self.fp64 = fp64 and not opcount
+        self.fp64 = (fp64 or ad) and not opcount
         self.dtype = np.float64 if self.fp64 else np.float32
         self.opcount = opcount
-        self.lib = _lib(self.fp64, opcount)
+        self.ad = ad
+        self.lib = _lib(self.fp64, opcount, ad)
         self._c = make_config(cfg)
         m = np.ascontiguousarray(mask, dtype=np.uint8).ravel()
         st = ctypes.c_int(0)
@@ -132,6 +138,28 @@ class Oracle:
         if st != 0:
             raise ValueError(f"oracle prepare_propagation status {st}")
         return int(self.scalar("COUNTER")), float(self.scalar("CFL_MAX"))
+
+    def step_ad(self, fl3, tauw, z0, u, v, depth, roairn=None, wstar=None, dx=None, ybar=None):
+        """One model step, PROPAGS then IMPLSCH, in FP64 from the wind
+        components u, v (U10 = sqrt(u^2 + v^2), UDIR = atan2(u, v)):
+        returns y = (FL3', TAUW', Z0') packed, and with dx (packed as
+        x = (FL3, TAUW, Z0, U, V)) also the tangent-linear dy, or with ybar
+        (packed as y) also the adjoint xbar."""
+        assert self.ad
+        n = self.nsea
+        d = lambda a: np.ascontiguousarray(a, np.float64)
+        ro = np.full(n, 1.225) if roairn is None else d(roairn)
+        ws = np.zeros(n) if wstar is None else d(wstar)
+        nb = self.kl * self.ml * n
+        y = np.zeros(nb + 2 * n)
+        mode, arg, out = 0, None, None
+        if dx is not None:
+            mode, arg, out = 1, d(dx), np.zeros(nb + 2 * n)
+        elif ybar is not None:
+            mode, arg, out = 2, d(ybar), np.zeros(nb + 4 * n)
+        self.lib.orc_ad_step(self.ctx, mode, _p(d(fl3)), _p(d(tauw)), _p(d(z0)), _p(d(u)), _p(d(v)), _p(ro), _p(ws),
+                             _p(d(depth)), _p(arg), _p(y), _p(out))
+        return (y, out) if mode else y
 
     def force_counter(self, counter):
         """For a window of a larger grid: CHECK_CFL's sub-step count is that

@@ -20,7 +20,8 @@
  * Build FP32 (default, the reference's REAL) or FP64 (-DORC_FP64: the
  * reference built with 8-byte reals); -DORC_OPCOUNT (C++) is the FP32
  * arithmetic with every operation counted (orc_opcount.h), per stage at the
- * ORC_STAGE markers.  The FP32 build evaluates the Fortran
+ * ORC_STAGE markers; -DORC_AD (C++) is the FP64 arithmetic recorded on a tape
+ * for its tangent-linear model and adjoint (orc_ad.h).  The FP32 build evaluates the Fortran
  * intrinsics with wam_math.h (deterministic single precision); the FP64 build
  * uses libm double.
  */
@@ -58,6 +59,31 @@ static inline int INTR(real x) { return (int)trunc(x); }
 static inline real FLOORR(real x) { return floor(x); }
 static inline real CEILR(real x) { return ceil(x); }
 static inline real SIGNR(real a, real b) { return b >= 0 ? fabs(a) : -fabs(a); }
+#elif defined(ORC_AD)
+/* the FP64 arithmetic on the tape of orc_ad.h; the context's tables are the
+   FP64 build's (passive) */
+#include "orc_ad.h"
+typedef adr real;
+typedef double treal;
+#define R(x) (x)
+#define EXP(x) ad_exp(real(x))
+#define LOG(x) ad_log(real(x))
+#define LOG10(x) ad_log10(real(x))
+#define TANH(x) ad_tanh(real(x))
+#define COSH(x) ad_cosh(real(x))
+#define SINH(x) ad_sinh(real(x))
+#define COS(x) ad_cos(real(x))
+#define SIN(x) ad_sin(real(x))
+#define SQRT(x) ad_sqrt(real(x))
+#define ACOS(x) ad_acos(real(x))
+#define POW(x, y) ad_pow(real(x), real(y))
+#define ABS(x) ad_abs(real(x))
+#define IPOW(x, n) ad_ipow(real(x), n)
+static inline int NINT(real x) { return (int)lround(x.v); }
+static inline int INTR(real x) { return (int)trunc(x.v); }
+static inline real FLOORR(real x) { return floor(x.v); }
+static inline real CEILR(real x) { return ceil(x.v); }
+static inline real SIGNR(real a, real b) { return b.v >= 0 ? ad_abs(a) : -ad_abs(a); }
 #elif defined(ORC_OPCOUNT)
 #include "orc_opcount.h"
 typedef oc_real real;
@@ -104,6 +130,13 @@ static inline real CEILR(real x) { return wam_ceilf(x); }
 static inline real SIGNR(real a, real b) { return wam_signf(a, b); }
 #endif
 
+/* The type of the context's stored tables and constants: the arithmetic type
+   in every build except the differentiating one (-DORC_AD, C++), where the
+   per-point work is active and the set-up (built in FP64) is passive. */
+#ifndef ORC_AD
+typedef real treal;
+#endif
+
 /* Stage markers of the op-counting build (no-ops otherwise). */
 #ifdef ORC_OPCOUNT
 #define ORC_STAGE(n) oc_stage(n)
@@ -138,45 +171,45 @@ typedef struct {
     orc_config cfg;
     int KL, ML;
     /* constants (wam_general_module.f90:41-85) */
-    real G, PI, ZPI, DEG, RAD, CIRC, RADIUS, ROAIR, RNUAIR, RNUAIRM, ROWATER,
+    treal G, PI, ZPI, DEG, RAD, CIRC, RADIUS, ROAIR, RNUAIR, RNUAIRM, ROWATER,
          XKAPPA, XNLEV, DKMAX, BETAMAX, ZALP, ALPHA, CDFAC, EMIN, EPS1;
     /* fre_dir (wam_fre_dir_module.f90:297-461) */
-    real CO, INV_LOG_CO, DELTH, DELTR, MO_TAIL, MM1_TAIL, MP1_TAIL, MP2_TAIL, FMIN;
-    real *FR, *DF, *DF_FR, *DF_FR2, *DFIM, *DFIMOFR, *DFIM_FR, *DFIM_FR2, *FR5, *FRM5,
+    treal CO, INV_LOG_CO, DELTH, DELTR, MO_TAIL, MM1_TAIL, MP1_TAIL, MP2_TAIL, FMIN;
+    treal *FR, *DF, *DF_FR, *DF_FR2, *DFIM, *DFIMOFR, *DFIM_FR, *DFIM_FR2, *FR5, *FRM5,
          *RHOWG_DFIM, *GOM, *C, *TH, *COSTH, *SINTH;
     int *MPM, *KPM, *JXO, *JYO;            /* [ML][3], [KL][3], [KL][2], [KL][2] */
     /* tables (wam_tables_module.f90) */
-    int JUMAX; real UMAX, DELU, FLMIN;
-    real *FLMINFR;                          /* [ML][JUMAX]: FLMINFR(JU,M) */
-    real DEPTHA, DEPTHD;
-    real *TFAK, *TCGOND, *TSIHKD, *T_TAIL;  /* [ML][NDEPTH] */
+    int JUMAX; treal UMAX, DELU, FLMIN;
+    treal *FLMINFR;                          /* [ML][JUMAX]: FLMINFR(JU,M) */
+    treal DEPTHA, DEPTHD;
+    treal *TFAK, *TCGOND, *TSIHKD, *T_TAIL;  /* [ML][NDEPTH] */
     /* DIA (wam_source_module.f90:1056-1523) */
     int ISP, ISM, MFRSTLW, MLSTHG, KFRH;
     int *IKP, *IKP1, *IKM, *IKM1;           /* index MFRSTLW..MLSTHG -> [M-MFRSTLW] */
     int *K1W, *K2W, *K11W, *K21W;           /* [2][KL] (KH, K) */
-    real *AF11, *FKLAP, *FKLAP1, *FKLAM, *FKLAM1, *FRH, *FTRF;
-    real ACL1, ACL2, CL11, CL21, DAL1, DAL2;
+    treal *AF11, *FKLAP, *FKLAP1, *FKLAM, *FKLAM1, *FRH, *FTRF;
+    treal ACL1, ACL2, CL11, CL21, DAL1, DAL2;
     int *INLCOEF;                           /* [MLSTHG][5] */
-    real *RNLCOEF;                          /* [MLSTHG][25] */
+    treal *RNLCOEF;                          /* [MLSTHG][25] */
     /* TAU_PHI_HF (wam_source_module.f90:3784-3849) */
-    real X0TAUHF, WTAUHF[19];
+    treal X0TAUHF, WTAUHF[19];
     /* grid (preproc_module.f90:2043-2381) */
     int NSEA;
-    real DELPHI;
-    real *SINPH, *COSPH, *DELLAM;           /* [NY] */
+    treal DELPHI;
+    treal *SINPH, *COSPH, *DELLAM;           /* [NY] */
     int *ZDELLO, *IXLG, *KXLT;              /* 1-based grid indices per sea point */
     int *KLAT;                              /* [2 (ICL)][2 (IC)][NSEA] : KLAT(IJ,IC,ICL) 1-based, 0 = land */
     int *KLON;                              /* [2][NSEA] */
-    real *WLAT;                             /* [2][NSEA] */
-    real *DCO;                              /* [NSEA+1], index 0 = land slot */
-    real *DPSN;                             /* [2][NSEA] */
+    treal *WLAT;                             /* [2][NSEA] */
+    treal *DCO;                              /* [NSEA+1], index 0 = land slot */
+    treal *DPSN;                             /* [2][NSEA] */
     /* propagation */
-    int counter; real cfl_max, cfl_min;
+    int counter; treal cfl_max, cfl_min;
     int force_counter;   /* > 0: COUNTER of the grid this one is a window of (CHECK_CFL's rescale by it) */
     int ncoef;                              /* coefficient planes present */
-    real *COEF;                             /* [ncoef][ML][KL][NSEA] */
+    treal *COEF;                             /* [ncoef][ML][KL][NSEA] */
     int *INDEP;                             /* [NSEA] */
-    real *WN, *CGG;                         /* WAVNU2 results [ML][NSEA] (IPHYS=2) */
+    treal *WN, *CGG;                         /* WAVNU2 results [ML][NSEA] (IPHYS=2) */
 } orc_ctx;
 
 /* coefficient plane order in COEF: */

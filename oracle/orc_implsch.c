@@ -614,7 +614,7 @@ static void snonlin(const orc_ctx *c, orc_work *w, real DEPTH, real AKMEAN)
         int MP = c->IKP[i], MP1 = c->IKP1[i], MM = c->IKM[i], MM1 = c->IKM1[i];
         const int *ic = &c->INLCOEF[(MC - 1) * 5];
         int IC = ic[0], IP = ic[1], IP1 = ic[2], IM = ic[3], IM1 = ic[4];
-        const real *rc = &c->RNLCOEF[(MC - 1) * 25];
+        const treal *rc = &c->RNLCOEF[(MC - 1) * 25];
         real FTAIL = rc[0], GW1 = rc[1], GW2 = rc[2], GW3 = rc[3], GW4 = rc[4];
         real FKLAMPA = rc[5], FKLAMPB = rc[6], FKLAMP2 = rc[7], FKLAMP1 = rc[8];
         real FKLAPA2 = rc[9], FKLAPB2 = rc[10], FKLAP12 = rc[11], FKLAP22 = rc[12];
@@ -795,7 +795,7 @@ static void implsch_point(const orc_ctx *c, orc_work *w, real U10, real UDIR, re
     real DELT = (real)c->cfg.idelt;
     real c5g = R(5.0E-07) * c->G;
     real fmx = MAXR(FMEANWS, FMEAN);
-    const real *FLM = &c->FLMINFR[JU - 1];
+    const treal *FLM = &c->FLMINFR[JU - 1];
     for (int m = 0; m < ML; m++) {
         real DELFL = (c5g / IPOW(c->FR[m], 4)) * DELT;
         real TEMP2 = (ust * DELFL) * fmx;
