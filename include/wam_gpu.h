@@ -226,6 +226,57 @@ int wam_gpu_multigrid_state(wam_multigrid *mg, int32_t grid, float **d_cur);
 const char *wam_gpu_multigrid_last_error(wam_multigrid *mg);
 void wam_gpu_multigrid_free(wam_multigrid *mg);
 
+/* The adjoint of one model step, PROPAGS then IMPLSCH (wamodel.f90:332,354),
+   for NMEM members laid out as the ensemble entry points (no counterpart in
+   the reference, which has no adjoint model).  It is the transpose of this
+   library's forward step linearised at the given state, every branch (MIJ,
+   LLWS, the limiter, AIRSEA's iteration count) taken as the forward step
+   takes it there: given the step's input state d_fl3 (before PROPAGS) and
+   d_tauw, its forcing and the bars of its outputs (d_fl3_bar; d_tauw_bar,
+   d_ustar_bar, d_z0_bar may be NULL for zero), it writes the bars of its
+   inputs: d_fl3_bar_out (may be d_fl3_bar), d_u10_bar, d_udir_bar and
+   d_tauw_bar_out (Z0 and USTAR in are not read by IPHYS = 0, their bars are
+   zero).  IPHYS = 0 in deep water, on a grid whose neighbour relation is
+   symmetric (regular grids), else WAM_E_ARG.  HBM pointers, asynchronous on
+   the context stream. */
+int wam_gpu_step_adjoint_dev(wam_ctx *ctx, int32_t nmem, const float *d_fl3, const float *d_u10,
+                             const float *d_udir, const float *d_tauw, const float *d_roairn, const float *d_wstar,
+                             const float *d_fl3_bar, const float *d_tauw_bar, const float *d_ustar_bar,
+                             const float *d_z0_bar, float *d_fl3_bar_out, float *d_u10_bar, float *d_udir_bar,
+                             float *d_tauw_bar_out);
+
+/* The adjoint of PROPAGS alone (the transposed stencil of PROP_NO_CURR /
+   PROP_CURR, wam_propagation_module.f90:750-825, COUNTER sub-steps): from the
+   bar of its output d_in_bar into the bar of its input d_out_bar, NMEM
+   members.  Any configuration, on a grid whose neighbour relation is
+   symmetric. */
+int wam_gpu_propags_adjoint_dev(wam_ctx *ctx, int32_t nmem, const float *d_in_bar, float *d_out_bar);
+
+/* Wind components to speed and direction, as WAM_WIND does
+   (wam_wind_module.f90:1157-1167): U10 = SQRT(U**2 + V**2), UDIR =
+   ATAN2(U, V) (0 where U10 = 0) plus ZPI where negative; and the adjoint of
+   that map (zero where U10 = 0).  n values, HBM pointers, asynchronous. */
+int wam_gpu_wind_uv_dev(wam_ctx *ctx, const float *d_u, const float *d_v, float *d_u10, float *d_udir, int64_t n);
+int wam_gpu_wind_uv_adjoint_dev(wam_ctx *ctx, const float *d_u, const float *d_v, const float *d_u10_bar,
+                                const float *d_udir_bar, float *d_u_bar, float *d_v_bar, int64_t n);
+
+/* Checkpointing schedule for the adjoint of NSTEPS model steps with NSNAP
+   checkpoint slots besides slot 0, which holds the initial state (no
+   counterpart in the reference).  The schedule recomputes the fewest forward
+   steps (the binomial schedule, by dynamic programming); it is a list of
+   actions {op, a, b}: WAM_CK_ADVANCE runs steps a .. b-1 from the current
+   state (at step a); WAM_CK_STORE copies the current state (step a) into
+   slot b; WAM_CK_RESTORE makes slot b (step a) the current state;
+   WAM_CK_ADJOINT runs the adjoint of step a at the current state (step a),
+   after which the current state is undefined.  The first action stores step
+   0 in slot 0, the adjoint steps come in the order NSTEPS-1 .. 0.  Writes up
+   to cap actions (3 int32 each) and returns how many the schedule has, or
+   -WAM_E_ARG (NSTEPS outside 1..20000, or a table too long to build).
+   wam_checkpoint_recomputed_steps returns the forward steps ADVANCE runs. */
+enum { WAM_CK_ADVANCE = 0, WAM_CK_STORE = 1, WAM_CK_RESTORE = 2, WAM_CK_ADJOINT = 3 };
+int64_t wam_checkpoint_schedule(int32_t nsteps, int32_t nsnap, int32_t *actions, int64_t cap);
+int64_t wam_checkpoint_recomputed_steps(int32_t nsteps, int32_t nsnap);
+
 /* Stream used by the context (a hipStream_t; NULL = the context's own). */
 int wam_gpu_set_stream(wam_ctx *ctx, void *stream);
 

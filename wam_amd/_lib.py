@@ -46,6 +46,12 @@ SIGNATURES = {
     "wam_gpu_get_tuning": (_I, [_P, _P]),
     "wam_gpu_set_tuning": (_I, [_P, _P]),
     "wam_gpu_get_tuning_log": (ctypes.c_int64, [_P, _P, ctypes.c_int64]),
+    "wam_gpu_step_adjoint_dev": (_I, [_P, _I] + [_P] * 14),
+    "wam_gpu_propags_adjoint_dev": (_I, [_P, _I, _P, _P]),
+    "wam_gpu_wind_uv_dev": (_I, [_P] * 5 + [ctypes.c_int64]),
+    "wam_gpu_wind_uv_adjoint_dev": (_I, [_P] * 7 + [ctypes.c_int64]),
+    "wam_checkpoint_schedule": (ctypes.c_int64, [_I, _I, _P, ctypes.c_int64]),
+    "wam_checkpoint_recomputed_steps": (ctypes.c_int64, [_I, _I]),
     "wam_gpu_multigrid_create": (_I, [ctypes.POINTER(_P), _I, _P]),
     "wam_gpu_multigrid_add_nest": (_I, [_P, _I, _I, _I, _P, _I, _P, _P, _P, _P]),
     "wam_gpu_multigrid_run": (_I, [_P, _I, _I]),

@@ -17,8 +17,8 @@ LIB = os.path.join(HERE, "libwamgpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("WAM_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["wam_prop.hip", "wam_implsch.hip", "wam_output.hip", "wam_nest.hip", "wam_capi.hip"]
-CPP_SOURCES = ["wam_setup.cpp", "wam_restart.cpp", "wam_boundary_file.cpp", "wam_map_file.cpp"]
+HIP_SOURCES = ["wam_prop.hip", "wam_implsch.hip", "wam_adjoint.hip", "wam_output.hip", "wam_nest.hip", "wam_capi.hip"]
+CPP_SOURCES = ["wam_setup.cpp", "wam_restart.cpp", "wam_boundary_file.cpp", "wam_map_file.cpp", "wam_checkpoint.cpp"]
 HIP_FLAGS = ["-O3", f"--offload-arch={ARCH}", "-fPIC", "-std=c++17", "-ffp-contract=off",
              "-fhip-fp32-correctly-rounded-divide-sqrt", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops", "-Xarch_host", "-mfma", "-I" + CSRC]

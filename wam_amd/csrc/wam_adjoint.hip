@@ -1,0 +1,968 @@
+// wam_adjoint.hip -- the adjoint of one model step, PROPAGS then IMPLSCH
+// (wamodel.f90:332,354), on gfx950.
+//
+// The reference has no tangent-linear or adjoint model; these kernels are
+// the transpose of the linearisation of this library's forward step at a
+// given state, with every branch (MAX/MIN, the frequency cut-off MIJ, the
+// wind-sea mask LLWS, the limiter's clamp and floor, AIRSEA's iteration
+// count) taken as the forward step takes it there.
+//
+//  * implsch_adj_kernel: IMPLSCH for IPHYS = 0 in deep water.  One workgroup
+//    per (member-)point recomputes the forward step in LDS with the stages
+//    of the forward kernels (bit-identical values, so the same branches),
+//    keeping what the reverse sweep needs, then runs the reverse sweep:
+//    cut-off tail, limiter, SNONLIN, SDISSIP, STRESSO + SINPUT (second
+//    call), AIRSEA, tail, STRESSO + SINPUT (first call), mean parameters,
+//    AIRSEA.  The per-point scalar chains (AIRSEA's Newton iteration,
+//    WSIGSTAR, TAU_PHI_HF's 19 nodes) carry forward derivatives along their
+//    two inputs (dual numbers), whose local Jacobians the reverse sweep
+//    applies.  SNONLIN's quadruplet terms are reversed per quadruplet and
+//    their derivatives gathered per bin over host-built reader lists, so
+//    every sum has a fixed order and the result is deterministic.
+//  * propags_adj_kernel: the transposed stencil of PROP_NO_CURR / PROP_CURR
+//    (wam_propagation_module.f90:750-825) over the stored weight planes,
+//    as a gather: on a regular grid the neighbour relation is symmetric
+//    (KLAT_S(i) = j exactly when KLAT_N(j) = i; the host checks it), so the
+//    points whose stencil reads point j are j's own opposite neighbours.
+//  * wind_uv_kernel / wind_uv_adj_kernel: U10 = SQRT(U**2 + V**2), UDIR =
+//    ATAN2(U, V) (+ ZPI when negative), wam_wind_module.f90:1157-1167, and
+//    its adjoint.
+#include "wam_source_dev.hpp"
+
+namespace wam {
+
+// ---------------------------------------------------------------------------
+// dual numbers: a value and its derivatives along two input directions.  The
+// value part performs the forward kernels' operations in their order, so it
+// is bit-identical to them and takes the same branches.
+// ---------------------------------------------------------------------------
+struct D2 {
+    float v, a, b;
+};
+__device__ __forceinline__ D2 dc(float v) { return {v, 0.f, 0.f}; }
+__device__ __forceinline__ D2 operator+(D2 x, D2 y) { return {x.v + y.v, x.a + y.a, x.b + y.b}; }
+__device__ __forceinline__ D2 operator-(D2 x, D2 y) { return {x.v - y.v, x.a - y.a, x.b - y.b}; }
+__device__ __forceinline__ D2 operator*(D2 x, D2 y)
+{
+    return {x.v * y.v, x.a * y.v + x.v * y.a, x.b * y.v + x.v * y.b};
+}
+__device__ __forceinline__ D2 operator/(D2 x, D2 y)
+{
+    const float q = x.v / y.v;
+    return {q, (x.a - q * y.a) / y.v, (x.b - q * y.b) / y.v};
+}
+__device__ __forceinline__ D2 operator+(D2 x, float c) { return {x.v + c, x.a, x.b}; }
+__device__ __forceinline__ D2 operator+(float c, D2 x) { return {c + x.v, x.a, x.b}; }
+__device__ __forceinline__ D2 operator-(D2 x, float c) { return {x.v - c, x.a, x.b}; }
+__device__ __forceinline__ D2 operator-(float c, D2 x) { return {c - x.v, -x.a, -x.b}; }
+__device__ __forceinline__ D2 operator*(D2 x, float c) { return {x.v * c, x.a * c, x.b * c}; }
+__device__ __forceinline__ D2 operator*(float c, D2 x) { return {c * x.v, c * x.a, c * x.b}; }
+__device__ __forceinline__ D2 operator/(D2 x, float c) { return {x.v / c, x.a / c, x.b / c}; }
+__device__ __forceinline__ D2 operator/(float c, D2 y)
+{
+    const float q = c / y.v;
+    return {q, -(q * y.a) / y.v, -(q * y.b) / y.v};
+}
+__device__ __forceinline__ D2 dsqrt(D2 x)
+{
+    const float r = sqrtf(x.v), g = r > 0.f ? 0.5f / r : 0.f;
+    return {r, g * x.a, g * x.b};
+}
+__device__ __forceinline__ D2 dlog(D2 x) { return {wam_logf(x.v), x.a / x.v, x.b / x.v}; }
+__device__ __forceinline__ D2 dexp(D2 x)
+{
+    const float e = wam_expf(x.v);
+    return {e, e * x.a, e * x.b};
+}
+__device__ __forceinline__ D2 dpow(D2 x, float y)
+{
+    const float p = wam_powf(x.v, y), g = x.v != 0.f ? (y * p) / x.v : 0.f;
+    return {p, g * x.a, g * x.b};
+}
+__device__ __forceinline__ D2 dipow(D2 x, int n)
+{
+    const float g = (float)n * wam_ipowf(x.v, n - 1);
+    return {wam_ipowf(x.v, n), g * x.a, g * x.b};
+}
+// MAX and MIN as the forward kernels take them (ties select the second)
+__device__ __forceinline__ D2 dmax(D2 x, D2 y) { return x.v > y.v ? x : y; }
+__device__ __forceinline__ D2 dmin(D2 x, D2 y) { return x.v < y.v ? x : y; }
+
+// AIRSEA (wam_source_module.f90:868-971) with derivatives along (UTOP, TAUW);
+// the same operations as airsea().  ja = dUST/dUTOP, dUST/dTAUW, dZ0/dUTOP,
+// dZ0/dTAUW.
+__device__ __forceinline__ void airsea_d(const DevTables &t, float utop_v, float tauw_v, float &ust_v, float &z0_v,
+                                         float *ja)
+{
+    const D2 utop = {utop_v, 1.f, 0.f}, tauw = {tauw_v, 0.f, 1.f};
+    const float xlogxl = wam_logf(t.XNLEV), alphaog = t.ALPHA / t.G;
+    const D2 xkutop = t.XKAPPA * utop;
+    const D2 ustold = utop * dsqrt(8.0E-4f + 8.0E-5f * utop);
+    D2 tauold = dmax(ustold * ustold, tauw + t.EPS1);
+    D2 u = dsqrt(tauold);
+    D2 ustm1 = 1.0f / dmax(u, dc(1.0E-6f));
+    D2 z0ch = dc(0.f);
+    for (int it = 1; it <= 15; it++) {
+        const D2 x = tauw / tauold;
+        z0ch = (alphaog * tauold) / dsqrt(dmax(1.0f - x, dc(t.EPS1)));
+        const D2 z0vis = t.RNUAIRM * ustm1;
+        const D2 z0tot = z0ch + z0vis;
+        const D2 xologz0 = 1.0f / (xlogxl - dlog(z0tot));
+        const D2 f = u - xkutop * xologz0;
+        const D2 zz = (ustm1 * ((z0ch * (2.0f - 3.0f * x)) / (1.0f - x) - z0vis)) / z0tot;
+        const D2 delf = 1.0f - (xkutop * (xologz0 * xologz0)) * zz;
+        u = u - f / delf;
+        const D2 taunew = dmax(u * u, tauw + t.EPS1);
+        u = dsqrt(taunew);
+        if (taunew.v == tauold.v) break;
+        ustm1 = 1.0f / dmax(u, dc(1.0E-6f));
+        tauold = taunew;
+    }
+    ust_v = u.v;
+    z0_v = z0ch.v;
+    ja[0] = u.a; ja[1] = u.b; ja[2] = z0ch.a; ja[3] = z0ch.b;
+}
+
+// WSIGSTAR (:4250-4339) with derivatives along (USTAR, Z0).
+__device__ __forceinline__ D2 wsigstar_d(const DevTables &t, float ust_v, float z0_v, float wstar)
+{
+    const D2 ust = {ust_v, 1.f, 0.f}, z0 = {z0_v, 0.f, 1.f};
+    const float p1 = 1.48f, p2 = -0.21f, c1 = 1.03E-3f, c2 = 0.04E-3f;
+    D2 u10 = (ust * (1.0f / t.XKAPPA)) * (2.30258509299405f - dlog(z0));
+    u10 = dmax(u10, dc(1.0f));
+    const D2 u10m1 = 1.0f / u10;
+    const D2 c2u = c2 * dpow(u10, p1);
+    const D2 u10p2 = dpow(u10, p2);
+    const D2 c_d = (c1 + c2u) * u10p2;
+    const D2 dcddu = (((p2 * c1) + (p1 + p2) * c2u) * u10p2) * u10m1;
+    const D2 sigc = 1.0f + ((0.5f * u10) / c_d) * dcddu;
+    const D2 base = 0.0f * dipow(ust, 3) + 0.2f * wam_ipowf(wstar, 3);
+    return dmin(dc(0.1f), (sigc * u10m1) * dpow(base, 1.0f / 3.0f));
+}
+
+// TAU_PHI_HF's node j (1..19) of the stress integral (:3853-3966, XLEVTAIL =
+// 0) with derivatives along (USTAR, Z0), as the parallel form in stresso().
+__device__ __forceinline__ D2 tau_node_d(const DevTables &t, const float *fr, int mij, float ust_v, float z0_v, int j)
+{
+    const D2 ust0 = {ust_v, 1.f, 0.f}, z0 = {z0_v, 0.f, 1.f};
+    const float gm1 = 1.0f / t.G;
+    const float omegac = t.ZPI * fr[mij - 1];
+    const D2 tauw0 = ust0 * ust0;
+    const D2 xloggz0 = dlog(t.G * z0);
+    const D2 omegacc = dmax(dc(omegac), (t.X0TAUHF * t.G) / ust0);
+    const D2 sqz = dsqrt(z0 * gm1), sqg = 1.0f / sqz;
+    const D2 zinf = dlog(omegacc * sqz);
+    const D2 delz = dmax((0.0f - zinf) / 18.f, dc(0.0f));
+    const D2 ust = j == 1 ? ust0 : dsqrt(tauw0);
+    const D2 y = dexp(zinf + (float)(j - 1) * delz);
+    const D2 cm1 = (y * sqg) * gm1;
+    const D2 zx = ust * cm1 + t.ZALP;
+    const D2 zarg = t.XKAPPA / zx;
+    D2 zlog = (xloggz0 + 2.0f * dlog(cm1)) + zarg;
+    zlog = dmin(zlog, dc(0.0f));
+    const D2 zbeta = dexp(zlog) * dipow(zlog, 4);
+    return ((zbeta * tauw0) * t.WTAUHF[j - 1]) * delz;
+}
+
+// SINPUT of one bin (sin_bin's operations) with what its reverse needs: the
+// wind-input terms' ZLOG, X and EXP(ZLOG) of each active NSIN term.
+struct SinLin {
+    float sp, fl;
+    float z[2], x[2], e[2];
+    bool on[2];
+};
+
+__device__ __forceinline__ SinLin sin_bin_lin(const DevTables &t, const SinFreq &r, float tk, float const3)
+{
+    SinLin L;
+    float ufac1 = 0.f;
+    L.on[0] = L.on[1] = false;
+    if (tk > 0.01f) {
+#pragma unroll
+        for (int is = 0; is < 2; is++) {
+            const float x = tk * (is == 0 ? r.ucn1 : r.ucn2);
+            const float zlog = r.zcn + t.XKAPPA / x;
+            if (zlog < 0.f) {
+                const float z2x = (zlog * zlog) * x;
+                const float e = wam_expf(zlog);
+                ufac1 = ufac1 + ((0.5f * e) * z2x) * z2x;
+                L.on[is] = true; L.z[is] = zlog; L.x[is] = x; L.e[is] = e;
+            }
+        }
+    }
+    float ufac2 = (const3 * (tk - r.xv1)) * wam_ipowf(r.ucn1, 2);
+    ufac2 = 0.5f * ufac2 + ((0.5f * const3) * (tk - r.xv2)) * wam_ipowf(r.ucn2, 2);
+    L.sp = r.cnsn * ufac1;
+    L.fl = L.sp + r.cnsn * ufac2;
+    return L;
+}
+
+// Reverse of one bin's SINPUT given the bars of SPOS/F (spb) and FL (flb):
+// adds to the bar of COS(TH - UDIR) (dtk) and of the frequency's factors
+// dm = {ZCN, UCN1, UCN2, XV1, XV2}.  A term 0.5 EXP(Z) Z**4 X**2 with
+// Z = ZCN + XKAPPA/X has dterm/dZCN = 0.5 EXP(Z) X**2 Z**3 (Z + 4) and
+// dterm/dX = EXP(Z) Z**3 (Z X - 0.5 XKAPPA (Z + 4)).
+__device__ __forceinline__ void sin_bin_rev(const DevTables &t, const SinFreq &r, const SinLin &L, float tk,
+                                            float const3, float spb, float flb, float &dtk, float (&dm)[5])
+{
+    const float u1b = r.cnsn * (spb + flb), u2b = r.cnsn * flb;
+#pragma unroll
+    for (int is = 0; is < 2; is++) {
+        if (!L.on[is]) continue;
+        const float z = L.z[is], x = L.x[is], e = L.e[is], z3 = (z * z) * z;
+        const float dz = (((0.5f * e) * (x * x)) * z3) * (z + 4.f);
+        const float dx = (e * z3) * (z * x - (0.5f * t.XKAPPA) * (z + 4.f));
+        dm[0] += u1b * dz;
+        const float xb = u1b * dx;
+        dtk += xb * (is == 0 ? r.ucn1 : r.ucn2);
+        dm[1 + is] += xb * tk;
+    }
+    const float h = 0.5f * const3;
+    const float a1 = r.ucn1 * r.ucn1, a2 = r.ucn2 * r.ucn2;
+    dtk += u2b * (h * (a1 + a2));
+    dm[1] += u2b * ((const3 * (tk - r.xv1)) * r.ucn1);
+    dm[2] += u2b * ((const3 * (tk - r.xv2)) * r.ucn2);
+    dm[3] -= u2b * (h * a1);
+    dm[4] -= u2b * (h * a2);
+}
+
+__device__ __forceinline__ float wave_sum(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Per-point scalars of the adjoint: the forward values it linearises at,
+// the local Jacobians of the scalar chains and the running bars.
+struct AdjScalars {
+    float ja1[4], ja2[4];          // AIRSEA: d(UST, Z0)/d(U10, TAUW), first and second call
+    float js1[2], js2[2];          // WSIGSTAR: dSIG_N/d(UST, Z0)
+    float jt1[2], jt2[2];          // TAU_PHI_HF: dTAU1/d(UST, Z0)
+    float ust1, z01, sig1, ust2, z02, sig2;
+    float k1[8], k2[8];            // STRESSO's kept values, first and second call
+    float s_e, s_fm, s_tm, s_ews, s_fmws, emean, fmean, f1mean, emeanws, fmeanws;
+    int mij, mij2;
+    // bars
+    float ustb, z0b, twb, u10b, udirb;
+    float fmeanb, fmeanwsb, emeanb, f1meanb;
+    float xsb, ysb, t1b;
+    float seb, sfmb, stmb, sewsb, sfmwsb;
+};
+
+// STRESSO's stress modulus reversed to its direction sums (per bin later),
+// the cos**3 sum of F(:, MIJ) and the scalars: given the bar twb of the
+// clamped TAUW; k the kept values (stresso's keep), jt dTAU1/d(UST, Z0).
+__device__ __forceinline__ void stresso_rev_scalar(const DevTables &t, const float *k, float udir, float ust,
+                                                   float twb, const float *jt, AdjScalars &A)
+{
+    const float xs = k[0], ys = k[1], tauhf = k[2], tau1 = k[3], t1 = k[4], tw = k[5], c1 = k[6];
+    const float cap = ust * ust - t.EPS1;
+    const float tmin = wam_minf(tw, cap);
+    float tb = 0.f;
+    if (tmin > 0.f) {
+        if (tw < cap) tb = twb;
+        else A.ustb += twb * (2.f * ust);
+    }
+    float xb = 0.f, yb = 0.f;
+    if (tw > 0.f) { xb = tb * (xs / tw); yb = tb * (ys / tw); }
+    const float su = wam_sinf(udir), co = wam_cosf(udir);
+    const float hb = xb * su + yb * co;                  // bar of TAUHF = (CONST1 * T1) * TAU1
+    A.udirb += tauhf * (xb * co - yb * su);
+    A.t1b = hb * (c1 * tau1);
+    const float tau1b = hb * (c1 * t1);
+    A.ustb += tau1b * jt[0];
+    A.z0b += tau1b * jt[1];
+    const float ra = wam_maxf(t.ROAIR, 1.f);
+    A.xsb = xb / ra;
+    A.ysb = yb / ra;
+}
+
+// The frequencies' factors of SINPUT (sin_freq) reversed to USN1, USN2 and
+// Z0, then USTAR and SIG_N: thread m < ML writes its three bars over dm's
+// first rows; thread 0 then adds them up (after a barrier, by the caller).
+__device__ __forceinline__ void sin_freq_rev(const DevTables &t, const SinFreq &r, float fr_m, float ust, float sig,
+                                             float z0, float (&mb)[5][MAXF], int m)
+{
+    const float usn1 = ust * (1.0f - sig), usn2 = ust * (1.0f + sig);
+    const float cm = (t.ZPI * fr_m) / t.G;
+    const float d1 = ((usn1 / t.XKAPPA) * r.zcn) * cm, d2 = ((usn2 / t.XKAPPA) * r.zcn) * cm;
+    const float g1 = mb[3][m] / (d1 * d1), g2 = mb[4][m] / (d2 * d2);
+    const float zc = (r.zcn * cm) / t.XKAPPA;
+    const float u1 = mb[1][m] * cm + g1 * zc;
+    const float u2 = mb[2][m] * cm + g2 * zc;
+    const float zb = (mb[0][m] + g1 * ((usn1 / t.XKAPPA) * cm)) + g2 * ((usn2 / t.XKAPPA) * cm);
+    mb[0][m] = u1;
+    mb[1][m] = u2;
+    mb[2][m] = zb / z0;
+}
+
+__device__ __forceinline__ void sin_scalar_rev(const DevTables &t, const float (&mb)[5][MAXF], float ust, float sig,
+                                               const float *js, AdjScalars &A)
+{
+    float u1 = 0.f, u2 = 0.f, z = 0.f;
+    for (int m = 0; m < t.ML; m++) { u1 += mb[0][m]; u2 += mb[1][m]; z += mb[2][m]; }
+    const float sigb = ust * (u2 - u1);
+    A.ustb += (u1 * (1.0f - sig) + u2 * (1.0f + sig)) + sigb * js[0];
+    A.z0b += z + sigb * js[1];
+}
+
+__host__ __device__ __forceinline__ size_t adj_lds_floats(int KL, int ML, int MH, int CHUNK)
+{
+    return (size_t)scr_floats(KL, ML, CHUNK, 0) + 7 * (size_t)ML + 3 * (size_t)KL + 26 * (size_t)MH +
+           5 * (size_t)MH + 8 * (size_t)KL + 8 * (size_t)KL + 6 * (size_t)KL * ML;
+}
+
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) implsch_adj_kernel(DevTables t, AdjArgs a)
+{
+    extern __shared__ __align__(16) float lds_raw[];
+    __shared__ PointScalars P;
+    __shared__ AdjScalars A;
+    __shared__ float T[MAXF], T2[MAXF], cu[MAXF], sprd[MAXF], t2[MAXF], t2b[MAXF], flm[MAXF], tmp2[MAXF], sdd[MAXF];
+    __shared__ float pm[3 * MAXF + 64];
+    __shared__ SinFreq q1[MAXF], q2[MAXF];
+    __shared__ float mb[5][MAXF], kp[4][MAXF], cub[MAXF];
+    __shared__ float hd[2][19][2];
+    const int KL = t.KL, ML = t.ML, NB = KL * ML, NS = t.NSEA, tid = threadIdx.x, MH = t.MLSTHG;
+    const int wv = tid >> 6, lane = tid & 63;
+
+    Smem s;
+    float *p = lds_raw;
+    s.SCR = p;
+    p += scr_floats(KL, ML, t.CHUNK, 0);
+    s.fr = p; p += ML; s.dfim = p; p += ML; s.dfimofr = p; p += ML; s.dfim_fr = p; p += ML;
+    s.rhowg = p; p += ML; s.fr5 = p; p += ML; s.frm5 = p; p += ML;
+    s.th = p; p += KL; s.costh = p; p += KL; s.sinth = p; p += KL;
+    s.rnl = p; p += 25 * MH; s.af11 = p; p += MH;
+    s.sig10 = nullptr; s.dsii10 = nullptr;
+    s.inl = (int *)p; p += 5 * MH; s.kw = (int *)p; p += 8 * KL;
+    int *kinv = (int *)p; p += 8 * KL;
+    s.F = p; p += NB; s.SL = p; p += NB; s.FL = p; p += NB; s.SPOS = p; p += NB;
+    float *F0 = p; p += NB;
+    float *FB = p; p += NB;
+    s.SMIN = nullptr; s.SLBOT = nullptr;
+    s.LLWS = (unsigned char *)p;
+    unsigned char *LL1 = s.LLWS + NB;
+
+    for (int i = tid; i < ML; i += NT) {
+        s.fr[i] = t.FR[i]; s.dfim[i] = t.DFIM[i]; s.dfimofr[i] = t.DFIMOFR[i]; s.dfim_fr[i] = t.DFIM_FR[i];
+        s.rhowg[i] = t.RHOWG_DFIM[i]; s.fr5[i] = t.FR5[i]; s.frm5[i] = t.FRM5[i];
+    }
+    for (int i = tid; i < KL; i += NT) { s.th[i] = t.TH[i]; s.costh[i] = t.COSTH[i]; s.sinth[i] = t.SINTH[i]; }
+    for (int i = tid; i < 25 * MH; i += NT) s.rnl[i] = t.RNLCOEF[i];
+    for (int i = tid; i < 5 * MH; i += NT) s.inl[i] = t.INLCOEF[i];
+    for (int i = tid; i < MH; i += NT) s.af11[i] = t.AF11[i];
+    for (int i = tid; i < 2 * KL; i += NT) {
+        s.kw[i] = t.K1W[i]; s.kw[2 * KL + i] = t.K2W[i]; s.kw[4 * KL + i] = t.K11W[i]; s.kw[6 * KL + i] = t.K21W[i];
+    }
+    for (int i = tid; i < 8 * KL; i += NT) kinv[i] = t.AKINV[i];
+    int qslot[2];
+    for (int i = 0; i < 2; i++) {
+        const int ql = tid + i * NT, mcl = ql / (2 * KL);
+        qslot[i] = ql < 511 && mcl < t.CHUNK ? mcl | (ql - mcl * 2 * KL) << 8 : 255;
+    }
+    const float delt = t.DELT, gm1 = 1.0f / t.G;
+    const float const3 = ((2.f * t.XKAPPA) / t.BETAMAX) * (t.XKAPPA * t.XKAPPA);
+
+    const int NMP = a.NMP;
+    const int nblk = gridDim.x, xcd = blockIdx.x % 8, lb = blockIdx.x / 8;
+    const int nb_x = (nblk + 7 - xcd) / 8;
+    const int per = (NMP + 7) / 8;
+    const int p0 = xcd * per, p1 = min(NMP, p0 + per);
+    for (int mp = p0 + lb; mp < p1; mp += nb_x) {
+        const int mem = mp / NS, pt = mp - mem * NS;
+        const size_t fo = (size_t)mem * NB * NS + pt;
+        // ---------------- forward recomputation ----------------
+        for (int b = tid; b < NB; b += NT) {
+            const float f = a.fl3[fo + (size_t)b * NS];
+            s.F[b] = f;
+            F0[b] = f;
+            FB[b] = a.fl3_bar[fo + (size_t)b * NS];
+        }
+        if (tid == 0) {
+            P.u10 = a.u10[mp]; P.udir = a.udir[mp]; P.tauw = a.tauw[mp]; P.z0 = 0.f;
+            P.roairn = a.roairn[mp]; P.wstar = a.wstar[mp]; P.depth = t.DEPTH[pt];
+            P.indep = 1;
+            P.ju = min(t.JUMAX, max(wam_nint(P.u10 / t.DELU), 1));
+            A.ustb = a.ustar_bar ? a.ustar_bar[mp] : 0.f;
+            A.z0b = a.z0_bar ? a.z0_bar[mp] : 0.f;
+            A.twb = a.tauw_bar ? a.tauw_bar[mp] : 0.f;
+            A.u10b = 0.f; A.udirb = 0.f;
+        }
+        __syncthreads();
+        if (tid < KL) {
+            const float c = wam_cosf(s.th[tid] - P.udir);
+            cu[tid] = c;
+            sprd[tid] = wam_ipowf(wam_maxf(0.f, c), 2);
+        }
+        if (tid >= 64 && tid - 64 < ML) flm[tid - 64] = t.FLMINFR[(size_t)(tid - 64) * t.JUMAX + P.ju - 1];
+        if (tid == 128) {
+            airsea_d(t, P.u10, P.tauw, P.ust, P.z0, A.ja1);
+            A.ust1 = P.ust; A.z01 = P.z0;
+        }
+        dirsum(t, s.F, nullptr, T, tid);
+        __syncthreads();
+        if (tid == 0) P.s_e = integ(ML, t.MO_TAIL, T, s.dfim);
+        else if (tid == 64) P.s_fm = integ(ML, t.MM1_TAIL, T, s.dfimofr);
+        else if (tid == 192) P.s_tm = integ(ML, t.MP1_TAIL, T, s.dfim_fr);
+        __syncthreads();
+        if (tid == 0) {
+            const float e = wam_maxf(P.s_e, t.EMIN);
+            P.emean = e;
+            P.fmean = e / wam_maxf(P.s_fm, t.EMIN);
+            P.f1mean = 1.f / (e > t.EMIN ? e / P.s_tm : 1.f);
+            A.s_e = P.s_e; A.s_fm = P.s_fm; A.s_tm = P.s_tm;
+            A.emean = P.emean; A.fmean = P.fmean; A.f1mean = P.f1mean;
+        }
+        __syncthreads();
+        sinput<0>(t, s, P, cu, nullptr, q1);
+        dirsum(t, s.F, s.LLWS, T, tid);
+        for (int b = tid; b < NB; b += NT) LL1[b] = s.LLWS[b];
+        __syncthreads();
+        if (tid == 0) P.s_e = integ(ML, t.MO_TAIL, T, s.dfim);
+        else if (tid == 64) P.s_fm = integ(ML, t.MM1_TAIL, T, s.dfimofr);
+        else if (tid == 128) {
+            const D2 sg = wsigstar_d(t, A.ust1, A.z01, P.wstar);
+            A.sig1 = P.sig_n; A.js1[0] = sg.a; A.js1[1] = sg.b;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            const float e = wam_maxf(P.s_e, t.EMIN);
+            P.emeanws = e;
+            P.fmeanws = e / wam_maxf(P.s_fm, t.EMIN);
+            P.mij = frcut(t, s.fr, P.fmean, P.fmeanws, P.ust);
+            A.s_ews = P.s_e; A.s_fmws = P.s_fm; A.emeanws = P.emeanws; A.fmeanws = P.fmeanws; A.mij = P.mij;
+        }
+        __syncthreads();
+        stresso<0>(t, s, P, cu, nullptr, pm, true, A.k1);
+        if (tid == 0) airsea_d(t, P.u10, P.tauw, P.ust, P.z0, A.ja2);
+        imphftail<0>(t, s, P.mij, nullptr, nullptr, t2, 64);
+        if (tid >= 128 && tid < 128 + 19) {
+            const D2 h = tau_node_d(t, s.fr, A.mij, A.ust1, A.z01, tid - 128 + 1);
+            hd[0][tid - 128][0] = h.a; hd[0][tid - 128][1] = h.b;
+        }
+        __syncthreads();
+        if (tid >= 64) {
+            const int mij = P.mij;
+            for (int b = tid - 64 + KL * mij; b < NB; b += NT - 64) {
+                const int m = b / KL, k = b - KL * m;
+                s.F[b] = t2[m] * s.F[k + KL * (mij - 1)];
+            }
+        }
+        if (tid == 0) { A.ust2 = P.ust; A.z02 = P.z0; }
+        __syncthreads();
+        sinput<0>(t, s, P, cu, nullptr, q2);
+        stresso<0>(t, s, P, cu, nullptr, pm, true, A.k2);
+        // SDISSIP (:1593-1700) with FMEAN = F1MEAN, AKMEAN = XKMEAN (deep water)
+        if (tid < ML) {
+            const int m = tid;
+            const float e2 = P.emean * P.emean;
+            const float sds = (t.CONSD * e2) * wam_ipowf(P.f1mean, 9);
+            const float x = wam_ipowf(s.fr[m] / P.f1mean, 2);
+            sdd[m] = sds * (0.5f * x + 0.5f * wam_ipowf(x, 2));
+        } else if (tid == 64) {
+            const D2 sg = wsigstar_d(t, A.ust2, A.z02, P.wstar);
+            A.sig2 = P.sig_n; A.js2[0] = sg.a; A.js2[1] = sg.b;
+        } else if (tid >= 128 && tid < 128 + 19) {
+            const D2 h = tau_node_d(t, s.fr, A.mij, A.ust2, A.z02, tid - 128 + 1);
+            hd[1][tid - 128][0] = h.a; hd[1][tid - 128][1] = h.b;
+        }
+        __syncthreads();
+        for (int b = tid; b < NB; b += NT) {
+            const int m = b / KL;
+            s.SL[b] = s.SL[b] + sdd[m] * s.F[b];
+            s.FL[b] = s.FL[b] + sdd[m];
+        }
+        if (tid == 64) {
+            float a0 = 0.f, a1 = 0.f, c0 = 0.f, c1 = 0.f;
+            for (int j = 0; j < 19; j++) { a0 += hd[0][j][0]; a1 += hd[0][j][1]; c0 += hd[1][j][0]; c1 += hd[1][j][1]; }
+            A.jt1[0] = a0; A.jt1[1] = a1; A.jt2[0] = c0; A.jt2[1] = c1;
+        }
+        __syncthreads();
+        snonlin(t, s, 1.f, qslot);
+        if (tid < ML) {
+            const float delfl = (t.C5G / wam_ipowf(s.fr[tid], 4)) * delt;
+            tmp2[tid] = (P.ust * delfl) * wam_maxf(P.fmeanws, P.fmean);
+        }
+        __syncthreads();
+        for (int b = tid; b < NB; b += NT) {
+            const int m = b / KL, k = b - KL * m;
+            const float temp1 = (delt * s.SL[b]) / wam_maxf(1.f, 1.f - delt * s.FL[b]);
+            const float temp3 = wam_minf(wam_fabsf(temp1), tmp2[m]);
+            const float f = s.F[b] + wam_signf(temp3, temp1);
+            s.SPOS[b] = wam_maxf(f, flm[m] * sprd[k]);
+        }
+        __syncthreads();
+        dirsum(t, s.SPOS, nullptr, T, tid);
+        dirsum(t, s.SPOS, s.LLWS, T2, tid - 64);
+        __syncthreads();
+        if (tid == 0) P.s_e = integ(ML, t.MO_TAIL, T, s.dfim);
+        else if (tid == 64) P.s_fm = integ(ML, t.MM1_TAIL, T, s.dfimofr);
+        else if (tid == 128) P.s_w1 = integ(ML, t.MO_TAIL, T2, s.dfim);
+        else if (tid == 192) P.s_w2 = integ(ML, t.MM1_TAIL, T2, s.dfimofr);
+        __syncthreads();
+        if (tid == 0) {
+            const float e = wam_maxf(P.s_e, t.EMIN);
+            const float fm = e / wam_maxf(P.s_fm, t.EMIN);
+            const float ew = wam_maxf(P.s_w1, t.EMIN);
+            const float fmw = ew / wam_maxf(P.s_w2, t.EMIN);
+            A.mij2 = frcut(t, s.fr, fm, fmw, P.ust);
+        }
+        __syncthreads();
+        imphftail<0>(t, s, A.mij2, nullptr, nullptr, t2b, 0);
+        __syncthreads();
+
+        // ---------------- reverse sweep ----------------
+        // cut-off tail of the new spectrum (MIJ of the new mean parameters)
+        if (tid < KL) {
+            const int k = tid, mij2 = A.mij2;
+            float acc = 0.f;
+            for (int m = mij2; m < ML; m++) {
+                acc = acc + t2b[m] * FB[k + KL * m];
+                FB[k + KL * m] = 0.f;
+            }
+            FB[k + KL * (mij2 - 1)] += acc;
+        }
+        __syncthreads();
+        // limiter and implicit update: FB <- bar of F, SL <- bar of SL, FL <- bar of FL
+        {
+            float kacc = 0.f;
+            for (int m = wv; m < ML; m += 4) {
+                float t2s = 0.f;
+                if (lane < KL) {
+                    const int k = lane, b = k + KL * m;
+                    const float fnb = FB[b];
+                    const float y = 1.f - delt * s.FL[b];
+                    const float den = wam_maxf(1.f, y);
+                    const float temp1 = (delt * s.SL[b]) / den;
+                    const float a1 = wam_fabsf(temp1);
+                    const float temp3 = wam_minf(a1, tmp2[m]);
+                    const float f = s.F[b] + wam_signf(temp3, temp1);
+                    float t1b = 0.f, ftb = 0.f;
+                    if (f > flm[m] * sprd[k]) {
+                        ftb = fnb;
+                        if (a1 < tmp2[m]) t1b = fnb;
+                        else t2s = temp1 >= 0.f ? fnb : -fnb;
+                    } else if (cu[k] > 0.f) {
+                        kacc += fnb * (flm[m] * (2.f * cu[k]));
+                    }
+                    FB[b] = ftb;
+                    s.SL[b] = (t1b * delt) / den;
+                    s.FL[b] = !(1.f > y) ? ((t1b * temp1) * delt) / den : 0.f;
+                }
+                t2s = wave_sum(t2s);
+                if (lane == 0) mb[0][m] = t2s;
+            }
+            if (lane < KL) kp[wv][lane] = kacc;
+        }
+        __syncthreads();
+        if (tid < KL) cub[tid] = ((kp[0][tid] + kp[1][tid]) + kp[2][tid]) + kp[3][tid];
+        if (tid == 64) {
+            const float fmx = wam_maxf(A.fmeanws, A.fmean);
+            float ub = 0.f, fb = 0.f;
+            for (int m = 0; m < ML; m++) {
+                const float delfl = (t.C5G / wam_ipowf(s.fr[m], 4)) * delt;
+                ub += mb[0][m] * (delfl * fmx);
+                fb += mb[0][m] * (A.ust2 * delfl);
+            }
+            A.ustb += ub;
+            const bool ws = A.fmeanws > A.fmean;
+            A.fmeanwsb = ws ? fb : 0.f;
+            A.fmeanb = ws ? 0.f : fb;
+        }
+        __syncthreads();
+        // SNONLIN: per chunk of frequencies, every quadruplet's bars of FIJ,
+        // SAP and SAM from the bars of its nine updates, then every bin
+        // gathers them over the quadruplets that read it
+        {
+            float *QF = s.SCR, *QP = s.SCR + 512, *QM = s.SCR + 1024;
+            const float dal1 = t.DAL1, dal2 = t.DAL2;
+            for (int c = 0; c < t.ANCH; c++) {
+                const int mc0 = c * t.ACH, nmc = min(t.ACH, MH - mc0), nq = nmc * 2 * KL;
+                for (int ql = tid; ql < nq; ql += NT) {
+                    const int mcl = ql / (2 * KL), r = ql - mcl * 2 * KL, mcm1 = mc0 + mcl;
+                    const int K = r < KL ? r : r - KL;
+                    const int *ic = s.inl + mcm1 * 5;
+                    const float *rc = s.rnl + mcm1 * 25;
+                    const int k1 = s.kw[r], k2 = s.kw[2 * KL + r], k11 = s.kw[4 * KL + r], k21 = s.kw[6 * KL + r];
+                    const int IC = (ic[0] - 1) * KL, IP = (ic[1] - 1) * KL, IP1 = (ic[2] - 1) * KL;
+                    const int IM = (ic[3] - 1) * KL, IM1 = (ic[4] - 1) * KL;
+                    const float *F = s.F;
+                    const float sap = rc[1] * F[k1 + IP] + rc[2] * F[k11 + IP] + rc[3] * F[k1 + IP1] + rc[4] * F[k11 + IP1];
+                    const float sam = rc[13] * F[k2 + IM] + rc[14] * F[k21 + IM] + rc[15] * F[k2 + IM1] + rc[16] * F[k21 + IM1];
+                    const float fij = F[K + IC] * rc[0];
+                    const float ftemp = s.af11[mcm1];
+                    const float fcen = ftemp * fij;
+                    const float fad1 = fij * (sap + sam), fad2 = fad1 - (2.f * sap) * sam;
+                    const int8_t *tr = t.AQ_ROW + 9 * mcm1;
+                    float adb = 0.f, ddb = 0.f, dmb = 0.f, dpb = 0.f;
+                    if (tr[0] >= 0) {
+                        const int b = K + KL * tr[0];
+                        adb -= 2.f * s.SL[b];
+                        ddb -= 2.f * s.FL[b];
+                    }
+#pragma unroll
+                    for (int ty = 1; ty <= 8; ty++) {
+                        if (tr[ty] < 0) continue;
+                        const int kk = ty <= 4 ? ((ty & 1) ? k2 : k21) : ((ty & 1) ? k1 : k11);
+                        const int b = kk + KL * tr[ty];
+                        adb += rc[SLOT_SL[ty]] * s.SL[b];
+                        const float d = rc[SLOT_FL[ty]] * s.FL[b];
+                        if (ty <= 4) dmb += d;
+                        else dpb += d;
+                    }
+                    // AD = FAD2*FCEN, DELAD = (FAD1 + FAD2)*FTEMP,
+                    // DELAP = ((FIJ - 2 SAM)*DAL1)*FCEN, DELAM = ((FIJ - 2 SAP)*DAL2)*FCEN
+                    float fad2b = adb * fcen, fcenb = adb * fad2;
+                    const float fad1pb = ddb * ftemp;
+                    fcenb += dpb * ((fij - 2.f * sam) * dal1) + dmb * ((fij - 2.f * sap) * dal2);
+                    float fijb = dpb * (dal1 * fcen) + dmb * (dal2 * fcen);
+                    float samb = -2.f * (dpb * (dal1 * fcen)), sapb = -2.f * (dmb * (dal2 * fcen));
+                    fijb += fcenb * ftemp;
+                    float fad1b = fad1pb;
+                    fad2b += fad1pb;
+                    fad1b += fad2b;
+                    sapb -= (2.f * sam) * fad2b;
+                    samb -= (2.f * sap) * fad2b;
+                    fijb += fad1b * (sap + sam);
+                    sapb += fad1b * fij;
+                    samb += fad1b * fij;
+                    QF[ql] = fijb; QP[ql] = sapb; QM[ql] = samb;
+                }
+                __syncthreads();
+                for (int m = wv; m < ML; m += 4) {
+                    const int e0 = t.AR_OFF[c * ML + m], e1 = t.AR_OFF[c * ML + m + 1];
+                    if (lane < KL && e1 > e0) {
+                        float acc = 0.f;
+                        for (int e = e0; e < e1; e++) {
+                            const int v = t.AR_E[e], mcm1 = v >> 4, sl = v & 15;
+                            const float w = s.rnl[mcm1 * 25 + (sl <= 4 ? sl : sl + 8)];
+                            const float *X = sl == 0 ? QF : (sl <= 4 ? QP : QM);
+                            const int mpk = sl <= 4 ? ((sl - 1) & 1) : 2 + ((sl - 5) & 1);
+                            const int base = (mcm1 - mc0) * 2 * KL;
+#pragma unroll
+                            for (int kh = 0; kh < 2; kh++) {
+                                const int K = sl == 0 ? lane : kinv[(mpk * 2 + kh) * KL + lane];
+                                acc = acc + w * X[base + kh * KL + K];
+                            }
+                        }
+                        FB[lane + KL * m] += acc;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // SDISSIP: bar of its per-frequency factor, then EMEAN and F1MEAN
+        for (int m = wv; m < ML; m += 4) {
+            float d = 0.f;
+            if (lane < KL) {
+                const int b = lane + KL * m;
+                FB[b] += s.SL[b] * sdd[m];
+                d = s.SL[b] * s.F[b] + s.FL[b];
+            }
+            d = wave_sum(d);
+            if (lane == 0) T2[m] = d;
+        }
+        if (tid == 64) stresso_rev_scalar(t, A.k2, P.udir, A.ust2, A.twb, A.jt2, A);
+        __syncthreads();
+        if (tid == 0) {
+            const float e = A.emean, f1 = A.f1mean;
+            const float sds = (t.CONSD * (e * e)) * wam_ipowf(f1, 9);
+            float sdsb = 0.f, f1b = 0.f;
+            for (int m = 0; m < ML; m++) {
+                const float x = wam_ipowf(s.fr[m] / f1, 2);
+                sdsb += T2[m] * (0.5f * x + 0.5f * (x * x));
+                const float xb = T2[m] * (sds * (0.5f + x));
+                f1b += xb * ((-2.f * x) / f1);
+            }
+            A.emeanb = sdsb * ((t.CONSD * wam_ipowf(f1, 9)) * (2.f * e));
+            A.f1meanb = f1b + sdsb * ((t.CONSD * (e * e)) * (9.f * wam_ipowf(f1, 8)));
+        }
+        // SINPUT and STRESSO, second call: bins
+        {
+            const int mij = A.mij;
+            float kacc = 0.f;
+            for (int m = wv; m < ML; m += 4) {
+                float dm[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+                if (lane < KL) {
+                    const int k = lane, b = k + KL * m;
+                    const float ft = s.F[b], slb = s.SL[b], flb = s.FL[b];
+                    const SinLin L = sin_bin_lin(t, q2[m], cu[k], const3);
+                    float sposb = 0.f;
+                    if (m < mij && L.sp * ft > 0.f) {
+                        float rw = s.rhowg[m];
+                        if (m == mij - 1 && mij != ML) rw = 0.5f * rw;
+                        const float cmrw = ((t.ZPI * s.fr[m]) * gm1) * rw;
+                        sposb = (A.xsb * cmrw) * s.sinth[k] + (A.ysb * cmrw) * s.costh[k];
+                    }
+                    float fbadd = slb * L.fl + sposb * L.sp;
+                    float dtk = 0.f;
+                    sin_bin_rev(t, q2[m], L, cu[k], const3, sposb * ft, slb * ft + flb, dtk, dm);
+                    if (m == mij - 1 && cu[k] > 0.f) {
+                        const float c = cu[k];
+                        fbadd += A.t1b * ((c * c) * c);
+                        dtk += (A.t1b * ft) * (3.f * (c * c));
+                    }
+                    FB[b] += fbadd;
+                    kacc += dtk;
+                }
+#pragma unroll
+                for (int i = 0; i < 5; i++) {
+                    const float v = wave_sum(dm[i]);
+                    if (lane == 0) mb[i][m] = v;
+                }
+            }
+            if (lane < KL) kp[wv][lane] = kacc;
+        }
+        __syncthreads();
+        if (tid < KL) cub[tid] += ((kp[0][tid] + kp[1][tid]) + kp[2][tid]) + kp[3][tid];
+        if (tid >= 64 && tid - 64 < ML)
+            sin_freq_rev(t, q2[tid - 64], s.fr[tid - 64], A.ust2, A.sig2, A.z02, mb, tid - 64);
+        __syncthreads();
+        if (tid == 0) {
+            sin_scalar_rev(t, mb, A.ust2, A.sig2, A.js2, A);
+            // AIRSEA, second call: (UST, Z0) from (U10, TAUW of the first STRESSO)
+            A.u10b += A.ustb * A.ja2[0] + A.z0b * A.ja2[2];
+            A.twb = A.ustb * A.ja2[1] + A.z0b * A.ja2[3];
+            A.ustb = 0.f;
+            A.z0b = 0.f;
+        }
+        // cut-off tail of the first pass (MIJ)
+        if (tid >= 64 && tid - 64 < KL) {
+            const int k = tid - 64, mij = A.mij;
+            float acc = 0.f;
+            for (int m = mij; m < ML; m++) {
+                acc = acc + t2[m] * FB[k + KL * m];
+                FB[k + KL * m] = 0.f;
+            }
+            FB[k + KL * (mij - 1)] += acc;
+        }
+        __syncthreads();
+        if (tid == 0) stresso_rev_scalar(t, A.k1, P.udir, A.ust1, A.twb, A.jt1, A);
+        if (tid == 64) {
+            // mean parameters of the input spectrum: EMEAN, FMEAN (through
+            // MAX(FMEANWS, FMEAN)), F1MEAN; and FMEANWS of the first LLWS
+            const float e = A.emean;
+            const float dfm = wam_maxf(A.s_fm, t.EMIN);
+            float eb = A.emeanb + A.fmeanb / dfm, stmb = 0.f;
+            if (e > t.EMIN) {
+                const float q = e / A.s_tm;
+                const float qb = -A.f1meanb / (q * q);
+                eb += qb / A.s_tm;
+                stmb = -(qb * q) / A.s_tm;
+            }
+            A.sfmb = A.s_fm > t.EMIN ? -(A.fmeanb * A.fmean) / dfm : 0.f;
+            A.seb = A.s_e > t.EMIN ? eb : 0.f;
+            A.stmb = stmb;
+            const float dws = wam_maxf(A.s_fmws, t.EMIN);
+            A.sewsb = A.s_ews > t.EMIN ? A.fmeanwsb / dws : 0.f;
+            A.sfmwsb = A.s_fmws > t.EMIN ? -(A.fmeanwsb * A.fmeanws) / dws : 0.f;
+        }
+        __syncthreads();
+        // SINPUT and STRESSO, first call, and the mean parameters: bins
+        {
+            const int mij = A.mij;
+            float kacc = 0.f;
+            for (int m = wv; m < ML; m += 4) {
+                float dm[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+                if (lane < KL) {
+                    const int k = lane, b = k + KL * m;
+                    const float f0 = F0[b];
+                    const SinLin L = sin_bin_lin(t, q1[m], cu[k], const3);
+                    float sposb = 0.f;
+                    if (m < mij && L.sp * f0 > 0.f) {
+                        float rw = s.rhowg[m];
+                        if (m == mij - 1 && mij != ML) rw = 0.5f * rw;
+                        const float cmrw = ((t.ZPI * s.fr[m]) * gm1) * rw;
+                        sposb = (A.xsb * cmrw) * s.sinth[k] + (A.ysb * cmrw) * s.costh[k];
+                    }
+                    const bool last = m == ML - 1;
+                    const float tb = (A.seb * (s.dfim[m] + (last ? t.MO_TAIL : 0.f)) +
+                                      A.sfmb * (s.dfimofr[m] + (last ? t.MM1_TAIL : 0.f))) +
+                                     A.stmb * (s.dfim_fr[m] + (last ? t.MP1_TAIL : 0.f));
+                    float fbadd = sposb * L.sp + tb;
+                    if (LL1[b])
+                        fbadd += A.sewsb * (s.dfim[m] + (last ? t.MO_TAIL : 0.f)) +
+                                 A.sfmwsb * (s.dfimofr[m] + (last ? t.MM1_TAIL : 0.f));
+                    float dtk = 0.f;
+                    sin_bin_rev(t, q1[m], L, cu[k], const3, sposb * f0, 0.f, dtk, dm);
+                    if (m == mij - 1 && cu[k] > 0.f) {
+                        const float c = cu[k];
+                        fbadd += A.t1b * ((c * c) * c);
+                        dtk += (A.t1b * f0) * (3.f * (c * c));
+                    }
+                    FB[b] += fbadd;
+                    kacc += dtk;
+                }
+#pragma unroll
+                for (int i = 0; i < 5; i++) {
+                    const float v = wave_sum(dm[i]);
+                    if (lane == 0) mb[i][m] = v;
+                }
+            }
+            if (lane < KL) kp[wv][lane] = kacc;
+        }
+        __syncthreads();
+        if (tid < KL) cub[tid] += ((kp[0][tid] + kp[1][tid]) + kp[2][tid]) + kp[3][tid];
+        if (tid >= 64 && tid - 64 < ML)
+            sin_freq_rev(t, q1[tid - 64], s.fr[tid - 64], A.ust1, A.sig1, A.z01, mb, tid - 64);
+        __syncthreads();
+        if (tid == 0) {
+            sin_scalar_rev(t, mb, A.ust1, A.sig1, A.js1, A);
+            // AIRSEA, first call: (UST, Z0) from (U10, TAUW at the step's start)
+            A.u10b += A.ustb * A.ja1[0] + A.z0b * A.ja1[2];
+            const float tw0b = A.ustb * A.ja1[1] + A.z0b * A.ja1[3];
+            // UDIR through COS(TH(K) - UDIR): d/dUDIR = SIN(TH(K) - UDIR)
+            float ub = A.udirb;
+            for (int k = 0; k < KL; k++) ub += cub[k] * wam_sinf(s.th[k] - P.udir);
+            a.u10_bar[mp] = A.u10b;
+            a.udir_bar[mp] = ub;
+            a.tauw_bar_out[mp] = tw0b;
+        }
+        for (int b = tid; b < NB; b += NT) a.fl3_bar_out[fo + (size_t)b * NS] = FB[b];
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PROPAGS transposed.  Input j of row (M, K) receives, from every output i
+// whose stencil reads it, that output's weight times its bar.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int adj_chunk(int nchunks)
+{
+    const int per = (nchunks + 7) / 8;
+    return (blockIdx.x % 8) * per + blockIdx.x / 8;
+}
+
+template <bool CURR>
+__global__ void __launch_bounds__(256) propags_adj_kernel(DevTables t, const float *__restrict__ coef,
+                                                           const float *__restrict__ gin, float *__restrict__ gout)
+{
+    const size_t moff = (size_t)blockIdx.y * t.KL * t.ML * t.NSEA;
+    gin += moff;
+    gout += moff;
+    const int wg = blockDim.x, nch = (t.NSEA + wg - 1) / wg;
+    const int ch = adj_chunk(nch);
+    if (ch >= nch) return;
+    const int j = ch * wg + threadIdx.x;
+    if (j >= t.NSEA) return;
+    const int KL = t.KL, ML = t.ML, NS = t.NSEA;
+    const size_t NB = (size_t)KL * ML * NS;
+    const int ns = t.KLAT_S[j], nn = t.KLAT_N[j], nw = t.KLON_W[j], ne = t.KLON_E[j];
+    auto term = [&](int plane, size_t row, int i) -> float {
+        return i >= 0 ? coef[plane * NB + row + i] * gin[row + i] : 0.f;
+    };
+    for (int m = 0; m < ML; m++) {
+        for (int k = 0; k < KL; k++) {
+            const size_t row = ((size_t)m * KL + k) * NS;
+            const size_t rkm = ((size_t)m * KL + (t.KPM_M[k] - 1)) * NS;   // output K-1 reads K through TH1
+            const size_t rkp = ((size_t)m * KL + (t.KPM_P[k] - 1)) * NS;   // output K+1 reads K through TH2
+            float g = coef[P_SUM * NB + row + j] * gin[row + j];
+            if (!CURR) {
+                // PROP_NO_CURR reads KLAT_S when JYO1 = 1 (else KLAT_N) and
+                // KLON_W when JXO1 = 1 (else KLON_E)
+                g = g + term(1, row, t.JYO1[k] == 1 ? nn : ns);
+                g = g + term(2, row, t.JXO1[k] == 1 ? ne : nw);
+                g = g + term(3, rkm, j);
+                g = g + term(4, rkp, j);
+            } else {
+                g = g + term(P_LAT2, row, ns);     // output KLAT_S(j) reads its north neighbour j
+                g = g + term(P_LAT1, row, nn);
+                g = g + term(P_LON2, row, nw);
+                g = g + term(P_LON1, row, ne);
+                g = g + term(P_TH1, rkm, j);
+                g = g + term(P_TH2, rkp, j);
+                // SIG1 of output M' reads MIN(M'+1, ML); SIG2 reads MAX(M'-1, 1)
+                if (m > 0) g = g + term(P_SIG1, ((size_t)(m - 1) * KL + k) * NS, j);
+                if (m == ML - 1) g = g + term(P_SIG1, row, j);
+                if (m < ML - 1) g = g + term(P_SIG2, ((size_t)(m + 1) * KL + k) * NS, j);
+                if (m == 0) g = g + term(P_SIG2, row, j);
+            }
+            gout[row + j] = g;
+        }
+    }
+}
+
+__global__ void wind_uv_kernel(float zpi, const float *__restrict__ u, const float *__restrict__ v,
+                               float *__restrict__ u10, float *__restrict__ udir, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float uu = u[i], vv = v[i];
+    const float s = sqrtf(uu * uu + vv * vv);
+    float d = s != 0.f ? wam_atan2f(uu, vv) : 0.f;
+    if (d < 0.f) d = d + zpi;
+    u10[i] = s;
+    udir[i] = d;
+}
+
+__global__ void wind_uv_adj_kernel(const float *__restrict__ u, const float *__restrict__ v,
+                                   const float *__restrict__ u10b, const float *__restrict__ udirb,
+                                   float *__restrict__ ub, float *__restrict__ vb, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float uu = u[i], vv = v[i];
+    const float r2 = uu * uu + vv * vv, s = sqrtf(r2);
+    float gu = 0.f, gv = 0.f;
+    if (s != 0.f) {
+        // dU10/du = u/U10, dUDIR/du = v/(u**2 + v**2), dUDIR/dv = -u/(u**2 + v**2)
+        const float a = u10b[i], d = udirb[i];
+        gu = a * (uu / s) + d * (vv / r2);
+        gv = a * (vv / s) - d * (uu / r2);
+    }
+    ub[i] = gu;
+    vb[i] = gv;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+size_t implsch_adj_lds_bytes(const DevTables &t)
+{
+    return adj_lds_floats(t.KL, t.ML, t.MLSTHG, t.CHUNK) * sizeof(float) + 2 * (size_t)t.KL * t.ML;
+}
+
+hipError_t launch_implsch_adjoint(const DevTables &t, const AdjArgs &a, int ncu, hipStream_t st)
+{
+    const size_t lds = implsch_adj_lds_bytes(t);
+    hipError_t e = hipFuncSetAttribute((const void *)implsch_adj_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e != hipSuccess) return e;
+    int per_cu = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, implsch_adj_kernel, NT, lds);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) return hipErrorInvalidConfiguration;
+    const long nblocks = std::min<long>((long)per_cu * ncu, a.NMP);
+    hipLaunchKernelGGL(implsch_adj_kernel, dim3((unsigned)nblocks), dim3(NT), lds, st, t, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_propags_adjoint(const DevTables &t, const float *coef, int nmem, const float *gin, float *gout,
+                                  hipStream_t st)
+{
+    const int wg = 256, nch = (t.NSEA + wg - 1) / wg;
+    const dim3 grid(((nch + 7) / 8) * 8, nmem), block(wg);
+    if (t.refr_c) hipLaunchKernelGGL(propags_adj_kernel<true>, grid, block, 0, st, t, coef, gin, gout);
+    else hipLaunchKernelGGL(propags_adj_kernel<false>, grid, block, 0, st, t, coef, gin, gout);
+    return hipGetLastError();
+}
+
+hipError_t launch_wind_uv(float zpi, const float *u, const float *v, float *u10, float *udir, int64_t n,
+                          hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(wind_uv_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, zpi, u, v, u10, udir, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_wind_uv_adjoint(const float *u, const float *v, const float *u10b, const float *udirb, float *ub,
+                                  float *vb, int64_t n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(wind_uv_adj_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, u, v, u10b, udirb,
+                       ub, vb, n);
+    return hipGetLastError();
+}
+
+}  // namespace wam

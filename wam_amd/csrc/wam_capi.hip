@@ -39,6 +39,13 @@ hipError_t launch_implsch(const DevTables &t, const ImplArgs &a, int ncu, hipStr
 hipError_t launch_implsch_split(const DevTables &t, const ImplArgs &a, const ImplScratch &x, int ncu, hipStream_t st,
                                 const hipEvent_t *ev, const LaunchTune &tu);
 bool implsch_split_supported(const DevTables &t);
+hipError_t launch_implsch_adjoint(const DevTables &t, const AdjArgs &a, int ncu, hipStream_t st);
+hipError_t launch_propags_adjoint(const DevTables &t, const float *coef, int nmem, const float *gin, float *gout,
+                                  hipStream_t st);
+hipError_t launch_wind_uv(float zpi, const float *u, const float *v, float *u10, float *udir, int64_t n,
+                          hipStream_t st);
+hipError_t launch_wind_uv_adjoint(const float *u, const float *v, const float *u10b, const float *udirb, float *ub,
+                                  float *vb, int64_t n, hipStream_t st);
 hipError_t launch_output_params(const DevTables &t, const float *fl3, const float *u10, const float *udir,
                                 const float *ustar, const float *tauw, const float *z0, int orient, float *out,
                                 hipStream_t st);
@@ -132,6 +139,14 @@ struct wam_ctx {
     std::vector<float> depth_h;           // the DEPTH of the last PREPARE_PROPAGATION (host copy)
     bool prepared = false;
     void *noalloc = nullptr;              // this device's copy of scratch (wam_gpu_implsch_dev_noalloc)
+    // the step adjoint (wam_gpu_step_adjoint_dev): IMPLSCH's input spectra
+    // and their bars for adj_n values, the weight planes where PROPAGS
+    // computes its weights in the stencil (a point allocation), and whether
+    // the grid's neighbour relation is symmetric (-1: not yet checked)
+    float *adj_fp = nullptr, *adj_fb = nullptr;
+    size_t adj_n = 0;
+    float *adj_coef = nullptr;
+    int adj_sym = -1;
     int ncu = 256;                 // compute units of the device
     std::string err;
 };
@@ -247,6 +262,8 @@ static int upload_static(wam_ctx *ctx)
     UP(o, t.SN_E, SN_E); UP(o, t.SN_CH, SN_CH); UP(o, t.SN_WW, SN_WW);
     d.SP_CHUNK = t.SP_CHUNK; d.SP_NCHUNK = t.SP_NCHUNK; d.SP_NRUN = t.SP_NRUN; d.SP_PAD = t.SP_PAD;
     UP(o, t.SP_E, SP_E); UP(o, t.SP_RW, SP_RW);
+    d.ACH = t.ACH; d.ANCH = t.ANCH;
+    UP(o, t.AQ_ROW, AQ_ROW); UP(o, t.AR_E, AR_E); UP(o, t.AR_OFF, AR_OFF); UP(o, t.AKINV, AKINV);
     std::vector<int> ks = zero_based(t.KLAT_S), kn = zero_based(t.KLAT_N), kw = zero_based(t.KLON_W), ke = zero_based(t.KLON_E);
     UP(o, ks, KLAT_S); UP(o, kn, KLAT_N); UP(o, kw, KLON_W); UP(o, ke, KLON_E);
     UP(o, t.DPSN_S, DPSN_S); UP(o, t.DPSN_N, DPSN_N);
@@ -426,6 +443,7 @@ int wam_gpu_prepare_propagation(wam_ctx *ctx, const float *depth, const float *u
     if (rc) return rc;
     free_list(ctx->point_allocs);
     ctx->coef = nullptr; ctx->wn = nullptr; ctx->cgg = nullptr; ctx->dt.SBOT = nullptr;
+    ctx->adj_coef = nullptr;
     const int ns = t.NSEA;
     auto &o = ctx->point_allocs;
     std::vector<float> d(depth, depth + ns), uu(ns, 0.f), vv(ns, 0.f);
@@ -447,6 +465,7 @@ int wam_gpu_prepare_propagation_dev(wam_ctx *ctx, const float *d_depth, const fl
     if (t.cfg.refr_c && (!d_u || !d_v)) { ctx->err = "current refraction needs u and v"; return WAM_E_ARG; }
     free_list(ctx->point_allocs);
     ctx->coef = nullptr; ctx->wn = nullptr; ctx->cgg = nullptr; ctx->dt.SBOT = nullptr;
+    ctx->adj_coef = nullptr;
     const int ns = t.NSEA;
     auto &o = ctx->point_allocs;
     hipStream_t st = ctx->stream;
@@ -1819,6 +1838,166 @@ int wam_gpu_get_timing(wam_ctx *ctx, const char *name, double *total_ms, int64_t
     return WAM_OK;
 }
 
+// The grid's neighbour relation is symmetric (KLAT_S(i) = j exactly when
+// KLAT_N(j) = i, KLON_W(i) = j exactly when KLON_E(j) = i): the transposed
+// PROPAGS gathers over it.
+static bool neighbours_symmetric(const Tables &t)
+{
+    for (int j = 0; j < t.NSEA; j++) {
+        const int s = t.KLAT_S[j] - 1, n = t.KLAT_N[j] - 1, w = t.KLON_W[j] - 1, e = t.KLON_E[j] - 1;
+        if (s >= 0 && t.KLAT_N[s] - 1 != j) return false;
+        if (n >= 0 && t.KLAT_S[n] - 1 != j) return false;
+        if (w >= 0 && t.KLON_E[w] - 1 != j) return false;
+        if (e >= 0 && t.KLON_W[e] - 1 != j) return false;
+    }
+    return true;
+}
+
+static int adjoint_planes(wam_ctx *ctx);
+
+// Checks and work arrays of the step adjoint for nmem members.
+static int adjoint_ready(wam_ctx *ctx, int32_t nmem)
+{
+    int rc = ensemble_args_ok(ctx, nmem);
+    if (rc) return rc;
+    const Tables &t = ctx->tab;
+    if (t.cfg.iphys != 0 || t.cfg.shallow) {
+        ctx->err = "the step adjoint is built for IPHYS = 0 in deep water";
+        return WAM_E_ARG;
+    }
+    if (t.KL > 64 || t.ML > 64) { ctx->err = "the step adjoint needs KL, ML <= 64"; return WAM_E_ARG; }
+    if (!t.adj_err.empty()) { ctx->err = t.adj_err; return WAM_E_ARG; }
+    if (ctx->adj_sym < 0) ctx->adj_sym = neighbours_symmetric(t) ? 1 : 0;
+    if (!ctx->adj_sym) { ctx->err = "the transposed PROPAGS needs a symmetric neighbour relation"; return WAM_E_ARG; }
+    const size_t nb = (size_t)t.KL * t.ML * t.NSEA, n = (size_t)nmem * nb;
+    if (ctx->adj_n < n) {
+        if (ctx->adj_fp) (void)hipFree(ctx->adj_fp);
+        if (ctx->adj_fb) (void)hipFree(ctx->adj_fb);
+        ctx->adj_fp = ctx->adj_fb = nullptr;
+        ctx->adj_n = 0;
+        HIPCHK(hipMalloc(&ctx->adj_fp, n * sizeof(float)));
+        HIPCHK(hipMalloc(&ctx->adj_fb, n * sizeof(float)));
+        ctx->adj_n = n;
+    }
+    rc = adjoint_planes(ctx);
+    if (rc) return rc;
+    return ensure_ensemble_fscratch(ctx, nmem);
+}
+
+// The weight planes the transposed stencil reads: PROPAGS's own, or where
+// it computes its weights in the stencil, planes made for the adjoint (as
+// CHECK_CFL scaled them).
+static int adjoint_planes(wam_ctx *ctx)
+{
+    const Tables &t = ctx->tab;
+    const size_t nb = (size_t)t.KL * t.ML * t.NSEA;
+    if (!ctx->coef && !ctx->adj_coef) {
+        void *p = nullptr;
+        HIPCHK(hipMalloc(&p, nb * ctx->ncoef * sizeof(float)));
+        ctx->point_allocs.push_back(p);
+        ctx->adj_coef = (float *)p;
+        hipLaunchKernelGGL(prep_coef_kernel, dim3((t.NSEA + 255) / 256, t.KL * t.ML), dim3(256), 0, ctx->stream,
+                           ctx->dt, ctx->adj_coef);
+        HIPCHK(hipGetLastError());
+        if (ctx->counter > 1) {
+            hipLaunchKernelGGL(rescale_kernel, dim3(4096), dim3(256), 0, ctx->stream, ctx->adj_coef, nb, ctx->ncoef,
+                               (float)ctx->counter);
+            HIPCHK(hipGetLastError());
+        }
+    }
+    return WAM_OK;
+}
+
+// PROPAGS transposed: COUNTER sub-steps from src, the last into dst, the
+// others alternating between the two scratch arrays (src may be one of them).
+static int propags_transposed(wam_ctx *ctx, int nmem, const float *src, float *dst, float *sa, float *sb)
+{
+    const float *coef = ctx->coef ? ctx->coef : ctx->adj_coef;
+    for (int i = 0; i < ctx->counter; i++) {
+        const int left = ctx->counter - 1 - i;
+        float *to = left % 2 == 0 ? dst : (src == sa ? sb : sa);
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        const int rc = timing_events(ctx, ev, 2);
+        if (rc) return rc;
+        timing_record(ctx, ev[0]);
+        HIPCHK(launch_propags_adjoint(ctx->dt, coef, nmem, src, to, ctx->stream));
+        timing_record(ctx, ev[1]);
+        timing_keep(ctx, "propags_adjoint", ev[0], ev[1]);
+        src = to;
+    }
+    return WAM_OK;
+}
+
+int wam_gpu_step_adjoint_dev(wam_ctx *ctx, int32_t nmem, const float *d_fl3, const float *d_u10,
+                             const float *d_udir, const float *d_tauw, const float *d_roairn, const float *d_wstar,
+                             const float *d_fl3_bar, const float *d_tauw_bar, const float *d_ustar_bar,
+                             const float *d_z0_bar, float *d_fl3_bar_out, float *d_u10_bar, float *d_udir_bar,
+                             float *d_tauw_bar_out)
+{
+    if (!d_fl3 || !d_u10 || !d_udir || !d_tauw || !d_roairn || !d_wstar || !d_fl3_bar || !d_fl3_bar_out ||
+        !d_u10_bar || !d_udir_bar || !d_tauw_bar_out || d_fl3_bar_out == d_fl3)
+        return WAM_E_ARG;
+    int rc = adjoint_ready(ctx, nmem);
+    if (rc) return rc;
+    const Tables &t = ctx->tab;
+    // 1. IMPLSCH's input: PROPAGS of the step's input state
+    rc = propags_members(ctx, nmem, d_fl3, ctx->adj_fp, ctx->fscratch_e, true);
+    if (rc) return rc;
+    // 2. IMPLSCH's adjoint, the input spectrum's bar into adj_fb
+    AdjArgs a{};
+    a.fl3 = ctx->adj_fp; a.u10 = d_u10; a.udir = d_udir; a.tauw = d_tauw; a.roairn = d_roairn; a.wstar = d_wstar;
+    a.fl3_bar = d_fl3_bar; a.tauw_bar = d_tauw_bar; a.ustar_bar = d_ustar_bar; a.z0_bar = d_z0_bar;
+    a.fl3_bar_out = ctx->adj_fb; a.u10_bar = d_u10_bar; a.udir_bar = d_udir_bar; a.tauw_bar_out = d_tauw_bar_out;
+    a.NMP = nmem * t.NSEA;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    rc = timing_events(ctx, ev, 2);
+    if (rc) return rc;
+    timing_record(ctx, ev[0]);
+    HIPCHK(launch_implsch_adjoint(ctx->dt, a, ctx->ncu, ctx->stream));
+    timing_record(ctx, ev[1]);
+    timing_keep(ctx, "implsch_adjoint", ev[0], ev[1]);
+    // 3. PROPAGS transposed into d_fl3_bar_out
+    return propags_transposed(ctx, nmem, ctx->adj_fb, d_fl3_bar_out, ctx->adj_fp, ctx->adj_fb);
+}
+
+int wam_gpu_propags_adjoint_dev(wam_ctx *ctx, int32_t nmem, const float *d_in_bar, float *d_out_bar)
+{
+    if (!d_in_bar || !d_out_bar || d_in_bar == d_out_bar) return WAM_E_ARG;
+    int rc = ensemble_args_ok(ctx, nmem);
+    if (rc) return rc;
+    if (ctx->adj_sym < 0) ctx->adj_sym = neighbours_symmetric(ctx->tab) ? 1 : 0;
+    if (!ctx->adj_sym) { ctx->err = "the transposed PROPAGS needs a symmetric neighbour relation"; return WAM_E_ARG; }
+    rc = adjoint_planes(ctx);
+    if (rc) return rc;
+    const size_t n = (size_t)nmem * ctx->tab.KL * ctx->tab.ML * ctx->tab.NSEA;
+    if (ctx->counter > 1 && ctx->adj_n < n) {
+        if (ctx->adj_fp) (void)hipFree(ctx->adj_fp);
+        if (ctx->adj_fb) (void)hipFree(ctx->adj_fb);
+        ctx->adj_fp = ctx->adj_fb = nullptr;
+        ctx->adj_n = 0;
+        HIPCHK(hipMalloc(&ctx->adj_fp, n * sizeof(float)));
+        HIPCHK(hipMalloc(&ctx->adj_fb, n * sizeof(float)));
+        ctx->adj_n = n;
+    }
+    return propags_transposed(ctx, nmem, d_in_bar, d_out_bar, ctx->adj_fp, ctx->adj_fb);
+}
+
+int wam_gpu_wind_uv_dev(wam_ctx *ctx, const float *d_u, const float *d_v, float *d_u10, float *d_udir, int64_t n)
+{
+    if (!ctx || n < 0 || (n > 0 && (!d_u || !d_v || !d_u10 || !d_udir))) return WAM_E_ARG;
+    HIPCHK(launch_wind_uv(ctx->tab.k.ZPI, d_u, d_v, d_u10, d_udir, n, ctx->stream));
+    return WAM_OK;
+}
+
+int wam_gpu_wind_uv_adjoint_dev(wam_ctx *ctx, const float *d_u, const float *d_v, const float *d_u10_bar,
+                                const float *d_udir_bar, float *d_u_bar, float *d_v_bar, int64_t n)
+{
+    if (!ctx || n < 0 || (n > 0 && (!d_u || !d_v || !d_u10_bar || !d_udir_bar || !d_u_bar || !d_v_bar)))
+        return WAM_E_ARG;
+    HIPCHK(launch_wind_uv_adjoint(d_u, d_v, d_u10_bar, d_udir_bar, d_u_bar, d_v_bar, n, ctx->stream));
+    return WAM_OK;
+}
+
 int wam_gpu_set_stream(wam_ctx *ctx, void *stream)
 {
     if (!ctx) return WAM_E_ARG;
@@ -2008,6 +2187,8 @@ void wam_gpu_finalize(wam_ctx *ctx)
     free_impl_scratch(ctx->isx);
     free_impl_scratch(ctx->esx);
     if (ctx->fscratch_e) (void)hipFree(ctx->fscratch_e);
+    if (ctx->adj_fp) (void)hipFree(ctx->adj_fp);
+    if (ctx->adj_fb) (void)hipFree(ctx->adj_fb);
     if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
     if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
     for (hipEvent_t e : ctx->gev)

@@ -54,6 +54,13 @@ struct DevTables {
     int SP_CHUNK, SP_NCHUNK, SP_NRUN, SP_PAD;
     const uint16_t *SP_E;
     const int *SP_RW;
+    // SNONLIN's adjoint by chunks of ACH frequencies of quadruplets: update
+    // rows per (MC, update type), -1 = skipped; per (chunk, read row) the
+    // reading quadruplets (MC - 1) << 4 | read slot; inverse direction maps
+    int ACH, ANCH;
+    const int8_t *AQ_ROW;
+    const uint16_t *AR_E;
+    const int *AR_OFF, *AKINV;
     // per sea point (PREPARE_PROPAGATION), 0-based neighbour indices, -1 = land
     const int *KLAT_S, *KLAT_N, *KLON_W, *KLON_E;
     const int *INDEP, *INDEP_G;
@@ -87,6 +94,16 @@ struct ImplArgs {
     float *tauw, *ustar, *z0, *phiaw, *fluxes;
     int *mij;
     int lcflx;
+    int NMP;
+};
+
+// The adjoint of IMPLSCH for NMP member-points (laid out as ImplArgs): fl3
+// is IMPLSCH's input spectrum (after PROPAGS) and fl3_bar the bar of its
+// output; tauw_bar, ustar_bar, z0_bar may be NULL (zero bars).
+struct AdjArgs {
+    const float *fl3, *u10, *udir, *tauw, *roairn, *wstar;
+    const float *fl3_bar, *tauw_bar, *ustar_bar, *z0_bar;
+    float *fl3_bar_out, *u10_bar, *udir_bar, *tauw_bar_out;
     int NMP;
 };
 

@@ -452,6 +452,73 @@ static int snonlin_lists(Tables &t, std::string &err)
     return WAM_OK;
 }
 
+// Tables of SNONLIN's adjoint (the transpose of the DIA quadruplet terms):
+// the rows each quadruplet updates (the regimes of :2890-3043, as in
+// snonlin_lists), the quadruplets reading each frequency row, and the
+// inverses of the direction maps.  A map that is not a permutation of the
+// directions leaves adj_err set (the adjoint then refuses to run).
+static void snonlin_adjoint_tables(Tables &t)
+{
+    const int KL = t.KL, ML = t.ML, MF = t.MFRSTLW;
+    const int m1st = -MF + 1, mlst = ML - t.KFRH + m1st;
+    t.adj_err.clear();
+    t.AQ_ROW.assign((size_t)9 * t.MLSTHG, -1);
+    for (int mc = 1; mc <= t.MLSTHG; mc++) {
+        const int i = mc - MF;
+        const int mp = t.IKP[i], mp1 = t.IKP1[i], mm = t.IKM[i], mm1 = t.IKM1[i];
+        const int regime = (mc > m1st && mc < mlst) ? 0 : (mc >= mlst ? 1 : 2);
+        int8_t *r = &t.AQ_ROW[(size_t)9 * (mc - 1)];
+        auto set = [&](int ty, int m) { r[ty] = (int8_t)(m - 1); };
+        auto pair = [&](int ty, int m) { set(ty, m); set(ty + 1, m); };
+        if (regime == 0) {
+            set(0, mc); pair(1, mm); pair(3, mm1); pair(5, mp); pair(7, mp1);
+        } else if (regime == 1) {
+            pair(1, mm);
+            if (mm1 <= ML) {
+                pair(3, mm1);
+                if (mc <= ML) {
+                    set(0, mc);
+                    if (mp <= ML) { pair(5, mp); if (mp1 <= ML) pair(7, mp1); }
+                }
+            }
+        } else {
+            if (mm1 >= 1) pair(3, mm1);
+            set(0, mc); pair(5, mp); pair(7, mp1);
+        }
+    }
+    t.ACH = std::max(1, 512 / (2 * KL));
+    t.ANCH = (t.MLSTHG + t.ACH - 1) / t.ACH;
+    static const int rslot[9] = {0, 1, 1, 2, 2, 3, 3, 4, 4};          // INLCOEF column of the read row
+    static const int wslot[9] = {0, 1, 2, 3, 4, 13, 14, 15, 16};      // RNLCOEF slot of the read weight
+    t.AR_E.clear();
+    t.AR_OFF.assign((size_t)t.ANCH * ML + 1, 0);
+    for (int c = 0; c < t.ANCH; c++) {
+        const int mc_lo = 1 + c * t.ACH, mc_hi = std::min(t.MLSTHG, mc_lo + t.ACH - 1);
+        for (int m = 0; m < ML; m++) {
+            t.AR_OFF[(size_t)c * ML + m] = (int32_t)t.AR_E.size();
+            for (int mc = mc_lo; mc <= mc_hi; mc++)
+                for (int sl = 0; sl < 9; sl++) {
+                    const int row = t.INLCOEF[(size_t)(mc - 1) * 5 + rslot[sl]] - 1;
+                    if (row != m || t.RNLCOEF[(size_t)(mc - 1) * 25 + wslot[sl]] == 0.f) continue;
+                    t.AR_E.push_back((uint16_t)((mc - 1) << 4 | sl));
+                }
+        }
+    }
+    t.AR_OFF[(size_t)t.ANCH * ML] = (int32_t)t.AR_E.size();
+    t.AKINV.assign((size_t)8 * KL, -1);
+    const std::vector<int> *maps[4] = {&t.K1W, &t.K11W, &t.K2W, &t.K21W};
+    for (int mp = 0; mp < 4; mp++)
+        for (int kh = 0; kh < 2; kh++)
+            for (int k = 0; k < KL; k++) {
+                const int to = (*maps[mp])[(size_t)kh * KL + k] - 1;
+                int32_t &inv = t.AKINV[((size_t)mp * 2 + kh) * KL + to];
+                if (to < 0 || to >= KL || inv >= 0) t.adj_err = "a DIA direction map is not a permutation";
+                else inv = k;
+            }
+    for (int i = 0; i < t.MLSTHG * 5; i++)
+        if (t.INLCOEF[i] < 1 || t.INLCOEF[i] > ML) t.adj_err = "a DIA read row outside 1..ML";
+}
+
 // INIT_X0TAUHF, wam_source_module.f90:3784-3849
 static void x0tauhf(Tables &t)
 {
@@ -558,6 +625,7 @@ int build_tables(Tables &t, const wam_config &cfg, const uint8_t *mask, std::str
     dia_tables(t);
     rc = snonlin_lists(t, err);
     if (rc) return rc;
+    snonlin_adjoint_tables(t);
     x0tauhf(t);
     lfactor_grid(t);
     if (cfg.iphys == 2 && t.NK10 > 128) { err = "LFACTOR 10-Hz grid longer than 128 frequencies (FR(1) too low)"; return WAM_E_ARG; }

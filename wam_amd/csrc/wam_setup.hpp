@@ -97,6 +97,22 @@ struct Tables {
     int SP_CHUNK = 0, SP_NCHUNK = 0, SP_NRUN = 0, SP_PAD = 0;
     std::vector<uint16_t> SP_E;
     std::vector<int32_t> SP_RW;                          // [SP_NCHUNK][SP_NRUN][2]
+    // the adjoint of SNONLIN, by chunks of ACH frequencies of quadruplets
+    // (2*KL*ACH <= 512).  AQ_ROW[9*(MC-1) + type] is the 0-based frequency
+    // row of the quadruplet's update of that type (the scatter targets of
+    // snonlin_lists), -1 where the regime skips it.  A quadruplet reads the
+    // nine bins (K, IC), (K1, IP), (K11, IP), (K1, IP1), (K11, IP1), (K2, IM),
+    // (K21, IM), (K2, IM1), (K21, IM1) (read slots 0..8); AR_E[AR_OFF[c*ML +
+    // m] .. AR_OFF[c*ML + m + 1]) lists the (MC - 1) << 4 | slot of chunk c
+    // whose read row is m (nonzero weights only).  AKINV[(map*2 + KH)*KL + k]
+    // inverts the direction maps K -> K1W, K11W, K2W, K21W (map 0..3).
+    // adj_err is empty when every direction map is a permutation.
+    int ACH = 0, ANCH = 0;
+    std::vector<int8_t> AQ_ROW;
+    std::vector<uint16_t> AR_E;
+    std::vector<int32_t> AR_OFF;
+    std::vector<int32_t> AKINV;
+    std::string adj_err;
 };
 
 // Builds constants, fre_dir, tables, DIA, X0TAUHF and the grid.  Returns a

@@ -454,6 +454,51 @@ class WamGpu:
                                                          _ptr(ustar), _ptr(z0), _ptr(roairn), _ptr(wstar),
                                                          int(bool(lcflx)), _ptr(phiaw), _ptr(fluxes), _ptr(mij)))
 
+    def step_adjoint_dev(self, fl3, u10, udir, tauw, roairn, wstar, fl3_bar, fl3_bar_out, u10_bar, udir_bar,
+                         tauw_bar_out, tauw_bar=None, ustar_bar=None, z0_bar=None, nmem=1):
+        """Adjoint of one PROPAGS + IMPLSCH step at the input state (fl3, tauw)
+        with its forcing: from the bars of the step's outputs (fl3_bar and the
+        optional tauw_bar, ustar_bar, z0_bar) into the bars of its inputs
+        (fl3_bar_out, which may be fl3_bar, u10_bar, udir_bar, tauw_bar_out)."""
+        self._check(self.lib.wam_gpu_step_adjoint_dev(
+            self.ctx, int(nmem), _ptr(fl3), _ptr(u10), _ptr(udir), _ptr(tauw), _ptr(roairn), _ptr(wstar),
+            _ptr(fl3_bar), _ptr(tauw_bar), _ptr(ustar_bar), _ptr(z0_bar), _ptr(fl3_bar_out), _ptr(u10_bar),
+            _ptr(udir_bar), _ptr(tauw_bar_out)))
+
+    def propags_adjoint_dev(self, d_in_bar, d_out_bar, nmem=1):
+        """Adjoint of PROPAGS: the transposed stencil, COUNTER sub-steps."""
+        self._check(self.lib.wam_gpu_propags_adjoint_dev(self.ctx, int(nmem), _ptr(d_in_bar), _ptr(d_out_bar)))
+
+    def wind_uv_dev(self, u, v, u10, udir):
+        """U10 = SQRT(U**2 + V**2), UDIR = ATAN2(U, V) (+ ZPI where negative)."""
+        self._check(self.lib.wam_gpu_wind_uv_dev(self.ctx, _ptr(u), _ptr(v), _ptr(u10), _ptr(udir), int(u.numel())))
+
+    def wind_uv_adjoint_dev(self, u, v, u10_bar, udir_bar, u_bar, v_bar):
+        self._check(self.lib.wam_gpu_wind_uv_adjoint_dev(self.ctx, _ptr(u), _ptr(v), _ptr(u10_bar), _ptr(udir_bar),
+                                                         _ptr(u_bar), _ptr(v_bar), int(u.numel())))
+
+
+def checkpoint_schedule(nsteps, nsnap):
+    """The binomial checkpointing schedule of wam_checkpoint_schedule: a list
+    of (op, a, b) with op one of CK_ADVANCE, CK_STORE, CK_RESTORE, CK_ADJOINT."""
+    lib = _lib.load()
+    n = lib.wam_checkpoint_schedule(int(nsteps), int(nsnap), None, 0)
+    if n < 0:
+        raise WamError(-n, "bad checkpointing request")
+    buf = np.zeros(3 * n, np.int32)
+    lib.wam_checkpoint_schedule(int(nsteps), int(nsnap), _ptr(buf), n)
+    return [tuple(int(x) for x in buf[3 * i:3 * i + 3]) for i in range(n)]
+
+
+def checkpoint_recomputed_steps(nsteps, nsnap):
+    n = _lib.load().wam_checkpoint_recomputed_steps(int(nsteps), int(nsnap))
+    if n < 0:
+        raise WamError(-n, "bad checkpointing request")
+    return int(n)
+
+
+CK_ADVANCE, CK_STORE, CK_RESTORE, CK_ADJOINT = 0, 1, 2, 3
+
 
 class _GridState(ctypes.Structure):
     """struct wam_grid_state of include/wam_gpu.h."""

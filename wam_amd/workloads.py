@@ -236,6 +236,48 @@ def c4_case(scale=1.0, seed=20261205):
     return [g1, g2, g3], (1, 2, link)
 
 
+def c5_case(ny=400, nx=360, seed=20261206, nsteps=144, dlat=0.25, dlon=0.25, south=-50.0, west=0.0,
+            land_frac=0.30, nobs=500, obs_every=12, truth_amp=0.10, obs_sd=0.1):
+    """Adjoint workload (configuration 5): a deep-water grid (DEPTH = 999 m as
+    in a run with SHALLOW_RUN = F, no currents), IPHYS = 0, 24 x 25 spectra,
+    300 s steps; at the default size 400 x 360 cells of 0.25 deg with a
+    seeded fractal land mask, about 100 k sea points.  The winds are u, v
+    components for each of NSTEPS steps, blending two seeded k**-3 random
+    fields (mean speed 10 m/s, sd 5 m/s, 500 km decorrelation) linearly over
+    the window.  NOBS seeded sea points are observed every OBS_EVERY steps.
+    Returns (cfg, mask, fields, u[nsteps][nsea], v[nsteps][nsea], obs_points,
+    truth_u, truth_v): the truth winds are the winds times (1 + truth_amp *
+    GRF) with a 200 km Gaussian random field; the caller makes observations
+    from the truth run (Hs plus N(0, obs_sd) noise)."""
+    cfg = base_config(iphys=0, kl=24, ml=25, shallow=0, refr_d=0, refr_c=0, idelpro=300, idelt=300)
+    cfg = grid_config(cfg, nx, ny, south, west, dlat, dlon)
+    rng = np.random.default_rng(seed)
+    land = _fractal(ny, nx, 3.0, rng)
+    mask = (land < np.quantile(land, 1.0 - land_frac)).astype(np.uint8)
+    sea = mask.astype(bool)
+    n = int(sea.sum())
+    cells = 500e3 / (dlat * 111e3)
+
+    def wind():
+        spd = np.clip(10.0 + 5.0 * _kminus3_field(ny, nx, cells, rng), 0.5, None)
+        wdir = math.pi / 2 + 0.8 * _smooth_field(ny, nx, cells / 2.0, rng)
+        return (spd * np.sin(wdir))[sea], (spd * np.cos(wdir))[sea]
+
+    (ua, va), (ub, vb) = wind(), wind()
+    w = (np.arange(nsteps, dtype=np.float64) / max(nsteps - 1, 1))[:, None]
+    u = ((1.0 - w) * ua[None, :] + w * ub[None, :]).astype(np.float32)
+    v = ((1.0 - w) * va[None, :] + w * vb[None, :]).astype(np.float32)
+    g = _smooth_field(ny, nx, 200e3 / (dlat * 111e3), rng)[sea]
+    tu = (u * (1.0 + truth_amp * g)[None, :]).astype(np.float32)
+    tv = (v * (1.0 + truth_amp * g)[None, :]).astype(np.float32)
+    obs = np.sort(rng.choice(n, size=min(nobs, n), replace=False)).astype(np.int64)
+    spd0 = np.hypot(u[0], v[0])
+    udir0 = np.mod(np.arctan2(u[0], v[0]), 2 * math.pi)
+    fields = dict(depth=np.full(n, 999.0, np.float32), u=None, v=None, u10=spd0.astype(np.float32),
+                  udir=udir0.astype(np.float32), obs_every=obs_every, obs_sd=obs_sd)
+    return cfg, mask, fields, u, v, obs, tu, tv
+
+
 def frequencies(cfg):
     fr = np.empty(cfg["ml"], np.float64)
     fr[0] = cfg["fr1"] * 1.1 ** (1 - cfg["iref"])
