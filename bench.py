@@ -359,6 +359,164 @@ def bench_nested(args, torch):
     return res
 
 
+def bench_adjoint(args, torch):
+    """Configuration 5: the gradient of J = sum of squared Hs misfits at 500
+    seeded points every hour of a 12 h window (144 steps of 300 s) with
+    respect to the wind components u, v of every step, on a deep-water 0.25
+    deg grid of 100 800 sea points (24 x 25, IPHYS = 0).  Observations are
+    the Hs of a truth run with perturbed winds plus N(0, 0.1 m).  Timed: one
+    forward step (wind map, PROPAGS, IMPLSCH) against one adjoint step (wind
+    map, PROPAGS recomputed, IMPLSCH's adjoint, PROPAGS transposed, the wind
+    map's adjoint) at the same state; the whole window's forward and gradient
+    with every state stored and with the binomial schedule at args.adj_snaps
+    slots (the two gradients must be equal bit for bit); and the gradient of
+    a batch of args.adj_members members over args.adj_member_steps steps."""
+    from wam_amd import WamGpu, workloads as W
+    from wam_amd.adjoint import WindowModel, energy_weights, hs_from_spectra, window
+    from wam_amd.model import checkpoint_recomputed_steps
+    t0 = time.time()
+    nsteps = args.adj_steps
+    cfg, mask, f, u, v, obs, tu, tv = W.c5_case(nsteps=nsteps)
+    g = WamGpu(cfg, mask)
+    n = g.nsea
+    counter, cfl = g.prepare_propagation(f["depth"])
+    dev = torch.device("cuda:0")
+    t = lambda x: torch.from_numpy(np.ascontiguousarray(x, np.float32)).to(dev)
+    U, V = t(u), t(v)
+    fl0 = t(W.jonswap_spectrum(cfg, f["udir"]).ravel())
+    tw0 = torch.zeros(n, device=dev)
+    w = energy_weights(g)
+    every = f["obs_every"]
+    log(f"[adjoint] {n} sea points, COUNTER={counter}, {nsteps} steps, {obs.size} observed points "
+        f"({time.time() - t0:.1f}s)")
+    # observations: the truth run (perturbed winds) plus N(0, 0.1 m)
+    truth = WindowModel(g, nsteps, obs, obs_every=every, nsnap=0)
+    with torch.no_grad():
+        hs_t = hs_from_spectra(truth.forward(t(tu), t(tv), fl0, tw0), w)
+    rng = np.random.default_rng(20261207)
+    obs_v = hs_t + t(rng.normal(0.0, f["obs_sd"], tuple(hs_t.shape)))
+    del truth
+    # one forward step and one adjoint step at the same state, K times each
+    ro, ws = torch.full((n,), 1.225, device=dev), torch.zeros(n, device=dev)
+    fa, fb = fl0.clone(), torch.empty_like(fl0)
+    tauw, ustar, z0, u10, udir = (torch.zeros(n, device=dev) for _ in range(5))
+    for i in range(12):                                 # an hour of spin-up
+        g.wind_uv_dev(U[i], V[i], u10, udir)
+        g.propags_dev(fa, fb)
+        g.implsch_dev(fb, u10, udir, tauw, ustar, z0, ro, ws)
+        fa, fb = fb, fa
+    K = args.adj_reps
+    fs, ts = fa.clone(), tauw.clone()
+    fbar = torch.randn_like(fa)
+    fbo, u10b, udirb, twb, ub, vb = torch.empty_like(fa), *(torch.empty(n, device=dev) for _ in range(5))
+
+    def fwd_step():
+        nonlocal fa, fb
+        g.wind_uv_dev(U[12], V[12], u10, udir)
+        g.propags_dev(fa, fb)
+        g.implsch_dev(fb, u10, udir, tauw, ustar, z0, ro, ws)
+        fa, fb = fb, fa
+
+    def adj_step():
+        g.wind_uv_dev(U[12], V[12], u10, udir)
+        g.step_adjoint_dev(fs, u10, udir, ts, ro, ws, fbar, fbo, u10b, udirb, twb)
+        g.wind_uv_adjoint_dev(U[12], V[12], u10b, udirb, ub, vb)
+
+    res = {"workload": "c5: 0.25deg 400x360 deep water (999 m), 30% land, 24x25, IPHYS=0, 300 s; "
+                       f"J = sum (Hs - obs)^2 at {obs.size} points every {every} steps over {nsteps} steps, "
+                       "gradient w.r.t. u, v of every step",
+           "nsea": n, "counter": counter, "steps": nsteps, "obs_points": int(obs.size)}
+    timing = {}
+    for name, fn in (("forward", fwd_step), ("adjoint", adj_step)):
+        fn()
+        torch.cuda.synchronize()
+        g.set_timing(True)
+        w0 = time.perf_counter()
+        for _ in range(K):
+            fn()
+        torch.cuda.synchronize()
+        timing[name] = (time.perf_counter() - w0) / K
+        for kname in ("propags", "implsch_pre", "implsch_snl", "implsch_post", "implsch_adjoint", "propags_adjoint"):
+            ms, calls = g.kernel_time(kname)
+            if calls:
+                res.setdefault("kernel_ms_" + name, {})[kname] = round(ms / K, 3)
+        g.set_timing(False)
+    res["ms_forward_step"] = round(timing["forward"] * 1e3, 3)
+    res["ms_adjoint_step"] = round(timing["adjoint"] * 1e3, 3)
+    res["adjoint_over_forward_step"] = round(timing["adjoint"] / timing["forward"], 3)
+    ka = res.get("kernel_ms_adjoint", {}).get("implsch_adjoint")
+    if ka:
+        # compulsory HBM traffic of the kernel: F and the bar of FL3 out read,
+        # the input spectrum's bar written (4 bytes each per bin), per-point
+        # fields negligible
+        gbs = 12.0 * g.kl * g.ml * n / (ka * 1e-3) / 1e9
+        res["roofline_implsch_adjoint"] = dict(kernel="implsch_adj_kernel", bound="valu (per-point scalar chains, "
+                                               "LDS)", hbm_achieved=round(gbs, 1), hbm_peak=HBM_PEAK_GBS,
+                                               unit="GB/s", hbm_frac=round(gbs / HBM_PEAK_GBS, 4))
+    del fa, fb, fs, ts, fbar, fbo
+    # the whole window: forward, then the gradient, every state stored and binomial
+    grads = {}
+    for label, nsnap in (("store_all", None), ("binomial", args.adj_snaps)):
+        uu, vv = U.clone().requires_grad_(True), V.clone().requires_grad_(True)
+        model = WindowModel(g, nsteps, obs, obs_every=every, nsnap=nsnap)
+        torch.cuda.synchronize()
+        w0 = time.perf_counter()
+        spec = window(uu, vv, fl0, tw0, model)
+        hs = hs_from_spectra(spec, w)
+        j = ((hs - obs_v) ** 2).sum()
+        torch.cuda.synchronize()
+        t_f = time.perf_counter() - w0
+        w0 = time.perf_counter()
+        j.backward()
+        torch.cuda.synchronize()
+        t_b = time.perf_counter() - w0
+        grads[label] = (uu.grad, vv.grad)
+        res[label] = {"nsnap": model.nsnap, "forward_s": round(t_f, 3), "reverse_s": round(t_b, 3),
+                      "gradient_over_forward": round((t_f + t_b) / t_f, 3),
+                      "forward_steps_run": model.forward_steps,
+                      "recomputed_steps_schedule": checkpoint_recomputed_steps(nsteps, model.nsnap),
+                      "J": float(j)}
+        log(f"[adjoint] {label}: forward {t_f:.2f}s, reverse {t_b:.2f}s, J = {float(j):.6g}")
+        del model, spec
+        torch.cuda.empty_cache()
+    gu0, gv0 = grads["store_all"]
+    gu1, gv1 = grads["binomial"]
+    res["binomial_equals_store_all"] = bool(torch.equal(gu0, gu1) and torch.equal(gv0, gv1))
+    res["gradient_norm"] = float(torch.sqrt((gu0.double() ** 2).sum() + (gv0.double() ** 2).sum()))
+    res["gradient_finite"] = bool(torch.isfinite(gu0).all() and torch.isfinite(gv0).all())
+    del grads, gu0, gv0, gu1, gv1
+    torch.cuda.empty_cache()
+    # a batch of members, binomial checkpointing
+    nm, ms = args.adj_members, args.adj_member_steps
+    if nm > 1:
+        rngm = np.random.default_rng(20261208)
+        fac = 1.0 + 0.05 * rngm.standard_normal((1, nm, 1))
+        um = t((u[:ms, None, :] * fac).reshape(ms, -1))
+        vm = t((v[:ms, None, :] * fac).reshape(ms, -1))
+        uu, vv = um.requires_grad_(True), vm.requires_grad_(True)
+        model = WindowModel(g, ms, obs, obs_every=every, nsnap=args.adj_snaps, nmem=nm)
+        torch.cuda.synchronize()
+        w0 = time.perf_counter()
+        spec = window(uu, vv, fl0.repeat(nm), torch.zeros(nm * n, device=dev), model)
+        hs = hs_from_spectra(spec, w)
+        j = ((hs - obs_v[:ms // every, None, :]) ** 2).sum()
+        torch.cuda.synchronize()
+        t_f = time.perf_counter() - w0
+        w0 = time.perf_counter()
+        j.backward()
+        torch.cuda.synchronize()
+        t_b = time.perf_counter() - w0
+        res["batched"] = {"members": nm, "steps": ms, "nsnap": model.nsnap, "forward_s": round(t_f, 3),
+                          "reverse_s": round(t_b, 3), "gradient_over_forward": round((t_f + t_b) / t_f, 3),
+                          "member_point_steps_per_s_gradient": round(nm * n * ms / (t_f + t_b), 1),
+                          "gradient_finite": bool(torch.isfinite(uu.grad).all() and torch.isfinite(vv.grad).all())}
+        log(f"[adjoint] {nm} members x {ms} steps: forward {t_f:.2f}s, reverse {t_b:.2f}s")
+        del model, spec, uu, vv
+    g.close()
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -378,6 +536,12 @@ def main():
     ap.add_argument("--ens-steps", type=int, default=4)
     ap.add_argument("--no-nested", action="store_true", help="leave out the nested-grid measurement")
     ap.add_argument("--nest-steps", type=int, default=6)
+    ap.add_argument("--no-adjoint", action="store_true", help="leave out the adjoint (gradient) measurement")
+    ap.add_argument("--adj-steps", type=int, default=144, help="steps of the adjoint window (300 s each)")
+    ap.add_argument("--adj-reps", type=int, default=10)
+    ap.add_argument("--adj-snaps", type=int, default=12, help="checkpoint slots of the binomial schedule")
+    ap.add_argument("--adj-members", type=int, default=16)
+    ap.add_argument("--adj-member-steps", type=int, default=24)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one plain 1 GiB device copy before the timed steps (reference for HBM counters)")
     args = ap.parse_args()
@@ -533,6 +697,9 @@ def main():
     if not args.no_nested:
         torch.cuda.empty_cache()
         out["nested"] = bench_nested(args, torch)
+    if not args.no_adjoint:
+        torch.cuda.empty_cache()
+        out["adjoint"] = bench_adjoint(args, torch)
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(args)

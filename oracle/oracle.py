@@ -44,7 +44,10 @@ _LIBS = {}
 
 
 def _lib(fp64, opcount=False, ad=False):
-    key = "liborc_ad.so" if ad else ("liborc_opcount.so" if opcount else ("liborc64.so" if fp64 else "liborc32.so"))
+    if ad:
+        key = "liborc_ad.so" if fp64 else "liborc_ad32.so"
+    else:
+        key = "liborc_opcount.so" if opcount else ("liborc64.so" if fp64 else "liborc32.so")
     if key not in _LIBS:
         path = os.path.join(_HERE, key)
         if not os.path.exists(path):
@@ -94,11 +97,13 @@ def _p(a):
 class Oracle:
     """One model configuration on one regular grid (sea mask nx*ny, row-major from the south-west)."""
 
-    def __init__(self, cfg, mask, fp64=False, opcount=False, ad=False):
+    def __init__(self, cfg, mask, fp64=False, opcount=False, ad=False, ad32=False):
         """ad=True: the FP64 restatement together with one model step's
-        tangent-linear model and adjoint (step_ad)."""
+        tangent-linear model and adjoint (step_ad); ad32=True: the same with
+        the REAL*4 restatement's values and branches (derivatives in FP64)."""
         self.cfg = dict(cfg)
-        self.fp64 = (fp64 or ad) and not opcount
+        ad = ad or ad32
+        self.fp64 = (fp64 or ad) and not opcount and not ad32
         self.dtype = np.float64 if self.fp64 else np.float32
         self.opcount = opcount
         self.ad = ad
@@ -140,7 +145,7 @@ class Oracle:
         return int(self.scalar("COUNTER")), float(self.scalar("CFL_MAX"))
 
     def step_ad(self, fl3, tauw, z0, u, v, depth, roairn=None, wstar=None, dx=None, ybar=None):
-        """One model step, PROPAGS then IMPLSCH, in FP64 from the wind
+        """One model step, PROPAGS then IMPLSCH, in FP64 (or REAL*4) from the wind
         components u, v (U10 = sqrt(u^2 + v^2), UDIR = atan2(u, v)):
         returns y = (FL3', TAUW', Z0') packed, and with dx (packed as
         x = (FL3, TAUW, Z0, U, V)) also the tangent-linear dy, or with ybar

@@ -2,14 +2,16 @@
  * ORACLE -- test infrastructure only (see orc_common.h).
  *
  * orc_ad.cpp: one model step of the CPU restatement, PROPAGS then IMPLSCH
- * (wamodel.f90:332,354), in FP64 on the tape of orc_ad.h, with the wind
+ * (wamodel.f90:332,354), on the tape of orc_ad.h (FP64, or with ORC_AD32
+ * the REAL*4 values of the FP32 build), with the wind
  * given as components: U10 = SQRT(U**2 + V**2), UDIR = ATAN2(U, V) (+ 2 pi
  * when negative), wam_wind_module.f90:1157-1167.  It evaluates the step, its
  * tangent-linear model and its adjoint with respect to
  *     x = (FL3, TAUW, Z0, U, V)  ->  y = (FL3', TAUW', Z0')
  * at the given state, exactly (to FP64 round-off) for the linearisation the
  * restatement's branches define there.  The context is the FP64 build's
- * (orc_api_create of liborc64 linked in the same library).
+ * (FP32 with ORC_AD32: orc_api_create of the build linked in the same
+ * library).
  */
 #include <math.h>
 #include <stddef.h>
@@ -47,7 +49,7 @@ void record(const void *ctx, const double *F, const double *tauw, const double *
     for (int i = 0; i < ns; i++) { zz[i] = ad_input(z0[i]); xin.push_back(zz[i].i); }
     for (int i = 0; i < ns; i++) { uu[i] = ad_input(u[i]); xin.push_back(uu[i].i); }
     for (int i = 0; i < ns; i++) { vv[i] = ad_input(v[i]); xin.push_back(vv[i].i); }
-    const double zpi = 2.0 * 3.14159265358979323846;
+    const treal zpi = c->ZPI;
     for (int i = 0; i < ns; i++) {
         u10[i] = ad_sqrt(uu[i] * uu[i] + vv[i] * vv[i]);
         udir[i] = u10[i].v != 0.0 ? ad_atan2(uu[i], vv[i]) : adr(0.0);

@@ -60,12 +60,18 @@ static inline real FLOORR(real x) { return floor(x); }
 static inline real CEILR(real x) { return ceil(x); }
 static inline real SIGNR(real a, real b) { return b >= 0 ? fabs(a) : -fabs(a); }
 #elif defined(ORC_AD)
-/* the FP64 arithmetic on the tape of orc_ad.h; the context's tables are the
-   FP64 build's (passive) */
+/* the FP64 arithmetic on the tape of orc_ad.h, the context's tables the
+   FP64 build's (passive); with ORC_AD32 the REAL*4 arithmetic on the tape,
+   the tables the FP32 build's */
 #include "orc_ad.h"
 typedef adr real;
+#ifdef ORC_AD32
+typedef float treal;
+#define R(x) (x##f)
+#else
 typedef double treal;
 #define R(x) (x)
+#endif
 #define EXP(x) ad_exp(real(x))
 #define LOG(x) ad_log(real(x))
 #define LOG10(x) ad_log10(real(x))
@@ -79,10 +85,17 @@ typedef double treal;
 #define POW(x, y) ad_pow(real(x), real(y))
 #define ABS(x) ad_abs(real(x))
 #define IPOW(x, n) ad_ipow(real(x), n)
+#ifdef ORC_AD32
+static inline int NINT(real x) { return wam_nint(x.v); }
+static inline int INTR(real x) { return wam_int(x.v); }
+static inline real FLOORR(real x) { return wam_floorf(x.v); }
+static inline real CEILR(real x) { return wam_ceilf(x.v); }
+#else
 static inline int NINT(real x) { return (int)lround(x.v); }
 static inline int INTR(real x) { return (int)trunc(x.v); }
 static inline real FLOORR(real x) { return floor(x.v); }
 static inline real CEILR(real x) { return ceil(x.v); }
+#endif
 static inline real SIGNR(real a, real b) { return b.v >= 0 ? ad_abs(a) : -ad_abs(a); }
 #elif defined(ORC_OPCOUNT)
 #include "orc_opcount.h"

@@ -2,10 +2,14 @@
 (wam_gpu_step_adjoint_dev, wam_gpu_propags_adjoint_dev, the wind map and its
 adjoint), on deep-water grids of configuration 5's kind (IPHYS = 0, 24 x 25):
 
-  - the step adjoint against the oracle's taped FP64 adjoint of the same step
-    at the same state (liborc_ad, pinned by finite differences and the
-    dot-product identity in test_oracle_adjoint.py), as relative L2 errors of
-    each input's bar (the GPU evaluates in FP32);
+  - the step adjoint against the oracle's taped adjoint of the same step at
+    the same state with the REAL*4 restatement's values (liborc_ad32: the
+    FP32 build's values bit for bit, hence its branches, which the GPU's
+    forward reproduces; derivatives in FP64; its FP64 twin is pinned by
+    finite differences and the dot-product identity in
+    test_oracle_adjoint.py), as relative L2 errors of each input's bar.  A
+    comparison with the FP64 tape would differ wherever FP32 and FP64 take
+    different sides of a kink (the limiter's floor at nearly empty bins);
   - PROPAGS's adjoint against its forward by the dot-product identity
     <P x, y> = <x, P^T y>, with and without currents and with COUNTER > 1;
   - ensemble members equal single runs bit for bit, and a checkpointed
@@ -130,7 +134,7 @@ def test_propags_adjoint_is_the_transpose(case):
 
 
 def oracle_step_adjoint(cfg, mask, f, fl3, tauw, u, v, ybar):
-    o = Oracle(cfg, mask, ad=True)
+    o = Oracle(cfg, mask, ad32=True)
     o.prepare_propagation(f["depth"])
     z0 = np.full(tauw.size, 1e-3)
     _, xbar = o.step_ad(fl3, tauw, z0, u, v, f["depth"], ybar=ybar)
@@ -171,8 +175,11 @@ def test_step_adjoint_matches_the_oracle(which):
                  v=(vb, xbar[nb + 3 * n:]))
     errs = {k: rel(a.cpu().numpy(), b) for k, (a, b) in parts.items()}
     print(f"[case] {which} NSEA={n} COUNTER={counter} rel errors {errs}", flush=True)
+    # FL3's bar to FP32 round-off; TAUW's and the winds' bars pass through
+    # AIRSEA's iteration derivatives, accumulated in FP32
+    bars = dict(fl3=1e-6, tauw=2e-4, u=1e-4, v=1e-4)
     for k, e in errs.items():
-        assert e < 1e-4, (k, e)
+        assert e < bars[k], (k, e)
     g.close()
 
 
@@ -264,7 +271,10 @@ def test_window_gradient_matches_fp64_finite_differences():
     over 12 steps, along 50 random directions: the GPU adjoint (FP32 state,
     checkpointed with 3 slots) against central differences of the FP64
     oracle's window, |g.d - FD| <= 1e-3 |FD| where the perturbations flip no
-    cut-off index MIJ."""
+    cut-off index MIJ.  The step h = 1e-5 m/s per unit of the direction: the
+    model is piecewise smooth (MAX/MIN in the limiter, the floor FLMINFR
+    chosen by the rounded wind speed), and differences over 1e-3 m/s already
+    straddle kinks on a grid of this size."""
     import torch
     nsteps, every = 12, 3
     cfg, mask, f, u, v, obs, g, counter = setup(ny=10, nx=12, seed=21, nsteps=nsteps)
@@ -276,7 +286,7 @@ def test_window_gradient_matches_fp64_finite_differences():
     target = np.full((nsteps // every, obs.size), 1.0) + 0.1 * rng.standard_normal((nsteps // every, obs.size))
     j, gu, gv, hs, _ = window_grad(torch, g, cfg, f, u, v, obs, nsteps, every, 3, T(target))
     gu, gv = gu.cpu().numpy().astype(np.float64), gv.cpu().numpy().astype(np.float64)
-    h = 1e-3
+    h = 1e-5
     ok, flips = 0, 0
     worst = 0.0
     for d in range(50):

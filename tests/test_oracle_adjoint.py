@@ -70,3 +70,34 @@ def test_tangent_linear_matches_finite_differences_and_adjoint_is_its_transpose(
         e[k] = 1.0
         _, col = o.step_ad(fl3, tw, z0, u, v, f["depth"], dx=e)
         assert abs(np.dot(col, ybar) - xbar[k]) <= 1e-12 * max(1.0, np.abs(col).sum() * np.abs(ybar).max())
+
+
+def test_fp32_tape_is_the_fp32_step_and_its_adjoint_the_transpose(case):
+    """The REAL*4-valued tape (liborc_ad32): its value is the FP32
+    restatement's step bit for bit (so its branches are that build's, which
+    the GPU kernels reproduce), and its adjoint is the transpose of its
+    tangent-linear model."""
+    cfg, mask, f, o, n, fl3, u, v, tw, z0 = case
+    o32 = Oracle(cfg, mask, ad32=True)
+    o32.prepare_propagation(f["depth"])
+    f32 = fl3.astype(np.float32)
+    u32, v32 = u.astype(np.float32), v.astype(np.float32)
+    y = o32.step_ad(f32, tw.astype(np.float32), z0.astype(np.float32), u32, v32, f["depth"])
+    p = Oracle(cfg, mask)
+    p.prepare_propagation(f["depth"])
+    u10 = np.sqrt(u32 * u32 + v32 * v32)
+    from oracle.oracle import shared_math
+    ud = np.where(u10 != 0, shared_math(11, u32, v32), np.float32(0)).astype(np.float32)
+    ud = np.where(ud < 0, (ud + np.float32(2 * np.pi)).astype(np.float32), ud)
+    r = p.implsch(p.propags(f32), u10, ud, tw.astype(np.float32), z0.astype(np.float32), f["depth"])
+    nb = fl3.size
+    assert np.array_equal(y[:nb].astype(np.float32).view(np.int32), r["fl3"].view(np.int32))
+    assert np.array_equal(y[nb:nb + n].astype(np.float32).view(np.int32), r["tauw"].view(np.int32))
+    assert np.array_equal(y[nb + n:].astype(np.float32).view(np.int32), r["z0"].view(np.int32))
+    rng = np.random.default_rng(2)
+    dx = rng.standard_normal(nb + 4 * n)
+    _, dy = o32.step_ad(f32, tw, z0, u32, v32, f["depth"], dx=dx)
+    ybar = rng.standard_normal(nb + 2 * n)
+    _, xbar = o32.step_ad(f32, tw, z0, u32, v32, f["depth"], ybar=ybar)
+    lhs, rhs = np.dot(dy, ybar), np.dot(dx, xbar)
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)

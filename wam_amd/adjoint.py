@@ -22,18 +22,22 @@ class WindowModel:
     """A window of nsteps steps on the prepared context ``gpu`` (WamGpu),
     observed at the sea points ``obs_points`` (0-based) after every
     ``obs_every``-th step; ``nsnap`` checkpoint slots besides the initial
-    state (None: one per step, nothing recomputed).  ROAIRN and WSTAR are the
-    constants of the reference's runs without air-density or gustiness
+    state (None: one per step, nothing recomputed).  ``nmem`` > 1 advances an
+    ensemble (member-major arrays, as the ensemble entry points): winds
+    [nsteps][nmem*NSEA], spectra [nmem][ML][KL][NSEA].  ROAIRN and WSTAR are
+    the constants of the reference's runs without air-density or gustiness
     input (1.225 and 0, wam_initial_module.f90:193-194) unless given."""
 
-    def __init__(self, gpu, nsteps, obs_points, obs_every=1, nsnap=None, roairn=None, wstar=None):
+    def __init__(self, gpu, nsteps, obs_points, obs_every=1, nsnap=None, roairn=None, wstar=None, nmem=1):
         import torch
         self.gpu, self.nsteps, self.obs_every = gpu, int(nsteps), int(obs_every)
         self.nsnap = self.nsteps - 1 if nsnap is None else int(nsnap)
+        self.nmem = int(nmem)
         dev = torch.device("cuda:0")
         self.dev = dev
-        n = gpu.nsea
-        self.n, self.nb = n, gpu.kl * gpu.ml * n
+        n = gpu.nsea * self.nmem
+        self.nbin = gpu.kl * gpu.ml
+        self.n, self.nb = n, self.nbin * n
         self.pts = torch.as_tensor(np.asarray(obs_points, np.int64), device=dev)
         f = lambda: torch.zeros(self.nb, device=dev)
         p = lambda: torch.zeros(n, device=dev)
@@ -70,13 +74,20 @@ class WindowModel:
     def _step(self, u, v, i):
         g = self.gpu
         g.wind_uv_dev(u[i], v[i], self.u10, self.udir)
-        g.propags_dev(self.fa, self.fb)
-        g.implsch_dev(self.fb, self.u10, self.udir, self.tauw, self.ustar, self.z0, self.roairn, self.wstar)
+        if self.nmem == 1:
+            g.propags_dev(self.fa, self.fb)
+            g.implsch_dev(self.fb, self.u10, self.udir, self.tauw, self.ustar, self.z0, self.roairn, self.wstar)
+        else:
+            g.propags_ensemble_dev(self.nmem, self.fa, self.fb)
+            g.implsch_ensemble_dev(self.nmem, self.fb, self.u10, self.udir, self.tauw, self.ustar, self.z0,
+                                   self.roairn, self.wstar)
         self.fa, self.fb = self.fb, self.fa
         self.forward_steps += 1
 
     def _sample(self):
-        return self.fa.view(self.nb // self.n, self.n).index_select(1, self.pts)
+        """[KL*ML][npts] (one member) or [nmem][KL*ML][npts]."""
+        s = self.fa.view(self.nmem * self.nbin, self.gpu.nsea).index_select(1, self.pts)
+        return s if self.nmem == 1 else s.view(self.nmem, self.nbin, -1)
 
     def forward(self, u, v, fl3, tauw):
         """Runs the window; returns [nobs_t][KL*ML][npts] spectra at the
@@ -91,7 +102,8 @@ class WindowModel:
             if op == CK_STORE:
                 lead.append((a, b))
         self.first = next(i for i, (op, _, _) in enumerate(self.actions) if op == CK_ADJOINT)
-        out = torch.empty(self.nobs_t, self.nb // self.n, self.pts.numel(), device=self.dev)
+        shape = (self.nbin, self.pts.numel()) if self.nmem == 1 else (self.nmem, self.nbin, self.pts.numel())
+        out = torch.empty((self.nobs_t,) + shape, device=self.dev)
         self.last = (torch.empty(self.nb, device=self.dev), torch.empty(self.n, device=self.dev))
         for i in range(self.nsteps):
             for a, b in lead:
@@ -118,7 +130,7 @@ class WindowModel:
         u10b, udirb = torch.zeros(self.n, device=self.dev), torch.zeros(self.n, device=self.dev)
         self.fa.copy_(self.last[0])
         self.tauw.copy_(self.last[1])
-        nbin = self.nb // self.n
+        ns = self.gpu.nsea
         for op, a, b in self.actions[self.first:]:
             if op == CK_ADVANCE:
                 for i in range(a, b):
@@ -130,11 +142,11 @@ class WindowModel:
             else:
                 if (a + 1) % self.obs_every == 0:
                     k = (a + 1) // self.obs_every - 1
-                    fv = fbar.view(nbin, self.n)
-                    fv.index_add_(1, self.pts, g_out[k])
+                    fv = fbar.view(self.nmem * self.nbin, ns)
+                    fv.index_add_(1, self.pts, g_out[k].reshape(self.nmem * self.nbin, -1))
                 gpu.wind_uv_dev(u[a], v[a], self.u10, self.udir)
                 gpu.step_adjoint_dev(self.fa, self.u10, self.udir, self.tauw, self.roairn, self.wstar, fbar, fbar,
-                                     u10b, udirb, twb_out, tauw_bar=twb)
+                                     u10b, udirb, twb_out, tauw_bar=twb, nmem=self.nmem)
                 twb, twb_out = twb_out, twb
                 gpu.wind_uv_adjoint_dev(u[a], v[a], u10b, udirb, ub[a], vb[a])
         return ub, vb, fbar, twb
