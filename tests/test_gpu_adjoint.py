@@ -141,12 +141,14 @@ def oracle_step_adjoint(cfg, mask, f, fl3, tauw, u, v, ybar):
     return xbar
 
 
-@pytest.mark.parametrize("which", ["fl3", "tauw", "z0", "all"])
-def test_step_adjoint_matches_the_oracle(which):
+@pytest.mark.parametrize("which,kl", [("fl3", 24), ("tauw", 24), ("z0", 24), ("all", 24), ("all", 30)])
+def test_step_adjoint_matches_the_oracle(which, kl):
     """Bars of a step's inputs for random bars of its outputs (FL3', TAUW',
-    Z0' all, or one at a time), against the oracle's FP64 tape."""
+    Z0' all, or one at a time), against the oracle's REAL*4-valued tape; 24
+    directions run the split kernels, 30 (not a multiple of 4) the
+    one-workgroup-per-point kernel."""
     import torch
-    cfg, mask, f, u, v, obs, g, counter = setup(ny=8, nx=10, seed=7, nsteps=4)
+    cfg, mask, f, u, v, obs, g, counter = setup(ny=8, nx=10, seed=7, nsteps=4, kl=kl)
     n, nb = g.nsea, g.kl * g.ml * g.nsea
     fl3, tauw, ro, ws = spun_up_state(torch, g, cfg, f, u, v)
     rng = np.random.default_rng(17)
@@ -174,7 +176,7 @@ def test_step_adjoint_matches_the_oracle(which):
     parts = dict(fl3=(fbo, xbar[:nb]), tauw=(tbo, xbar[nb:nb + n]), u=(ub, xbar[nb + 2 * n:nb + 3 * n]),
                  v=(vb, xbar[nb + 3 * n:]))
     errs = {k: rel(a.cpu().numpy(), b) for k, (a, b) in parts.items()}
-    print(f"[case] {which} NSEA={n} COUNTER={counter} rel errors {errs}", flush=True)
+    print(f"[case] {which} KL={kl} NSEA={n} COUNTER={counter} rel errors {errs}", flush=True)
     # FL3's bar to FP32 round-off; TAUW's and the winds' bars pass through
     # AIRSEA's iteration derivatives, accumulated in FP32
     bars = dict(fl3=1e-6, tauw=2e-4, u=1e-4, v=1e-4)

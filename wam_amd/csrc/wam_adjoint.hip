@@ -28,141 +28,9 @@
 //    ATAN2(U, V) (+ ZPI when negative), wam_wind_module.f90:1157-1167, and
 //    its adjoint.
 #include "wam_source_dev.hpp"
+#include "wam_dual.hpp"
 
 namespace wam {
-
-// ---------------------------------------------------------------------------
-// dual numbers: a value and its derivatives along two input directions.  The
-// value part performs the forward kernels' operations in their order, so it
-// is bit-identical to them and takes the same branches.
-// ---------------------------------------------------------------------------
-struct D2 {
-    float v, a, b;
-};
-__device__ __forceinline__ D2 dc(float v) { return {v, 0.f, 0.f}; }
-__device__ __forceinline__ D2 operator+(D2 x, D2 y) { return {x.v + y.v, x.a + y.a, x.b + y.b}; }
-__device__ __forceinline__ D2 operator-(D2 x, D2 y) { return {x.v - y.v, x.a - y.a, x.b - y.b}; }
-__device__ __forceinline__ D2 operator*(D2 x, D2 y)
-{
-    return {x.v * y.v, x.a * y.v + x.v * y.a, x.b * y.v + x.v * y.b};
-}
-__device__ __forceinline__ D2 operator/(D2 x, D2 y)
-{
-    const float q = x.v / y.v;
-    return {q, (x.a - q * y.a) / y.v, (x.b - q * y.b) / y.v};
-}
-__device__ __forceinline__ D2 operator+(D2 x, float c) { return {x.v + c, x.a, x.b}; }
-__device__ __forceinline__ D2 operator+(float c, D2 x) { return {c + x.v, x.a, x.b}; }
-__device__ __forceinline__ D2 operator-(D2 x, float c) { return {x.v - c, x.a, x.b}; }
-__device__ __forceinline__ D2 operator-(float c, D2 x) { return {c - x.v, -x.a, -x.b}; }
-__device__ __forceinline__ D2 operator*(D2 x, float c) { return {x.v * c, x.a * c, x.b * c}; }
-__device__ __forceinline__ D2 operator*(float c, D2 x) { return {c * x.v, c * x.a, c * x.b}; }
-__device__ __forceinline__ D2 operator/(D2 x, float c) { return {x.v / c, x.a / c, x.b / c}; }
-__device__ __forceinline__ D2 operator/(float c, D2 y)
-{
-    const float q = c / y.v;
-    return {q, -(q * y.a) / y.v, -(q * y.b) / y.v};
-}
-__device__ __forceinline__ D2 dsqrt(D2 x)
-{
-    const float r = sqrtf(x.v), g = r > 0.f ? 0.5f / r : 0.f;
-    return {r, g * x.a, g * x.b};
-}
-__device__ __forceinline__ D2 dlog(D2 x) { return {wam_logf(x.v), x.a / x.v, x.b / x.v}; }
-__device__ __forceinline__ D2 dexp(D2 x)
-{
-    const float e = wam_expf(x.v);
-    return {e, e * x.a, e * x.b};
-}
-__device__ __forceinline__ D2 dpow(D2 x, float y)
-{
-    const float p = wam_powf(x.v, y), g = x.v != 0.f ? (y * p) / x.v : 0.f;
-    return {p, g * x.a, g * x.b};
-}
-__device__ __forceinline__ D2 dipow(D2 x, int n)
-{
-    const float g = (float)n * wam_ipowf(x.v, n - 1);
-    return {wam_ipowf(x.v, n), g * x.a, g * x.b};
-}
-// MAX and MIN as the forward kernels take them (ties select the second)
-__device__ __forceinline__ D2 dmax(D2 x, D2 y) { return x.v > y.v ? x : y; }
-__device__ __forceinline__ D2 dmin(D2 x, D2 y) { return x.v < y.v ? x : y; }
-
-// AIRSEA (wam_source_module.f90:868-971) with derivatives along (UTOP, TAUW);
-// the same operations as airsea().  ja = dUST/dUTOP, dUST/dTAUW, dZ0/dUTOP,
-// dZ0/dTAUW.
-__device__ __forceinline__ void airsea_d(const DevTables &t, float utop_v, float tauw_v, float &ust_v, float &z0_v,
-                                         float *ja)
-{
-    const D2 utop = {utop_v, 1.f, 0.f}, tauw = {tauw_v, 0.f, 1.f};
-    const float xlogxl = wam_logf(t.XNLEV), alphaog = t.ALPHA / t.G;
-    const D2 xkutop = t.XKAPPA * utop;
-    const D2 ustold = utop * dsqrt(8.0E-4f + 8.0E-5f * utop);
-    D2 tauold = dmax(ustold * ustold, tauw + t.EPS1);
-    D2 u = dsqrt(tauold);
-    D2 ustm1 = 1.0f / dmax(u, dc(1.0E-6f));
-    D2 z0ch = dc(0.f);
-    for (int it = 1; it <= 15; it++) {
-        const D2 x = tauw / tauold;
-        z0ch = (alphaog * tauold) / dsqrt(dmax(1.0f - x, dc(t.EPS1)));
-        const D2 z0vis = t.RNUAIRM * ustm1;
-        const D2 z0tot = z0ch + z0vis;
-        const D2 xologz0 = 1.0f / (xlogxl - dlog(z0tot));
-        const D2 f = u - xkutop * xologz0;
-        const D2 zz = (ustm1 * ((z0ch * (2.0f - 3.0f * x)) / (1.0f - x) - z0vis)) / z0tot;
-        const D2 delf = 1.0f - (xkutop * (xologz0 * xologz0)) * zz;
-        u = u - f / delf;
-        const D2 taunew = dmax(u * u, tauw + t.EPS1);
-        u = dsqrt(taunew);
-        if (taunew.v == tauold.v) break;
-        ustm1 = 1.0f / dmax(u, dc(1.0E-6f));
-        tauold = taunew;
-    }
-    ust_v = u.v;
-    z0_v = z0ch.v;
-    ja[0] = u.a; ja[1] = u.b; ja[2] = z0ch.a; ja[3] = z0ch.b;
-}
-
-// WSIGSTAR (:4250-4339) with derivatives along (USTAR, Z0).
-__device__ __forceinline__ D2 wsigstar_d(const DevTables &t, float ust_v, float z0_v, float wstar)
-{
-    const D2 ust = {ust_v, 1.f, 0.f}, z0 = {z0_v, 0.f, 1.f};
-    const float p1 = 1.48f, p2 = -0.21f, c1 = 1.03E-3f, c2 = 0.04E-3f;
-    D2 u10 = (ust * (1.0f / t.XKAPPA)) * (2.30258509299405f - dlog(z0));
-    u10 = dmax(u10, dc(1.0f));
-    const D2 u10m1 = 1.0f / u10;
-    const D2 c2u = c2 * dpow(u10, p1);
-    const D2 u10p2 = dpow(u10, p2);
-    const D2 c_d = (c1 + c2u) * u10p2;
-    const D2 dcddu = (((p2 * c1) + (p1 + p2) * c2u) * u10p2) * u10m1;
-    const D2 sigc = 1.0f + ((0.5f * u10) / c_d) * dcddu;
-    const D2 base = 0.0f * dipow(ust, 3) + 0.2f * wam_ipowf(wstar, 3);
-    return dmin(dc(0.1f), (sigc * u10m1) * dpow(base, 1.0f / 3.0f));
-}
-
-// TAU_PHI_HF's node j (1..19) of the stress integral (:3853-3966, XLEVTAIL =
-// 0) with derivatives along (USTAR, Z0), as the parallel form in stresso().
-__device__ __forceinline__ D2 tau_node_d(const DevTables &t, const float *fr, int mij, float ust_v, float z0_v, int j)
-{
-    const D2 ust0 = {ust_v, 1.f, 0.f}, z0 = {z0_v, 0.f, 1.f};
-    const float gm1 = 1.0f / t.G;
-    const float omegac = t.ZPI * fr[mij - 1];
-    const D2 tauw0 = ust0 * ust0;
-    const D2 xloggz0 = dlog(t.G * z0);
-    const D2 omegacc = dmax(dc(omegac), (t.X0TAUHF * t.G) / ust0);
-    const D2 sqz = dsqrt(z0 * gm1), sqg = 1.0f / sqz;
-    const D2 zinf = dlog(omegacc * sqz);
-    const D2 delz = dmax((0.0f - zinf) / 18.f, dc(0.0f));
-    const D2 ust = j == 1 ? ust0 : dsqrt(tauw0);
-    const D2 y = dexp(zinf + (float)(j - 1) * delz);
-    const D2 cm1 = (y * sqg) * gm1;
-    const D2 zx = ust * cm1 + t.ZALP;
-    const D2 zarg = t.XKAPPA / zx;
-    D2 zlog = (xloggz0 + 2.0f * dlog(cm1)) + zarg;
-    zlog = dmin(zlog, dc(0.0f));
-    const D2 zbeta = dexp(zlog) * dipow(zlog, 4);
-    return ((zbeta * tauw0) * t.WTAUHF[j - 1]) * delz;
-}
 
 // SINPUT of one bin (sin_bin's operations) with what its reverse needs: the
 // wind-input terms' ZLOG, X and EXP(ZLOG) of each active NSIN term.
@@ -825,6 +693,542 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) im
 }
 
 // ---------------------------------------------------------------------------
+// IMPLSCH's adjoint in kernels of the forward split's shape (IPHYS = 0, deep
+// water, KL % 4 == 0): the per-point scalar chains run one sea point per
+// lane, SNONLIN one point per workgroup, on the tiled work arrays.
+//   implsch_pre_kernel<0, 1>, implsch_snl_kernel<0, 1> (wam_implsch.hip): the
+//       forward recomputation up to the limiter, keeping what the reverse
+//       needs (x: F after the tail, the final SL and FL, LLWS; e: the
+//       limiter's output, LLWS of the first SINPUT, COS(TH - UDIR), scalars)
+//   adj_post_kernel: the new cut-off index, then the bar of the limiter's
+//       output (IMPHFTAIL transposed)
+//   adj_snl_kernel: the limiter and SNONLIN reversed (per point)
+//   adj_rev_kernel: SDISSIP, the second SINPUT and STRESSO, AIRSEA, the
+//       tail, the first SINPUT and STRESSO, the mean parameters and AIRSEA
+//       reversed (per lane), the input spectrum's bar out
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) adj_post_kernel(DevTables t, AdjArgs a, ImplScratch x, AdjExtra e)
+{
+    const int KL = t.KL, ML = t.ML, NS = t.NSEA, NB4 = x.NB4, XS = x.NX, lane = threadIdx.x;
+    const int lp0 = blockIdx.x * blockDim.x + lane;
+    const bool on = lp0 < x.NP;
+    const int lp = on ? lp0 : x.NP - 1, mp = x.P0 + lp;
+    const int mem = mp / NS, pt = mp - mem * NS;
+    float *FQ = e.fn + quad_offset(lp, 0, NB4);
+    const unsigned *LLW = x.llws + lp;
+    const float ust = e.sc[(size_t)AS_UST2 * XS + lp];
+    // the new mean parameters and the cut-off index, as implsch_post_kernel
+    float s_e = 0.f, s_fm = 0.f, s_w1 = 0.f, s_w2 = 0.f, Tl = 0.f, Ql = 0.f;
+    for (int i = 0; i < ML; i++) {
+        const int m = i == 0 ? ML - 1 : i - 1;
+        float T = 0.f, Q = 0.f;
+        bool qfirst = true;
+        for (int k0 = 0; k0 < KL; k0 += 4) {
+            const int b0 = k0 + KL * m;
+            const float4 f4 = *reinterpret_cast<const float4 *>(FQ + (size_t)(b0 >> 2) * 256);
+            const unsigned w = LLW[(size_t)(b0 >> 5) * XS] >> (b0 & 31);
+            const float fv[4] = {f4.x, f4.y, f4.z, f4.w};
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int k = k0 + u;
+                const float f = fv[u];
+                T = k == 0 ? f : T + f;
+                if ((w >> u) & 1u) {
+                    Q = qfirst ? f : Q + f;
+                    qfirst = false;
+                }
+            }
+        }
+        if (i == 0) {
+            Tl = T; Ql = Q;
+            s_e = t.MO_TAIL * T; s_fm = t.MM1_TAIL * T; s_w1 = t.MO_TAIL * Q; s_w2 = t.MM1_TAIL * Q;
+        } else {
+            s_e = s_e + T * t.DFIM[m]; s_fm = s_fm + T * t.DFIMOFR[m];
+            s_w1 = s_w1 + Q * t.DFIM[m]; s_w2 = s_w2 + Q * t.DFIMOFR[m];
+        }
+    }
+    {
+        const int m = ML - 1;
+        s_e = s_e + Tl * t.DFIM[m]; s_fm = s_fm + Tl * t.DFIMOFR[m];
+        s_w1 = s_w1 + Ql * t.DFIM[m]; s_w2 = s_w2 + Ql * t.DFIMOFR[m];
+    }
+    const float ev = wam_maxf(s_e, t.EMIN);
+    const float fmean = ev / wam_maxf(s_fm, t.EMIN);
+    const float ew = wam_maxf(s_w1, t.EMIN);
+    const float fmeanws = ew / wam_maxf(s_w2, t.EMIN);
+    const int mij = frcut(t, t.FR, fmean, fmeanws, ust);
+    // IMPHFTAIL transposed: F(K, M >= MIJ) = T2(M) F(K, MIJ) sends T2(M) times
+    // their bars to row MIJ, the rows above get none
+    const float tden = t.FRM5[mij - 1];
+    const float *G = a.fl3_bar + (size_t)mem * KL * ML * NS + pt;
+    for (int k0 = 0; k0 < KL; k0 += 4) {
+        float acc[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int m = mij; m < ML; m++) {
+            const float t2 = t.FRM5[m] / tden;
+#pragma unroll
+            for (int u = 0; u < 4; u++) acc[u] = acc[u] + t2 * G[(size_t)(k0 + u + KL * m) * NS];
+        }
+        for (int m = 0; m < ML; m++) {
+            float v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const float g = m < mij ? G[(size_t)(k0 + u + KL * m) * NS] : 0.f;
+                v[u] = m == mij - 1 ? g + acc[u] : g;
+            }
+            if (on)
+                *reinterpret_cast<float4 *>(FQ + (size_t)((k0 + KL * m) >> 2) * 256) = make_float4(v[0], v[1], v[2], v[3]);
+        }
+    }
+    if (on) e.sc[(size_t)AS_MIJ2 * XS + lp] = (float)mij;
+}
+
+// LDS floats of adj_snl_kernel besides its static arrays
+__host__ __device__ __forceinline__ size_t adj_snl_lds_floats(int KL, int ML, int MH)
+{
+    return 3 * 512 + 25 * (size_t)MH + (size_t)MH + 5 * (size_t)MH + 16 * (size_t)KL + 4 * (size_t)KL * ML;
+}
+
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2)))
+adj_snl_kernel(DevTables t, AdjArgs a, ImplScratch x, AdjExtra e)
+{
+    extern __shared__ __align__(16) float lds_raw[];
+    __shared__ float cu[MAXF], sprd[MAXF], flm[MAXF], tmp2[MAXF], mb0[MAXF], kp[4][MAXF];
+    __shared__ float Pv[3];
+    const int KL = t.KL, ML = t.ML, NB = KL * ML, NB4 = x.NB4, XS = x.NX, tid = threadIdx.x, MH = t.MLSTHG;
+    const int wv = tid >> 6, lane = tid & 63;
+    float *QF = lds_raw, *QP = lds_raw + 512, *QM = lds_raw + 1024;
+    float *p = lds_raw + 1536;
+    float *rnl = p; p += 25 * MH;
+    float *af11 = p; p += MH;
+    int *inl = (int *)p; p += 5 * MH;
+    int *kw = (int *)p; p += 8 * KL;
+    int *kinv = (int *)p; p += 8 * KL;
+    float *F = p; p += NB;
+    float *SL = p; p += NB;
+    float *FL = p; p += NB;
+    float *FB = p; p += NB;
+    for (int i = tid; i < 25 * MH; i += NT) rnl[i] = t.RNLCOEF[i];
+    for (int i = tid; i < 5 * MH; i += NT) inl[i] = t.INLCOEF[i];
+    for (int i = tid; i < MH; i += NT) af11[i] = t.AF11[i];
+    for (int i = tid; i < 2 * KL; i += NT) {
+        kw[i] = t.K1W[i]; kw[2 * KL + i] = t.K2W[i]; kw[4 * KL + i] = t.K11W[i]; kw[6 * KL + i] = t.K21W[i];
+    }
+    for (int i = tid; i < 8 * KL; i += NT) kinv[i] = t.AKINV[i];
+    const float delt = t.DELT;
+    const int nblk = gridDim.x, xcd = blockIdx.x % 8, lb = blockIdx.x / 8;
+    const int nb_x = (nblk + 7 - xcd) / 8;
+    const int per = (x.NP + 7) / 8;
+    const int p0 = xcd * per, p1 = min(x.NP, p0 + per);
+    for (int lp = p0 + lb; lp < p1; lp += nb_x) {
+        const int mp = x.P0 + lp;
+        const size_t q0 = quad_offset(lp, 0, NB4);
+        for (int q = tid; q < NB4; q += NT) {
+            const size_t o = q0 + (size_t)q * 256;
+            reinterpret_cast<float4 *>(F)[q] = *reinterpret_cast<const float4 *>(x.ft + o);
+            reinterpret_cast<float4 *>(SL)[q] = *reinterpret_cast<const float4 *>(x.slt + o);
+            reinterpret_cast<float4 *>(FL)[q] = *reinterpret_cast<const float4 *>(x.flt + o);
+            reinterpret_cast<float4 *>(FB)[q] = *reinterpret_cast<const float4 *>(e.fn + o);
+        }
+        if (tid < KL) {
+            const float c = e.cu[(size_t)tid * XS + lp];
+            cu[tid] = c;
+            sprd[tid] = wam_ipowf(wam_maxf(0.f, c), 2);
+        }
+        if (tid >= 64 && tid - 64 < ML) {
+            const int m = tid - 64;
+            const int ju = min(t.JUMAX, max(wam_nint(a.u10[mp] / t.DELU), 1));
+            flm[m] = t.FLMINFR[(size_t)m * t.JUMAX + ju - 1];
+        }
+        if (tid == 128) {
+            Pv[0] = e.sc[(size_t)AS_UST2 * XS + lp];
+            Pv[1] = e.sc[(size_t)AS_FMEANWS * XS + lp];
+            Pv[2] = e.sc[(size_t)AS_FMEAN * XS + lp];
+        }
+        __syncthreads();
+        if (tid < ML) {
+            const float delfl = (t.C5G / wam_ipowf(t.FR[tid], 4)) * delt;
+            tmp2[tid] = (Pv[0] * delfl) * wam_maxf(Pv[1], Pv[2]);
+        }
+        __syncthreads();
+        // the limiter and implicit update reversed: FB <- the bar of F after
+        // the tail, SL and FL <- their bars
+        {
+            float kacc = 0.f;
+            for (int m = wv; m < ML; m += 4) {
+                float t2s = 0.f;
+                if (lane < KL) {
+                    const int k = lane, b = k + KL * m;
+                    const float fnb = FB[b];
+                    const float y = 1.f - delt * FL[b];
+                    const float den = wam_maxf(1.f, y);
+                    const float temp1 = (delt * SL[b]) / den;
+                    const float a1 = wam_fabsf(temp1);
+                    const float temp3 = wam_minf(a1, tmp2[m]);
+                    const float f = F[b] + wam_signf(temp3, temp1);
+                    float t1b = 0.f, ftb = 0.f;
+                    if (f > flm[m] * sprd[k]) {
+                        ftb = fnb;
+                        if (a1 < tmp2[m]) t1b = fnb;
+                        else t2s = temp1 >= 0.f ? fnb : -fnb;
+                    } else if (cu[k] > 0.f) {
+                        kacc += fnb * (flm[m] * (2.f * cu[k]));
+                    }
+                    FB[b] = ftb;
+                    SL[b] = (t1b * delt) / den;
+                    FL[b] = !(1.f > y) ? ((t1b * temp1) * delt) / den : 0.f;
+                }
+                t2s = wave_sum(t2s);
+                if (lane == 0) mb0[m] = t2s;
+            }
+            if (lane < KL) kp[wv][lane] = kacc;
+        }
+        __syncthreads();
+        if (tid < KL) e.cub[(size_t)tid * XS + lp] = ((kp[0][tid] + kp[1][tid]) + kp[2][tid]) + kp[3][tid];
+        if (tid == 64) {
+            const float fmx = wam_maxf(Pv[1], Pv[2]);
+            float ub = 0.f, fb = 0.f;
+            for (int m = 0; m < ML; m++) {
+                const float delfl = (t.C5G / wam_ipowf(t.FR[m], 4)) * delt;
+                ub += mb0[m] * (delfl * fmx);
+                fb += mb0[m] * (Pv[0] * delfl);
+            }
+            e.sc[(size_t)AS_UST2B * XS + lp] = ub;
+            e.sc[(size_t)AS_FMXB * XS + lp] = fb;
+        }
+        // SNONLIN reversed, chunk by chunk of frequencies
+        {
+            const float dal1 = t.DAL1, dal2 = t.DAL2;
+            for (int c = 0; c < t.ANCH; c++) {
+                const int mc0 = c * t.ACH, nmc = min(t.ACH, MH - mc0), nq = nmc * 2 * KL;
+                for (int ql = tid; ql < nq; ql += NT) {
+                    const int mcl = ql / (2 * KL), r = ql - mcl * 2 * KL, mcm1 = mc0 + mcl;
+                    const int K = r < KL ? r : r - KL;
+                    const int *ic = inl + mcm1 * 5;
+                    const float *rc = rnl + mcm1 * 25;
+                    const int k1 = kw[r], k2 = kw[2 * KL + r], k11 = kw[4 * KL + r], k21 = kw[6 * KL + r];
+                    const int IC = (ic[0] - 1) * KL, IP = (ic[1] - 1) * KL, IP1 = (ic[2] - 1) * KL;
+                    const int IM = (ic[3] - 1) * KL, IM1 = (ic[4] - 1) * KL;
+                    const float sap = rc[1] * F[k1 + IP] + rc[2] * F[k11 + IP] + rc[3] * F[k1 + IP1] + rc[4] * F[k11 + IP1];
+                    const float sam = rc[13] * F[k2 + IM] + rc[14] * F[k21 + IM] + rc[15] * F[k2 + IM1] + rc[16] * F[k21 + IM1];
+                    const float fij = F[K + IC] * rc[0];
+                    const float ftemp = af11[mcm1];
+                    const float fcen = ftemp * fij;
+                    const float fad1 = fij * (sap + sam), fad2 = fad1 - (2.f * sap) * sam;
+                    const int8_t *tr = t.AQ_ROW + 9 * mcm1;
+                    float adb = 0.f, ddb = 0.f, dmb = 0.f, dpb = 0.f;
+                    if (tr[0] >= 0) {
+                        const int b = K + KL * tr[0];
+                        adb -= 2.f * SL[b];
+                        ddb -= 2.f * FL[b];
+                    }
+#pragma unroll
+                    for (int ty = 1; ty <= 8; ty++) {
+                        if (tr[ty] < 0) continue;
+                        const int kk = ty <= 4 ? ((ty & 1) ? k2 : k21) : ((ty & 1) ? k1 : k11);
+                        const int b = kk + KL * tr[ty];
+                        adb += rc[SLOT_SL[ty]] * SL[b];
+                        const float d = rc[SLOT_FL[ty]] * FL[b];
+                        if (ty <= 4) dmb += d;
+                        else dpb += d;
+                    }
+                    float fad2b = adb * fcen, fcenb = adb * fad2;
+                    const float fad1pb = ddb * ftemp;
+                    fcenb += dpb * ((fij - 2.f * sam) * dal1) + dmb * ((fij - 2.f * sap) * dal2);
+                    float fijb = dpb * (dal1 * fcen) + dmb * (dal2 * fcen);
+                    float samb = -2.f * (dpb * (dal1 * fcen)), sapb = -2.f * (dmb * (dal2 * fcen));
+                    fijb += fcenb * ftemp;
+                    float fad1b = fad1pb;
+                    fad2b += fad1pb;
+                    fad1b += fad2b;
+                    sapb -= (2.f * sam) * fad2b;
+                    samb -= (2.f * sap) * fad2b;
+                    fijb += fad1b * (sap + sam);
+                    sapb += fad1b * fij;
+                    samb += fad1b * fij;
+                    QF[ql] = fijb; QP[ql] = sapb; QM[ql] = samb;
+                }
+                __syncthreads();
+                for (int m = wv; m < ML; m += 4) {
+                    const int e0 = t.AR_OFF[c * ML + m], e1 = t.AR_OFF[c * ML + m + 1];
+                    if (lane < KL && e1 > e0) {
+                        float acc = 0.f;
+                        for (int en = e0; en < e1; en++) {
+                            const int v = t.AR_E[en], mcm1 = v >> 4, sl = v & 15;
+                            const float w = rnl[mcm1 * 25 + (sl <= 4 ? sl : sl + 8)];
+                            const float *X = sl == 0 ? QF : (sl <= 4 ? QP : QM);
+                            const int mpk = sl <= 4 ? ((sl - 1) & 1) : 2 + ((sl - 5) & 1);
+                            const int base = (mcm1 - mc0) * 2 * KL;
+#pragma unroll
+                            for (int kh = 0; kh < 2; kh++) {
+                                const int K = sl == 0 ? lane : kinv[(mpk * 2 + kh) * KL + lane];
+                                acc = acc + w * X[base + kh * KL + K];
+                            }
+                        }
+                        FB[lane + KL * m] += acc;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (int q = tid; q < NB4; q += NT) {
+            const size_t o = q0 + (size_t)q * 256;
+            *reinterpret_cast<float4 *>(e.fn + o) = reinterpret_cast<const float4 *>(FB)[q];
+            *reinterpret_cast<float4 *>(x.slt + o) = reinterpret_cast<const float4 *>(SL)[q];
+            *reinterpret_cast<float4 *>(x.flt + o) = reinterpret_cast<const float4 *>(FL)[q];
+        }
+        __syncthreads();
+    }
+}
+
+// STRESSO's modulus reversed (one lane): as stresso_rev_scalar, on plain values
+__device__ __forceinline__ void stresso_rev_lane(const DevTables &t, const float *k, float udir, float ust, float twb,
+                                                 float jt0, float jt1, float &ustb, float &z0b, float &udirb,
+                                                 float &xsb, float &ysb, float &t1b)
+{
+    const float xs = k[0], ys = k[1], tauhf = k[2], tau1 = k[3], t1 = k[4], tw = k[5], c1 = k[6];
+    const float cap = ust * ust - t.EPS1;
+    const float tmin = wam_minf(tw, cap);
+    float tb = 0.f;
+    if (tmin > 0.f) {
+        if (tw < cap) tb = twb;
+        else ustb += twb * (2.f * ust);
+    }
+    float xb = 0.f, yb = 0.f;
+    if (tw > 0.f) { xb = tb * (xs / tw); yb = tb * (ys / tw); }
+    const float su = wam_sinf(udir), co = wam_cosf(udir);
+    const float hb = xb * su + yb * co;
+    udirb += tauhf * (xb * co - yb * su);
+    t1b = hb * (c1 * tau1);
+    const float tau1b = hb * (c1 * t1);
+    ustb += tau1b * jt0;
+    z0b += tau1b * jt1;
+    const float ra = wam_maxf(t.ROAIR, 1.f);
+    xsb = xb / ra;
+    ysb = yb / ra;
+}
+
+// sin_freq reversed for one frequency (one lane): adds the bars of USN1,
+// USN2 and Z0 from the frequency's factor bars dm = {ZCN, UCN1, UCN2, XV1, XV2}
+__device__ __forceinline__ void sin_freq_rev_lane(const DevTables &t, const SinFreq &r, float fr_m, float ust,
+                                                  float sig, float z0, const float (&dm)[5], float &u1, float &u2,
+                                                  float &zs)
+{
+    const float usn1 = ust * (1.0f - sig), usn2 = ust * (1.0f + sig);
+    const float cm = (t.ZPI * fr_m) / t.G;
+    const float d1 = ((usn1 / t.XKAPPA) * r.zcn) * cm, d2 = ((usn2 / t.XKAPPA) * r.zcn) * cm;
+    const float g1 = dm[3] / (d1 * d1), g2 = dm[4] / (d2 * d2);
+    const float zc = (r.zcn * cm) / t.XKAPPA;
+    u1 += dm[1] * cm + g1 * zc;
+    u2 += dm[2] * cm + g2 * zc;
+    zs += ((dm[0] + g1 * ((usn1 / t.XKAPPA) * cm)) + g2 * ((usn2 / t.XKAPPA) * cm)) / z0;
+}
+
+__global__ void __launch_bounds__(256) adj_rev_kernel(DevTables t, AdjArgs a, ImplScratch x, AdjExtra e)
+{
+    extern __shared__ float rev_lds[];            // [3][KL][W]: COS(TH - UDIR), its bar, the tail's sums
+    const int KL = t.KL, ML = t.ML, NB = KL * ML, NS = t.NSEA, NB4 = x.NB4, XS = x.NX, lane = threadIdx.x;
+    const int W = blockDim.x;
+    const int lp0 = blockIdx.x * W + lane;
+    const bool on = lp0 < x.NP;
+    const int lp = on ? lp0 : x.NP - 1, mp = x.P0 + lp;
+    const int mem = mp / NS, pt = mp - mem * NS;
+    float *cu = rev_lds + lane, *cub = rev_lds + (size_t)KL * W + lane, *acc = rev_lds + (size_t)2 * KL * W + lane;
+    for (int k = 0; k < KL; k++) {
+        cu[k * W] = e.cu[(size_t)k * XS + lp];
+        cub[k * W] = e.cub[(size_t)k * XS + lp];
+        acc[k * W] = 0.f;
+    }
+    auto S = [&](int slot) { return e.sc[(size_t)slot * XS + lp]; };
+    const float udir = a.udir[mp], roairn = a.roairn[mp];
+    const int mij = (int)S(AS_MIJ);
+    const float ust1 = S(AS_UST1), z01 = S(AS_Z01), sig1 = S(AS_SIG1);
+    const float ust2 = S(AS_UST2), z02 = S(AS_Z02), sig2 = S(AS_SIG2);
+    const float emean = S(AS_EMEAN), f1mean = S(AS_F1MEAN), fmean = S(AS_FMEAN), fmeanws = S(AS_FMEANWS);
+    const float const3 = ((2.f * t.XKAPPA) / t.BETAMAX) * (t.XKAPPA * t.XKAPPA);
+    const float gm1 = 1.0f / t.G;
+    float ustb = S(AS_UST2B) + (a.ustar_bar ? a.ustar_bar[mp] : 0.f);
+    float z0b = a.z0_bar ? a.z0_bar[mp] : 0.f;
+    float udirb = 0.f, u10b = 0.f;
+    const float fmxb = S(AS_FMXB);
+    const bool ws = fmeanws > fmean;
+    const float fmeanwsb = ws ? fmxb : 0.f, fmeanb = ws ? 0.f : fmxb;
+    // STRESSO, second call: the modulus's bar back to its sums
+    float xsb, ysb, t1b;
+    {
+        float k2[7];
+        for (int i = 0; i < 7; i++) k2[i] = S(AS_K2 + i);
+        stresso_rev_lane(t, k2, udir, ust2, a.tauw_bar ? a.tauw_bar[mp] : 0.f, S(AS_JT2), S(AS_JT2 + 1), ustb, z0b,
+                         udirb, xsb, ysb, t1b);
+    }
+    // pass I: SDISSIP and the second SINPUT/STRESSO reversed; rows at and
+    // above MIJ add their final bars to the tail's sums
+    const float e2 = emean * emean;
+    const float sds = (t.CONSD * e2) * wam_ipowf(f1mean, 9);
+    const float tden = t.FRM5[mij - 1];
+    float sdsb = 0.f, f1b = 0.f, U1 = 0.f, U2 = 0.f, Z = 0.f;
+    const size_t q0 = quad_offset(lp, 0, NB4);
+    {
+        const float usn1 = ust2 * (1.0f - sig2), usn2 = ust2 * (1.0f + sig2);
+        for (int m = 0; m < ML; m++) {
+            const SinFreq r = sin_freq<0>(t, t.FR[m], 0.f, usn1, usn2, z02, roairn);
+            const float xd = wam_ipowf(t.FR[m] / f1mean, 2);
+            const float t2d = sds * (0.5f * xd + 0.5f * wam_ipowf(xd, 2));
+            float cmrw = 0.f;
+            if (m < mij) {
+                float rw = t.RHOWG_DFIM[m];
+                if (m == mij - 1 && mij != ML) rw = 0.5f * rw;
+                cmrw = ((t.ZPI * t.FR[m]) * gm1) * rw;
+            }
+            const float t2m = m >= mij ? t.FRM5[m] / tden : 0.f;
+            float dsum = 0.f;
+            float dm[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+            for (int k0 = 0; k0 < KL; k0 += 4) {
+                const size_t o = q0 + (size_t)((k0 + KL * m) >> 2) * 256;
+                const float4 f4 = *reinterpret_cast<const float4 *>(x.ft + o);
+                const float4 s4 = *reinterpret_cast<const float4 *>(x.slt + o);
+                const float4 l4 = *reinterpret_cast<const float4 *>(x.flt + o);
+                const float4 b4 = *reinterpret_cast<const float4 *>(e.fn + o);
+                const float fv[4] = {f4.x, f4.y, f4.z, f4.w}, sv[4] = {s4.x, s4.y, s4.z, s4.w};
+                const float lv[4] = {l4.x, l4.y, l4.z, l4.w};
+                float bv[4] = {b4.x, b4.y, b4.z, b4.w};
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int k = k0 + u;
+                    const float ft = fv[u], slb = sv[u], flb = lv[u], ck = cu[k * W];
+                    float fb = bv[u] + slb * t2d;
+                    dsum = dsum + (slb * ft + flb);
+                    const SinLin L = sin_bin_lin(t, r, ck, const3);
+                    float sposb = 0.f;
+                    if (m < mij && L.sp * ft > 0.f) sposb = (xsb * cmrw) * t.SINTH[k] + (ysb * cmrw) * t.COSTH[k];
+                    float fbadd = slb * L.fl + sposb * L.sp;
+                    float dtk = 0.f;
+                    sin_bin_rev(t, r, L, ck, const3, sposb * ft, slb * ft + flb, dtk, dm);
+                    if (m == mij - 1 && ck > 0.f) {
+                        fbadd += t1b * ((ck * ck) * ck);
+                        dtk += (t1b * ft) * (3.f * (ck * ck));
+                    }
+                    fb = fb + fbadd;
+                    cub[k * W] += dtk;
+                    if (m >= mij) acc[k * W] += t2m * fb;
+                    bv[u] = fb;
+                }
+                if (on && m < mij) *reinterpret_cast<float4 *>(e.fn + o) = make_float4(bv[0], bv[1], bv[2], bv[3]);
+            }
+            sdsb += dsum * (0.5f * xd + 0.5f * (xd * xd));
+            const float xb = dsum * (sds * (0.5f + xd));
+            f1b += xb * ((-2.f * xd) / f1mean);
+            sin_freq_rev_lane(t, r, t.FR[m], ust2, sig2, z02, dm, U1, U2, Z);
+        }
+    }
+    const float emeanb = sdsb * ((t.CONSD * wam_ipowf(f1mean, 9)) * (2.f * emean));
+    const float f1meanb = f1b + sdsb * ((t.CONSD * e2) * (9.f * wam_ipowf(f1mean, 8)));
+    {
+        const float sigb = ust2 * (U2 - U1);
+        ustb += (U1 * (1.0f - sig2) + U2 * (1.0f + sig2)) + sigb * S(AS_JS2);
+        z0b += Z + sigb * S(AS_JS2 + 1);
+    }
+    // AIRSEA, second call: (UST, Z0) from (U10, TAUW of the first STRESSO)
+    u10b += ustb * S(AS_JA2) + z0b * S(AS_JA2 + 2);
+    const float tw1b = ustb * S(AS_JA2 + 1) + z0b * S(AS_JA2 + 3);
+    ustb = 0.f;
+    z0b = 0.f;
+    // STRESSO, first call
+    {
+        float k1[7];
+        for (int i = 0; i < 7; i++) k1[i] = S(AS_K1 + i);
+        stresso_rev_lane(t, k1, udir, ust1, tw1b, S(AS_JT1), S(AS_JT1 + 1), ustb, z0b, udirb, xsb, ysb, t1b);
+    }
+    // the mean parameters' integrals (EMEAN, FMEAN, F1MEAN; FMEANWS of the first LLWS)
+    float seb, sfmb, stmb = 0.f, sewsb, sfmwsb;
+    {
+        const float s_fm = S(AS_SFM), s_tm = S(AS_STM);
+        const float dfm = wam_maxf(s_fm, t.EMIN);
+        float eb = emeanb + fmeanb / dfm;
+        if (emean > t.EMIN) {
+            const float q = emean / s_tm;
+            const float qb = -f1meanb / (q * q);
+            eb += qb / s_tm;
+            stmb = -(qb * q) / s_tm;
+        }
+        sfmb = s_fm > t.EMIN ? -(fmeanb * fmean) / dfm : 0.f;
+        seb = S(AS_SE) > t.EMIN ? eb : 0.f;
+        const float s_fmws = S(AS_SFMWS);
+        const float dws = wam_maxf(s_fmws, t.EMIN);
+        sewsb = S(AS_SEWS) > t.EMIN ? fmeanwsb / dws : 0.f;
+        sfmwsb = s_fmws > t.EMIN ? -(fmeanwsb * fmeanws) / dws : 0.f;
+    }
+    // pass II: the tail transposed, the first SINPUT/STRESSO and the mean
+    // parameters reversed, the input spectrum's bar out
+    U1 = 0.f; U2 = 0.f; Z = 0.f;
+    {
+        const float usn1 = ust1 * (1.0f - sig1), usn2 = ust1 * (1.0f + sig1);
+        const float *F0 = a.fl3 + (size_t)mem * NB * NS + pt;
+        float *OUT = a.fl3_bar_out + (size_t)mem * NB * NS + pt;
+        const unsigned char *L1 = e.llws1 + lp;
+        for (int m = 0; m < ML; m++) {
+            const SinFreq r = sin_freq<0>(t, t.FR[m], 0.f, usn1, usn2, z01, roairn);
+            float cmrw = 0.f;
+            if (m < mij) {
+                float rw = t.RHOWG_DFIM[m];
+                if (m == mij - 1 && mij != ML) rw = 0.5f * rw;
+                cmrw = ((t.ZPI * t.FR[m]) * gm1) * rw;
+            }
+            const bool last = m == ML - 1;
+            const float tb = (seb * (t.DFIM[m] + (last ? t.MO_TAIL : 0.f)) +
+                              sfmb * (t.DFIMOFR[m] + (last ? t.MM1_TAIL : 0.f))) +
+                             stmb * (t.DFIM_FR[m] + (last ? t.MP1_TAIL : 0.f));
+            const float twsb = sewsb * (t.DFIM[m] + (last ? t.MO_TAIL : 0.f)) +
+                               sfmwsb * (t.DFIMOFR[m] + (last ? t.MM1_TAIL : 0.f));
+            float dm[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+            for (int k0 = 0; k0 < KL; k0 += 4) {
+                float bv[4] = {0.f, 0.f, 0.f, 0.f};
+                if (m < mij) {
+                    const float4 b4 = *reinterpret_cast<const float4 *>(e.fn + q0 + (size_t)((k0 + KL * m) >> 2) * 256);
+                    bv[0] = b4.x; bv[1] = b4.y; bv[2] = b4.z; bv[3] = b4.w;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int k = k0 + u, b = k + KL * m;
+                    const float f0 = F0[(size_t)b * NS], ck = cu[k * W];
+                    float base = bv[u];
+                    if (m == mij - 1) base = base + acc[k * W];
+                    const SinLin L = sin_bin_lin(t, r, ck, const3);
+                    float sposb = 0.f;
+                    if (m < mij && L.sp * f0 > 0.f) sposb = (xsb * cmrw) * t.SINTH[k] + (ysb * cmrw) * t.COSTH[k];
+                    float fbadd = sposb * L.sp + tb;
+                    if (L1[(size_t)b * XS]) fbadd += twsb;
+                    float dtk = 0.f;
+                    sin_bin_rev(t, r, L, ck, const3, sposb * f0, 0.f, dtk, dm);
+                    if (m == mij - 1 && ck > 0.f) {
+                        fbadd += t1b * ((ck * ck) * ck);
+                        dtk += (t1b * f0) * (3.f * (ck * ck));
+                    }
+                    cub[k * W] += dtk;
+                    if (on) OUT[(size_t)b * NS] = base + fbadd;
+                }
+            }
+            sin_freq_rev_lane(t, r, t.FR[m], ust1, sig1, z01, dm, U1, U2, Z);
+        }
+    }
+    {
+        const float sigb = ust1 * (U2 - U1);
+        ustb += (U1 * (1.0f - sig1) + U2 * (1.0f + sig1)) + sigb * S(AS_JS1);
+        z0b += Z + sigb * S(AS_JS1 + 1);
+    }
+    // AIRSEA, first call: (UST, Z0) from (U10, TAUW at the step's start)
+    u10b += ustb * S(AS_JA1) + z0b * S(AS_JA1 + 2);
+    const float tw0b = ustb * S(AS_JA1 + 1) + z0b * S(AS_JA1 + 3);
+    float ub = udirb;
+    for (int k = 0; k < KL; k++) ub += cub[k * W] * wam_sinf(t.TH[k] - udir);
+    if (on) {
+        a.u10_bar[mp] = u10b;
+        a.udir_bar[mp] = ub;
+        a.tauw_bar_out[mp] = tw0b;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // PROPAGS transposed.  Input j of row (M, K) receives, from every output i
 // whose stencil reads it, that output's weight times its bar.
 // ---------------------------------------------------------------------------
@@ -935,6 +1339,42 @@ hipError_t launch_implsch_adjoint(const DevTables &t, const AdjArgs &a, int ncu,
     if (per_cu < 1) return hipErrorInvalidConfiguration;
     const long nblocks = std::min<long>((long)per_cu * ncu, a.NMP);
     hipLaunchKernelGGL(implsch_adj_kernel, dim3((unsigned)nblocks), dim3(NT), lds, st, t, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_implsch_adj_forward(const DevTables &t, const ImplArgs &a, const ImplScratch &x,
+                                      const AdjExtra &ex, int ncu, hipStream_t st, const LaunchTune &tu);
+
+// IMPLSCH's adjoint in the split form on the chunk of member-points of x.
+hipError_t launch_implsch_adjoint_split(const DevTables &t, const AdjArgs &a, const ImplScratch &x, const AdjExtra &ex,
+                                        int ncu, hipStream_t st, const LaunchTune &tu)
+{
+    ImplArgs ia{};
+    ia.fl3 = const_cast<float *>(a.fl3);
+    ia.u10 = a.u10; ia.udir = a.udir; ia.roairn = a.roairn; ia.wstar = a.wstar;
+    // read only by the recomputation (TAUW at the step's start; Z0 in is not used by AIRSEA)
+    ia.tauw = const_cast<float *>(a.tauw);
+    ia.z0 = const_cast<float *>(a.tauw);
+    ia.NMP = a.NMP;
+    hipError_t e = launch_implsch_adj_forward(t, ia, x, ex, ncu, st, tu);
+    if (e != hipSuccess) return e;
+    constexpr int W = 64;
+    const int nwg = (x.NP + W - 1) / W;
+    hipLaunchKernelGGL(adj_post_kernel, dim3(nwg), dim3(W), 0, st, t, a, x, ex);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const size_t lds = adj_snl_lds_floats(t.KL, t.ML, t.MLSTHG) * sizeof(float);
+    e = hipFuncSetAttribute((const void *)adj_snl_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    int per_cu = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, adj_snl_kernel, NT, lds);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) return hipErrorInvalidConfiguration;
+    const long nblocks = std::min<long>((long)per_cu * ncu, x.NP);
+    hipLaunchKernelGGL(adj_snl_kernel, dim3((unsigned)nblocks), dim3(NT), lds, st, t, a, x, ex);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(adj_rev_kernel, dim3(nwg), dim3(W), (size_t)3 * t.KL * W * sizeof(float), st, t, a, x, ex);
     return hipGetLastError();
 }
 

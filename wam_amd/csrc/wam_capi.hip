@@ -40,6 +40,8 @@ hipError_t launch_implsch_split(const DevTables &t, const ImplArgs &a, const Imp
                                 const hipEvent_t *ev, const LaunchTune &tu);
 bool implsch_split_supported(const DevTables &t);
 hipError_t launch_implsch_adjoint(const DevTables &t, const AdjArgs &a, int ncu, hipStream_t st);
+hipError_t launch_implsch_adjoint_split(const DevTables &t, const AdjArgs &a, const ImplScratch &x, const AdjExtra &ex,
+                                        int ncu, hipStream_t st, const LaunchTune &tu);
 hipError_t launch_propags_adjoint(const DevTables &t, const float *coef, int nmem, const float *gin, float *gout,
                                   hipStream_t st);
 hipError_t launch_wind_uv(float zpi, const float *u, const float *v, float *u10, float *udir, int64_t n,
@@ -147,6 +149,9 @@ struct wam_ctx {
     size_t adj_n = 0;
     float *adj_coef = nullptr;
     int adj_sym = -1;
+    // the split IMPLSCH adjoint's work arrays for a chunk of member-points
+    ImplScratch asx{};
+    AdjExtra aex{};
     int ncu = 256;                 // compute units of the device
     std::string err;
 };
@@ -766,6 +771,33 @@ static int ensure_impl_scratch(wam_ctx *ctx, ImplScratch &x, size_t cap, int lcf
         HIPCHK(hipMalloc(&x.tick, 8 * 32 * sizeof(unsigned)));
     }
     if (lcflx && !x.smt) HIPCHK(hipMalloc(&x.smt, tiled * sizeof(float)));
+    return WAM_OK;
+}
+
+static void free_adj_extra(AdjExtra &e)
+{
+    for (void *p : {(void *)e.fn, (void *)e.llws1, (void *)e.cu, (void *)e.cub, (void *)e.sc})
+        if (p) (void)hipFree(p);
+    e = AdjExtra{};
+}
+
+// The split IMPLSCH adjoint's work arrays for cap member-points: those of
+// the forward split (ctx->asx) and AdjExtra's.
+static int ensure_adj_scratch(wam_ctx *ctx, size_t cap)
+{
+    const Tables &t = ctx->tab;
+    const size_t nb = (size_t)t.KL * t.ML;
+    if (ctx->asx.ft && (size_t)ctx->asx.NX != cap) free_adj_extra(ctx->aex);
+    int rc = ensure_impl_scratch(ctx, ctx->asx, cap, 0);
+    if (rc) return rc;
+    if (!ctx->aex.fn) {
+        const size_t tiled = (cap + 63) / 64 * 64 * nb;
+        HIPCHK(hipMalloc(&ctx->aex.fn, tiled * sizeof(float)));
+        HIPCHK(hipMalloc(&ctx->aex.llws1, nb * cap));
+        HIPCHK(hipMalloc(&ctx->aex.cu, (size_t)t.KL * cap * sizeof(float)));
+        HIPCHK(hipMalloc(&ctx->aex.cub, (size_t)t.KL * cap * sizeof(float)));
+        HIPCHK(hipMalloc(&ctx->aex.sc, (size_t)AS_NSC * cap * sizeof(float)));
+    }
     return WAM_OK;
 }
 
@@ -1953,7 +1985,22 @@ int wam_gpu_step_adjoint_dev(wam_ctx *ctx, int32_t nmem, const float *d_fl3, con
     rc = timing_events(ctx, ev, 2);
     if (rc) return rc;
     timing_record(ctx, ev[0]);
-    HIPCHK(launch_implsch_adjoint(ctx->dt, a, ctx->ncu, ctx->stream));
+    if (implsch_split_supported(ctx->dt)) {
+        // in chunks of member-points, as the ensemble's IMPLSCH
+        const size_t nmp = (size_t)a.NMP, cap = ensemble_chunk(ctx, nmp);
+        rc = ensure_adj_scratch(ctx, cap);
+        if (rc) return rc;
+        ImplScratch &x = ctx->asx;
+        for (size_t p0 = 0; p0 < nmp; p0 += cap) {
+            x.P0 = (int)p0;
+            x.NP = (int)std::min(cap, nmp - p0);
+            HIPCHK(launch_implsch_adjoint_split(ctx->dt, a, x, ctx->aex, ctx->ncu, ctx->stream, ctx->tune));
+        }
+        x.P0 = 0;
+        x.NP = x.NX;
+    } else {
+        HIPCHK(launch_implsch_adjoint(ctx->dt, a, ctx->ncu, ctx->stream));
+    }
     timing_record(ctx, ev[1]);
     timing_keep(ctx, "implsch_adjoint", ev[0], ev[1]);
     // 3. PROPAGS transposed into d_fl3_bar_out
@@ -2189,6 +2236,8 @@ void wam_gpu_finalize(wam_ctx *ctx)
     if (ctx->fscratch_e) (void)hipFree(ctx->fscratch_e);
     if (ctx->adj_fp) (void)hipFree(ctx->adj_fp);
     if (ctx->adj_fb) (void)hipFree(ctx->adj_fb);
+    free_impl_scratch(ctx->asx);
+    free_adj_extra(ctx->aex);
     if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
     if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
     for (hipEvent_t e : ctx->gev)

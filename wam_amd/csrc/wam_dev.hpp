@@ -137,6 +137,27 @@ struct ImplScratch {
     unsigned *tick;                // [8][32]: point counters of implsch_snl_kernel, one per XCD (own 128-byte line)
 };
 
+// Per-point scalars the adjoint's forward recomputation keeps (rows of
+// AdjExtra::sc): AIRSEA's values and Jacobians d(UST, Z0)/d(U10, TAUW) of
+// both calls, WSIGSTAR's and TAU_PHI_HF's derivatives along (UST, Z0),
+// STRESSO's kept values (stresso_close), the mean parameters and their
+// integrals, the cut-off indices and the limiter's bars.
+enum {
+    AS_UST1 = 0, AS_Z01, AS_JA1, AS_SIG1 = AS_JA1 + 4, AS_JS1, AS_JT1 = AS_JS1 + 2, AS_K1 = AS_JT1 + 2,
+    AS_SE = AS_K1 + 7, AS_SFM, AS_STM, AS_EMEAN, AS_FMEAN, AS_F1MEAN, AS_SEWS, AS_SFMWS, AS_EMEANWS, AS_FMEANWS,
+    AS_MIJ, AS_TW1, AS_UST2, AS_Z02, AS_JA2, AS_SIG2 = AS_JA2 + 4, AS_JS2, AS_JT2 = AS_JS2 + 2, AS_K2 = AS_JT2 + 2,
+    AS_MIJ2 = AS_K2 + 7, AS_UST2B, AS_FMXB, AS_NSC
+};
+
+// The adjoint's work arrays besides ImplScratch's (rows NX apart, as there).
+struct AdjExtra {
+    float *fn;              // tiled: the limiter's output, then its bar, then the bar of F after the tail
+    unsigned char *llws1;   // [KL*ML][NX]: LLWS of the first SINPUT
+    float *cu;              // [KL][NX]: COS(TH(K) - UDIR)
+    float *cub;             // [KL][NX]: the bar of cu from the limiter's floor
+    float *sc;              // [AS_NSC][NX]
+};
+
 // Launch shapes, chosen by the autotuner (wam_gpu_autotune); every choice
 // gives the same results, bit for bit.
 struct LaunchTune {

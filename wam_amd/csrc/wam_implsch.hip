@@ -17,6 +17,7 @@
 // parallel into LDS, then every bin adds its contributions from that chunk in
 // the order of the reference's sequential scatter (host-built lists).
 #include "wam_source_dev.hpp"
+#include "wam_dual.hpp"
 
 namespace wam {
 
@@ -384,9 +385,13 @@ constexpr int PW = 256;   // most sea points per workgroup of the per-lane kerne
 
 // STRESSO's closing part (:3164-3356) for one point: the direction sums phi,
 // xs, ys over M < MIJ, the tail moments h1, h2 of F(., MIJ) and TAU_PHI_HF in
-// its sequential form.  Returns TAUW; PHIAW in phiaw.
+// its sequential form.  Returns TAUW; PHIAW in phiaw.  keep (the adjoint's
+// recomputation) receives the stress components after the high-frequency
+// part, TAUHF, TAU_PHI_HF's stress integral, h1, the modulus before its
+// clamps and CONST1.
 __device__ __forceinline__ float stresso_close(const DevTables &t, int mij, float ust, float z0, float udir, float phi,
-                                               float xs, float ys, float h1, float h2, float &phiaw)
+                                               float xs, float ys, float h1, float h2, float &phiaw,
+                                               float *keep = nullptr)
 {
     const float gm1 = 1.0f / t.G;
     const float cst = (t.DELTH * t.ZPI4) * gm1;
@@ -401,6 +406,9 @@ __device__ __forceinline__ float stresso_close(const DevTables &t, int mij, floa
     xs = xs + tauhf * wam_sinf(udir);
     ys = ys + tauhf * wam_cosf(udir);
     float tw = sqrtf(xs * xs + ys * ys);
+    if (keep) {
+        keep[0] = xs; keep[1] = ys; keep[2] = tauhf; keep[3] = tau1; keep[4] = h1; keep[5] = tw; keep[6] = c1;
+    }
     tw = wam_minf(tw, ust * ust - t.EPS1);
     tw = wam_maxf(tw, 0.f);
     phiaw = phi;
@@ -421,8 +429,12 @@ __device__ __forceinline__ float stresso_cm(const DevTables &t, int m, float tfk
     return SHALLOW ? tfk_m * (1.0f / (t.ZPI * t.FR[m])) : (t.ZPI * t.FR[m]) * (1.0f / t.G);
 }
 
-template <int SHALLOW>
-__global__ void __launch_bounds__(PW) __attribute__((amdgpu_waves_per_eu(4))) implsch_pre_kernel(DevTables t, ImplArgs a, ImplScratch x)
+// ADJ = 1 is the adjoint's recomputation of the same stages (identical
+// values): it keeps LLWS of the first SINPUT, COS(TH - UDIR), the scalar
+// chains' derivatives and STRESSO's values in e, and leaves the caller's
+// TAUW, USTAR, Z0 and PHIAW as they are.
+template <int SHALLOW, int ADJ = 0>
+__global__ void __launch_bounds__(PW) __attribute__((amdgpu_waves_per_eu(4))) implsch_pre_kernel(DevTables t, ImplArgs a, ImplScratch x, AdjExtra e)
 // (workgroups of 64, 128 or 256 lanes, one sea point each)
 {
     extern __shared__ float cu_lds[];            // [KL][W]: COS(TH(K) - UDIR) of each lane's point
@@ -443,8 +455,14 @@ __global__ void __launch_bounds__(PW) __attribute__((amdgpu_waves_per_eu(4))) im
         cu[k * W] = c;
         // the limiter's floor FLMINFR(JU, M) * SPRD(K) (:494-498) is applied by implsch_snl_kernel
         if (on) x.sprd[(size_t)k * XS + lp] = wam_ipowf(wam_maxf(0.f, c), 2);
+        if (ADJ && on) e.cu[(size_t)k * XS + lp] = c;
     }
-    airsea(t, u10, tauw, ust, z0);
+    float ja[4];
+    auto put = [&](int slot, float v) {
+        if (ADJ && on) e.sc[(size_t)slot * XS + lp] = v;
+    };
+    if (ADJ) airsea_d(t, u10, tauw, ust, z0, ja);
+    else airsea(t, u10, tauw, ust, z0);
 
     const float const3 = ((2.f * t.XKAPPA) / t.BETAMAX) * (t.XKAPPA * t.XKAPPA);
     const float d1 = sqrtf(t.G) / t.ZPI, d2 = t.ZPI / sqrtf(t.G);
@@ -474,6 +492,7 @@ __global__ void __launch_bounds__(PW) __attribute__((amdgpu_waves_per_eu(4))) im
                     float sp, fl;
                     bool ll;
                     sin_bin(t, r, cu[k * W], const3, sp, fl, ll);
+                    if (ADJ && on) e.llws1[(size_t)(k + KL * m) * XS + lp] = ll;
                     if (ll) {
                         Q = qfirst ? f : Q + f;
                         qfirst = false;
@@ -544,10 +563,25 @@ __global__ void __launch_bounds__(PW) __attribute__((amdgpu_waves_per_eu(4))) im
             xs = xs + axy[(size_t)ML * XS] * cmrw;
             ys = ys + axy[(size_t)2 * ML * XS] * cmrw;
         }
-        float phiaw;
-        tauw = stresso_close(t, mij, ust, z0, udir, phi, xs, ys, h1, h2, phiaw);
+        float phiaw, keep[7];
+        tauw = stresso_close(t, mij, ust, z0, udir, phi, xs, ys, h1, h2, phiaw, ADJ ? keep : nullptr);
+        if (ADJ) {
+            put(AS_UST1, ust); put(AS_Z01, z0);
+            for (int i = 0; i < 4; i++) put(AS_JA1 + i, ja[i]);
+            const D2 sg = wsigstar_d(t, ust, z0, wstar);
+            put(AS_SIG1, sg.v); put(AS_JS1, sg.a); put(AS_JS1 + 1, sg.b);
+            float j0, j1;
+            tau_jac(t, t.FR, mij, ust, z0, j0, j1);
+            put(AS_JT1, j0); put(AS_JT1 + 1, j1);
+            for (int i = 0; i < 7; i++) put(AS_K1 + i, keep[i]);
+            put(AS_SE, s_e); put(AS_SFM, s_fm); put(AS_STM, s_tm);
+            put(AS_EMEAN, emean); put(AS_FMEAN, fmean); put(AS_F1MEAN, f1mean);
+            put(AS_SEWS, q_e); put(AS_SFMWS, q_fm); put(AS_EMEANWS, ews); put(AS_FMEANWS, fmeanws);
+            put(AS_MIJ, (float)mij); put(AS_TW1, tauw);
+        }
     }
-    airsea(t, u10, tauw, ust, z0);
+    if (ADJ) airsea_d(t, u10, tauw, ust, z0, ja);
+    else airsea(t, u10, tauw, ust, z0);
 
     // 3. IMPHFTAIL, the second SINPUT, STRESSO's sums and SDISSIP in one pass;
     //    F, SL and FL go to the tiled work arrays, LLWS to bits.
@@ -623,13 +657,25 @@ __global__ void __launch_bounds__(PW) __attribute__((amdgpu_waves_per_eu(4))) im
             }
         }
     }
-    float phiaw;
-    tauw = stresso_close(t, mij, ust, z0, udir, phi, xs, ys, h1, h2, phiaw);
+    float phiaw, keep[7];
+    tauw = stresso_close(t, mij, ust, z0, udir, phi, xs, ys, h1, h2, phiaw, ADJ ? keep : nullptr);
+    if (ADJ) {
+        put(AS_UST2, ust); put(AS_Z02, z0);
+        for (int i = 0; i < 4; i++) put(AS_JA2 + i, ja[i]);
+        const D2 sg = wsigstar_d(t, ust, z0, wstar);
+        put(AS_SIG2, sg.v); put(AS_JS2, sg.a); put(AS_JS2 + 1, sg.b);
+        float j0, j1;
+        tau_jac(t, t.FR, mij, ust, z0, j0, j1);
+        put(AS_JT2, j0); put(AS_JT2 + 1, j1);
+        for (int i = 0; i < 7; i++) put(AS_K2 + i, keep[i]);
+    }
     if (on) {
-        a.tauw[mp] = tauw;
-        a.ustar[mp] = ust;
-        a.z0[mp] = z0;
-        if (a.phiaw) a.phiaw[mp] = phiaw;
+        if (!ADJ) {
+            a.tauw[mp] = tauw;
+            a.ustar[mp] = ust;
+            a.z0[mp] = z0;
+            if (a.phiaw) a.phiaw[mp] = phiaw;
+        }
         float enhfr = 1.f;
         if (SHALLOW) {
             const float enh = wam_maxf((0.75f * t.DEPTH[pt]) * akmean, .5f);
@@ -653,9 +699,12 @@ constexpr int SNL_NJ = 3;    // bins per thread: KL*ML <= SNL_NT * SNL_NJ
 constexpr int SNL_QT = 256 * SP_QPT_DEV / SNL_NT;
 
 // Three workgroups (LDS-limited) of 8 waves per CU: registers for 6 waves per SIMD.
-template <int SHALLOW>
+// ADJ = 1 (the adjoint's recomputation) reads USTAR from e.sc and writes
+// the final SL and FL over the work arrays' and the limiter's output to e.fn,
+// leaving F after the tail in x.ft.
+template <int SHALLOW, int ADJ = 0>
 __global__ void __launch_bounds__(SNL_NT) __attribute__((amdgpu_waves_per_eu(6)))
-implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x)
+implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x, AdjExtra e)
 {
     extern __shared__ __align__(16) float lds_raw[];
     __shared__ float sprd[MAXF], flm[MAXF], tfk[MAXF], tlim[MAXF], tbot[MAXF];
@@ -740,7 +789,8 @@ implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x)
             }
         }
         if (tid == 0) {
-            Ps[0] = a.ustar[mp]; Ps[1] = x.pp[lp]; Ps[2] = x.pp[(size_t)XS + lp]; Ps[3] = x.pp[(size_t)3 * XS + lp];
+            Ps[0] = ADJ ? e.sc[(size_t)AS_UST2 * XS + lp] : a.ustar[mp];
+            Ps[1] = x.pp[lp]; Ps[2] = x.pp[(size_t)XS + lp]; Ps[3] = x.pp[(size_t)3 * XS + lp];
             Pmij = (int)x.pp[(size_t)2 * XS + lp];
         }
         // no elementary functions here: SPRD comes from implsch_pre_kernel and
@@ -803,7 +853,19 @@ implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x)
         __syncthreads();
         {
             const float4 *F4 = reinterpret_cast<const float4 *>(s.F);
-            for (int q = tid; q < NB4; q += SNL_NT) *reinterpret_cast<float4 *>(x.ft + q0 + (size_t)q * 256) = F4[q];
+            float *dst = ADJ ? e.fn : x.ft;
+            for (int q = tid; q < NB4; q += SNL_NT) *reinterpret_cast<float4 *>(dst + q0 + (size_t)q * 256) = F4[q];
+        }
+        if (ADJ) {
+#pragma unroll
+            for (int j = 0; j < SNL_NJ; j++) {
+                const int b = tid + SNL_NT * j;
+                if (b < NB) {
+                    const size_t o = q0 + (size_t)(b >> 2) * 256 + (b & 3);
+                    x.slt[o] = sl[j];
+                    x.flt[o] = fl[j];
+                }
+            }
         }
         if (tid == 0) Pnext = p0 + tnext;
         __syncthreads();
@@ -919,12 +981,12 @@ hipError_t launch_implsch_split(const DevTables &t, const ImplArgs &a, const Imp
     auto mark = [&](int i) { if (ev && ev[i]) (void)hipEventRecord(ev[i], st); };
     mark(0);
     const size_t lds1 = (size_t)t.KL * W * sizeof(float);
-    if (t.shallow) hipLaunchKernelGGL(implsch_pre_kernel<1>, dim3(nwg), dim3(W), lds1, st, t, a, x);
-    else hipLaunchKernelGGL(implsch_pre_kernel<0>, dim3(nwg), dim3(W), lds1, st, t, a, x);
+    if (t.shallow) hipLaunchKernelGGL(implsch_pre_kernel<1>, dim3(nwg), dim3(W), lds1, st, t, a, x, AdjExtra{});
+    else hipLaunchKernelGGL(implsch_pre_kernel<0>, dim3(nwg), dim3(W), lds1, st, t, a, x, AdjExtra{});
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     mark(1);
-    void (*k)(DevTables, ImplArgs, ImplScratch) = t.shallow ? implsch_snl_kernel<1> : implsch_snl_kernel<0>;
+    void (*k)(DevTables, ImplArgs, ImplScratch, AdjExtra) = t.shallow ? implsch_snl_kernel<1> : implsch_snl_kernel<0>;
     const size_t lds = implsch_snl_lds_bytes(t, a.lcflx);
     e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
@@ -935,7 +997,7 @@ hipError_t launch_implsch_split(const DevTables &t, const ImplArgs &a, const Imp
     if (tu.snl_per_cu > 0 && tu.snl_per_cu < per_cu) per_cu = tu.snl_per_cu;
     const long want = (long)per_cu * ncu;
     const int nblocks = (int)(want < x.NP ? want : x.NP);
-    hipLaunchKernelGGL(k, dim3(nblocks), dim3(SNL_NT), lds, st, t, a, x);
+    hipLaunchKernelGGL(k, dim3(nblocks), dim3(SNL_NT), lds, st, t, a, x, AdjExtra{});
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     mark(2);
@@ -944,6 +1006,32 @@ hipError_t launch_implsch_split(const DevTables &t, const ImplArgs &a, const Imp
     e = hipGetLastError();
     mark(3);
     return e;
+}
+
+// The adjoint's recomputation of IMPLSCH up to the limiter (deep water):
+// implsch_pre_kernel and implsch_snl_kernel with ADJ = 1 on the chunk of x.
+hipError_t launch_implsch_adj_forward(const DevTables &t, const ImplArgs &a, const ImplScratch &x,
+                                      const AdjExtra &ex, int ncu, hipStream_t st, const LaunchTune &tu)
+{
+    const int W = tu.src_wg;
+    const int nwg = (x.NP + W - 1) / W;
+    const size_t lds1 = (size_t)t.KL * W * sizeof(float);
+    hipLaunchKernelGGL((implsch_pre_kernel<0, 1>), dim3(nwg), dim3(W), lds1, st, t, a, x, ex);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    void (*k)(DevTables, ImplArgs, ImplScratch, AdjExtra) = implsch_snl_kernel<0, 1>;
+    const size_t lds = implsch_snl_lds_bytes(t, 0);
+    e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    int per_cu = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, SNL_NT, lds);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) return hipErrorInvalidConfiguration;
+    if (tu.snl_per_cu > 0 && tu.snl_per_cu < per_cu) per_cu = tu.snl_per_cu;
+    const long want = (long)per_cu * ncu;
+    const int nblocks = (int)(want < x.NP ? want : x.NP);
+    hipLaunchKernelGGL(k, dim3(nblocks), dim3(SNL_NT), lds, st, t, a, x, ex);
+    return hipGetLastError();
 }
 
 size_t implsch_lds_bytes(const DevTables &t, int lcflx)

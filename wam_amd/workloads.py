@@ -237,9 +237,9 @@ def c4_case(scale=1.0, seed=20261205):
 
 
 def c5_case(ny=400, nx=360, seed=20261206, nsteps=144, dlat=0.25, dlon=0.25, south=-50.0, west=0.0,
-            land_frac=0.30, nobs=500, obs_every=12, truth_amp=0.10, obs_sd=0.1):
+            land_frac=0.30, nobs=500, obs_every=12, truth_amp=0.10, obs_sd=0.1, kl=24, ml=25):
     """Adjoint workload (configuration 5): a deep-water grid (DEPTH = 999 m as
-    in a run with SHALLOW_RUN = F, no currents), IPHYS = 0, 24 x 25 spectra,
+    in a run with SHALLOW_RUN = F, no currents), IPHYS = 0, KL x ML = 24 x 25 spectra,
     300 s steps; at the default size 400 x 360 cells of 0.25 deg with a
     seeded fractal land mask, about 100 k sea points.  The winds are u, v
     components for each of NSTEPS steps, blending two seeded k**-3 random
@@ -249,7 +249,7 @@ def c5_case(ny=400, nx=360, seed=20261206, nsteps=144, dlat=0.25, dlon=0.25, sou
     truth_u, truth_v): the truth winds are the winds times (1 + truth_amp *
     GRF) with a 200 km Gaussian random field; the caller makes observations
     from the truth run (Hs plus N(0, obs_sd) noise)."""
-    cfg = base_config(iphys=0, kl=24, ml=25, shallow=0, refr_d=0, refr_c=0, idelpro=300, idelt=300)
+    cfg = base_config(iphys=0, kl=kl, ml=ml, shallow=0, refr_d=0, refr_c=0, idelpro=300, idelt=300)
     cfg = grid_config(cfg, nx, ny, south, west, dlat, dlon)
     rng = np.random.default_rng(seed)
     land = _fractal(ny, nx, 3.0, rng)
