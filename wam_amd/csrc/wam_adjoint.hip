@@ -783,10 +783,13 @@ __global__ void __launch_bounds__(256) adj_post_kernel(DevTables t, AdjArgs a, I
     if (on) e.sc[(size_t)AS_MIJ2 * XS + lp] = (float)mij;
 }
 
-// LDS floats of adj_snl_kernel besides its static arrays
-__host__ __device__ __forceinline__ size_t adj_snl_lds_floats(int KL, int ML, int MH)
+// LDS words of adj_snl_kernel besides its static arrays: the quadruplet
+// bars, the DIA tables, the reader lists and their offsets, the update rows,
+// and the point's F, SL, FL and bar of F
+__host__ __device__ __forceinline__ size_t adj_snl_lds_floats(int KL, int ML, int MH, int NAR, int ANCH)
 {
-    return 3 * 512 + 25 * (size_t)MH + (size_t)MH + 5 * (size_t)MH + 16 * (size_t)KL + 4 * (size_t)KL * ML;
+    return 3 * 512 + 25 * (size_t)MH + (size_t)MH + 5 * (size_t)MH + 16 * (size_t)KL + (size_t)NAR +
+           ((size_t)ANCH * ML + 1) + (9 * (size_t)MH + 3) / 4 + 4 * (size_t)KL * ML;
 }
 
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2)))
@@ -804,10 +807,16 @@ adj_snl_kernel(DevTables t, AdjArgs a, ImplScratch x, AdjExtra e)
     int *inl = (int *)p; p += 5 * MH;
     int *kw = (int *)p; p += 8 * KL;
     int *kinv = (int *)p; p += 8 * KL;
+    int *are = (int *)p; p += t.NAR;
+    int *aro = (int *)p; p += t.ANCH * ML + 1;
+    signed char *aqr = (signed char *)p; p += (9 * MH + 3) / 4;
     float *F = p; p += NB;
     float *SL = p; p += NB;
     float *FL = p; p += NB;
     float *FB = p; p += NB;
+    for (int i = tid; i < t.NAR; i += NT) are[i] = t.AR_E[i];
+    for (int i = tid; i < t.ANCH * ML + 1; i += NT) aro[i] = t.AR_OFF[i];
+    for (int i = tid; i < 9 * MH; i += NT) aqr[i] = t.AQ_ROW[i];
     for (int i = tid; i < 25 * MH; i += NT) rnl[i] = t.RNLCOEF[i];
     for (int i = tid; i < 5 * MH; i += NT) inl[i] = t.INLCOEF[i];
     for (int i = tid; i < MH; i += NT) af11[i] = t.AF11[i];
@@ -915,20 +924,23 @@ adj_snl_kernel(DevTables t, AdjArgs a, ImplScratch x, AdjExtra e)
                     const float ftemp = af11[mcm1];
                     const float fcen = ftemp * fij;
                     const float fad1 = fij * (sap + sam), fad2 = fad1 - (2.f * sap) * sam;
-                    const int8_t *tr = t.AQ_ROW + 9 * mcm1;
+                    const signed char *tr = aqr + 9 * mcm1;
                     float adb = 0.f, ddb = 0.f, dmb = 0.f, dpb = 0.f;
                     if (tr[0] >= 0) {
                         const int b = K + KL * tr[0];
                         adb -= 2.f * SL[b];
                         ddb -= 2.f * FL[b];
                     }
+                    // RNLCOEF slots of the SL and FL weights of update types 1..8
+                    constexpr int wsl[9] = {0, 20, 19, 17, 18, 8, 7, 5, 6};
+                    constexpr int wfl[9] = {0, 23, 24, 21, 22, 11, 12, 9, 10};
 #pragma unroll
                     for (int ty = 1; ty <= 8; ty++) {
                         if (tr[ty] < 0) continue;
                         const int kk = ty <= 4 ? ((ty & 1) ? k2 : k21) : ((ty & 1) ? k1 : k11);
                         const int b = kk + KL * tr[ty];
-                        adb += rc[SLOT_SL[ty]] * SL[b];
-                        const float d = rc[SLOT_FL[ty]] * FL[b];
+                        adb += rc[wsl[ty]] * SL[b];
+                        const float d = rc[wfl[ty]] * FL[b];
                         if (ty <= 4) dmb += d;
                         else dpb += d;
                     }
@@ -949,23 +961,25 @@ adj_snl_kernel(DevTables t, AdjArgs a, ImplScratch x, AdjExtra e)
                     QF[ql] = fijb; QP[ql] = sapb; QM[ql] = samb;
                 }
                 __syncthreads();
-                for (int m = wv; m < ML; m += 4) {
-                    const int e0 = t.AR_OFF[c * ML + m], e1 = t.AR_OFF[c * ML + m + 1];
-                    if (lane < KL && e1 > e0) {
+                // one bin per thread (every lane busy), its row's reader list
+                for (int b = tid; b < NB; b += NT) {
+                    const int m = b / KL, k = b - KL * m;
+                    const int e0 = aro[c * ML + m], e1 = aro[c * ML + m + 1];
+                    if (e1 > e0) {
                         float acc = 0.f;
                         for (int en = e0; en < e1; en++) {
-                            const int v = t.AR_E[en], mcm1 = v >> 4, sl = v & 15;
+                            const int v = are[en], mcm1 = v >> 4, sl = v & 15;
                             const float w = rnl[mcm1 * 25 + (sl <= 4 ? sl : sl + 8)];
                             const float *X = sl == 0 ? QF : (sl <= 4 ? QP : QM);
                             const int mpk = sl <= 4 ? ((sl - 1) & 1) : 2 + ((sl - 5) & 1);
                             const int base = (mcm1 - mc0) * 2 * KL;
 #pragma unroll
                             for (int kh = 0; kh < 2; kh++) {
-                                const int K = sl == 0 ? lane : kinv[(mpk * 2 + kh) * KL + lane];
+                                const int K = sl == 0 ? k : kinv[(mpk * 2 + kh) * KL + k];
                                 acc = acc + w * X[base + kh * KL + K];
                             }
                         }
-                        FB[lane + KL * m] += acc;
+                        FB[b] += acc;
                     }
                 }
                 __syncthreads();
@@ -1024,7 +1038,7 @@ __device__ __forceinline__ void sin_freq_rev_lane(const DevTables &t, const SinF
     zs += ((dm[0] + g1 * ((usn1 / t.XKAPPA) * cm)) + g2 * ((usn2 / t.XKAPPA) * cm)) / z0;
 }
 
-__global__ void __launch_bounds__(256) adj_rev_kernel(DevTables t, AdjArgs a, ImplScratch x, AdjExtra e)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) adj_rev_kernel(DevTables t, AdjArgs a, ImplScratch x, AdjExtra e)
 {
     extern __shared__ float rev_lds[];            // [3][KL][W]: COS(TH - UDIR), its bar, the tail's sums
     const int KL = t.KL, ML = t.ML, NB = KL * ML, NS = t.NSEA, NB4 = x.NB4, XS = x.NX, lane = threadIdx.x;
@@ -1253,15 +1267,17 @@ __global__ void __launch_bounds__(256) propags_adj_kernel(DevTables t, const flo
     const int KL = t.KL, ML = t.ML, NS = t.NSEA;
     const size_t NB = (size_t)KL * ML * NS;
     const int ns = t.KLAT_S[j], nn = t.KLAT_N[j], nw = t.KLON_W[j], ne = t.KLON_E[j];
+    // every weight is read once: non-temporal loads keep them from evicting
+    // the bars that neighbouring points read again
     auto term = [&](int plane, size_t row, int i) -> float {
-        return i >= 0 ? coef[plane * NB + row + i] * gin[row + i] : 0.f;
+        return i >= 0 ? __builtin_nontemporal_load(coef + plane * NB + row + i) * gin[row + i] : 0.f;
     };
     for (int m = 0; m < ML; m++) {
         for (int k = 0; k < KL; k++) {
             const size_t row = ((size_t)m * KL + k) * NS;
             const size_t rkm = ((size_t)m * KL + (t.KPM_M[k] - 1)) * NS;   // output K-1 reads K through TH1
             const size_t rkp = ((size_t)m * KL + (t.KPM_P[k] - 1)) * NS;   // output K+1 reads K through TH2
-            float g = coef[P_SUM * NB + row + j] * gin[row + j];
+            float g = __builtin_nontemporal_load(coef + P_SUM * NB + row + j) * gin[row + j];
             if (!CURR) {
                 // PROP_NO_CURR reads KLAT_S when JYO1 = 1 (else KLAT_N) and
                 // KLON_W when JXO1 = 1 (else KLON_E)
@@ -1282,7 +1298,67 @@ __global__ void __launch_bounds__(256) propags_adj_kernel(DevTables t, const flo
                 if (m < ML - 1) g = g + term(P_SIG2, ((size_t)(m + 1) * KL + k) * NS, j);
                 if (m == 0) g = g + term(P_SIG2, row, j);
             }
-            gout[row + j] = g;
+            __builtin_nontemporal_store(g, gout + row + j);
+        }
+    }
+}
+
+// The no-current weights as the transposed stencil reads them: plane p at
+// point j holds the weight that multiplies the bar gathered from j's
+// neighbour in the transposed sum (SUM of j, LAT and LON of the points that
+// read j, TH1 of row K-1 and TH2 of row K+1 at j; 0 for a land neighbour).
+__global__ void transpose_weights_kernel(DevTables t, const float *__restrict__ coef, float *__restrict__ tw)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= t.NSEA) return;
+    const int KL = t.KL, NS = t.NSEA;
+    const int bin = blockIdx.y, m = bin / KL, k = bin - KL * m;
+    const size_t NB = (size_t)KL * t.ML * NS, row = ((size_t)m * KL + k) * NS;
+    const int il = t.JYO1[k] == 1 ? t.KLAT_N[j] : t.KLAT_S[j];
+    const int io = t.JXO1[k] == 1 ? t.KLON_E[j] : t.KLON_W[j];
+    tw[row + j] = coef[row + j];
+    tw[NB + row + j] = il >= 0 ? coef[NB + row + il] : 0.f;
+    tw[2 * NB + row + j] = io >= 0 ? coef[2 * NB + row + io] : 0.f;
+    tw[3 * NB + row + j] = coef[3 * NB + ((size_t)m * KL + (t.KPM_M[k] - 1)) * NS + j];
+    tw[4 * NB + row + j] = coef[4 * NB + ((size_t)m * KL + (t.KPM_P[k] - 1)) * NS + j];
+}
+
+// PROP_NO_CURR transposed on the transposed weights: the same products in
+// the same order as propags_adj_kernel<false>, every weight load at the
+// point's own place.
+__global__ void __launch_bounds__(256) propags_adjt_kernel(DevTables t, const float *__restrict__ tw,
+                                                            const float *__restrict__ gin, float *__restrict__ gout)
+{
+    const size_t moff = (size_t)blockIdx.y * t.KL * t.ML * t.NSEA;
+    gin += moff;
+    gout += moff;
+    const int wg = blockDim.x, nch = (t.NSEA + wg - 1) / wg;
+    const int ch = adj_chunk(nch);
+    if (ch >= nch) return;
+    const int j = ch * wg + threadIdx.x;
+    if (j >= t.NSEA) return;
+    const int KL = t.KL, ML = t.ML, NS = t.NSEA;
+    const size_t NB = (size_t)KL * ML * NS;
+    const int ns = t.KLAT_S[j], nn = t.KLAT_N[j], nw = t.KLON_W[j], ne = t.KLON_E[j];
+    for (int m = 0; m < ML; m++) {
+        for (int k = 0; k < KL; k++) {
+            const size_t row = ((size_t)m * KL + k) * NS, at = row + j;
+            const size_t rkm = ((size_t)m * KL + (t.KPM_M[k] - 1)) * NS;
+            const size_t rkp = ((size_t)m * KL + (t.KPM_P[k] - 1)) * NS;
+            const int il = t.JYO1[k] == 1 ? nn : ns, io = t.JXO1[k] == 1 ? ne : nw;
+            // every load issued up front (a land neighbour reads the point itself, unused)
+            const float w0 = __builtin_nontemporal_load(tw + at), w1 = __builtin_nontemporal_load(tw + NB + at);
+            const float w2 = __builtin_nontemporal_load(tw + 2 * NB + at);
+            const float w3 = __builtin_nontemporal_load(tw + 3 * NB + at);
+            const float w4 = __builtin_nontemporal_load(tw + 4 * NB + at);
+            const float g0 = gin[at], gl = gin[row + (il >= 0 ? il : j)], go = gin[row + (io >= 0 ? io : j)];
+            const float g3 = gin[rkm + j], g4 = gin[rkp + j];
+            float g = w0 * g0;
+            g = g + (il >= 0 ? w1 * gl : 0.f);
+            g = g + (io >= 0 ? w2 * go : 0.f);
+            g = g + w3 * g3;
+            g = g + w4 * g4;
+            __builtin_nontemporal_store(g, gout + at);
         }
     }
 }
@@ -1363,7 +1439,7 @@ hipError_t launch_implsch_adjoint_split(const DevTables &t, const AdjArgs &a, co
     hipLaunchKernelGGL(adj_post_kernel, dim3(nwg), dim3(W), 0, st, t, a, x, ex);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const size_t lds = adj_snl_lds_floats(t.KL, t.ML, t.MLSTHG) * sizeof(float);
+    const size_t lds = adj_snl_lds_floats(t.KL, t.ML, t.MLSTHG, t.NAR, t.ANCH) * sizeof(float);
     e = hipFuncSetAttribute((const void *)adj_snl_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     int per_cu = 0;
@@ -1378,13 +1454,22 @@ hipError_t launch_implsch_adjoint_split(const DevTables &t, const AdjArgs &a, co
     return hipGetLastError();
 }
 
-hipError_t launch_propags_adjoint(const DevTables &t, const float *coef, int nmem, const float *gin, float *gout,
-                                  hipStream_t st)
+// tw: the transposed no-current weights (transpose_weights_kernel), else NULL
+hipError_t launch_propags_adjoint(const DevTables &t, const float *coef, const float *tw, int nmem, const float *gin,
+                                  float *gout, hipStream_t st)
 {
     const int wg = 256, nch = (t.NSEA + wg - 1) / wg;
     const dim3 grid(((nch + 7) / 8) * 8, nmem), block(wg);
     if (t.refr_c) hipLaunchKernelGGL(propags_adj_kernel<true>, grid, block, 0, st, t, coef, gin, gout);
+    else if (tw) hipLaunchKernelGGL(propags_adjt_kernel, grid, block, 0, st, t, tw, gin, gout);
     else hipLaunchKernelGGL(propags_adj_kernel<false>, grid, block, 0, st, t, coef, gin, gout);
+    return hipGetLastError();
+}
+
+hipError_t launch_transpose_weights(const DevTables &t, const float *coef, float *tw, hipStream_t st)
+{
+    hipLaunchKernelGGL(transpose_weights_kernel, dim3((t.NSEA + 255) / 256, t.KL * t.ML), dim3(256), 0, st, t, coef,
+                       tw);
     return hipGetLastError();
 }
 

@@ -42,8 +42,9 @@ bool implsch_split_supported(const DevTables &t);
 hipError_t launch_implsch_adjoint(const DevTables &t, const AdjArgs &a, int ncu, hipStream_t st);
 hipError_t launch_implsch_adjoint_split(const DevTables &t, const AdjArgs &a, const ImplScratch &x, const AdjExtra &ex,
                                         int ncu, hipStream_t st, const LaunchTune &tu);
-hipError_t launch_propags_adjoint(const DevTables &t, const float *coef, int nmem, const float *gin, float *gout,
-                                  hipStream_t st);
+hipError_t launch_propags_adjoint(const DevTables &t, const float *coef, const float *tw, int nmem, const float *gin,
+                                  float *gout, hipStream_t st);
+hipError_t launch_transpose_weights(const DevTables &t, const float *coef, float *tw, hipStream_t st);
 hipError_t launch_wind_uv(float zpi, const float *u, const float *v, float *u10, float *udir, int64_t n,
                           hipStream_t st);
 hipError_t launch_wind_uv_adjoint(const float *u, const float *v, const float *u10b, const float *udirb, float *ub,
@@ -148,6 +149,7 @@ struct wam_ctx {
     float *adj_fp = nullptr, *adj_fb = nullptr;
     size_t adj_n = 0;
     float *adj_coef = nullptr;
+    float *adj_tw = nullptr;              // the no-current weights transposed (a point allocation)
     int adj_sym = -1;
     // the split IMPLSCH adjoint's work arrays for a chunk of member-points
     ImplScratch asx{};
@@ -267,7 +269,7 @@ static int upload_static(wam_ctx *ctx)
     UP(o, t.SN_E, SN_E); UP(o, t.SN_CH, SN_CH); UP(o, t.SN_WW, SN_WW);
     d.SP_CHUNK = t.SP_CHUNK; d.SP_NCHUNK = t.SP_NCHUNK; d.SP_NRUN = t.SP_NRUN; d.SP_PAD = t.SP_PAD;
     UP(o, t.SP_E, SP_E); UP(o, t.SP_RW, SP_RW);
-    d.ACH = t.ACH; d.ANCH = t.ANCH;
+    d.ACH = t.ACH; d.ANCH = t.ANCH; d.NAR = (int)t.AR_E.size();
     UP(o, t.AQ_ROW, AQ_ROW); UP(o, t.AR_E, AR_E); UP(o, t.AR_OFF, AR_OFF); UP(o, t.AKINV, AKINV);
     std::vector<int> ks = zero_based(t.KLAT_S), kn = zero_based(t.KLAT_N), kw = zero_based(t.KLON_W), ke = zero_based(t.KLON_E);
     UP(o, ks, KLAT_S); UP(o, kn, KLAT_N); UP(o, kw, KLON_W); UP(o, ke, KLON_E);
@@ -448,7 +450,7 @@ int wam_gpu_prepare_propagation(wam_ctx *ctx, const float *depth, const float *u
     if (rc) return rc;
     free_list(ctx->point_allocs);
     ctx->coef = nullptr; ctx->wn = nullptr; ctx->cgg = nullptr; ctx->dt.SBOT = nullptr;
-    ctx->adj_coef = nullptr;
+    ctx->adj_coef = nullptr; ctx->adj_tw = nullptr;
     const int ns = t.NSEA;
     auto &o = ctx->point_allocs;
     std::vector<float> d(depth, depth + ns), uu(ns, 0.f), vv(ns, 0.f);
@@ -470,7 +472,7 @@ int wam_gpu_prepare_propagation_dev(wam_ctx *ctx, const float *d_depth, const fl
     if (t.cfg.refr_c && (!d_u || !d_v)) { ctx->err = "current refraction needs u and v"; return WAM_E_ARG; }
     free_list(ctx->point_allocs);
     ctx->coef = nullptr; ctx->wn = nullptr; ctx->cgg = nullptr; ctx->dt.SBOT = nullptr;
-    ctx->adj_coef = nullptr;
+    ctx->adj_coef = nullptr; ctx->adj_tw = nullptr;
     const int ns = t.NSEA;
     auto &o = ctx->point_allocs;
     hipStream_t st = ctx->stream;
@@ -1937,6 +1939,13 @@ static int adjoint_planes(wam_ctx *ctx)
             HIPCHK(hipGetLastError());
         }
     }
+    if (!t.cfg.refr_c && !ctx->adj_tw) {
+        void *p = nullptr;
+        HIPCHK(hipMalloc(&p, nb * 5 * sizeof(float)));
+        ctx->point_allocs.push_back(p);
+        ctx->adj_tw = (float *)p;
+        HIPCHK(launch_transpose_weights(ctx->dt, ctx->coef ? ctx->coef : ctx->adj_coef, ctx->adj_tw, ctx->stream));
+    }
     return WAM_OK;
 }
 
@@ -1952,7 +1961,7 @@ static int propags_transposed(wam_ctx *ctx, int nmem, const float *src, float *d
         const int rc = timing_events(ctx, ev, 2);
         if (rc) return rc;
         timing_record(ctx, ev[0]);
-        HIPCHK(launch_propags_adjoint(ctx->dt, coef, nmem, src, to, ctx->stream));
+        HIPCHK(launch_propags_adjoint(ctx->dt, coef, ctx->adj_tw, nmem, src, to, ctx->stream));
         timing_record(ctx, ev[1]);
         timing_keep(ctx, "propags_adjoint", ev[0], ev[1]);
         src = to;

@@ -57,7 +57,7 @@ struct DevTables {
     // SNONLIN's adjoint by chunks of ACH frequencies of quadruplets: update
     // rows per (MC, update type), -1 = skipped; per (chunk, read row) the
     // reading quadruplets (MC - 1) << 4 | read slot; inverse direction maps
-    int ACH, ANCH;
+    int ACH, ANCH, NAR;             // NAR: entries of AR_E
     const int8_t *AQ_ROW;
     const uint16_t *AR_E;
     const int *AR_OFF, *AKINV;
