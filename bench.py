@@ -472,10 +472,11 @@ def bench_adjoint(args, torch):
         t_b = time.perf_counter() - w0
         grads[label] = (uu.grad, vv.grad)
         res[label] = {"nsnap": model.nsnap, "forward_s": round(t_f, 3), "reverse_s": round(t_b, 3),
+                      "reverse_over_forward": round(t_b / t_f, 3),
                       "gradient_over_forward": round((t_f + t_b) / t_f, 3),
                       "forward_steps_run": model.forward_steps,
                       "recomputed_steps_schedule": checkpoint_recomputed_steps(nsteps, model.nsnap),
-                      "J": float(j)}
+                      "J": float(j.detach())}
         log(f"[adjoint] {label}: forward {t_f:.2f}s, reverse {t_b:.2f}s, J = {float(j):.6g}")
         del model, spec
         torch.cuda.empty_cache()
@@ -507,7 +508,8 @@ def bench_adjoint(args, torch):
         torch.cuda.synchronize()
         t_b = time.perf_counter() - w0
         res["batched"] = {"members": nm, "steps": ms, "nsnap": model.nsnap, "forward_s": round(t_f, 3),
-                          "reverse_s": round(t_b, 3), "gradient_over_forward": round((t_f + t_b) / t_f, 3),
+                          "reverse_s": round(t_b, 3), "reverse_over_forward": round(t_b / t_f, 3),
+                          "gradient_over_forward": round((t_f + t_b) / t_f, 3),
                           "member_point_steps_per_s_gradient": round(nm * n * ms / (t_f + t_b), 1),
                           "gradient_finite": bool(torch.isfinite(uu.grad).all() and torch.isfinite(vv.grad).all())}
         log(f"[adjoint] {nm} members x {ms} steps: forward {t_f:.2f}s, reverse {t_b:.2f}s")
