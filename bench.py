@@ -544,6 +544,8 @@ def main():
     ap.add_argument("--adj-snaps", type=int, default=12, help="checkpoint slots of the binomial schedule")
     ap.add_argument("--adj-members", type=int, default=16)
     ap.add_argument("--adj-member-steps", type=int, default=24)
+    ap.add_argument("--only", choices=("ensemble", "nested", "adjoint"),
+                    help="run one of the configurations alone and print its object as the JSON line (profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one plain 1 GiB device copy before the timed steps (reference for HBM counters)")
     args = ap.parse_args()
@@ -552,6 +554,10 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.only:
+        fn = {"ensemble": bench_ensemble, "nested": bench_nested, "adjoint": bench_adjoint}[args.only]
+        print(json.dumps(fn(args, torch)), flush=True)
+        return
     from wam_amd import WamGpu, workloads as W
 
     t0 = time.time()
