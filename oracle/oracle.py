@@ -65,6 +65,7 @@ def _lib(fp64, opcount=False, ad=False):
         lib.orc_api_prepare_propagation.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 3
         lib.orc_api_propags.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         lib.orc_api_force_counter.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        lib.orc_api_source_options.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         lib.orc_api_implsch.argtypes = ([ctypes.c_void_p] + [ctypes.c_void_p] * 9 +
                                         [ctypes.c_int] + [ctypes.c_void_p] * 3 + [ctypes.c_int])
         lib.orc_api_wavnu2.argtypes = [ctypes.c_void_p] * 4
@@ -116,6 +117,10 @@ class Oracle:
             raise ValueError(f"oracle create failed with status {st.value}")
         self.kl, self.ml = cfg["kl"], cfg["ml"]
         self.nsea = int(self.scalar("NSEA"))
+        # the off-by-default sources: WAVE_BREAKING_RUN (SDEPTHLIM, SFBRK) and PHILLIPS_RUN
+        if cfg.get("wave_breaking") or cfg.get("phillips"):
+            self.lib.orc_api_source_options(self.ctx, int(bool(cfg.get("wave_breaking"))),
+                                            int(bool(cfg.get("phillips"))))
 
     def __del__(self):
         if getattr(self, "ctx", None):

@@ -83,6 +83,13 @@ void orc_api_propags(const orc_ctx *c, real *fl3) { orc_propags(c, fl3); }
 /* CHECK_CFL's sub-step count of the grid this context is a window of (0:
    its own); set before orc_api_prepare_propagation */
 void orc_api_force_counter(orc_ctx *c, int counter) { c->force_counter = counter; }
+/* SET_MODEL_OPTION's WAVE_BREAKING and PHILLIPS (wam_timopt_module.f90:176-177):
+   breaking needs a shallow-water run and is switched off otherwise (:213-227). */
+void orc_api_source_options(orc_ctx *c, int wave_breaking, int phillips)
+{
+    c->wave_breaking = wave_breaking && c->cfg.shallow;
+    c->phillips = phillips != 0;
+}
 void orc_api_implsch(const orc_ctx *c, real *fl3, const real *u10, const real *udir, real *tauw,
                      real *ustar, real *z0, const real *roairn, const real *wstar, const real *depth,
                      int lcflx, real *phiaw, real *out_flux, int *mij, int nthreads)

@@ -219,6 +219,9 @@ typedef struct {
     /* propagation */
     int counter; treal cfl_max, cfl_min;
     int force_counter;   /* > 0: COUNTER of the grid this one is a window of (CHECK_CFL's rescale by it) */
+    /* WAVE_BREAKING_RUN and PHILLIPS_RUN (wam_timopt_module.f90:60-63, set by
+       SET_MODEL_OPTION :176-177; breaking is switched off in deep water :213-227) */
+    int wave_breaking, phillips;
     int ncoef;                              /* coefficient planes present */
     treal *COEF;                             /* [ncoef][ML][KL][NSEA] */
     int *INDEP;                             /* [NSEA] */

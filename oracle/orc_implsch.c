@@ -731,6 +731,78 @@ static void wnfluxes(const orc_ctx *c, orc_work *w, const real *SS, real USTAR, 
     out[0] = phioc; out[1] = tx; out[2] = ty; out[3] = pb; out[4] = bx; out[5] = by;
 }
 
+/* GAMD, the parameter of the depth-limited wave height (wam_source_module.f90:67). */
+#define GAMD R(0.8)
+
+/* SDEPTHLIM, wam_source_module.f90:3735-3780 (IMPLSCH :513-515 with
+   WAVE_BREAKING_RUN): the spectrum scaled by EM = MIN((0.25 GAMD DEPTH)**2 /
+   EMEAN, 1), and EMEAN by the same factor. */
+static void sdepthlim(const orc_ctx *c, orc_work *w, real DEPTH, real *EMEAN)
+{
+    int NB = c->KL * c->ML;
+    real h = (R(0.25) * GAMD) * DEPTH;
+    real EM = MINR((h * h) / *EMEAN, R(1.0));
+    for (int b = 0; b < NB; b++) w->F[b] = w->F[b] * EM;
+    *EMEAN = EM * *EMEAN;
+}
+
+/* SFBRK, wam_source_module.f90:1966-2037, with CMPQB :2041-2107 (Battjes and
+   Janssen's fraction of breaking waves by one Newton-Raphson step): SL +=
+   SBR * F, FL += DSBR (IMPLSCH :640-642). */
+static void sfbrk(const orc_ctx *c, orc_work *w, real EMEAN, real FMEAN, real DEPTH)
+{
+    int NB = c->KL * c->ML;
+    real gd = GAMD * DEPTH;
+    real BB = (R(8.) * EMEAN) / (gd * gd);
+    /* CMPQB */
+    real Q0 = R(0.);
+    if (BB >= R(0.25)) {
+        real q = R(2.) * SQRT(BB) - R(1.);
+        Q0 = q * q;
+    }
+    real QB;
+    if (BB < R(1.)) {
+        real e = EXP((Q0 - R(1.)) / BB);
+        QB = Q0 - (BB * (Q0 - e)) / (BB - e);
+    } else {
+        QB = R(1.);
+    }
+    QB = MINR(R(1.), QB);
+    real SBR = -(R(2.) * FMEAN);                    /* -ALPHA*2.*FMEAN, ALPHA = 1 */
+    if (BB <= R(1.)) SBR = (SBR * QB) / BB;
+    real DSBR = R(0.);
+    if (BB < R(1.) && ABS(BB - QB) > R(0.)) DSBR = (SBR * (R(1.) - QB)) / (BB - QB);
+    for (int b = 0; b < NB; b++) {
+        w->SL[b] = w->SL[b] + SBR * w->F[b];
+        w->FL[b] = w->FL[b] + DSBR;
+    }
+}
+
+/* SOURCE_PHILLIPS, wam_source_module.f90:3051-3160 (IMPLSCH :644-646):
+   SL(K,M) += TEMP(M) * TPHOLD(K), the constants folded in REAL*4 as
+   written. */
+static void source_phillips(const orc_ctx *c, orc_work *w, real USTAR, real UDIR, real DEPTH, int indep)
+{
+    int KL = c->KL, ML = c->ML;
+    real TPH[64];
+    real CONST1 = R(28.) / (R(0.13) * c->G);
+    real FPM = MAXR(R(0.0000001), CONST1 * USTAR);
+    real XEPS = c->ROAIR / c->ROWATER;
+    CONST1 = ((R(80.) * R(16.0)) * (XEPS * XEPS)) / ((R(0.5) * R(3.0)) * (c->G * c->G));
+    for (int k = 0; k < KL; k++) TPH[k] = IPOW(USTAR * MAXR(R(0.), COS(c->TH[k] - UDIR)), 4);
+    for (int m = 0; m < ML; m++) {
+        real x = c->FR[m] * FPM;
+        real TEMP = EXP(-(R(1.) / IPOW(x, 4)));     /* (FR*FPM)**(-4) */
+        if (c->cfg.shallow) {
+            real KD = MINR(TB(c->TFAK, m) * DEPTH, R(40.));
+            TEMP = (CONST1 * TEMP) / (R(1.0) + (R(2.0) * KD) / SINH(R(2.) * KD));
+        } else {
+            TEMP = CONST1 * TEMP;
+        }
+        for (int k = 0; k < KL; k++) w->SL[k + KL * m] = w->SL[k + KL * m] + TEMP * TPH[k];
+    }
+}
+
 /* One sea point through IMPLSCH (wam_source_module.f90:374-700). */
 static void implsch_point(const orc_ctx *c, orc_work *w, real U10, real UDIR, real *TAUW,
                           real *USTAR, real *Z0, real ROAIRN, real WSTAR, real DEPTH, int indep,
@@ -744,6 +816,10 @@ static void implsch_point(const orc_ctx *c, orc_work *w, real U10, real UDIR, re
 
     dirsum(c, w->F, NULL, T);
     real EMEAN = total_energy(c, T);
+    if (c->wave_breaking) {
+        sdepthlim(c, w, DEPTH, &EMEAN);
+        dirsum(c, w->F, NULL, T);
+    }
     real FMEAN = femean(c, T, EMEAN);
     real F1MEAN = R(1.) / tm1(c, T, EMEAN);
     real AKMEAN = R(0.), XKMEAN = R(0.);
@@ -790,6 +866,8 @@ static void implsch_point(const orc_ctx *c, orc_work *w, real U10, real UDIR, re
         wnfluxes(c, w, w->SMIN, ust, UDIR, phiaw, MIJ, indep, flux_out);
     }
     if (c->cfg.shallow) for (int b = 0; b < NB; b++) w->SL[b] = w->SL[b] + w->SLBOT[b];
+    if (c->wave_breaking) sfbrk(c, w, EMEAN, FMEAN, DEPTH);
+    if (c->phillips) source_phillips(c, w, ust, UDIR, DEPTH, indep);
 
     /* 4. limiter and implicit update */
     real DELT = (real)c->cfg.idelt;

@@ -8,7 +8,15 @@ Run in the container that holds the reference checkout:
 Each case writes ref_implsch_<case>.npz: the model options, the grid (so the
 tables, including MIN_ENERGY's fetch-dependent floor, can be rebuilt), the
 inputs of IMPLSCH for a few seeded sea points, and the reference's outputs
-after NSTEP consecutive IMPLSCH calls (no propagation in between).
+after NSTEP consecutive IMPLSCH calls (no propagation in between).  Case
+names given on the command line write only those cases.
+
+The *_break_* and *_phillips_* cases switch on the off-by-default sources
+(SET_MODEL_OPTION's WAVE_BREAKING: SDEPTHLIM and SFBRK; PHILLIPS:
+SOURCE_PHILLIPS).  Their shallow points get depths of 0.6 to 24 m, so that
+the depth limit (0.25 GAMD DEPTH)**2 caps the energy of the shallowest ones,
+and the Phillips cases start from a calm sea (the JONSWAP spectrum scaled by
+1e-4), where the linear growth term is a visible part of the first steps.
 """
 import json
 import os
@@ -40,6 +48,17 @@ def cases():
         "c1_st6_1step": dict(gen="c1", cfg=c1small, nstep=1, lcflx=0),
         "c1_st6_6h": dict(gen="c1", cfg=c1small, nstep=36, lcflx=0),
         "c1_janssen_1step": dict(gen="c1", cfg=dict(c1small, iphys=0), nstep=1, lcflx=0),
+        "c2sub_break_phillips_1step": dict(gen="c2", iphys=0, nstep=1, lcflx=1, breaking=1, phillips=1,
+                                           shallow_depths=True),
+        "c2sub_break_phillips_6h": dict(gen="c2", iphys=0, nstep=72, lcflx=0, breaking=1, phillips=1,
+                                        shallow_depths=True),
+        "c2sub_st6_break_phillips_1step": dict(gen="c2", iphys=2, nstep=1, lcflx=1, breaking=1, phillips=1,
+                                               shallow_depths=True),
+        # deep water: the reference switches breaking off there (wam_timopt_module.f90:213-227)
+        "c1_janssen_phillips_calm_1step": dict(gen="c1", cfg=dict(c1small, iphys=0), nstep=1, lcflx=0,
+                                               breaking=1, phillips=1, scale=1e-4),
+        "c1_janssen_phillips_calm_6h": dict(gen="c1", cfg=dict(c1small, iphys=0), nstep=36, lcflx=0,
+                                            breaking=1, phillips=1, scale=1e-4),
     }
 
 
@@ -50,6 +69,8 @@ def inputs(case):
         cfg, mask, f = W.c2_case(ny=6, nx=8, seed=3, iphys=case["iphys"])
         depth = f["depth"]
         u10, udir = f["u10"], f["udir"]
+        if case.get("shallow_depths"):
+            depth = np.geomspace(0.6, 24.0, depth.size).astype(np.float32)[np.random.default_rng(5).permutation(depth.size)]
     else:
         cfg = case["cfg"]
         mask = np.ones((cfg["ny"], cfg["nx"]), np.uint8)
@@ -58,10 +79,12 @@ def inputs(case):
         depth = np.full(n, 999.0, np.float32)
         u10 = (20.0 + rng.uniform(-6, 6, n)).astype(np.float32)
         udir = (np.pi / 2 + rng.uniform(-0.5, 0.5, n)).astype(np.float32)
+    if case.get("breaking") or case.get("phillips"):
+        cfg = dict(cfg, wave_breaking=int(case.get("breaking", 0)), phillips=int(case.get("phillips", 0)))
     o = Oracle(cfg, mask)
     o.prepare_propagation(depth, None, None)
     n = o.nsea
-    fl3 = W.jonswap_spectrum(cfg, udir).ravel()
+    fl3 = W.jonswap_spectrum(cfg, udir).ravel() * case.get("scale", 1.0)
     indep = o.get("INDEP").astype(np.int32) if cfg["shallow"] else np.zeros(n, np.int32)
     return cfg, mask, dict(fl3=fl3.astype(np.float32), u10=np.asarray(u10, np.float32),
                            udir=np.asarray(udir, np.float32), tauw=np.zeros(n, np.float32),
@@ -74,8 +97,9 @@ def run_reference(cfg, inp, delphi, nstep, lcflx, tag):
     n = inp["u10"].size
     fin, fout = f"/tmp/ref_implsch_{tag}.in", f"/tmp/ref_implsch_{tag}.out"
     with open(fin, "wb") as fh:
-        fh.write(struct.pack("11i", cfg["kl"], cfg["ml"], cfg["iref"], cfg["iphys"], cfg["shallow"], cfg["ndepth"],
-                             cfg["idelpro"], cfg["idelt"], n, lcflx, nstep))
+        fh.write(struct.pack("13i", cfg["kl"], cfg["ml"], cfg["iref"], cfg["iphys"], cfg["shallow"], cfg["ndepth"],
+                             cfg["idelpro"], cfg["idelt"], n, lcflx, nstep, cfg.get("wave_breaking", 0),
+                             cfg.get("phillips", 0)))
         fh.write(struct.pack("8f", cfg["fr1"], cfg["deptha"], cfg["depthd"], cfg["alpha"], cfg["betamax"],
                              cfg["zalp"], cfg["cdfac"], delphi))
         for k in ("fl3", "u10", "udir", "tauw", "z0", "roairn", "wstar", "depth"):
@@ -95,7 +119,10 @@ def run_reference(cfg, inp, delphi, nstep, lcflx, tag):
 
 def main():
     subprocess.check_call(["make", "-s", "-C", ORACLE, "-f", "Makefile.ref"])
+    only = set(sys.argv[1:])
     for name, case in cases().items():
+        if only and name not in only:
+            continue
         cfg, mask, inp, delphi = inputs(case)
         out = run_reference(cfg, inp, delphi, case["nstep"], case["lcflx"], name)
         np.savez_compressed(os.path.join(HERE, f"ref_implsch_{name}.npz"),
