@@ -186,6 +186,8 @@ class WamGpu:
         for name, _ in _lib.WamConfig._fields_:
             setattr(c, name, cfg[name])
         self._c = c
+        if cfg.get("wave_breaking") or cfg.get("phillips"):
+            raise WamError(1, "WAVE_BREAKING and PHILLIPS sources are not available on the device")
         m = np.ascontiguousarray(mask, dtype=np.uint8).ravel()
         if m.size != cfg["nx"] * cfg["ny"]:
             raise ValueError("mask size must be nx*ny")
