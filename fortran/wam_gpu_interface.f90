@@ -91,6 +91,11 @@ INTERFACE
       TYPE(C_PTR), VALUE :: ctx
       CHARACTER(KIND=C_CHAR), DIMENSION(*) :: name
    END FUNCTION
+   INTEGER(C_INT) FUNCTION c_wam_gpu_set_source_options(ctx, brk, phil) BIND(C, NAME='wam_gpu_set_source_options')
+      IMPORT :: C_INT, C_INT32_T, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT32_T), VALUE :: brk, phil
+   END FUNCTION
    SUBROUTINE c_wam_gpu_finalize(ctx) BIND(C, NAME='wam_gpu_finalize')
       IMPORT :: C_PTR
       TYPE(C_PTR), VALUE :: ctx
@@ -105,7 +110,8 @@ END INTERFACE
 TYPE(C_PTR), SAVE :: CTX = C_NULL_PTR
 INTEGER, SAVE, PUBLIC :: NSEA_GPU = 0
 
-PUBLIC :: WAM_GPU_SETUP, WAM_GPU_PREPARE_PROPAGATION, WAM_GPU_PREPARE_PROPAGATION_DEV, PROPAGS_GPU, &
+PUBLIC :: WAM_GPU_SETUP, WAM_GPU_SET_SOURCE_OPTIONS, WAM_GPU_PREPARE_PROPAGATION, WAM_GPU_PREPARE_PROPAGATION_DEV, &
+&         PROPAGS_GPU, &
 &         IMPLSCH_GPU, OUTPUT_PARAMETERS_GPU, BOUNDARY_OUTPUT_GPU, BOUNDARY_INPUT_GPU, WAM_GPU_END
 
 CONTAINS
@@ -130,6 +136,18 @@ SUBROUTINE WAM_GPU_SETUP (CFG, MASK)
    CALL CHECK(c_wam_gpu_init(CTX, CFG, C_LOC(MASK)), 'WAM_GPU_SETUP')
    NSEA_GPU = NINT(c_wam_gpu_get_scalar(CTX, 'NSEA'//C_NULL_CHAR))
 END SUBROUTINE WAM_GPU_SETUP
+
+!  SET_MODEL_OPTION's WAVE_BREAKING and PHILLIPS switches (wam_timopt_module.f90:176-177):
+!  call after WAM_GPU_SETUP with the host's WAVE_BREAKING_RUN and PHILLIPS_RUN.
+SUBROUTINE WAM_GPU_SET_SOURCE_OPTIONS (WAVE_BREAKING, PHILLIPS)
+   LOGICAL, INTENT(IN) :: WAVE_BREAKING, PHILLIPS
+   INTEGER(C_INT32_T) :: B, P
+   B = 0_C_INT32_T
+   P = 0_C_INT32_T
+   IF (WAVE_BREAKING) B = 1_C_INT32_T
+   IF (PHILLIPS) P = 1_C_INT32_T
+   CALL CHECK(c_wam_gpu_set_source_options(CTX, B, P), 'WAM_GPU_SET_SOURCE_OPTIONS')
+END SUBROUTINE WAM_GPU_SET_SOURCE_OPTIONS
 
 SUBROUTINE WAM_GPU_PREPARE_PROPAGATION (DEPTH, U, V, INDEP)
    REAL(C_FLOAT), INTENT(IN), TARGET, CONTIGUOUS :: DEPTH(:)

@@ -104,6 +104,17 @@ int wam_gpu_prepare_propagation_dev(wam_ctx *ctx, const float *d_depth, const fl
    next wam_gpu_prepare_propagation. */
 int wam_gpu_set_propagation_weights(wam_ctx *ctx, int32_t stored);
 
+/* SET_MODEL_OPTION's WAVE_BREAKING and PHILLIPS switches
+   (wam_timopt_module.f90:176-177, the module variables WAVE_BREAKING_RUN and
+   PHILLIPS_RUN read by IMPLSCH, wam_source_module.f90:513-515,639-646): wave
+   breaking adds SDEPTHLIM (:3735-3780) and SFBRK with CMPQB (:1966-2107),
+   PHILLIPS adds SOURCE_PHILLIPS (:3051-3160).  Both are off by default.  As
+   in SET_MODEL_OPTION (:213-227), breaking stays off in deep water.  Takes
+   effect at the next IMPLSCH call; the step adjoint refuses PHILLIPS. */
+int wam_gpu_set_source_options(wam_ctx *ctx, int32_t wave_breaking, int32_t phillips);
+/* The switches as they stand (after the deep-water rule); either pointer may be NULL. */
+int wam_gpu_get_source_options(wam_ctx *ctx, int32_t *wave_breaking, int32_t *phillips);
+
 /* Drop-in PROPAGS: F3(NIJS:NIJL,KL,ML) in host memory, updated in place. */
 int wam_gpu_propags(wam_ctx *ctx, int32_t nijs, int32_t nijl, int32_t kl, int32_t ml, float *f3);
 

@@ -35,7 +35,8 @@ def write_case(d, cfg, mask, f, fl3, nstep):
         for k in keys:
             v = cfg[k]
             fh.write(f"  {k.upper()} = {repr(float(v)) + 'D0' if isinstance(v, float) else int(v)},\n")
-        fh.write(f"  NSTEP = {nstep}, CURRENT = {cur}, NSEA = {n}\n/\n")
+        fh.write(f"  NSTEP = {nstep}, CURRENT = {cur}, NSEA = {n},\n")
+        fh.write(f"  WAVE_BREAKING = {int(cfg.get('wave_breaking', 0))}, PHILLIPS = {int(cfg.get('phillips', 0))}\n/\n")
     # MASK(NX,NY) in Fortran order is the row-major [NY][NX] array
     np.ascontiguousarray(mask, np.int8).tofile(os.path.join(d, "mask.i8"))
     for k in ("depth", "u10", "udir"):
@@ -51,13 +52,18 @@ def test_fortran_host_builds():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("which", ["c1", "c2cut"])
+@pytest.mark.parametrize("which", ["c1", "c2cut", "c2cut_sources"])
 def test_fortran_host_matches_oracle(tmp_path, which):
     exe = build_driver()
     if which == "c1":
         cfg, mask, f = W.c1_case()
     else:
         cfg, mask, f = W.c2_case(ny=30, nx=40, seed=12, kl=24, ml=25)
+    if which == "c2cut_sources":
+        # WAVE_BREAKING and PHILLIPS through WAM_GPU_SET_SOURCE_OPTIONS, on depths of 0.6 to 24 m
+        cfg = dict(cfg, wave_breaking=1, phillips=1)
+        n = f["depth"].size
+        f = dict(f, depth=np.geomspace(0.6, 24.0, n).astype(np.float32)[np.random.default_rng(4).permutation(n)])
     nstep = 3
     fl3 = W.jonswap_spectrum(cfg, f["udir"]).ravel()
     d = str(tmp_path / which)

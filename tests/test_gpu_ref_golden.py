@@ -18,8 +18,6 @@ def test_gpu_matches_reference_implsch(path):
     from wam_amd import WamGpu
     z, cfg = load(path)
     print(f"[case] {os.path.basename(path)}", flush=True)
-    if cfg.get("wave_breaking") or cfg.get("phillips"):
-        pytest.skip("SDEPTHLIM, SFBRK and SOURCE_PHILLIPS are not built on the device")
     g = WamGpu(cfg, z["mask"])
     n = z["in_u10"].size
     cur = np.zeros(n, np.float32) if cfg["refr_c"] else None      # IMPLSCH does not read currents

@@ -59,6 +59,8 @@ SIGNATURES = {
     "wam_gpu_multigrid_last_error": (ctypes.c_char_p, [_P]),
     "wam_gpu_multigrid_free": (None, [_P]),
     "wam_gpu_set_propagation_weights": (_I, [_P, _I]),
+    "wam_gpu_set_source_options": (_I, [_P, _I, _I]),
+    "wam_gpu_get_source_options": (_I, [_P, _P, _P]),
     "wam_gpu_set_stream": (_I, [_P, _P]),
     "wam_gpu_output_parameters_dev": (_I, [_P] * 7 + [_I, _P]),
     "wam_gpu_output_parameters": (_I, [_P, _I, _I, _I] + [_P] * 6 + [_I, _P]),

@@ -767,7 +767,7 @@ static int ensure_impl_scratch(wam_ctx *ctx, ImplScratch &x, size_t cap, int lcf
         HIPCHK(hipMalloc(&x.slt, tiled * sizeof(float)));
         HIPCHK(hipMalloc(&x.flt, tiled * sizeof(float)));
         HIPCHK(hipMalloc(&x.axy, 3 * (size_t)t.ML * cap * sizeof(float)));
-        HIPCHK(hipMalloc(&x.pp, 6 * cap * sizeof(float)));
+        HIPCHK(hipMalloc(&x.pp, 8 * cap * sizeof(float)));
         HIPCHK(hipMalloc(&x.sprd, (size_t)t.KL * cap * sizeof(float)));
         HIPCHK(hipMalloc(&x.llws, nw * cap * sizeof(unsigned)));
         HIPCHK(hipMalloc(&x.tick, 8 * 32 * sizeof(unsigned)));
@@ -807,7 +807,7 @@ static int ensure_adj_scratch(wam_ctx *ctx, size_t cap)
 static size_t impl_scratch_per_point(const Tables &t, int lcflx)
 {
     const size_t nb = (size_t)t.KL * t.ML, nw = (nb + 31) / 32;
-    return ((3 + (lcflx ? 1 : 0)) * nb + 3 * (size_t)t.ML + 6 + t.KL + nw) * 4;
+    return ((3 + (lcflx ? 1 : 0)) * nb + 3 * (size_t)t.ML + 8 + t.KL + nw) * 4;
 }
 
 // The member-points per IMPLSCH launch sequence of an ensemble: the tuned
@@ -992,7 +992,8 @@ int wam_gpu_ensemble_run_dev(wam_ctx *ctx, int32_t nmem, int32_t nsteps, float *
         (uintptr_t)nmem, (uintptr_t)nsteps, (uintptr_t)d_fa, (uintptr_t)d_fb, (uintptr_t)d_u10, (uintptr_t)d_udir,
         (uintptr_t)d_tauw, (uintptr_t)d_ustar, (uintptr_t)d_z0, (uintptr_t)d_roairn, (uintptr_t)d_wstar,
         (uintptr_t)ctx->tune.prop_wg, (uintptr_t)ctx->tune.src_wg, (uintptr_t)ctx->tune.snl_per_cu,
-        (uintptr_t)ctx->esx.NX, (uintptr_t)ctx->counter, (uintptr_t)ctx->dt.DEPTH, (uintptr_t)ctx->coef};
+        (uintptr_t)ctx->esx.NX, (uintptr_t)ctx->counter, (uintptr_t)ctx->dt.DEPTH, (uintptr_t)ctx->coef,
+        (uintptr_t)(ctx->dt.brk | ctx->dt.phil << 1)};
     if (!ctx->gstream) {
         HIPCHK(hipStreamCreateWithFlags(&ctx->gstream, hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&ctx->gev[0], hipEventDisableTiming));
@@ -1184,7 +1185,7 @@ struct wam_multigrid {
     std::vector<hipEvent_t> ev;         // [0] fork, [1..n] joins
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    int gsteps = -1, gparity = -1;
+    int gsteps = -1, gparity = -1, gopts = -1;
     int parity = 0;                     // swaps done: grid states are in d_fb when odd
     std::string err;
 };
@@ -1368,7 +1369,13 @@ int wam_gpu_multigrid_run(wam_multigrid *mg, int32_t nsteps, int32_t fused)
         const int rc = mg_steps(mg, nsteps, mg->parity, false);
         if (rc) return rc;
     } else {
-        if (!mg->gexec || mg->gsteps != nsteps || mg->gparity != mg->parity) {
+        // the grids' source options are baked into the captured launches
+        int opts = 0;
+        for (int i = 0; i < n; i++) {
+            const DevTables &d = mg->grids[i].st.ctx->dt;
+            opts = opts * 4 + (d.brk | d.phil << 1);
+        }
+        if (!mg->gexec || mg->gsteps != nsteps || mg->gparity != mg->parity || mg->gopts != opts) {
             if (mg->gexec) { (void)hipGraphExecDestroy(mg->gexec); mg->gexec = nullptr; }
             if (mg->graph) { (void)hipGraphDestroy(mg->graph); mg->graph = nullptr; }
             hipStream_t cap = nullptr;
@@ -1389,6 +1396,7 @@ int wam_gpu_multigrid_run(wam_multigrid *mg, int32_t nsteps, int32_t fused)
             MGCHK(hipGraphInstantiate(&mg->gexec, g, nullptr, nullptr, 0));
             mg->gsteps = nsteps;
             mg->gparity = mg->parity;
+            mg->gopts = opts;
         }
         MGCHK(hipGraphLaunch(mg->gexec, mg->origin));
     }
@@ -1467,7 +1475,7 @@ int64_t wam_gpu_get_tuning_log(wam_ctx *ctx, char *buf, int64_t n)
 static int noalloc_layout(const Tables &t, void *base, ImplScratch &x)
 {
     const size_t nb = (size_t)t.KL * t.ML, nw = (nb + 31) / 32;
-    const size_t per_pt = (4 * nb + 3 * (size_t)t.ML + 6 + t.KL + nw) * 4;
+    const size_t per_pt = (4 * nb + 3 * (size_t)t.ML + 8 + t.KL + nw) * 4;
     const size_t fixed = 1024 + 10 * 256;                       // tickets, alignment
     if (NOALLOC_SCRATCH_BYTES < fixed + 64 * per_pt) return 0;
     const size_t c = (NOALLOC_SCRATCH_BYTES - fixed) / per_pt / 64 * 64;
@@ -1480,7 +1488,7 @@ static int noalloc_layout(const Tables &t, void *base, ImplScratch &x)
     x.flt = (float *)take(c * nb * 4);
     x.smt = (float *)take(c * nb * 4);
     x.axy = (float *)take(3 * (size_t)t.ML * c * 4);
-    x.pp = (float *)take(6 * c * 4);
+    x.pp = (float *)take(8 * c * 4);
     x.sprd = (float *)take((size_t)t.KL * c * 4);
     x.llws = (unsigned *)take(nw * c * 4);
     x.tick = (unsigned *)take(8 * 32 * 4);
@@ -1846,6 +1854,23 @@ int wam_gpu_set_propagation_weights(wam_ctx *ctx, int32_t stored)
     return WAM_OK;
 }
 
+int wam_gpu_set_source_options(wam_ctx *ctx, int32_t wave_breaking, int32_t phillips)
+{
+    if (!ctx) return WAM_E_ARG;
+    // SET_MODEL_OPTION switches breaking off in deep water (wam_timopt_module.f90:213-227)
+    ctx->dt.brk = wave_breaking && ctx->tab.cfg.shallow ? 1 : 0;
+    ctx->dt.phil = phillips ? 1 : 0;
+    return WAM_OK;
+}
+
+int wam_gpu_get_source_options(wam_ctx *ctx, int32_t *wave_breaking, int32_t *phillips)
+{
+    if (!ctx) return WAM_E_ARG;
+    if (wave_breaking) *wave_breaking = ctx->dt.brk;
+    if (phillips) *phillips = ctx->dt.phil;
+    return WAM_OK;
+}
+
 int wam_gpu_set_timing(wam_ctx *ctx, int32_t on)
 {
     if (!ctx) return WAM_E_ARG;
@@ -1899,6 +1924,7 @@ static int adjoint_ready(wam_ctx *ctx, int32_t nmem)
         ctx->err = "the step adjoint is built for IPHYS = 0 in deep water";
         return WAM_E_ARG;
     }
+    if (ctx->dt.phil) { ctx->err = "the step adjoint does not differentiate SOURCE_PHILLIPS"; return WAM_E_ARG; }
     if (t.KL > 64 || t.ML > 64) { ctx->err = "the step adjoint needs KL, ML <= 64"; return WAM_E_ARG; }
     if (!t.adj_err.empty()) { ctx->err = t.adj_err; return WAM_E_ARG; }
     if (ctx->adj_sym < 0) ctx->adj_sym = neighbours_symmetric(t) ? 1 : 0;

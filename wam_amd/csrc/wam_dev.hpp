@@ -24,6 +24,7 @@ enum { P_SUM = 0, P_LAT1, P_LAT2, P_LON1, P_LON2, P_TH1, P_TH2, P_SIG1, P_SIG2, 
 struct DevTables {
     int KL, ML, NSEA, NDEPTH, JUMAX, MFRSTLW, MLSTHG, NQ, NCMAX, NK10;
     int iphys, shallow, refr_d, refr_c;
+    int brk, phil;                        // WAVE_BREAKING_RUN (SDEPTHLIM, SFBRK), PHILLIPS_RUN (SOURCE_PHILLIPS)
     float G, ZPI, ROAIR, RNUAIRM, ROWATER, XKAPPA, XNLEV, BETAMAX, ZALP, ALPHA, CDFAC, EMIN, EPS1, CO;
     float DELTH, INV_LOG_CO, MO_TAIL, MM1_TAIL, MP1_TAIL, MP2_TAIL, DELU, DAL1, DAL2, X0TAUHF, DELT;
     float DEG;                            // 180/PI
@@ -131,7 +132,7 @@ struct ImplScratch {
     int P0, NP, NX;
     float *ft, *slt, *flt, *smt;   // F, SL, FL, SL - SPOS of the second SINPUT (LCFLX only)
     float *axy;                    // [3][ML][NX]: STRESSO's direction sums of the first SINPUT
-    float *pp;                     // [6][NX]: ENHFR, MAX(FMEANWS, FMEAN), MIJ, PHIAW, SIN(UDIR), COS(UDIR)
+    float *pp;                     // [8][NX]: ENHFR, MAX(FMEANWS, FMEAN), MIJ, PHIAW, SIN(UDIR), COS(UDIR), SFBRK's SBR, DSBR
     float *sprd;                   // [KL][NX]: SPRD(K) = MAX(0, COS(TH(K) - UDIR))**2 of the call
     unsigned *llws;                // [(KL*ML+31)/32][NX]: LLWS bits of the second SINPUT
     unsigned *tick;                // [8][32]: point counters of implsch_snl_kernel, one per XCD (own 128-byte line)

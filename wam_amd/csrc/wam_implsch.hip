@@ -108,6 +108,22 @@ __global__ void __launch_bounds__(NT) implsch_kernel(DevTables t, ImplArgs a)
         }
         dirsum(t, s.F, nullptr, T, tid);
         __syncthreads();
+        if (t.brk) {
+            // SDEPTHLIM (:513-515): the spectrum and EMEAN scaled by EM; the
+            // other mean parameters come from the scaled spectrum
+            if (tid == 0) {
+                const float e0 = wam_maxf(integ(ML, t.MO_TAIL, T, s.dfim), t.EMIN);
+                const float em = sdepthlim_em(P.depth, e0);
+                P.em = em;
+                P.emean = em * e0;
+            }
+            __syncthreads();
+            const float em = P.em;
+            for (int b = tid; b < NB; b += NT) s.F[b] = s.F[b] * em;
+            __syncthreads();
+            dirsum(t, s.F, nullptr, T, tid);
+            __syncthreads();
+        }
         // 2. mean parameters (the integrals run on separate waves)
         if (tid == 0) P.s_e = integ(ML, t.MO_TAIL, T, s.dfim);
         else if (tid == 64) P.s_fm = integ(ML, t.MM1_TAIL, T, s.dfimofr);
@@ -128,7 +144,7 @@ __global__ void __launch_bounds__(NT) implsch_kernel(DevTables t, ImplArgs a)
         }
         __syncthreads();
         if (tid == 0) {
-            const float e = wam_maxf(P.s_e, t.EMIN);
+            const float e = t.brk ? P.emean : wam_maxf(P.s_e, t.EMIN);
             P.emean = e;
             P.fmean = e / wam_maxf(P.s_fm, t.EMIN);
             P.f1mean = 1.f / (e > t.EMIN ? e / P.s_tm : 1.f);
@@ -136,6 +152,7 @@ __global__ void __launch_bounds__(NT) implsch_kernel(DevTables t, ImplArgs a)
                 P.akmean = e > t.EMIN ? wam_ipowf(e / P.s_w1, 2) : 1.f;
                 P.xkmean = e > t.EMIN ? wam_ipowf(P.s_w2 / e, 2) : 1.f;
             }
+            if (t.brk) sfbrk_rates(e, P.fmean, P.depth, P.sbr, P.dsbr);
         }
         __syncthreads();
         // 3. input source
@@ -312,17 +329,25 @@ __global__ void __launch_bounds__(NT) implsch_kernel(DevTables t, ImplArgs a)
                 wnfluxes<SHALLOW>(t, s, P.phiaw, P.ust, wam_sinf(P.udir), wam_cosf(P.udir), P.mij, tfk, a.fluxes + pt,
                                   a.NMP);
         }
-        // 4. limiter and implicit update
+        // 4. limiter and implicit update; SOURCE_PHILLIPS' TEMP(M) and TPHOLD(K)
+        //    into pm (free after the dissipation)
         if (tid < ML) {
             const float delfl = (t.C5G / wam_ipowf(s.fr[tid], 4)) * delt;
             t2[tid] = (P.ust * delfl) * wam_maxf(P.fmeanws, P.fmean);
+            if (t.phil) pm[tid] = phillips_temp<SHALLOW>(t, s.fr[tid], P.ust, SHALLOW ? tfk[tid] : 0.f, P.depth);
         }
+        if (t.phil && tid >= 64 && tid - 64 < KL) pm[MAXF + tid - 64] = phillips_tph(s.th[tid - 64], P.udir, P.ust);
         __syncthreads();
         for (int b = tid; b < NB; b += NT) {
             const int m = b / KL, k = b - KL * m;
-            float sl = s.SL[b];
+            float sl = s.SL[b], flv = s.FL[b];
             if (SHALLOW) sl = sl + s.SLBOT[b];
-            const float temp1 = (delt * sl) / wam_maxf(1.f, 1.f - delt * s.FL[b]);
+            if (t.brk) {                          // SFBRK (:640-642)
+                sl = sl + P.sbr * s.F[b];
+                flv = flv + P.dsbr;
+            }
+            if (t.phil) sl = sl + pm[m] * pm[MAXF + k];   // SOURCE_PHILLIPS (:644-646)
+            const float temp1 = (delt * sl) / wam_maxf(1.f, 1.f - delt * flv);
             const float temp3 = wam_minf(wam_fabsf(temp1), t2[m]);
             const float f = s.F[b] + wam_signf(temp3, temp1);
             s.F[b] = wam_maxf(f, flm[m] * sprd[k]);
@@ -432,8 +457,10 @@ __device__ __forceinline__ float stresso_cm(const DevTables &t, int m, float tfk
 // ADJ = 1 is the adjoint's recomputation of the same stages (identical
 // values): it keeps LLWS of the first SINPUT, COS(TH - UDIR), the scalar
 // chains' derivatives and STRESSO's values in e, and leaves the caller's
-// TAUW, USTAR, Z0 and PHIAW as they are.
-template <int SHALLOW, int ADJ = 0>
+// TAUW, USTAR, Z0 and PHIAW as they are.  BRK = 1 (shallow water with
+// WAVE_BREAKING_RUN) scales the spectrum by SDEPTHLIM's factor as it is read
+// and leaves SFBRK's rates for implsch_snl_kernel.
+template <int SHALLOW, int ADJ = 0, int BRK = 0>
 __global__ void __launch_bounds__(PW) __attribute__((amdgpu_waves_per_eu(4))) implsch_pre_kernel(DevTables t, ImplArgs a, ImplScratch x, AdjExtra e)
 // (workgroups of 64, 128 or 256 lanes, one sea point each)
 {
@@ -466,6 +493,26 @@ __global__ void __launch_bounds__(PW) __attribute__((amdgpu_waves_per_eu(4))) im
 
     const float const3 = ((2.f * t.XKAPPA) / t.BETAMAX) * (t.XKAPPA * t.XKAPPA);
     const float d1 = sqrtf(t.G) / t.ZPI, d2 = t.ZPI / sqrtf(t.G);
+    // 0. SDEPTHLIM (:513-515, :3735-3780): TOTAL_ENERGY of the spectrum as
+    //    given, then the factor EM that every later read of F applies
+    float em = 1.f, emean_lim = 0.f;
+    if (BRK) {
+        float se = 0.f, Tl = 0.f;
+        for (int i = 0; i < ML; i++) {
+            const int m = i == 0 ? ML - 1 : i - 1;
+            float T = 0.f;
+            for (int k = 0; k < KL; k++) {
+                const float f = F[(size_t)(k + KL * m) * NS];
+                T = k == 0 ? f : T + f;
+            }
+            if (i == 0) { Tl = T; se = t.MO_TAIL * T; }
+            else se = se + T * t.DFIM[m];
+        }
+        se = se + Tl * t.DFIM[ML - 1];
+        const float e0 = wam_maxf(se, t.EMIN);
+        em = sdepthlim_em(t.DEPTH[pt], e0);
+        emean_lim = em * e0;
+    }
     // 1. one pass over the spectrum: the mean parameters' integrals, the
     //    first SINPUT's LLWS-masked integrals and STRESSO's direction sums.
     //    Frequency ML goes first, since it seeds every integral; its own
@@ -483,7 +530,10 @@ __global__ void __launch_bounds__(PW) __attribute__((amdgpu_waves_per_eu(4))) im
             for (int k0 = 0; k0 < KL; k0 += 4) {
                 float f4[4];
 #pragma unroll
-                for (int u = 0; u < 4; u++) f4[u] = F[(size_t)(k0 + u + KL * m) * NS];
+                for (int u = 0; u < 4; u++) {
+                    f4[u] = F[(size_t)(k0 + u + KL * m) * NS];
+                    if (BRK) f4[u] = f4[u] * em;
+                }
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
                     const int k = k0 + u;
@@ -532,7 +582,7 @@ __global__ void __launch_bounds__(PW) __attribute__((amdgpu_waves_per_eu(4))) im
         }
         q_e = q_e + Ql * t.DFIM[m]; q_fm = q_fm + Ql * t.DFIMOFR[m];
     }
-    const float emean = wam_maxf(s_e, t.EMIN);
+    const float emean = BRK ? emean_lim : wam_maxf(s_e, t.EMIN);
     const float fmean = emean / wam_maxf(s_fm, t.EMIN);
     const float f1mean = 1.f / (emean > t.EMIN ? emean / s_tm : 1.f);
     float akmean = 1.f, xkmean = 1.f;
@@ -549,7 +599,7 @@ __global__ void __launch_bounds__(PW) __attribute__((amdgpu_waves_per_eu(4))) im
     float h1 = 0.f, h2 = 0.f;
     for (int k = 0; k < KL; k++) {
         const float cosw = wam_maxf(cu[k * W], 0.f);
-        const float f = F[(size_t)(k + KL * (mij - 1)) * NS];
+        const float f = BRK ? F[(size_t)(k + KL * (mij - 1)) * NS] * em : F[(size_t)(k + KL * (mij - 1)) * NS];
         if (k == 0) { h2 = f * wam_ipowf(cosw, 2); h1 = h2 * cosw; }
         else { h1 = h1 + f * wam_ipowf(cosw, 3); h2 = h2 + f * wam_ipowf(cosw, 2); }
     }
@@ -617,7 +667,10 @@ __global__ void __launch_bounds__(PW) __attribute__((amdgpu_waves_per_eu(4))) im
             for (int k0 = 0; k0 < KL; k0 += 4) {
                 float f4[4], sl4[4], fl4[4], sm4[4];
 #pragma unroll
-                for (int u = 0; u < 4; u++) f4[u] = F[(size_t)(k0 + u + KL * mrow) * NS];
+                for (int u = 0; u < 4; u++) {
+                    f4[u] = F[(size_t)(k0 + u + KL * mrow) * NS];
+                    if (BRK) f4[u] = f4[u] * em;
+                }
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
                     const int k = k0 + u, b = k + KL * m;
@@ -689,6 +742,12 @@ __global__ void __launch_bounds__(PW) __attribute__((amdgpu_waves_per_eu(4))) im
             x.pp[(size_t)4 * XS + lp] = wam_sinf(udir);
             x.pp[(size_t)5 * XS + lp] = wam_cosf(udir);
         }
+        if (BRK) {       // SFBRK's rates on EMEAN after SDEPTHLIM and FMEAN of 2.3
+            float sbr, dsbr;
+            sfbrk_rates(emean, fmean, t.DEPTH[pt], sbr, dsbr);
+            x.pp[(size_t)6 * XS + lp] = sbr;
+            x.pp[(size_t)7 * XS + lp] = dsbr;
+        }
     }
     // implsch_snl_kernel's point tickets start from zero
     if (blockIdx.x == 0 && lane < 8) x.tick[32 * lane] = 0u;
@@ -701,14 +760,16 @@ constexpr int SNL_QT = 256 * SP_QPT_DEV / SNL_NT;
 // Three workgroups (LDS-limited) of 8 waves per CU: registers for 6 waves per SIMD.
 // ADJ = 1 (the adjoint's recomputation) reads USTAR from e.sc and writes
 // the final SL and FL over the work arrays' and the limiter's output to e.fn,
-// leaving F after the tail in x.ft.
-template <int SHALLOW, int ADJ = 0>
+// leaving F after the tail in x.ft.  OPT = 1 adds the off-by-default sources
+// the context switches on: SFBRK (rates from implsch_pre_kernel) and
+// SOURCE_PHILLIPS, after SBOTTOM as IMPLSCH adds them (:639-646).
+template <int SHALLOW, int ADJ = 0, int OPT = 0>
 __global__ void __launch_bounds__(SNL_NT) __attribute__((amdgpu_waves_per_eu(6)))
 implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x, AdjExtra e)
 {
     extern __shared__ __align__(16) float lds_raw[];
     __shared__ float sprd[MAXF], flm[MAXF], tfk[MAXF], tlim[MAXF], tbot[MAXF];
-    __shared__ float Ps[5];
+    __shared__ float Ps[OPT ? 7 : 5];
     __shared__ int Pmij, Pnext;
     constexpr int S = 256 * SP_QPT_DEV;
     const int KL = t.KL, ML = t.ML, NB = KL * ML, NS = t.NSEA, NB4 = x.NB4, XS = x.NX, tid = threadIdx.x;
@@ -727,6 +788,8 @@ implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x, AdjExtra e)
     s.sprw = reinterpret_cast<int *>(p); p += 2 * t.SP_NCHUNK * t.SP_NRUN;
     s.fr = p; p += ML; s.dfim = p; p += ML;
     s.th = p; p += KL; s.costh = p; p += KL; s.sinth = p; p += KL;
+    float *ptemp = p, *ptph = p + ML;     // OPT: SOURCE_PHILLIPS' TEMP(M), TPHOLD(K)
+    if (OPT) p += ML + KL;
 
     for (int i = tid; i < ML; i += SNL_NT) { s.fr[i] = t.FR[i]; s.dfim[i] = t.DFIM[i]; }
     for (int i = tid; i < KL; i += SNL_NT) { s.th[i] = t.TH[i]; s.costh[i] = t.COSTH[i]; s.sinth[i] = t.SINTH[i]; }
@@ -792,6 +855,16 @@ implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x, AdjExtra e)
             Ps[0] = ADJ ? e.sc[(size_t)AS_UST2 * XS + lp] : a.ustar[mp];
             Ps[1] = x.pp[lp]; Ps[2] = x.pp[(size_t)XS + lp]; Ps[3] = x.pp[(size_t)3 * XS + lp];
             Pmij = (int)x.pp[(size_t)2 * XS + lp];
+            if (OPT && t.brk) { Ps[5] = x.pp[(size_t)6 * XS + lp]; Ps[6] = x.pp[(size_t)7 * XS + lp]; }
+        }
+        if (OPT && t.phil) {
+            // SOURCE_PHILLIPS on USTAR of the second AIRSEA
+            const float ust = a.ustar[mp];
+            if (tid < ML) {
+                const float tk = SHALLOW ? t.TFAK[tb(t, tid, t.INDEP[pt])] : 0.f;
+                ptemp[tid] = phillips_temp<SHALLOW>(t, t.FR[tid], ust, tk, t.DEPTH[pt]);
+            }
+            if (tid >= 256 && tid - 256 < KL) ptph[tid - 256] = phillips_tph(t.TH[tid - 256], a.udir[mp], ust);
         }
         // no elementary functions here: SPRD comes from implsch_pre_kernel and
         // SBOTTOM's factor from PREPARE_PROPAGATION's table (their double-
@@ -844,6 +917,11 @@ implsch_snl_kernel(DevTables t, ImplArgs a, ImplScratch x, AdjExtra e)
                     fv = fv + tbot[m];
                     sv = sv + slb;
                 }
+                if (OPT && t.brk) {
+                    sv = sv + Ps[5] * s.F[b];
+                    fv = fv + Ps[6];
+                }
+                if (OPT && t.phil) sv = sv + ptemp[m] * ptph[k];
                 const float temp1 = (delt * sv) / wam_maxf(1.f, 1.f - delt * fv);
                 const float temp3 = wam_minf(wam_fabsf(temp1), tlim[m]);
                 const float f = s.F[b] + wam_signf(temp3, temp1);
@@ -953,12 +1031,13 @@ __global__ void __launch_bounds__(PW) implsch_post_kernel(DevTables t, ImplArgs 
     if (on && a.mij) a.mij[mp] = mij;
 }
 
-size_t implsch_snl_lds_bytes(const DevTables &t, int lcflx)
+size_t implsch_snl_lds_bytes(const DevTables &t, int lcflx, int opt)
 {
     const int ML = t.ML, KL = t.KL, MH = t.MLSTHG, NB = KL * ML;
     size_t n = 2 * 9 * (size_t)SP_TSTRIDE_DEV + (size_t)NB;
     if (lcflx) n += (t.shallow ? 2 : 1) * (size_t)NB;
     n += 2 * (size_t)ML + 3 * (size_t)KL + 36 * (size_t)MH + 2 * (size_t)t.SP_NCHUNK * t.SP_NRUN;
+    if (opt) n += (size_t)ML + KL;
     return n * sizeof(float);
 }
 
@@ -981,13 +1060,18 @@ hipError_t launch_implsch_split(const DevTables &t, const ImplArgs &a, const Imp
     auto mark = [&](int i) { if (ev && ev[i]) (void)hipEventRecord(ev[i], st); };
     mark(0);
     const size_t lds1 = (size_t)t.KL * W * sizeof(float);
-    if (t.shallow) hipLaunchKernelGGL(implsch_pre_kernel<1>, dim3(nwg), dim3(W), lds1, st, t, a, x, AdjExtra{});
+    const bool opt = t.brk || t.phil;
+    if (t.shallow && t.brk)
+        hipLaunchKernelGGL((implsch_pre_kernel<1, 0, 1>), dim3(nwg), dim3(W), lds1, st, t, a, x, AdjExtra{});
+    else if (t.shallow) hipLaunchKernelGGL(implsch_pre_kernel<1>, dim3(nwg), dim3(W), lds1, st, t, a, x, AdjExtra{});
     else hipLaunchKernelGGL(implsch_pre_kernel<0>, dim3(nwg), dim3(W), lds1, st, t, a, x, AdjExtra{});
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     mark(1);
-    void (*k)(DevTables, ImplArgs, ImplScratch, AdjExtra) = t.shallow ? implsch_snl_kernel<1> : implsch_snl_kernel<0>;
-    const size_t lds = implsch_snl_lds_bytes(t, a.lcflx);
+    void (*k)(DevTables, ImplArgs, ImplScratch, AdjExtra) =
+        opt ? (t.shallow ? implsch_snl_kernel<1, 0, 1> : implsch_snl_kernel<0, 0, 1>)
+            : (t.shallow ? implsch_snl_kernel<1> : implsch_snl_kernel<0>);
+    const size_t lds = implsch_snl_lds_bytes(t, a.lcflx, opt);
     e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     int per_cu = 0;
@@ -1020,7 +1104,7 @@ hipError_t launch_implsch_adj_forward(const DevTables &t, const ImplArgs &a, con
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     void (*k)(DevTables, ImplArgs, ImplScratch, AdjExtra) = implsch_snl_kernel<0, 1>;
-    const size_t lds = implsch_snl_lds_bytes(t, 0);
+    const size_t lds = implsch_snl_lds_bytes(t, 0, 0);
     e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     int per_cu = 0;

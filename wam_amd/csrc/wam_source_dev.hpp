@@ -17,6 +17,7 @@ struct PointScalars {
     float ust, emean, fmean, f1mean, akmean, xkmean, emeanws, fmeanws, phiaw, sig_n, b1, enhfr;
     int mij;
     float s_e, s_fm, s_tm, s_w1, s_w2;                     // partial integrals
+    float em, sbr, dsbr;                                   // SDEPTHLIM's factor, SFBRK's rates (WAVE_BREAKING_RUN)
     // LFACTOR state
     float twx, twy, tauvx, tauvy, tau_tot, rtau, drtau;
     int sign_new, overshot, done, apply;
@@ -51,6 +52,64 @@ __device__ __forceinline__ void dirsum(const DevTables &t, const float *F, const
         }
         T[m] = s;
     }
+}
+
+// ---- the off-by-default sources (WAVE_BREAKING_RUN, PHILLIPS_RUN) ----------
+
+constexpr float GAMD_DEV = 0.8f;   // breaker parameter of SDEPTHLIM and SFBRK (wam_source_module.f90:67)
+
+// SDEPTHLIM's factor (:3735-3780): EM = MIN((0.25 GAMD DEPTH)**2 / EMEAN, 1);
+// the spectrum and EMEAN are multiplied by it.
+__device__ __forceinline__ float sdepthlim_em(float depth, float emean)
+{
+    const float h = (0.25f * GAMD_DEV) * depth;
+    return wam_minf((h * h) / emean, 1.0f);
+}
+
+// SFBRK (:1966-2037): SL += SBR * F, FL += DSBR, with the fraction of
+// breaking waves QB from CMPQB's Newton-Raphson step (:2041-2107) on
+// BB = 8 EMEAN / (GAMD DEPTH)**2.
+__device__ __forceinline__ void sfbrk_rates(float emean, float fmean, float depth, float &sbr, float &dsbr)
+{
+    const float gd = GAMD_DEV * depth;
+    const float bb = (8.f * emean) / (gd * gd);
+    float q0 = 0.f;
+    if (bb >= 0.25f) {
+        const float q = 2.f * sqrtf(bb) - 1.f;
+        q0 = q * q;
+    }
+    float qb = 1.f;
+    if (bb < 1.f) {
+        const float ex = wam_expf((q0 - 1.f) / bb);
+        qb = q0 - (bb * (q0 - ex)) / (bb - ex);
+    }
+    qb = wam_minf(1.f, qb);
+    sbr = -(2.f * fmean);                   // -ALPHA*2.*FMEAN with ALPHA = 1
+    if (bb <= 1.f) sbr = (sbr * qb) / bb;
+    dsbr = 0.f;
+    if (bb < 1.f && wam_fabsf(bb - qb) > 0.f) dsbr = (sbr * (1.f - qb)) / (bb - qb);
+}
+
+// SOURCE_PHILLIPS (:3051-3160): SL(K,M) += TEMP(M) * TPHOLD(K).  TEMP of
+// frequency fr (tfk = TFAK(INDEP, M), shallow water only) ...
+template <int SHALLOW>
+__device__ __forceinline__ float phillips_temp(const DevTables &t, float fr, float ust, float tfk, float depth)
+{
+    const float fpm = wam_maxf(0.0000001f, (28.f / (0.13f * t.G)) * ust);
+    const float xeps = t.ROAIR / t.ROWATER;
+    const float c1 = ((80.f * 16.0f) * (xeps * xeps)) / ((0.5f * 3.0f) * (t.G * t.G));
+    const float tp = wam_expf(-(1.f / wam_ipowf(fr * fpm, 4)));     // EXP(-(FR*FPM)**(-4))
+    if (SHALLOW) {
+        const float kd = wam_minf(tfk * depth, 40.f);
+        return (c1 * tp) / (1.0f + (2.0f * kd) / wam_sinhf(2.f * kd));
+    }
+    return c1 * tp;
+}
+
+// ... and TPHOLD of direction th.
+__device__ __forceinline__ float phillips_tph(float th, float udir, float ust)
+{
+    return wam_ipowf(ust * wam_maxf(0.f, wam_cosf(th - udir)), 4);
 }
 
 // seeded sequential sum c0*T(ML) + SUM_M T(M)*w(M) (the frequency integrals of

@@ -186,8 +186,6 @@ class WamGpu:
         for name, _ in _lib.WamConfig._fields_:
             setattr(c, name, cfg[name])
         self._c = c
-        if cfg.get("wave_breaking") or cfg.get("phillips"):
-            raise WamError(1, "WAVE_BREAKING and PHILLIPS sources are not available on the device")
         m = np.ascontiguousarray(mask, dtype=np.uint8).ravel()
         if m.size != cfg["nx"] * cfg["ny"]:
             raise ValueError("mask size must be nx*ny")
@@ -198,6 +196,8 @@ class WamGpu:
         self._check(rc)
         self.kl, self.ml = cfg["kl"], cfg["ml"]
         self.nsea = int(self.scalar("NSEA"))
+        if not host_only and (cfg.get("wave_breaking") or cfg.get("phillips")):
+            self.set_source_options(bool(cfg.get("wave_breaking")), bool(cfg.get("phillips")))
 
     def _check(self, rc):
         if rc != 0:
@@ -223,6 +223,15 @@ class WamGpu:
         next prepare_propagation."""
         mode = -1 if stored is None else int(bool(stored))
         self._check(self.lib.wam_gpu_set_propagation_weights(self.ctx, mode))
+
+    def set_source_options(self, wave_breaking=False, phillips=False):
+        """SET_MODEL_OPTION's WAVE_BREAKING (SDEPTHLIM and SFBRK; stays off in
+        deep water) and PHILLIPS (SOURCE_PHILLIPS) switches, off by default;
+        they apply from the next IMPLSCH call.  Returns the switches as set."""
+        self._check(self.lib.wam_gpu_set_source_options(self.ctx, int(bool(wave_breaking)), int(bool(phillips))))
+        b, p = ctypes.c_int32(), ctypes.c_int32()
+        self._check(self.lib.wam_gpu_get_source_options(self.ctx, ctypes.byref(b), ctypes.byref(p)))
+        return bool(b.value), bool(p.value)
 
     def prepare_propagation(self, depth, u=None, v=None):
         d = np.ascontiguousarray(depth, dtype=np.float32)
