@@ -33,7 +33,7 @@ enum {
     WAM_OK = 0,
     WAM_E_ARG = 1,        /* bad argument, size mismatch or unsupported option */
     WAM_E_HIP = 2,        /* HIP runtime error (allocation, launch) */
-    WAM_E_CFL = 3,        /* negative propagation weight (CHECK_CFL abort) */
+    WAM_E_CFL = 3,        /* negative propagation weight (CHECK_CFL abort), or a CFL number not finite or above 2^20 */
     WAM_E_STATE = 4,      /* call order violated (e.g. IMPLSCH before PREPARE_PROPAGATION) */
     WAM_E_GRID = 5,       /* empty or invalid sea mask */
     WAM_E_IO = 6          /* restart or boundary file: cannot be opened, read or written, or inconsistent with the grid */
@@ -79,7 +79,8 @@ int wam_gpu_init_host(wam_ctx **ctx, const wam_config *cfg, const uint8_t *sea_m
    currents per sea point (host arrays of NSEA; u, v may be NULL = no
    current).  Computes INDEP, the propagation weights on the device, the CFL
    sub-step count, and the per-point wave numbers for ST6 (WAVNU2 depends on
-   depth only).  Must precede IMPLSCH and PROPAGS. */
+   depth only).  Must precede IMPLSCH and PROPAGS.  A depth that is not
+   positive and finite, or a current that is not finite, returns WAM_E_ARG. */
 int wam_gpu_prepare_propagation(wam_ctx *ctx, const float *depth, const float *u, const float *v);
 
 /* Device-resident PREPARE_PROPAGATION (wam_propagation_module.f90:279-598,
@@ -288,8 +289,12 @@ enum { WAM_CK_ADVANCE = 0, WAM_CK_STORE = 1, WAM_CK_RESTORE = 2, WAM_CK_ADJOINT 
 int64_t wam_checkpoint_schedule(int32_t nsteps, int32_t nsnap, int32_t *actions, int64_t cap);
 int64_t wam_checkpoint_recomputed_steps(int32_t nsteps, int32_t nsnap);
 
-/* Stream used by the context (a hipStream_t; NULL = the context's own). */
+/* Stream used by the context (a hipStream_t; NULL = the context's own).
+   No counterpart in the reference (a host-side CPU model). */
 int wam_gpu_set_stream(wam_ctx *ctx, void *stream);
+/* The stream the context's device work is ordered on, for callers that order
+   their own streams against it. */
+int wam_gpu_get_stream(wam_ctx *ctx, void **stream);
 
 /* Integrated output parameters (COMPUTE_OUTPUT_PARAMETER,
    wam_output_module.f90:481-601, with TOTAL_ENERGY, FEMEAN,

@@ -45,6 +45,32 @@ def test_single_bin_spectrum(k0, m0):
     assert np.all(out[IX["spread"]] < 0.05)
 
 
+def test_top_frequency_tails():
+    """Energy in the top frequency ML only: every moment then carries the
+    f**-5 tail beyond FR(ML) (wam_fre_dir_module.f90:406-409, EX_TAIL = -5),
+    M0 = a (DFIM + DELTH FR/4), M1 = a (DFIM FR + DELTH FR**2/3),
+    M2 = a (DFIM FR**2 + DELTH FR**3/2), so TM1 = M0/M1 and TM2 = SQRT(M0/M2)
+    (TM1_TM2_PERIODS_B, wam_interface_module.f90:1210-1243); a dropped tail or a
+    wrong tail exponent moves them by several per cent."""
+    cfg, mask = deep_case()
+    o = Oracle(cfg, mask)
+    n, kl, ml = o.nsea, cfg["kl"], cfg["ml"]
+    a = 0.04
+    fl3 = np.zeros((ml, kl, n), np.float32)
+    fl3[ml - 1, 5, :] = a
+    out = run(o, fl3.ravel(), n)
+    fr, dfim = float(o.get("FR")[ml - 1]), float(o.get("DFIM")[ml - 1])
+    delth = 2.0 * math.pi / kl
+    m0 = a * (dfim + delth * fr / 4.0)
+    m1 = a * (dfim * fr + delth * fr ** 2 / 3.0)
+    m2 = a * (dfim * fr ** 2 + delth * fr ** 3 / 2.0)
+    assert np.allclose(out[IX["hs"]], 4.0 * math.sqrt(m0), rtol=2e-6)
+    assert np.allclose(out[IX["tm1"]], m0 / m1, rtol=2e-6)
+    assert np.allclose(out[IX["tm2"]], math.sqrt(m0 / m2), rtol=2e-6)
+    # without the tail TM2 would be 1/FR: the tail's share is not negligible
+    assert abs(float(out[IX["tm2"]][0]) * fr - 1.0) > 0.02
+
+
 def test_cos2_spreading_about_the_wind():
     """(2/pi) cos^2 spreading: mean resultant length 8/(3 pi), spread
     sqrt(2 (1 - 8/(3 pi))) rad, mean direction the wind's."""

@@ -62,6 +62,7 @@ SIGNATURES = {
     "wam_gpu_set_source_options": (_I, [_P, _I, _I]),
     "wam_gpu_get_source_options": (_I, [_P, _P, _P]),
     "wam_gpu_set_stream": (_I, [_P, _P]),
+    "wam_gpu_get_stream": (_I, [_P, _P]),
     "wam_gpu_output_parameters_dev": (_I, [_P] * 7 + [_I, _P]),
     "wam_gpu_output_parameters": (_I, [_P, _I, _I, _I] + [_P] * 6 + [_I, _P]),
     "wam_gpu_save_restart": (_I, [_P, ctypes.c_char_p, ctypes.c_char_p] + [_P] * 4),

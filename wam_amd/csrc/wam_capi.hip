@@ -158,10 +158,14 @@ struct wam_ctx {
     std::string err;
 };
 
+// A failed HIP call also leaves its code as the thread's last error, which
+// a later launch check (hipGetLastError) would report again: it is cleared
+// with the failure, so the next call starts clean.
 #define HIPCHK(x)                                                                  \
     do {                                                                           \
         hipError_t e_ = (x);                                                       \
         if (e_ != hipSuccess) {                                                    \
+            (void)hipGetLastError();                                               \
             ctx->err = std::string(#x) + ": " + hipGetErrorString(e_);             \
             return WAM_E_HIP;                                                      \
         }                                                                          \
@@ -412,6 +416,9 @@ static int finish_prepare(wam_ctx *ctx)
     for (float x : {0.1f, lon2mn, lon2mn, th1mn, th2mn, s1mn, s2mn, summn}) cmin = fminf(cmin, x);
     ctx->cfl_max = cmax; ctx->cfl_min = cmin;
     if (cmin < 0.f) { ctx->err = "negative propagation weight (CHECK_CFL)"; return WAM_E_CFL; }
+    // COUNTER = CEILING(CFL) sub-steps: a CFL number that is not finite, or
+    // one that would need more than 2^20 sub-steps, is refused
+    if (!(cmax <= 1048576.f)) { ctx->err = "CFL maximum not finite or above 2^20 (CHECK_CFL)"; return WAM_E_CFL; }
     ctx->counter = 1;
     if (cmax > 0.999f) {
         ctx->counter = (int)std::ceil(cmax);
@@ -486,7 +493,7 @@ int wam_gpu_prepare_propagation_dev(wam_ctx *ctx, const float *d_depth, const fl
     int *ind = nullptr, *indg = nullptr;
     unsigned *bad = nullptr;
     int r = dev(ns * 4, (void **)&dd) | dev(ns * 4, (void **)&du) | dev(ns * 4, (void **)&dv) |
-            dev(ns * 4, (void **)&ind) | dev(ns * 4, (void **)&indg) | dev(4, (void **)&bad);
+            dev(ns * 4, (void **)&ind) | dev(ns * 4, (void **)&indg) | dev(8, (void **)&bad);
     for (int i = 0; i < 8; i++) r |= dev(ns * 4, (void **)&f[i]);
     if (r) return r;
     HIPCHK(hipMemcpyAsync(dd, d_depth, ns * 4, hipMemcpyDeviceToDevice, st));
@@ -494,7 +501,7 @@ int wam_gpu_prepare_propagation_dev(wam_ctx *ctx, const float *d_depth, const fl
     else HIPCHK(hipMemsetAsync(du, 0, ns * 4, st));
     if (d_v) HIPCHK(hipMemcpyAsync(dv, d_v, ns * 4, hipMemcpyDeviceToDevice, st));
     else HIPCHK(hipMemsetAsync(dv, 0, ns * 4, st));
-    HIPCHK(hipMemsetAsync(bad, 0, 4, st));
+    HIPCHK(hipMemsetAsync(bad, 0, 8, st));
     PrepPoints pp{dd, du, dv, ind, indg, f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], bad,
                   (float)t.cfg.deptha, wam_logf((float)t.cfg.depthd), (float)t.cfg.idelpro};
     DevTables &d = ctx->dt;
@@ -505,12 +512,12 @@ int wam_gpu_prepare_propagation_dev(wam_ctx *ctx, const float *d_depth, const fl
     HIPCHK(hipGetLastError());
     // the host copies of the per-point tables (INDEP is IMPLSCH's argument on
     // the host path), and the depth check
-    unsigned hbad = 0;
+    unsigned hbad[2] = {0u, 0u};
     t.INDEP.resize(ns); t.INDEP_G.resize(ns);
     std::vector<float> *hf[8] = {&t.DDPHI, &t.DDLAM, &t.DUPHI, &t.DULAM, &t.DVPHI, &t.DVLAM, &t.DELLA0, &t.DRGM};
     ctx->depth_h.resize(ns);
     HIPCHK(hipMemcpyAsync(ctx->depth_h.data(), dd, ns * 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hbad, bad, 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(t.INDEP.data(), ind, ns * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(t.INDEP_G.data(), indg, ns * 4, hipMemcpyDeviceToHost, st));
     for (int i = 0; i < 8; i++) {
@@ -518,7 +525,8 @@ int wam_gpu_prepare_propagation_dev(wam_ctx *ctx, const float *d_depth, const fl
         HIPCHK(hipMemcpyAsync(hf[i]->data(), f[i], ns * 4, hipMemcpyDeviceToHost, st));
     }
     HIPCHK(hipStreamSynchronize(st));
-    if (hbad) { ctx->err = "depth must be positive at every sea point"; return WAM_E_ARG; }
+    if (hbad[0]) { ctx->err = "depth must be positive and finite at every sea point"; return WAM_E_ARG; }
+    if (hbad[1]) { ctx->err = "currents must be finite at every sea point"; return WAM_E_ARG; }
     return finish_prepare(ctx);
 }
 
@@ -760,19 +768,38 @@ static int ensure_impl_scratch(wam_ctx *ctx, ImplScratch &x, size_t cap, int lcf
     const size_t nb = (size_t)t.KL * t.ML, nw = (nb + 31) / 32;
     if (x.ft && (size_t)x.NX != cap) free_impl_scratch(x);
     const size_t tiled = (cap + 63) / 64 * 64 * nb;      // whole 64-point tiles
+    auto failed = [&](hipError_t e) {
+        (void)hipGetLastError();
+        ctx->err = std::string("IMPLSCH work arrays: ") + hipGetErrorString(e);
+        return WAM_E_HIP;
+    };
     if (!x.ft) {
-        x.NB4 = (int)(nb / 4);
-        x.P0 = 0; x.NP = (int)cap; x.NX = (int)cap;
-        HIPCHK(hipMalloc(&x.ft, tiled * sizeof(float)));
-        HIPCHK(hipMalloc(&x.slt, tiled * sizeof(float)));
-        HIPCHK(hipMalloc(&x.flt, tiled * sizeof(float)));
-        HIPCHK(hipMalloc(&x.axy, 3 * (size_t)t.ML * cap * sizeof(float)));
-        HIPCHK(hipMalloc(&x.pp, 8 * cap * sizeof(float)));
-        HIPCHK(hipMalloc(&x.sprd, (size_t)t.KL * cap * sizeof(float)));
-        HIPCHK(hipMalloc(&x.llws, nw * cap * sizeof(unsigned)));
-        HIPCHK(hipMalloc(&x.tick, 8 * 32 * sizeof(unsigned)));
+        // all or nothing: a failed allocation leaves no partial set behind
+        ImplScratch n{};
+        n.NB4 = (int)(nb / 4);
+        n.P0 = 0; n.NP = (int)cap; n.NX = (int)cap;
+        const struct { void **p; size_t bytes; } req[] = {
+            {(void **)&n.ft, tiled * sizeof(float)}, {(void **)&n.slt, tiled * sizeof(float)},
+            {(void **)&n.flt, tiled * sizeof(float)}, {(void **)&n.axy, 3 * (size_t)t.ML * cap * sizeof(float)},
+            {(void **)&n.pp, 8 * cap * sizeof(float)}, {(void **)&n.sprd, (size_t)t.KL * cap * sizeof(float)},
+            {(void **)&n.llws, nw * cap * sizeof(unsigned)}, {(void **)&n.tick, 8 * 32 * sizeof(unsigned)}};
+        for (const auto &r : req) {
+            const hipError_t e = hipMalloc(r.p, r.bytes);
+            if (e != hipSuccess) {
+                *r.p = nullptr;
+                free_impl_scratch(n);
+                return failed(e);
+            }
+        }
+        x = n;
     }
-    if (lcflx && !x.smt) HIPCHK(hipMalloc(&x.smt, tiled * sizeof(float)));
+    if (lcflx && !x.smt) {
+        const hipError_t e = hipMalloc(&x.smt, tiled * sizeof(float));
+        if (e != hipSuccess) {
+            x.smt = nullptr;
+            return failed(e);
+        }
+    }
     return WAM_OK;
 }
 
@@ -794,11 +821,22 @@ static int ensure_adj_scratch(wam_ctx *ctx, size_t cap)
     if (rc) return rc;
     if (!ctx->aex.fn) {
         const size_t tiled = (cap + 63) / 64 * 64 * nb;
-        HIPCHK(hipMalloc(&ctx->aex.fn, tiled * sizeof(float)));
-        HIPCHK(hipMalloc(&ctx->aex.llws1, nb * cap));
-        HIPCHK(hipMalloc(&ctx->aex.cu, (size_t)t.KL * cap * sizeof(float)));
-        HIPCHK(hipMalloc(&ctx->aex.cub, (size_t)t.KL * cap * sizeof(float)));
-        HIPCHK(hipMalloc(&ctx->aex.sc, (size_t)AS_NSC * cap * sizeof(float)));
+        AdjExtra n{};
+        const struct { void **p; size_t bytes; } req[] = {
+            {(void **)&n.fn, tiled * sizeof(float)}, {(void **)&n.llws1, nb * cap},
+            {(void **)&n.cu, (size_t)t.KL * cap * sizeof(float)}, {(void **)&n.cub, (size_t)t.KL * cap * sizeof(float)},
+            {(void **)&n.sc, (size_t)AS_NSC * cap * sizeof(float)}};
+        for (const auto &r : req) {
+            const hipError_t e = hipMalloc(r.p, r.bytes);
+            if (e != hipSuccess) {
+                *r.p = nullptr;
+                free_adj_extra(n);
+                (void)hipGetLastError();
+                ctx->err = std::string("adjoint work arrays: ") + hipGetErrorString(e);
+                return WAM_E_HIP;
+            }
+        }
+        ctx->aex = n;
     }
     return WAM_OK;
 }
@@ -1199,7 +1237,10 @@ static int mg_fail(wam_multigrid *mg, const std::string &m, int rc)
 #define MGCHK(x)                                                                   \
     do {                                                                           \
         hipError_t e_ = (x);                                                       \
-        if (e_ != hipSuccess) return mg_fail(mg, std::string(#x) + ": " + hipGetErrorString(e_), WAM_E_HIP); \
+        if (e_ != hipSuccess) {                                                    \
+            (void)hipGetLastError();                                               \
+            return mg_fail(mg, std::string(#x) + ": " + hipGetErrorString(e_), WAM_E_HIP); \
+        }                                                                          \
     } while (0)
 
 // One model step of grid g on stream st (the context's stream redirected).
@@ -2084,6 +2125,13 @@ int wam_gpu_set_stream(wam_ctx *ctx, void *stream)
 {
     if (!ctx) return WAM_E_ARG;
     ctx->stream = stream ? (hipStream_t)stream : ctx->own_stream;
+    return WAM_OK;
+}
+
+int wam_gpu_get_stream(wam_ctx *ctx, void **stream)
+{
+    if (!ctx || !stream) return WAM_E_ARG;
+    *stream = (void *)ctx->stream;
     return WAM_OK;
 }
 

@@ -119,7 +119,7 @@ struct PrepPoints {
     const float *depth, *u, *v;
     int *indep, *indep_g;
     float *ddphi, *ddlam, *duphi, *dulam, *dvphi, *dvlam, *della0, *drgm;
-    unsigned *bad;                 // set where DEPTH is not positive
+    unsigned *bad;                 // [0] set where DEPTH is not positive and finite, [1] where U or V is not finite
     float deptha, lndepthd, delpro;
 };
 

@@ -17,6 +17,8 @@
 //    the north/south neighbour rows are served from that XCD's L2.
 //  * wavnu2_kernel: WAVNU2 (wam_source_module.f90:4343-4461), which depends
 //    on depth only and is tabulated once per depth field.
+#include <cfloat>
+
 #include "wam_dev.hpp"
 #include "wam_math.h"
 
@@ -179,7 +181,8 @@ __global__ void __launch_bounds__(256) prep_points_kernel(DevTables t, PrepPoint
     if (ij >= t.NSEA) return;
     const int ND = t.NDEPTH;
     const float d0 = p.depth[ij];
-    if (!(d0 > 0.f)) p.bad[0] = 1u;
+    if (!(d0 > 0.f && d0 <= FLT_MAX)) p.bad[0] = 1u;
+    if (t.refr_c && !(fabsf(p.u[ij]) <= FLT_MAX && fabsf(p.v[ij]) <= FLT_MAX)) p.bad[1] = 1u;
     auto index = [&](float dep) {
         const int id = wam_nint(wam_logf(dep / p.deptha) / p.lndepthd + 1.f);
         return id < 1 ? 1 : (id > ND ? ND : id);

@@ -1,6 +1,7 @@
 // wam_setup.cpp -- host-side set-up of the WAM hot path (see wam_setup.hpp).
 #include "wam_setup.hpp"
 
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 
@@ -639,7 +640,16 @@ int prepare_points(Tables &t, const float *depth, const float *u, const float *v
     const int ns = t.NSEA, ND = t.cfg.ndepth;
     const float da = (float)t.cfg.deptha, ldd = wam_logf((float)t.cfg.depthd);
     for (int ij = 0; ij < ns; ij++)
-        if (!(depth[ij] > 0.f)) { err = "depth must be positive at every sea point"; return WAM_E_ARG; }
+        if (!(depth[ij] > 0.f && depth[ij] <= FLT_MAX)) {
+            err = "depth must be positive and finite at every sea point";
+            return WAM_E_ARG;
+        }
+    if (t.cfg.refr_c && u && v)
+        for (int ij = 0; ij < ns; ij++)
+            if (!(std::fabs(u[ij]) <= FLT_MAX && std::fabs(v[ij]) <= FLT_MAX)) {
+                err = "currents must be finite at every sea point";
+                return WAM_E_ARG;
+            }
     t.INDEP.assign(ns, 1);
     if (t.cfg.shallow)
         for (int ij = 0; ij < ns; ij++) {

@@ -200,8 +200,9 @@ __device__ __forceinline__ int frcut(const DevTables &t, const float *fr, float 
     const float fm2 = wam_maxf(fmws, fm) * fpmh;
     const float fpm = fppm / wam_maxf(ust, 1.0E-6f);
     const float fpm4 = wam_maxf(fm2, fpm);
-    const int mij = wam_nint(wam_log10f(fpm4) * t.INV_LOG_CO) + 1;
-    return min(max(1, mij), t.ML);
+    // MIN(ML, MAX(1, NINT(...) + 1)), clamped before the + 1 (a saturated
+    // NINT of a non-finite argument must not wrap)
+    return min(max(0, wam_nint(wam_log10f(fpm4) * t.INV_LOG_CO)), t.ML - 1) + 1;
 }
 
 // SINPUT's factors of one frequency (:2111-2308): CNSN, UCN1, UCN2, the two

@@ -10,7 +10,9 @@ Errors raise ``WamError`` carrying the library's code and message, where the
 reference calls ABORT1.
 """
 import ctypes
+import functools
 import os
+import sys
 
 import numpy as np
 
@@ -21,6 +23,26 @@ class WamError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"WAM error {code}: {msg}")
         self.code = code
+
+
+def _stream_ordered(fn):
+    """Device-memory entry points run on the context's stream.  When torch is
+    in use and its current stream is another one, the context's stream waits
+    for it before the call (inputs written there) and it waits for the
+    context's stream after the call (outputs read there); both waits are on
+    the device, the host does not block."""
+    @functools.wraps(fn)
+    def wrapper(self, *args, **kwargs):
+        streams = self._torch_streams()
+        if streams is None:
+            return fn(self, *args, **kwargs)
+        cur, own = streams
+        own.wait_stream(cur)
+        try:
+            return fn(self, *args, **kwargs)
+        finally:
+            cur.wait_stream(own)
+    return wrapper
 
 
 def _ptr(a):
@@ -204,6 +226,24 @@ class WamGpu:
             msg = self.lib.wam_gpu_last_error(self.ctx)
             raise WamError(rc, msg.decode() if msg else "")
 
+    def _torch_streams(self):
+        """(torch's current stream, the context's stream as a torch stream),
+        or None when there is nothing to order (torch not in use on the
+        device, or the context already runs on torch's current stream)."""
+        torch = sys.modules.get("torch")
+        if torch is None or not torch.cuda.is_initialized():
+            return None
+        h = ctypes.c_void_p()
+        self._check(self.lib.wam_gpu_get_stream(self.ctx, ctypes.byref(h)))
+        cur = torch.cuda.current_stream()
+        handle = h.value or 0
+        if handle == cur.cuda_stream:
+            return None
+        cache = self.__dict__.setdefault("_own_streams", {})
+        if handle not in cache:
+            cache[handle] = torch.cuda.ExternalStream(handle)
+        return cur, cache[handle]
+
     def close(self):
         if getattr(self, "ctx", None):
             self.lib.wam_gpu_finalize(self.ctx)
@@ -243,6 +283,7 @@ class WamGpu:
         self._check(self.lib.wam_gpu_prepare_propagation(self.ctx, _ptr(d), _ptr(uu), _ptr(vv)))
         return int(self.scalar("COUNTER")), float(self.scalar("CFL_MAX"))
 
+    @_stream_ordered
     def prepare_propagation_dev(self, depth, u=None, v=None):
         """PREPARE_PROPAGATION from fields already in HBM (torch tensors of
         NSEA float32 on the context's device, or device pointers as ints)."""
@@ -284,18 +325,21 @@ class WamGpu:
                                                        *[_ptr(x) for x in a], int(orient), _ptr(out)))
         return out
 
+    @_stream_ordered
     def output_parameters_dev(self, fl3, u10, udir, ustar, tauw, z0, out, orient=1):
         """Device-resident form on torch CUDA tensors; out holds NOUT * NSEA floats."""
         p = lambda x: ctypes.c_void_p(x.data_ptr())
         self._check(self.lib.wam_gpu_output_parameters_dev(self.ctx, p(fl3), p(u10), p(udir), p(ustar), p(tauw), p(z0),
                                                            int(orient), p(out)))
 
+    @_stream_ordered
     def save_restart(self, path, dates, fl3, u10, udir, tauw):
         """SAVE_RESTART_FILE from device tensors (wam_gpu_save_restart); the
         depth and current records are the context's prepared fields."""
         self._check(self.lib.wam_gpu_save_restart(self.ctx, os.fsencode(path), _dates(dates), _ptr(fl3), _ptr(u10),
                                                   _ptr(udir), _ptr(tauw)))
 
+    @_stream_ordered
     def connect_restart(self, path, fl3, u10, udir, tauw, depth=None, u=None, v=None):
         """CONNECT_RESTART into device tensors (wam_gpu_connect_restart);
         returns the dates and which optional records the file held."""
@@ -306,6 +350,7 @@ class WamGpu:
                                                      ctypes.byref(present)))
         return _undates(raw.raw), dict(depth=bool(present.value & 1), current=bool(present.value & 2))
 
+    @_stream_ordered
     def boundary_output(self, fl3, ijarc):
         """BOUNDARY_OUTPUT's gather and mean parameters at the 1-based sea
         points ijarc: FL3 as a host array [ML][KL][NSEA] or a device tensor;
@@ -323,6 +368,7 @@ class WamGpu:
                                                              _ptr(mean)))
         return fbc, mean
 
+    @_stream_ordered
     def boundary_input(self, fl3, f1, m1, f2, m2, which, del12, del1l, ijarf, ibfl, ibfr, bfw, return_set=True):
         """BOUNDARY_INPUT's time and space interpolation into FL3 (a host
         array [ML][KL][NSEA], updated in place, or a device tensor): which = 1
@@ -348,11 +394,13 @@ class WamGpu:
             self._check(self.lib.wam_gpu_boundary_input_dev(self.ctx, nbinp, *args, _ptr(fl3), _ptr(fi), _ptr(mi)))
         return (fi, mi) if return_set else None
 
+    @_stream_ordered
     def output_grids_dev(self, params, out, grids):
         """The gridded fields of the parameter numbers params (1-16, 32) from
         out (device tensor, [NOUT][NSEA]) into grids ([len(params)][NY][NX])."""
         self._check(self.lib.wam_gpu_output_grids_dev(self.ctx, _ptr(_fflag(params)), _ptr(out), _ptr(grids)))
 
+    @_stream_ordered
     def write_map(self, path, cdtpro, cdatea, params, out, append=False):
         """One output time of the MAP file from the device parameters out
         ([NOUT][NSEA], output_parameters_dev) for the parameter numbers params."""
@@ -409,9 +457,11 @@ class WamGpu:
         return dict(fl3=f, tauw=tw, ustar=us, z0=z, phiaw=ph, flux=fx)
 
     # -- device resident (torch tensors in HBM) ----------------------------
+    @_stream_ordered
     def propags_dev(self, d_in, d_out):
         self._check(self.lib.wam_gpu_propags_dev(self.ctx, _ptr(d_in), _ptr(d_out)))
 
+    @_stream_ordered
     def implsch_dev(self, fl3, u10, udir, tauw, ustar, z0, roairn, wstar, lcflx=False, phiaw=None, fluxes=None,
                     mij=None):
         self._check(self.lib.wam_gpu_implsch_dev(self.ctx, _ptr(fl3), _ptr(u10), _ptr(udir), _ptr(tauw),
@@ -419,9 +469,11 @@ class WamGpu:
                                                  int(bool(lcflx)), _ptr(phiaw), _ptr(fluxes), _ptr(mij)))
 
     # -- ensembles: NMEM members on this grid, member arrays concatenated --
+    @_stream_ordered
     def propags_ensemble_dev(self, nmem, d_in, d_out):
         self._check(self.lib.wam_gpu_propags_ensemble_dev(self.ctx, int(nmem), _ptr(d_in), _ptr(d_out)))
 
+    @_stream_ordered
     def implsch_ensemble_dev(self, nmem, fl3, u10, udir, tauw, ustar, z0, roairn, wstar, lcflx=False, phiaw=None,
                              fluxes=None, mij=None):
         self._check(self.lib.wam_gpu_implsch_ensemble_dev(self.ctx, int(nmem), _ptr(fl3), _ptr(u10), _ptr(udir),
@@ -429,6 +481,7 @@ class WamGpu:
                                                           _ptr(wstar), int(bool(lcflx)), _ptr(phiaw), _ptr(fluxes),
                                                           _ptr(mij)))
 
+    @_stream_ordered
     def ensemble_run_dev(self, nmem, nsteps, fa, fb, u10, udir, tauw, ustar, z0, roairn, wstar, graph=True):
         """nsteps PROPAGS + IMPLSCH steps of the ensemble; returns the buffer
         holding the result (fa after an even number of steps, else fb)."""
@@ -437,6 +490,7 @@ class WamGpu:
                                                       _ptr(roairn), _ptr(wstar), int(bool(graph))))
         return fa if nsteps % 2 == 0 else fb
 
+    @_stream_ordered
     def autotune(self, nmem, fa, fb, u10, udir, tauw, z0, roairn, wstar, reps=2):
         self._check(self.lib.wam_gpu_autotune(self.ctx, int(nmem), _ptr(fa), _ptr(fb), _ptr(u10), _ptr(udir),
                                               _ptr(tauw), _ptr(z0), _ptr(roairn), _ptr(wstar), int(reps)))
@@ -458,6 +512,7 @@ class WamGpu:
         self.lib.wam_gpu_get_tuning_log(self.ctx, buf, n + 1)
         return [(w, int(v), float(ms)) for w, v, ms in (l.split() for l in buf.value.decode().splitlines())]
 
+    @_stream_ordered
     def implsch_dev_noalloc(self, fl3, u10, udir, tauw, ustar, z0, roairn, wstar, lcflx=False, phiaw=None,
                             fluxes=None, mij=None):
         """implsch_dev on the library's static device scratch array: allocates nothing."""
@@ -465,6 +520,7 @@ class WamGpu:
                                                          _ptr(ustar), _ptr(z0), _ptr(roairn), _ptr(wstar),
                                                          int(bool(lcflx)), _ptr(phiaw), _ptr(fluxes), _ptr(mij)))
 
+    @_stream_ordered
     def step_adjoint_dev(self, fl3, u10, udir, tauw, roairn, wstar, fl3_bar, fl3_bar_out, u10_bar, udir_bar,
                          tauw_bar_out, tauw_bar=None, ustar_bar=None, z0_bar=None, nmem=1):
         """Adjoint of one PROPAGS + IMPLSCH step at the input state (fl3, tauw)
@@ -476,14 +532,17 @@ class WamGpu:
             _ptr(fl3_bar), _ptr(tauw_bar), _ptr(ustar_bar), _ptr(z0_bar), _ptr(fl3_bar_out), _ptr(u10_bar),
             _ptr(udir_bar), _ptr(tauw_bar_out)))
 
+    @_stream_ordered
     def propags_adjoint_dev(self, d_in_bar, d_out_bar, nmem=1):
         """Adjoint of PROPAGS: the transposed stencil, COUNTER sub-steps."""
         self._check(self.lib.wam_gpu_propags_adjoint_dev(self.ctx, int(nmem), _ptr(d_in_bar), _ptr(d_out_bar)))
 
+    @_stream_ordered
     def wind_uv_dev(self, u, v, u10, udir):
         """U10 = SQRT(U**2 + V**2), UDIR = ATAN2(U, V) (+ ZPI where negative)."""
         self._check(self.lib.wam_gpu_wind_uv_dev(self.ctx, _ptr(u), _ptr(v), _ptr(u10), _ptr(udir), int(u.numel())))
 
+    @_stream_ordered
     def wind_uv_adjoint_dev(self, u, v, u10_bar, udir_bar, u_bar, v_bar):
         self._check(self.lib.wam_gpu_wind_uv_adjoint_dev(self.ctx, _ptr(u), _ptr(v), _ptr(u10_bar), _ptr(udir_bar),
                                                          _ptr(u_bar), _ptr(v_bar), int(u.numel())))
