@@ -961,7 +961,10 @@ __global__ void __launch_bounds__(PW) implsch_post_kernel(DevTables t, ImplArgs 
     const float *FQ = x.ft + quad_offset(lp, 0, NB4);   // bins 4q..4q+3 at FQ[q * 256]
     const unsigned *LLW = x.llws + lp;
     const float ust = a.ustar[mp];
-    // direction sums (plain and LLWS-masked) and their integrals; frequency ML first
+    float *out = a.fl3 + (size_t)mem * KL * ML * NS + pt;   // FL3(pt, k, m) at out[(k + KL*m) * NS]
+    // direction sums (plain and LLWS-masked) and their integrals, frequency ML
+    // first; every bin is written to FL3 as it is read (the tail rows are
+    // written again below, once MIJ is known)
     float s_e = 0.f, s_fm = 0.f, s_w1 = 0.f, s_w2 = 0.f, Tl = 0.f, Ql = 0.f;
     for (int i = 0; i < ML; i++) {
         const int m = i == 0 ? ML - 1 : i - 1;
@@ -976,6 +979,7 @@ __global__ void __launch_bounds__(PW) implsch_post_kernel(DevTables t, ImplArgs 
             for (int u = 0; u < 4; u++) {
                 const int k = k0 + u;
                 const float f = fv[u];
+                if (on) out[(size_t)(b0 + u) * NS] = f;
                 T = k == 0 ? f : T + f;
                 if ((w >> u) & 1u) {
                     Q = qfirst ? f : Q + f;
@@ -1009,22 +1013,17 @@ __global__ void __launch_bounds__(PW) implsch_post_kernel(DevTables t, ImplArgs 
     } else {
         tden = t.FRM5[mij - 1];
     }
-    for (int m = 0; m < ML; m++) {
-        const bool tail = m >= mij;
-        float ttail = 0.f;
-        if (tail) {
-            const float an = SHALLOW ? wam_ipowf(1.f / t.TFAK[tb(t, m, indep)], 3) / t.TCGOND[tb(t, m, indep)]
-                                     : t.FRM5[m];
-            ttail = an / tden;
-        }
-        const int mrow = tail ? mij - 1 : m;
+    // IMPHFTAIL: the rows from MIJ up, from row MIJ - 1
+    for (int m = mij; m < ML; m++) {
+        const float an = SHALLOW ? wam_ipowf(1.f / t.TFAK[tb(t, m, indep)], 3) / t.TCGOND[tb(t, m, indep)]
+                                 : t.FRM5[m];
+        const float ttail = an / tden;
         for (int k0 = 0; k0 < KL; k0 += 4) {
-            const float4 f4 = *reinterpret_cast<const float4 *>(FQ + (size_t)((k0 + KL * mrow) >> 2) * 256);
-            const float fv[4] = {f4.x, f4.y, f4.z, f4.w};
+            const float4 f4 = *reinterpret_cast<const float4 *>(FQ + (size_t)((k0 + KL * (mij - 1)) >> 2) * 256);
             if (on) {
+                const float fv[4] = {f4.x, f4.y, f4.z, f4.w};
 #pragma unroll
-                for (int u = 0; u < 4; u++)
-                    a.fl3[((size_t)mem * KL * ML + k0 + u + KL * m) * NS + pt] = tail ? ttail * fv[u] : fv[u];
+                for (int u = 0; u < 4; u++) out[(size_t)(k0 + u + KL * m) * NS] = ttail * fv[u];
             }
         }
     }

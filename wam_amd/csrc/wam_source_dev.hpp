@@ -891,13 +891,19 @@ __device__ __forceinline__ void snonlin_products(const DevTables &t, const Smem 
             if (np == 0) continue;
             const uint32_t *e = reinterpret_cast<const uint32_t *>(t.SP_E + roff[j]) + lane;
             float a = sl[j], b = fl[j];
-            gather_group(Ub, pv[j], a, b);
-            for (int i = G; i < np; i += G) {          // np is a whole number of groups
-                uint32_t v[G];
+            uint32_t cur[G];
 #pragma unroll
-                for (int u = 0; u < G; u++) v[u] = e[(i + u) * 64];
-                gather_group(Ub, v, a, b);
+            for (int u = 0; u < G; u++) cur[u] = pv[j][u];
+            for (int i = G; i < np; i += G) {          // np is a whole number of groups
+                // the next group's entries are loaded before this group's LDS reads
+                uint32_t nxt[G];
+#pragma unroll
+                for (int u = 0; u < G; u++) nxt[u] = e[(i + u) * 64];
+                gather_group(Ub, cur, a, b);
+#pragma unroll
+                for (int u = 0; u < G; u++) cur[u] = nxt[u];
             }
+            gather_group(Ub, cur, a, b);
             sl[j] = a;
             fl[j] = b;
         }
