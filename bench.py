@@ -207,6 +207,66 @@ def bench_nest(args, torch, g, cfg, mask, fl3):
     print(json.dumps(out), flush=True)
 
 
+def bench_st6(args, torch):
+    """The ST6 physics (IPHYS = 2): configuration 2's production grid with ST6
+    (its secondary physics) and configuration 1 (0.5 deg 60 x 60 deep water,
+    24 x 25, ST6), each a plain PROPAGS + IMPLSCH step on spectra in HBM.
+    IMPLSCH takes its automatic form: three kernels at c2's size, the single
+    kernel (one workgroup per sea point) at c1's 3600 points."""
+    from wam_amd import WamGpu, workloads as W
+    dev = torch.device("cuda:0")
+    t = lambda x: torch.from_numpy(np.ascontiguousarray(x, np.float32)).to(dev)
+    res = {}
+    for name, make, steps in (("c2", lambda: W.c2_case(ny=args.ny, nx=args.nx, iphys=2), args.st6_steps),
+                              ("c1", lambda: W.c1_case(), 20)):
+        cfg, mask, f = make()
+        g = WamGpu(cfg, mask)
+        n = g.nsea
+        counter, cfl = g.prepare_propagation(f["depth"], f["u"] if cfg["refr_c"] else None,
+                                             f["v"] if cfg["refr_c"] else None)
+        stream = torch.cuda.current_stream()
+        g.set_stream(stream.cuda_stream)
+        u10, udir = t(f["u10"]), t(f["udir"])
+        tauw, ustar, z0 = (torch.zeros(n, device=dev) for _ in range(3))
+        roairn, wstar = torch.full((n,), 1.225, device=dev), torch.zeros(n, device=dev)
+        fa = device_spectrum(torch, cfg, udir).reshape(-1)
+        fb = torch.empty_like(fa)
+
+        def step():
+            nonlocal fa, fb
+            g.propags_dev(fa, fb)
+            fa, fb = fb, fa
+            g.implsch_dev(fa, u10, udir, tauw, ustar, z0, roairn, wstar)
+
+        step()
+        torch.cuda.synchronize()
+        g.set_timing(True)
+        torch.cuda.synchronize()
+        w0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - w0) / steps * 1e3
+        kt = {}
+        for k in KERNELS:
+            kms, calls = g.kernel_time(k)
+            if calls:
+                kt[k] = round(kms / calls, 3)
+        g.set_timing(False)
+        res[name] = {"nsea": n, "kl": cfg["kl"], "ml": cfg["ml"], "shallow": cfg["shallow"], "counter": counter,
+                     "steps": steps, "ms_per_step": round(ms, 3), "value": n / (ms * 1e-3),
+                     "unit": "sea-point updates/s", "kernel_ms": kt,
+                     "state_finite": bool(torch.isfinite(fa).all().item())}
+        log(f"[bench] ST6 {name}: NSEA={n}, {ms:.2f} ms/step, kernels {kt}")
+        del fa, fb, u10, udir, tauw, ustar, z0, roairn, wstar
+        g.close()
+        torch.cuda.empty_cache()
+    res["workload"] = ("ST6 (IPHYS=2): c2's 0.1deg 901x1600 grid, 36x36, shallow + currents, 300 s; c1: 0.5deg "
+                       "60x60 deep water, 24x25, 600 s; PROPAGS + IMPLSCH per step (three-kernel IMPLSCH at c2, single kernel "
+                       "at c1)")
+    return res
+
+
 def bench_ensemble(args, torch):
     """The batched ensemble (configuration 3): NMEM members on the 0.1 deg
     450 x 800 sub-box (252 000 sea points, 36 x 36, shallow with currents),
@@ -536,6 +596,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="leave out the batched ensemble measurement")
     ap.add_argument("--ens-members", type=int, default=32)
     ap.add_argument("--ens-steps", type=int, default=4)
+    ap.add_argument("--no-st6", action="store_true", help="leave out the ST6 measurement")
+    ap.add_argument("--st6-steps", type=int, default=3)
     ap.add_argument("--no-nested", action="store_true", help="leave out the nested-grid measurement")
     ap.add_argument("--nest-steps", type=int, default=6)
     ap.add_argument("--no-adjoint", action="store_true", help="leave out the adjoint (gradient) measurement")
@@ -544,7 +606,7 @@ def main():
     ap.add_argument("--adj-snaps", type=int, default=12, help="checkpoint slots of the binomial schedule")
     ap.add_argument("--adj-members", type=int, default=16)
     ap.add_argument("--adj-member-steps", type=int, default=24)
-    ap.add_argument("--only", choices=("ensemble", "nested", "adjoint"),
+    ap.add_argument("--only", choices=("ensemble", "nested", "adjoint", "st6"),
                     help="run one of the configurations alone and print its object as the JSON line (profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one plain 1 GiB device copy before the timed steps (reference for HBM counters)")
@@ -555,7 +617,7 @@ def main():
 
     import torch
     if args.only:
-        fn = {"ensemble": bench_ensemble, "nested": bench_nested, "adjoint": bench_adjoint}[args.only]
+        fn = {"ensemble": bench_ensemble, "nested": bench_nested, "adjoint": bench_adjoint, "st6": bench_st6}[args.only]
         print(json.dumps(fn(args, torch)), flush=True)
         return
     from wam_amd import WamGpu, workloads as W
@@ -702,6 +764,9 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_ensemble:
         out["batched"] = bench_ensemble(args, torch)
+    if not args.no_st6:
+        torch.cuda.empty_cache()
+        out["st6"] = bench_st6(args, torch)
     if not args.no_nested:
         torch.cuda.empty_cache()
         out["nested"] = bench_nested(args, torch)

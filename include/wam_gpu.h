@@ -105,6 +105,16 @@ int wam_gpu_prepare_propagation_dev(wam_ctx *ctx, const float *d_depth, const fl
    next wam_gpu_prepare_propagation. */
 int wam_gpu_set_propagation_weights(wam_ctx *ctx, int32_t stored);
 
+/* IMPLSCH's form on the device (no counterpart in the reference): 1 = three
+   kernels (the per-lane stages up to the dissipation, SNONLIN with one
+   workgroup per sea point, the per-lane new mean parameters and tail), 0 =
+   one kernel with one workgroup per sea point, -1 = automatic (the default:
+   three kernels for IPHYS = 0, and for ST6 from 8192 sea points on).  Where
+   the three-kernel form does not apply (KL not a multiple of 4, more than
+   1536 bins) the single kernel runs whatever the setting.  Results are
+   bit-identical in both forms.  Takes effect at the next IMPLSCH call. */
+int wam_gpu_set_implsch_form(wam_ctx *ctx, int32_t form);
+
 /* SET_MODEL_OPTION's WAVE_BREAKING and PHILLIPS switches
    (wam_timopt_module.f90:176-177, the module variables WAVE_BREAKING_RUN and
    PHILLIPS_RUN read by IMPLSCH, wam_source_module.f90:513-515,639-646): wave

@@ -84,15 +84,16 @@ def test_windows_reproduce_the_full_grid():
 
 
 @pytest.mark.gpu
-def test_full_size_step_on_windows():
+@pytest.mark.parametrize("iphys", [0, 2])          # Janssen; ST6 (both in IMPLSCH's three-kernel form here)
+def test_full_size_step_on_windows(iphys):
     import torch
     from wam_amd import WamGpu
-    cfg, mask, f = W.c2_case()
+    cfg, mask, f = W.c2_case(iphys=iphys)
     g = WamGpu(cfg, mask)
     n = g.nsea
     assert n == 1009120
     counter, _ = g.prepare_propagation(f["depth"], f["u"], f["v"])
-    print(f"[full] NSEA={n} COUNTER={counter}", flush=True)
+    print(f"[full] IPHYS={iphys} NSEA={n} COUNTER={counter}", flush=True)
     assert counter == 2
     dev = torch.device("cuda:0")
     t = lambda x: torch.from_numpy(np.ascontiguousarray(x, np.float32)).to(dev)

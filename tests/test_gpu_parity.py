@@ -136,6 +136,30 @@ def test_implsch_one_step(case):
         assert bits_equal(rg[k], ro[k]), (k, maxrel(rg[k], ro[k]))
 
 
+@pytest.mark.parametrize("three", [False, True])
+def test_implsch_forms(case, three):
+    """Both IMPLSCH forms (three kernels; one kernel with one workgroup per
+    sea point), forced whatever the automatic choice, over three steps with
+    LCFLX on the first: each bit-identical to the oracle.  Where the three-
+    kernel form does not apply (30 directions) the single kernel runs."""
+    o, f, cfg = case["o"], case["f"], case["cfg"]
+    g = WamGpu(cfg, case["mask"])
+    g.set_implsch_form(three)
+    g.prepare_propagation(f["depth"], f["u"], f["v"])
+    n = g.nsea
+    fo = fg = o.propags(case["fl3"])
+    zo = zg = two = twg = np.zeros(n, np.float32)
+    for step in range(3):
+        lc = step == 0
+        ro = o.implsch(fo, f["u10"], f["udir"], two, zo, f["depth"], lcflx=lc)
+        rg = g.implsch(fg, f["u10"], f["udir"], twg, zg, lcflx=lc)
+        for k in ("fl3", "ustar", "tauw", "z0", "phiaw") + (("flux",) if lc else ()):
+            assert bits_equal(rg[k], ro[k]), (step, k, maxrel(rg[k], ro[k]))
+        fo, two, zo = ro["fl3"], ro["tauw"], ro["z0"]
+        fg, twg, zg = rg["fl3"], rg["tauw"], rg["z0"]
+    g.close()
+
+
 def test_output_parameters_bit_exact(case):
     """The integrated output parameters on the GPU (host and device-resident
     forms) against the oracle, both direction conventions."""

@@ -13,12 +13,14 @@ from test_ref_golden import FIXTURES, OUT_FLUX, PROP_FIXTURES, check, check_prop
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("form", ["auto", "three_kernels"])
 @pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(p)[12:-4] for p in FIXTURES])
-def test_gpu_matches_reference_implsch(path):
+def test_gpu_matches_reference_implsch(path, form):
     from wam_amd import WamGpu
     z, cfg = load(path)
-    print(f"[case] {os.path.basename(path)}", flush=True)
+    print(f"[case] {os.path.basename(path)} {form}", flush=True)
     g = WamGpu(cfg, z["mask"])
+    g.set_implsch_form(True if form == "three_kernels" else None)
     n = z["in_u10"].size
     cur = np.zeros(n, np.float32) if cfg["refr_c"] else None      # IMPLSCH does not read currents
     g.prepare_propagation(z["in_depth"], cur, cur)

@@ -46,15 +46,17 @@ CASES = {
     "split_shallow_36x36": lambda: shallow_case(kl=36, ml=36, seed=13),
     "single_kernel_kl30": lambda: shallow_case(kl=30, ml=25, seed=14),
     "st6_shallow": lambda: shallow_case(iphys=2, seed=15),
+    "st6_shallow_three_kernels": lambda: shallow_case(iphys=2, seed=17),
     "split_deep_calm": calm_deep_case,
 }
 OPTIONS = {"breaking": (1, 0), "phillips": (0, 1), "both": (1, 1)}
 
 
-def run_pair(cfg, mask, f, brk, phil, nstep, scale=1.0, lcflx_first=True):
+def run_pair(cfg, mask, f, brk, phil, nstep, scale=1.0, lcflx_first=True, three=None):
     cfg = dict(cfg, wave_breaking=brk, phillips=phil)
     o = Oracle(cfg, mask)
     g = WamGpu(cfg, mask)
+    g.set_implsch_form(three)
     u = f["u"] if cfg["refr_c"] else None
     v = f["v"] if cfg["refr_c"] else None
     o.prepare_propagation(f["depth"], u, v)
@@ -83,7 +85,8 @@ def test_sources_bit_exact(name, opt):
     brk, phil = OPTIONS[opt]
     scale = 1e-4 if name == "split_deep_calm" else 1.0
     print(f"[case] {name} {opt} iphys={cfg['iphys']} shallow={cfg['shallow']} {cfg['kl']}x{cfg['ml']}", flush=True)
-    fo, fg = run_pair(cfg, mask, f, brk, phil, nstep=3, scale=scale)
+    three = True if name.endswith("three_kernels") else None
+    fo, fg = run_pair(cfg, mask, f, brk, phil, nstep=3, scale=scale, three=three)
     # the sources change the result (a switch that did nothing would pass the comparison)
     plain, _ = run_pair(cfg, mask, f, 0, 0, nstep=3, scale=scale, lcflx_first=False)
     if cfg["shallow"] or phil:

@@ -60,6 +60,7 @@ SIGNATURES = {
     "wam_gpu_multigrid_free": (None, [_P]),
     "wam_gpu_set_propagation_weights": (_I, [_P, _I]),
     "wam_gpu_set_source_options": (_I, [_P, _I, _I]),
+    "wam_gpu_set_implsch_form": (_I, [_P, _I]),
     "wam_gpu_get_source_options": (_I, [_P, _P, _P]),
     "wam_gpu_set_stream": (_I, [_P, _P]),
     "wam_gpu_get_stream": (_I, [_P, _P]),

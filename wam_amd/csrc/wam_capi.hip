@@ -110,6 +110,7 @@ struct wam_ctx {
     float *coef = nullptr;          // stored weight planes (NULL: computed in the stencil)
     int ncoef = 0;
     int weights_mode = -1;          // wam_gpu_set_propagation_weights: -1 auto, 0 computed, 1 stored
+    int implsch_form = -1;          // wam_gpu_set_implsch_form: -1 auto, 0 single kernel, 1 three kernels
     int counter = 1;
     float cfl_max = 0.f, cfl_min = 0.f;
     float *wn = nullptr, *cgg = nullptr;
@@ -227,6 +228,17 @@ static_assert(SN_GROUP == SN_GROUP_DEV && SN_NONE == SN_NONE_DEV, "host and devi
 static_assert(SP_QPT == SP_QPT_DEV && SP_NONE == SP_NONE_DEV && SP_TSTRIDE == SP_TSTRIDE_DEV && SP_GROUP == SP_GROUP_DEV,
               "host and device SNONLIN products layouts differ");
 
+// IMPLSCH's form: three kernels (per-lane stages, SNONLIN per workgroup)
+// unless the single kernel is asked for; automatic: always for IPHYS = 0,
+// for ST6 from 8192 sea points on (its per-lane LFACTOR chain is long, and
+// below that the single kernel's one workgroup per point keeps the GPU
+// busier).
+static int implsch_form_split(int form, int iphys, int nsea)
+{
+    if (form >= 0) return form;
+    return iphys == 0 || nsea >= 8192 ? 1 : 0;
+}
+
 static int upload_static(wam_ctx *ctx)
 {
     Tables &t = ctx->tab;
@@ -235,6 +247,7 @@ static int upload_static(wam_ctx *ctx)
     d.MFRSTLW = t.MFRSTLW; d.MLSTHG = t.MLSTHG; d.NQ = t.NQ; d.NCMAX = t.NCMAX; d.NK10 = t.NK10;
     d.CHUNK = t.CHUNK; d.NCHUNK = t.NCHUNK;
     d.iphys = t.cfg.iphys; d.shallow = t.cfg.shallow; d.refr_d = t.cfg.refr_d; d.refr_c = t.cfg.refr_c;
+    d.split = implsch_form_split(ctx->implsch_form, t.cfg.iphys, t.NSEA);
     const Consts &k = t.k;
     d.G = k.G; d.ZPI = k.ZPI; d.ROAIR = k.ROAIR; d.RNUAIRM = k.RNUAIRM; d.ROWATER = k.ROWATER;
     d.XKAPPA = k.XKAPPA; d.XNLEV = k.XNLEV; d.BETAMAX = k.BETAMAX; d.ZALP = k.ZALP; d.ALPHA = k.ALPHA;
@@ -757,7 +770,7 @@ int wam_gpu_propags_dev(wam_ctx *ctx, const float *d_in, float *d_out)
 static void free_impl_scratch(ImplScratch &x)
 {
     for (void *p : {(void *)x.ft, (void *)x.slt, (void *)x.flt, (void *)x.smt, (void *)x.axy, (void *)x.pp,
-                    (void *)x.sprd, (void *)x.llws, (void *)x.tick})
+                    (void *)x.sprd, (void *)x.llws, (void *)x.tick, (void *)x.st6})
         if (p) (void)hipFree(p);
     x = ImplScratch{};
 }
@@ -782,8 +795,10 @@ static int ensure_impl_scratch(wam_ctx *ctx, ImplScratch &x, size_t cap, int lcf
             {(void **)&n.ft, tiled * sizeof(float)}, {(void **)&n.slt, tiled * sizeof(float)},
             {(void **)&n.flt, tiled * sizeof(float)}, {(void **)&n.axy, 3 * (size_t)t.ML * cap * sizeof(float)},
             {(void **)&n.pp, 8 * cap * sizeof(float)}, {(void **)&n.sprd, (size_t)t.KL * cap * sizeof(float)},
-            {(void **)&n.llws, nw * cap * sizeof(unsigned)}, {(void **)&n.tick, 8 * 32 * sizeof(unsigned)}};
+            {(void **)&n.llws, nw * cap * sizeof(unsigned)}, {(void **)&n.tick, 8 * 32 * sizeof(unsigned)},
+            {(void **)&n.st6, t.cfg.iphys == 2 ? 6 * (size_t)t.ML * cap * sizeof(float) : 0}};
         for (const auto &r : req) {
+            if (!r.bytes) continue;
             const hipError_t e = hipMalloc(r.p, r.bytes);
             if (e != hipSuccess) {
                 *r.p = nullptr;
@@ -845,7 +860,7 @@ static int ensure_adj_scratch(wam_ctx *ctx, size_t cap)
 static size_t impl_scratch_per_point(const Tables &t, int lcflx)
 {
     const size_t nb = (size_t)t.KL * t.ML, nw = (nb + 31) / 32;
-    return ((3 + (lcflx ? 1 : 0)) * nb + 3 * (size_t)t.ML + 8 + t.KL + nw) * 4;
+    return ((3 + (lcflx ? 1 : 0)) * nb + 3 * (size_t)t.ML + 8 + t.KL + nw + (t.cfg.iphys == 2 ? 6 * (size_t)t.ML : 0)) * 4;
 }
 
 // The member-points per IMPLSCH launch sequence of an ensemble: the tuned
@@ -1031,7 +1046,7 @@ int wam_gpu_ensemble_run_dev(wam_ctx *ctx, int32_t nmem, int32_t nsteps, float *
         (uintptr_t)d_tauw, (uintptr_t)d_ustar, (uintptr_t)d_z0, (uintptr_t)d_roairn, (uintptr_t)d_wstar,
         (uintptr_t)ctx->tune.prop_wg, (uintptr_t)ctx->tune.src_wg, (uintptr_t)ctx->tune.snl_per_cu,
         (uintptr_t)ctx->esx.NX, (uintptr_t)ctx->counter, (uintptr_t)ctx->dt.DEPTH, (uintptr_t)ctx->coef,
-        (uintptr_t)(ctx->dt.brk | ctx->dt.phil << 1)};
+        (uintptr_t)(ctx->dt.brk | ctx->dt.phil << 1 | ctx->dt.split << 2)};
     if (!ctx->gstream) {
         HIPCHK(hipStreamCreateWithFlags(&ctx->gstream, hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&ctx->gev[0], hipEventDisableTiming));
@@ -1414,7 +1429,7 @@ int wam_gpu_multigrid_run(wam_multigrid *mg, int32_t nsteps, int32_t fused)
         int opts = 0;
         for (int i = 0; i < n; i++) {
             const DevTables &d = mg->grids[i].st.ctx->dt;
-            opts = opts * 4 + (d.brk | d.phil << 1);
+            opts = opts * 8 + (d.brk | d.phil << 1 | d.split << 2);
         }
         if (!mg->gexec || mg->gsteps != nsteps || mg->gparity != mg->parity || mg->gopts != opts) {
             if (mg->gexec) { (void)hipGraphExecDestroy(mg->gexec); mg->gexec = nullptr; }
@@ -1516,8 +1531,8 @@ int64_t wam_gpu_get_tuning_log(wam_ctx *ctx, char *buf, int64_t n)
 static int noalloc_layout(const Tables &t, void *base, ImplScratch &x)
 {
     const size_t nb = (size_t)t.KL * t.ML, nw = (nb + 31) / 32;
-    const size_t per_pt = (4 * nb + 3 * (size_t)t.ML + 8 + t.KL + nw) * 4;
-    const size_t fixed = 1024 + 10 * 256;                       // tickets, alignment
+    const size_t per_pt = (4 * nb + 3 * (size_t)t.ML + 8 + t.KL + nw + (t.cfg.iphys == 2 ? 6 * (size_t)t.ML : 0)) * 4;
+    const size_t fixed = 1024 + 11 * 256;                       // tickets, alignment
     if (NOALLOC_SCRATCH_BYTES < fixed + 64 * per_pt) return 0;
     const size_t c = (NOALLOC_SCRATCH_BYTES - fixed) / per_pt / 64 * 64;
     unsigned char *p = (unsigned char *)base;
@@ -1533,6 +1548,7 @@ static int noalloc_layout(const Tables &t, void *base, ImplScratch &x)
     x.sprd = (float *)take((size_t)t.KL * c * 4);
     x.llws = (unsigned *)take(nw * c * 4);
     x.tick = (unsigned *)take(8 * 32 * 4);
+    if (t.cfg.iphys == 2) x.st6 = (float *)take(6 * (size_t)t.ML * c * 4);
     if ((size_t)(p - (unsigned char *)base) > NOALLOC_SCRATCH_BYTES) return 0;
     return (int)c;
 }
@@ -1892,6 +1908,15 @@ int wam_gpu_set_propagation_weights(wam_ctx *ctx, int32_t stored)
     if (stored < -1 || stored > 1) { ctx->err = "stored must be -1, 0 or 1"; return WAM_E_ARG; }
     ctx->weights_mode = stored;
     ctx->prepared = false;      // takes effect at the next wam_gpu_prepare_propagation
+    return WAM_OK;
+}
+
+int wam_gpu_set_implsch_form(wam_ctx *ctx, int32_t form)
+{
+    if (!ctx) return WAM_E_ARG;
+    if (form < -1 || form > 1) { ctx->err = "form must be -1, 0 or 1"; return WAM_E_ARG; }
+    ctx->implsch_form = form;
+    ctx->dt.split = implsch_form_split(form, ctx->tab.cfg.iphys, ctx->tab.NSEA);
     return WAM_OK;
 }
 

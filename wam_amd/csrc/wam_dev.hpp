@@ -25,6 +25,7 @@ struct DevTables {
     int KL, ML, NSEA, NDEPTH, JUMAX, MFRSTLW, MLSTHG, NQ, NCMAX, NK10;
     int iphys, shallow, refr_d, refr_c;
     int brk, phil;                        // WAVE_BREAKING_RUN (SDEPTHLIM, SFBRK), PHILLIPS_RUN (SOURCE_PHILLIPS)
+    int split;                            // IMPLSCH in three kernels (1) or the single kernel (0), where both apply
     float G, ZPI, ROAIR, RNUAIRM, ROWATER, XKAPPA, XNLEV, BETAMAX, ZALP, ALPHA, CDFAC, EMIN, EPS1, CO;
     float DELTH, INV_LOG_CO, MO_TAIL, MM1_TAIL, MP1_TAIL, MP2_TAIL, DELU, DAL1, DAL2, X0TAUHF, DELT;
     float DEG;                            // 180/PI
@@ -136,6 +137,7 @@ struct ImplScratch {
     float *sprd;                   // [KL][NX]: SPRD(K) = MAX(0, COS(TH(K) - UDIR))**2 of the call
     unsigned *llws;                // [(KL*ML+31)/32][NX]: LLWS bits of the second SINPUT
     unsigned *tick;                // [8][32]: point counters of implsch_snl_kernel, one per XCD (own 128-byte line)
+    float *st6;                    // IPHYS = 2: [6][ML][NX] per-frequency values of implsch_pre_st6_kernel
 };
 
 // Per-point scalars the adjoint's forward recomputation keeps (rows of

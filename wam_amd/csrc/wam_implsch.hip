@@ -753,6 +753,418 @@ __global__ void __launch_bounds__(PW) __attribute__((amdgpu_waves_per_eu(4))) im
     if (blockIdx.x == 0 && lane < 8) x.tick[32 * lane] = 0u;
 }
 
+// IMPLSCH up to the dissipation for IPHYS = 2 (ST6), one lane per sea point,
+// feeding the same SNONLIN and post kernels as IPHYS = 0.  The stages and
+// their operation order are the single kernel's (implsch_kernel<2, *>):
+// W3FLX4, the mean parameters, SINPUT_ST6 with LFACTOR and TAUWINDS
+// (:4704-4925, :4465-4700, :5555-5587), FRCUTINDEX_ST6, STRESSO (PHIAW only),
+// IMPHFTAIL, SDISSIP_ST6 and W3SWL6 (:4929-5252).  FRCUTINDEX_ST6 needs FMEAN
+// and USTAR only, so MIJ and the tail are known before SINPUT_ST6's last pass,
+// and W3FLX4 depends on U10 alone (its second call gives the same USTAR, Z0).
+// Three passes over the spectrum:
+//   A: the mean-parameter integrals and, per frequency, SINPUT_ST6's
+//      saturation factor SQRT(BN), C^-1 and the LFACTOR stress components;
+//   C: SDISSIP_ST6's and W3SWL6's per-frequency factors on the spectrum after
+//      IMPHFTAIL (rows from MIJ up replaced by the tail);
+//   B: SINPUT_ST6 per bin with the stress factor (its tanh terms evaluated
+//      again, the same values as in pass A), the dissipation, STRESSO's
+//      direction sums, the LLWS-masked integrals, the tiled outputs.
+// LFACTOR's iteration runs per lane between A and C.  x.st6 holds the per-
+// frequency values: SQRT(BN), C^-1, the two stress components, T12 and
+// SQRT(BNM) ([6][ML][NX]).  BRK = 1 applies SDEPTHLIM as in implsch_pre_kernel.
+template <int SHALLOW, int BRK = 0>
+__global__ void __launch_bounds__(PW) __attribute__((amdgpu_waves_per_eu(4)))
+implsch_pre_st6_kernel(DevTables t, ImplArgs a, ImplScratch x)
+{
+    extern __shared__ float cu_lds[];            // [KL][W]: COS(TH(K) - UDIR) of each lane's point
+    const int KL = t.KL, ML = t.ML, NB = KL * ML, NS = t.NSEA, NB4 = x.NB4, XS = x.NX, lane = threadIdx.x;
+    const int W = blockDim.x;
+    const int lp0 = blockIdx.x * W + lane;
+    const bool on = lp0 < x.NP;
+    const int lp = on ? lp0 : x.NP - 1, mp = x.P0 + lp;      // member-point
+    const int mem = mp / NS, pt = mp - mem * NS;            // ensemble member, sea point
+    float *cu = cu_lds + lane;
+    const float *F = a.fl3 + (size_t)mem * NB * NS + pt;   // F(b) = F[b * NS]
+    const float u10 = a.u10[mp], udir = a.udir[mp], roairn = a.roairn[mp];
+    const int indep = SHALLOW ? t.INDEP[pt] : 1;
+    const float depth = t.DEPTH[pt];
+    float *SQB = x.st6 + lp, *CIV = SQB + (size_t)ML * XS, *SXM = CIV + (size_t)ML * XS, *SYM = SXM + (size_t)ML * XS;
+    float *T12 = SYM + (size_t)ML * XS, *SBN = T12 + (size_t)ML * XS;    // row m at [m * XS]
+    for (int k = 0; k < KL; k++) {
+        const float c = wam_cosf(t.TH[k] - udir);
+        cu[k * W] = c;
+        if (on) x.sprd[(size_t)k * XS + lp] = wam_ipowf(wam_maxf(0.f, c), 2);
+    }
+    float ust, z0;
+    w3flx4(t, u10, ust, z0);
+    const float cosu = wam_cosf(udir), sinu = wam_sinf(udir);
+    const float uproxy = 32.f * ust, rwr = roairn / t.ROWATER;
+    const float ZPI = t.ZPI;
+    const float d1 = sqrtf(t.G) / ZPI, d2 = ZPI / sqrtf(t.G);
+    // SDEPTHLIM (:513-515): TOTAL_ENERGY of the spectrum as given first
+    float em = 1.f, emean_lim = 0.f;
+    if (BRK) {
+        float se = 0.f, Tl = 0.f;
+        for (int i = 0; i < ML; i++) {
+            const int m = i == 0 ? ML - 1 : i - 1;
+            float T = 0.f;
+            for (int k = 0; k < KL; k++) {
+                const float f = F[(size_t)(k + KL * m) * NS];
+                T = k == 0 ? f : T + f;
+            }
+            if (i == 0) { Tl = T; se = t.MO_TAIL * T; }
+            else se = se + T * t.DFIM[m];
+        }
+        se = se + Tl * t.DFIM[ML - 1];
+        const float e0 = wam_maxf(se, t.EMIN);
+        em = sdepthlim_em(depth, e0);
+        emean_lim = em * e0;
+    }
+    auto fread = [&](int b) { return BRK ? F[(size_t)b * NS] * em : F[(size_t)b * NS]; };
+    // SINPUT_ST6's input term D of one bin (and its W2), as the single kernel forms it
+    auto dterm = [&](int k, float sg, float ci, float sqb, float &w2) {
+        const float cosd = t.COSTH[k] * cosu + t.SINTH[k] * sinu;
+        const float xv = ((uproxy * ci) * cosd) - 1.f;
+        const float w1 = wam_ipowf(wam_maxf(0.f, xv), 2);
+        w2 = wam_ipowf(wam_minf(0.f, xv), 2);
+        return (((rwr * sg) * (2.8f - (1.f + wam_tanhf((10.f * sqb) * w1 - 11.f)))) * sqb) * w1;
+    };
+
+    // pass A
+    float s_e = 0.f, s_fm = 0.f, s_tm = 0.f, s_w1 = 0.f, s_w2 = 0.f, Tl = 0.f;
+    for (int i = 0; i < ML; i++) {
+        const int m = i == 0 ? ML - 1 : i - 1;
+        const float wnm = t.WN[(size_t)m * NS + pt], cgm = t.CGG[(size_t)m * NS + pt];
+        const float sg = ZPI * t.FR[m], c0 = ZPI * sg;
+        float T = 0.f, sm = 0.f, mx = 0.f;
+        for (int k = 0; k < KL; k++) {
+            const float f = fread(k + KL * m);
+            T = k == 0 ? f : T + f;
+            const float av = (f * cgm) / c0;
+            sm = k == 0 ? av : sm + av;
+            mx = k == 0 ? av : wam_maxf(mx, av);
+        }
+        float sn = 0.f;
+        for (int k = 0; k < KL; k++) {
+            const float av = (fread(k + KL * m) * cgm) / c0;
+            const float v = (mx < 1.0E-34f) ? 1.f : av / mx;
+            sn = k == 0 ? v : sn + v;
+        }
+        const float anar = 1.0f / (sn * t.DELTH);
+        const float adens = (sm * sg) * t.DELTH;
+        const float sqb = sqrtf((anar * adens) * wam_ipowf(wnm, 3));
+        const float ci = wnm / sg;
+        float bx = 0.f, by = 0.f;
+        for (int k = 0; k < KL; k++) {
+            float w2;
+            const float d = dterm(k, sg, ci, sqb, w2);
+            const float av = (fread(k + KL * m) * cgm) / c0;
+            const float sds = ((d * av) * sg) / cgm;
+            const float vp = wam_maxf(0.f, sds);
+            if (k == 0) { bx = vp * t.COSTH[k]; by = vp * t.SINTH[k]; }
+            else { bx = bx + vp * t.COSTH[k]; by = by + vp * t.SINTH[k]; }
+        }
+        if (on) {
+            SQB[(size_t)m * XS] = sqb;
+            CIV[(size_t)m * XS] = ci;
+            SXM[(size_t)m * XS] = bx * t.DELTH;
+            SYM[(size_t)m * XS] = by * t.DELTH;
+        }
+        if (i == 0) {
+            Tl = T;
+            s_e = t.MO_TAIL * T; s_fm = t.MM1_TAIL * T; s_tm = t.MP1_TAIL * T;
+            if (SHALLOW) { s_w1 = (t.MM1_TAIL * d1) * T; s_w2 = (t.MP1_TAIL * d2) * T; }
+        } else {
+            s_e = s_e + T * t.DFIM[m]; s_fm = s_fm + T * t.DFIMOFR[m]; s_tm = s_tm + T * t.DFIM_FR[m];
+            if (SHALLOW) {
+                const float sk = sqrtf(t.TFAK[tb(t, m, indep)]);
+                s_w1 = s_w1 + (T / sk) * t.DFIM[m];
+                s_w2 = s_w2 + (T * sk) * t.DFIM[m];
+            }
+        }
+    }
+    {
+        const int m = ML - 1;
+        s_e = s_e + Tl * t.DFIM[m]; s_fm = s_fm + Tl * t.DFIMOFR[m]; s_tm = s_tm + Tl * t.DFIM_FR[m];
+        if (SHALLOW) {
+            const float sk = sqrtf(t.TFAK[tb(t, m, indep)]);
+            s_w1 = s_w1 + (Tl / sk) * t.DFIM[m];
+            s_w2 = s_w2 + (Tl * sk) * t.DFIM[m];
+        }
+    }
+    const float emean = BRK ? emean_lim : wam_maxf(s_e, t.EMIN);
+    const float fmean = emean / wam_maxf(s_fm, t.EMIN);
+    float akmean = 1.f;
+    if (SHALLOW) akmean = emean > t.EMIN ? wam_ipowf(emean / s_w1, 2) : 1.f;
+    const int mij = frcut(t, t.FR, fmean, fmean, ust);      // FRCUTINDEX_ST6: FMEAN and USTAR
+
+    // LFACTOR (:4465-4700) with TAUWINDS: the stress supported by the waves
+    // over the frequencies and their 10-Hz extension, NK10 terms in order
+    const int NK = ML, NK10 = t.NK10;
+    const float GRW = t.G * t.ROWATER;
+    const float sxl = SXM[(size_t)(NK - 1) * XS], syl = SYM[(size_t)(NK - 1) * XS];
+    auto term = [&](int i, bool with_lf, float rtau, float &tx, float &ty) {
+        float ci, sx, sy;
+        if (i < NK) {
+            ci = CIV[(size_t)i * XS]; sx = SXM[(size_t)i * XS]; sy = SYM[(size_t)i * XS];
+        } else {
+            ci = t.SIG10[i] * 0.101978f;
+            const float r2 = wam_ipowf(t.SIG10[NK - 1] / t.SIG10[i], 2);
+            sx = sxl * r2; sy = syl * r2;
+        }
+        const float ds = t.DSII10[i];
+        if (with_lf) {
+            const float lf = wam_minf(1.0f, wam_expf((1.0f - uproxy * ci) * rtau));
+            tx = ((sx * lf) * ci) * ds;
+            ty = ((sy * lf) * ci) * ds;
+        } else {
+            tx = (sx * ci) * ds;
+            ty = (sy * ci) * ds;
+        }
+    };
+    auto stress = [&](bool with_lf, float rtau, float &twx, float &twy) {
+        term(0, with_lf, rtau, twx, twy);
+        for (int i = 1; i < NK10; i++) {
+            float tx, ty;
+            term(i, with_lf, rtau, tx, ty);
+            twx = twx + tx;
+            twy = twy + ty;
+        }
+        twx = GRW * twx;
+        twy = GRW * twy;
+    };
+    const float tau_tot = (ust * ust) * roairn;
+    float tau_vis = (wam_maxf(0.0f, -5.0E-5f * u10 + 1.1E-3f) * (u10 * u10)) * roairn;
+    tau_vis = wam_minf(0.95f * tau_tot, tau_vis);
+    const float tauvx = tau_vis * cosu, tauvy = tau_vis * sinu;
+    float twx, twy;
+    stress(false, 0.f, twx, twy);
+    float taux = tauvx + twx, tauy = tauvy + twy;
+    float tau = sqrtf(taux * taux + tauy * tauy);
+    float err = (tau - tau_tot) / tau_tot;
+    bool iterated = false;
+    float rtau_lf = 0.f;                   // the RTAU of the last stress factors evaluated
+    if (tau > tau_tot) {
+        bool overshot = false;
+        float rtau = err / 90.f, drtau = 2.0f;
+        int sign_new = err >= 0.f ? 1 : -1;
+        for (int it = 1; it <= 80; it++) {
+            iterated = true;
+            rtau_lf = rtau;
+            stress(true, rtau, twx, twy);
+            taux = tauvx + twx; tauy = tauvy + twy;
+            tau = sqrtf(taux * taux + tauy * tauy);
+            err = (tau - tau_tot) / tau_tot;
+            const int sign_old = sign_new;
+            sign_new = err >= 0.f ? 1 : -1;
+            if (sign_new != sign_old) overshot = true;
+            if (overshot) drtau = wam_maxf(0.5f * (1.0f + drtau), 1.00010f);
+            rtau = sign_new > 0 ? rtau * drtau : rtau * (1.f / drtau);
+            if (wam_fabsf(err) < 1.54E-4f) break;
+        }
+    }
+    auto lfac = [&](float ci) { return iterated ? wam_minf(1.0f, wam_expf((1.0f - uproxy * ci) * rtau_lf)) : 1.0f; };
+    bool apply;
+    {
+        float ls = lfac(CIV[0]);
+        for (int m = 1; m < NK; m++) ls = ls + lfac(CIV[(size_t)m * XS]);
+        apply = ls < (float)NK;
+    }
+    const float tauw = sqrtf(twx * twx + twy * twy);
+
+    // IMPHFTAIL's factors: rows from MIJ up are TTAIL(M) * F(., MIJ - 1)
+    float tden;
+    if (SHALLOW) {
+        const float akm0 = 1.f / t.TFAK[tb(t, mij - 1, indep)];
+        tden = wam_ipowf(akm0, 3) / t.TCGOND[tb(t, mij - 1, indep)];
+    } else {
+        tden = t.FRM5[mij - 1];
+    }
+    auto ttail = [&](int m) {
+        const float an = SHALLOW ? wam_ipowf(1.f / t.TFAK[tb(t, m, indep)], 3) / t.TCGOND[tb(t, m, indep)] : t.FRM5[m];
+        return an / tden;
+    };
+
+    // pass C: SDISSIP_ST6 (:4929-5102) and W3SWL6 (:5106-5252) per frequency
+    uint64_t abpos = 0;
+    float b1;
+    {
+        float run = 0.f, cur = 0.f, smt = 0.f;
+        int ikx = 0;
+        const float xfac = (1.0f - 1.0f / t.CO) / (t.CO - 1.0f / t.CO);
+        const float bnt = 0.035f * 0.035f;
+        for (int m = 0; m < ML; m++) {
+            const bool tl = m >= mij;
+            const float tt = tl ? ttail(m) : 1.f;
+            const int row = tl ? mij - 1 : m;
+            auto fp = [&](int k) { return tl ? tt * fread(k + KL * row) : fread(k + KL * m); };
+            const float wnm = t.WN[(size_t)m * NS + pt], cgm = t.CGG[(size_t)m * NS + pt];
+            float ed = fp(0);
+            for (int k = 1; k < KL; k++) ed = ed + fp(k);
+            const float edens = ed * t.DELTH;
+            const float sigm = ZPI * t.FR[m], c0 = ZPI * sigm;
+            float ab = 0.f, mx = 0.f;
+            for (int k = 0; k < KL; k++) {
+                const float v = (fp(k) * cgm) / c0;
+                ab = k == 0 ? v : ab + v;
+                mx = k == 0 ? v : wam_maxf(mx, v);
+            }
+            float sn = 0.f;
+            for (int k = 0; k < KL; k++) {
+                const float v = (fp(k) * cgm) / c0;
+                const float kn = (mx < 1.0E-34f) ? 1.f : v / mx;
+                sn = k == 0 ? kn : sn + kn;
+            }
+            const float anar = 1.0f / (sn * t.DELTH);
+            const float bnm = (anar * ((ab * sigm) * t.DELTH)) * wam_ipowf(wnm, 3);
+            const float dden = (ZPI * t.DFIM[m]) * sigm;
+            const float t2m = (ab * dden) / cgm;
+            const float et = (ZPI * bnt) / ((1.0f * cgm) * wam_ipowf(wnm, 3));
+            const float ex = wam_maxf(0.0f, edens - et);
+            const float nex = ex / et;
+            const float n4 = wam_ipowf(nex, 4);
+            const float t1 = ((4.75E-6f * 1.0f) * t.FR[m]) * n4;
+            float dfii = t.DF[m];
+            if (m > 0 && m < ML - 1) dfii = dfii * xfac;
+            const float termv = (1.0f * n4) * dfii;
+            run = m == 0 ? termv : run + termv;
+            if (m == 0 || ab > cur) { cur = ab; ikx = m; }
+            smt = m == 0 ? t2m : smt + t2m;
+            if (on) {
+                T12[(size_t)m * XS] = -1.0f * (wam_maxf(0.0f, t1) + wam_maxf(0.0f, 7.00E-5f * run));
+                SBN[(size_t)m * XS] = sqrtf(bnm);
+            }
+            if (ab > 1.E-30f) abpos |= 1ull << m;
+        }
+        b1 = 0.0041f * ((2.f * sqrtf(smt)) * t.WN[(size_t)ikx * NS + pt]);
+    }
+
+    // pass B: SINPUT_ST6 per bin, the dissipation, STRESSO's sums, the
+    // LLWS-masked integrals; F after the tail, SL, FL (SL - SPOS) and LLWS out
+    float q_e = 0.f, q_fm = 0.f, Ql = 0.f;
+    float phi = 0.f, xs = 0.f, ys = 0.f, sal = 0.f, sxl2 = 0.f, syl2 = 0.f;
+    {
+        const size_t q0 = quad_offset(lp, 0, NB4);
+        unsigned *LLW = x.llws + lp;
+        for (int i = 0; i < ML; i++) {
+            const int m = i == 0 ? ML - 1 : i - 1;
+            const bool tl = m >= mij;
+            const float tt = tl ? ttail(m) : 1.f;
+            const float cgm = t.CGG[(size_t)m * NS + pt];
+            const float sg = ZPI * t.FR[m];
+            const float ci = CIV[(size_t)m * XS], sqb = SQB[(size_t)m * XS];
+            const float lf = lfac(ci);
+            const float t12 = T12[(size_t)m * XS];
+            const float ddis = ((abpos >> m) & 1ull) ? (((-(2.f / 3.f)) * b1) * sg) * SBN[(size_t)m * XS] : 0.f;
+            float Q = 0.f, sa = 0.f, sx = 0.f, sy = 0.f;
+            bool qfirst = true;
+            unsigned word = 0;
+            for (int k0 = 0; k0 < KL; k0 += 4) {
+                float f4[4], sl4[4], fl4[4], sm4[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int k = k0 + u, b = k + KL * m;
+                    const float f = fread(b);
+                    const float fpv = tl ? tt * fread(k + KL * (mij - 1)) : f;
+                    float w2;
+                    float d = dterm(k, sg, ci, sqb, w2);
+                    if (apply) d = d * lf;
+                    const float dpos = d;
+                    d = d - ((((rwr * sg) * 0.09f) * (2.8f - (1.f + wam_tanhf((10.f * sqb) * w2 - 11.f)))) * sqb) * w2;
+                    const float sl0 = d * f, sp = dpos * f;
+                    const bool ll = sp > 0.f;
+                    sm4[u] = sl0 - sp;
+                    const float sl1 = sl0 + t12 * fpv, fl1 = d + t12;
+                    sl4[u] = sl1 + ddis * fpv;
+                    fl4[u] = fl1 + ddis;
+                    f4[u] = fpv;
+                    word |= (unsigned)ll << (b & 31);
+                    if ((b & 31) == 31 || b == NB - 1) {
+                        if (on) LLW[(size_t)(b >> 5) * XS] = word;
+                        word = 0;
+                    }
+                    if (ll) {
+                        Q = qfirst ? f : Q + f;
+                        qfirst = false;
+                    }
+                    const float spp = wam_maxf(sp, 0.f);
+                    if (k == 0) { sa = spp; sx = spp * t.SINTH[k]; sy = spp * t.COSTH[k]; }
+                    else { sa = sa + spp; sx = sx + spp * t.SINTH[k]; sy = sy + spp * t.COSTH[k]; }
+                }
+                if (on) {
+                    const size_t o = q0 + (size_t)((k0 + KL * m) >> 2) * 256;
+                    *reinterpret_cast<float4 *>(x.ft + o) = make_float4(f4[0], f4[1], f4[2], f4[3]);
+                    *reinterpret_cast<float4 *>(x.slt + o) = make_float4(sl4[0], sl4[1], sl4[2], sl4[3]);
+                    *reinterpret_cast<float4 *>(x.flt + o) = make_float4(fl4[0], fl4[1], fl4[2], fl4[3]);
+                    if (a.lcflx)
+                        *reinterpret_cast<float4 *>(x.smt + o) = make_float4(sm4[0], sm4[1], sm4[2], sm4[3]);
+                }
+            }
+            if (i == 0) {
+                Ql = Q;
+                q_e = t.MO_TAIL * Q; q_fm = t.MM1_TAIL * Q;
+                sal = sa; sxl2 = sx; syl2 = sy;          // frequency ML's STRESSO terms go last
+            } else {
+                q_e = q_e + Q * t.DFIM[m]; q_fm = q_fm + Q * t.DFIMOFR[m];
+                if (m < mij) {
+                    const float rw = stresso_rw(t, m, mij);
+                    const float cmrw = stresso_cm<SHALLOW>(t, m, SHALLOW ? t.TFAK[tb(t, m, indep)] : 0.f) * rw;
+                    phi = phi + sa * rw;
+                    xs = xs + sx * cmrw;
+                    ys = ys + sy * cmrw;
+                }
+            }
+        }
+        const int m = ML - 1;
+        q_e = q_e + Ql * t.DFIM[m]; q_fm = q_fm + Ql * t.DFIMOFR[m];
+        if (m < mij) {
+            const float rw = stresso_rw(t, m, mij);
+            const float cmrw = stresso_cm<SHALLOW>(t, m, SHALLOW ? t.TFAK[tb(t, m, indep)] : 0.f) * rw;
+            phi = phi + sal * rw;
+            xs = xs + sxl2 * cmrw;
+            ys = ys + syl2 * cmrw;
+        }
+    }
+    const float ews = wam_maxf(q_e, t.EMIN);
+    const float fmeanws = ews / wam_maxf(q_fm, t.EMIN);
+    // STRESSO (TAUW not kept for ST6) on the moments of F(., MIJ)
+    float h1 = 0.f, h2 = 0.f;
+    for (int k = 0; k < KL; k++) {
+        const float cosw = wam_maxf(cu[k * W], 0.f);
+        const float f = fread(k + KL * (mij - 1));
+        if (k == 0) { h2 = f * wam_ipowf(cosw, 2); h1 = h2 * cosw; }
+        else { h1 = h1 + f * wam_ipowf(cosw, 3); h2 = h2 + f * wam_ipowf(cosw, 2); }
+    }
+    float phiaw;
+    (void)stresso_close(t, mij, ust, z0, udir, phi, xs, ys, h1, h2, phiaw);
+    if (on) {
+        a.tauw[mp] = tauw;
+        a.ustar[mp] = ust;
+        a.z0[mp] = z0;
+        if (a.phiaw) a.phiaw[mp] = phiaw;
+        float enhfr = 1.f;
+        if (SHALLOW) {
+            const float enh = wam_maxf((0.75f * depth) * akmean, .5f);
+            enhfr = 1.f + ((5.5f / enh) * (1.f - .833f * enh)) * wam_expf(-1.25f * enh);
+        }
+        x.pp[lp] = enhfr;
+        x.pp[(size_t)XS + lp] = wam_maxf(fmeanws, fmean);
+        x.pp[(size_t)2 * XS + lp] = (float)mij;
+        x.pp[(size_t)3 * XS + lp] = phiaw;
+        if (a.lcflx) {
+            x.pp[(size_t)4 * XS + lp] = sinu;
+            x.pp[(size_t)5 * XS + lp] = cosu;
+        }
+        if (BRK) {
+            float sbr, dsbr;
+            sfbrk_rates(emean, fmean, depth, sbr, dsbr);
+            x.pp[(size_t)6 * XS + lp] = sbr;
+            x.pp[(size_t)7 * XS + lp] = dsbr;
+        }
+    }
+    if (blockIdx.x == 0 && lane < 8) x.tick[32 * lane] = 0u;
+}
+
 constexpr int SNL_NT = 512;  // threads of implsch_snl_kernel: one quadruplet of a chunk each
 constexpr int SNL_NJ = 3;    // bins per thread: KL*ML <= SNL_NT * SNL_NJ
 constexpr int SNL_QT = 256 * SP_QPT_DEV / SNL_NT;
@@ -1040,12 +1452,14 @@ size_t implsch_snl_lds_bytes(const DevTables &t, int lcflx, int opt)
     return n * sizeof(float);
 }
 
-// Whether the three-kernel form applies: Janssen physics, whole 4-bin quads
-// per direction row, at most SNL_NJ bins per thread, one chunk's quadruplets
-// of one frequency within the slots.
+// Whether the three-kernel form applies: Janssen or ST6 physics, whole 4-bin
+// quads per direction row, at most SNL_NJ bins per thread, one chunk's
+// quadruplets of one frequency within the slots (ST6: at most 64 frequencies,
+// as the single kernel assumes).
 bool implsch_split_supported(const DevTables &t)
 {
-    return t.iphys == 0 && t.KL % 4 == 0 && t.KL * t.ML <= SNL_NT * SNL_NJ && 2 * t.KL < 256 * SP_QPT_DEV;
+    return t.split && (t.iphys == 0 || (t.iphys == 2 && t.ML <= 64)) && t.KL % 4 == 0 &&
+           t.KL * t.ML <= SNL_NT * SNL_NJ && 2 * t.KL < 256 * SP_QPT_DEV;
 }
 
 // The three kernels of IPHYS = 0 on one stream for the chunk of sea points
@@ -1060,7 +1474,12 @@ hipError_t launch_implsch_split(const DevTables &t, const ImplArgs &a, const Imp
     mark(0);
     const size_t lds1 = (size_t)t.KL * W * sizeof(float);
     const bool opt = t.brk || t.phil;
-    if (t.shallow && t.brk)
+    if (t.iphys == 2) {
+        if (t.shallow && t.brk)
+            hipLaunchKernelGGL((implsch_pre_st6_kernel<1, 1>), dim3(nwg), dim3(W), lds1, st, t, a, x);
+        else if (t.shallow) hipLaunchKernelGGL(implsch_pre_st6_kernel<1>, dim3(nwg), dim3(W), lds1, st, t, a, x);
+        else hipLaunchKernelGGL(implsch_pre_st6_kernel<0>, dim3(nwg), dim3(W), lds1, st, t, a, x);
+    } else if (t.shallow && t.brk)
         hipLaunchKernelGGL((implsch_pre_kernel<1, 0, 1>), dim3(nwg), dim3(W), lds1, st, t, a, x, AdjExtra{});
     else if (t.shallow) hipLaunchKernelGGL(implsch_pre_kernel<1>, dim3(nwg), dim3(W), lds1, st, t, a, x, AdjExtra{});
     else hipLaunchKernelGGL(implsch_pre_kernel<0>, dim3(nwg), dim3(W), lds1, st, t, a, x, AdjExtra{});

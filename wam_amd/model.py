@@ -264,6 +264,12 @@ class WamGpu:
         mode = -1 if stored is None else int(bool(stored))
         self._check(self.lib.wam_gpu_set_propagation_weights(self.ctx, mode))
 
+    def set_implsch_form(self, three_kernels):
+        """IMPLSCH in three kernels (True), one kernel (False) or the automatic
+        choice (None); the results are bit-identical."""
+        form = -1 if three_kernels is None else int(bool(three_kernels))
+        self._check(self.lib.wam_gpu_set_implsch_form(self.ctx, form))
+
     def set_source_options(self, wave_breaking=False, phillips=False):
         """SET_MODEL_OPTION's WAVE_BREAKING (SDEPTHLIM and SFBRK; stays off in
         deep water) and PHILLIPS (SOURCE_PHILLIPS) switches, off by default;
