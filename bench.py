@@ -94,10 +94,16 @@ def cpu_baseline(args):
         r = o.implsch(fl3, f["u10"], f["udir"], tauw, z0, f["depth"], nthreads=threads)
         fl3, tauw, z0 = r["fl3"], r["tauw"], r["z0"]
     dt = (time.perf_counter() - t0) / steps
+    # the reference's own step is R = 1.111 times the oracle's on one core for
+    # this physics and size (a 100 800-point cut, profiles/cpu_reference_calibration.md)
+    r_ref = 1.111
     return dict(value=n / dt, unit="sea-point updates/s", cores=threads, nproc=nproc, cpu_model=model, kind="port",
                 sample=f"oracle REAL*4 on a {args.cpu_ny}x{args.cpu_nx} sub-box of the same workload "
                        f"({n} sea points, 36x36, COUNTER={counter}), {steps} steps, OpenMP over points",
-                seconds_per_step=dt)
+                seconds_per_step=dt,
+                reference_equivalent=dict(value=n / (dt * r_ref), R=r_ref,
+                                          note="oracle time x R, R = t(reference step) / t(oracle step) on one core "
+                                               "of the build container (flang -O2 build of the reference's sources)"))
 
 
 def nest_rings(mask):
