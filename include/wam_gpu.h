@@ -480,7 +480,8 @@ const char *wam_gpu_last_error(wam_ctx *ctx);
    checks of the device build against the host build (no counterpart in the
    reference, which calls its runtime's intrinsics): out[i] = f(x[i], y[i])
    for fn 0 EXP, 1 LOG, 2 LOG10, 3 TANH, 4 COSH, 5 SINH, 6 COS, 7 SIN,
-   8 x**y (real y), 9 NINT, 10 x**INT(y), 11 ATAN2(x, y); host arrays of n
+   8 x**y (real y), 9 NINT, 10 x**INT(y), 11 ATAN2(x, y), 12 x/1.1 (the FMA-corrected
+   quotient PROPAGS uses for its SIG1 weight); host arrays of n
    (y may be NULL for one-argument functions), device 0, synchronous. */
 int wam_gpu_eval_math(int32_t fn, const float *x, const float *y, float *out, int64_t n);
 

@@ -136,6 +136,7 @@ void orc_api_math(int fn, const float *x, const float *y, float *out, long n)
         case 9: r = (float)wam_nint(a); break;
         case 10: r = wam_ipowf(a, (int)b); break;
         case 11: r = wam_atan2f(a, b); break;
+        case 12: r = wam_div_1p1(a); break;
         }
         out[i] = r;
     }

@@ -63,6 +63,34 @@ def test_nonfinite_fields_refused(path):
     assert np.array_equal(bits(rg["fl3"]), bits(ro["fl3"]))
 
 
+@pytest.mark.parametrize("stored", [False, True])
+def test_nan_weight_refused(stored):
+    """Finite but extreme fields whose propagation weights are NaN at one sea
+    point: a depth step from 1 m to 3e38 m on both of its axes and a current
+    of 1e5 m/s, so that TEMPD = V*DDPHI + U*DDLAM is +Inf - Inf there.  The
+    oracle's weights there are NaN (no Inf); CHECK_CFL refuses them, in both
+    weight modes, before its CFL bounds."""
+    cfg, mask, f = case()
+    o = Oracle(cfg, mask)
+    n = o.nsea
+    klat, klon = o.get("KLAT"), o.get("KLON")
+    s_, n_, w_, e_ = klat[:n] - 1, klat[n:2 * n] - 1, klon[:n] - 1, klon[n:2 * n] - 1
+    inner = np.flatnonzero((s_ >= 0) & (n_ >= 0) & (w_ >= 0) & (e_ >= 0))
+    p = int(inner[inner.size // 2])
+    d, u, v = f["depth"].copy(), f["u"].copy(), f["v"].copy()
+    d[[p, n_[p], e_[p]]] = 3e38
+    d[[s_[p], w_[p]]] = 1.0
+    u[p], v[p] = -1e5, 1e5
+    o.prepare_propagation(d, u, v)
+    w = o.get("COEF").reshape(-1, n)[:, p]
+    assert np.isnan(w).any() and not np.isinf(w).any()
+    g = WamGpu(cfg, mask)
+    g.set_propagation_weights(stored)
+    with pytest.raises(WamError, match="not a number") as e:
+        g.prepare_propagation(d, u, v)
+    assert e.value.code == WAM_E_CFL, e.value
+
+
 @pytest.mark.parametrize("kl", [24, 30])          # the three-kernel IMPLSCH and the single kernel
 def test_nan_wind_stays_at_its_point(kl):
     cfg, mask, f = case(kl=kl, seed=32)

@@ -47,7 +47,7 @@ def same(a, b, x=None, y=None):
 
 @pytest.mark.parametrize("fn,lo,hi", [(0, 0.0, 104.0), (1, 1e-45, 3.4e38), (2, 1e-45, 3.4e38), (3, 0.0, 10.0),
                                       (4, 0.0, 90.0), (5, 0.0, 90.0), (6, 0.0, 65536.0), (7, 0.0, 65536.0),
-                                      (9, 0.0, 1e6)])
+                                      (9, 0.0, 1e6), (12, 1e-45, 3.4e38)])
 def test_device_math_equals_host(fn, lo, hi):
     x = floats(lo, hi, 257, signed=fn not in (1, 2))
     print(f"[case] fn={fn} n={x.size}", flush=True)

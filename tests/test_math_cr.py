@@ -35,3 +35,14 @@ def test_correctly_rounded(sweep, fn):
     assert r["ne_correctly_rounded"] == 0, r
     # every disagreement with glibc is one where glibc is not correctly rounded
     assert r["ne_glibc"] == r["glibc_ne_correctly_rounded"], r
+
+
+def test_division_by_1p1_is_ieee(tmp_path):
+    """wam_div_1p1 (PROPAGS' SIG1 weight, -MIN(0, DOP)/1.1) equals IEEE division
+    by 1.1f bit for bit on every float."""
+    exe = str(tmp_path / "div_sweep")
+    subprocess.check_call(["gcc", "-O2", "-mfma", "-std=gnu11", "-ffp-contract=off", "-fno-fast-math", "-fopenmp",
+                           "-I" + os.path.join(ROOT, "wam_amd", "csrc"), os.path.join(HERE, "math", "div_sweep.c"),
+                           "-o", exe])
+    r = json.loads(subprocess.check_output([exe, "1"], text=True))
+    assert r["n"] == 2 ** 32 and r["ne"] == 0, r

@@ -95,6 +95,7 @@ __global__ void eval_math_kernel(int fn, const float *x, const float *y, float *
     case 9: r = (float)wam_nint(a); break;
     case 10: r = wam_ipowf(a, (int)b); break;
     case 11: r = wam_atan2f(a, b); break;
+    case 12: r = wam_div_1p1(a); break;
     }
     out[i] = r;
 }
@@ -366,6 +367,7 @@ static int finish_prepare(wam_ctx *ctx)
     ctx->ncoef = t.cfg.refr_c ? 9 : 5;
     hipStream_t st = ctx->stream;
     std::vector<float> lo(ctx->ncoef, 3.4e38f), hi(ctx->ncoef, -3.4e38f);
+    bool nan_weight = false;
     // auto: computed in the stencil where the hoisted shallow-water + current
     // kernel applies (it is faster than streaming nine planes), else stored
     const bool stored = ctx->weights_mode == 1 ||
@@ -388,8 +390,10 @@ static int finish_prepare(wam_ctx *ctx)
         (void)hipFree(mm);
         for (int pl = 0; pl < ctx->ncoef; pl++)
             for (int b = 0; b < nblk; b++) {
-                lo[pl] = fminf(lo[pl], h[2 * nblk * pl + 2 * b]);
-                hi[pl] = fmaxf(hi[pl], h[2 * nblk * pl + 2 * b + 1]);
+                const float a = h[2 * nblk * pl + 2 * b], z = h[2 * nblk * pl + 2 * b + 1];
+                if (a != a || z != z) nan_weight = true;
+                lo[pl] = fminf(lo[pl], a);
+                hi[pl] = fmaxf(hi[pl], z);
             }
     } else {
         // CHECK_CFL over weights computed on the fly; PROPAGS recomputes them
@@ -405,8 +409,10 @@ static int finish_prepare(wam_ctx *ctx)
         (void)hipFree(mm);
         for (int b = 0; b < nblk; b++)
             for (int pl = 0; pl < ctx->ncoef; pl++) {
-                lo[pl] = fminf(lo[pl], h[((size_t)b * P_NPLANES + pl) * 2]);
-                hi[pl] = fmaxf(hi[pl], h[((size_t)b * P_NPLANES + pl) * 2 + 1]);
+                const float a = h[((size_t)b * P_NPLANES + pl) * 2], z = h[((size_t)b * P_NPLANES + pl) * 2 + 1];
+                if (a != a || z != z) nan_weight = true;
+                lo[pl] = fminf(lo[pl], a);
+                hi[pl] = fmaxf(hi[pl], z);
             }
     }
     // plane order: no current SUM LAT1 LON1 TH1 TH2; current SUM LAT1 LAT2 LON1 LON2 TH1 TH2 SIG1 SIG2
@@ -428,6 +434,10 @@ static int finish_prepare(wam_ctx *ctx)
     float cmin = lat11mn;
     for (float x : {0.1f, lon2mn, lon2mn, th1mn, th2mn, s1mn, s2mn, summn}) cmin = fminf(cmin, x);
     ctx->cfl_max = cmax; ctx->cfl_min = cmin;
+    // a NaN weight (from finite but extreme depths or currents) is refused:
+    // the propagation kernels take MIN(0, x) and MAX(0, x) of arguments that
+    // are then never NaN
+    if (nan_weight) { ctx->err = "a propagation weight is not a number (CHECK_CFL)"; return WAM_E_CFL; }
     if (cmin < 0.f) { ctx->err = "negative propagation weight (CHECK_CFL)"; return WAM_E_CFL; }
     // COUNTER = CEILING(CFL) sub-steps: a CFL number that is not finite, or
     // one that would need more than 2^20 sub-steps, is refused
@@ -735,8 +745,9 @@ static int propags_members(wam_ctx *ctx, int nmem, const float *d_in, float *d_o
         timing_record(ctx, ev[0]);
         if (ctx->coef)
             hipLaunchKernelGGL(kern, grid, block, 0, ctx->stream, ctx->dt, ctx->coef, src, dst);
-        else if (t.cfg.refr_c && t.cfg.shallow && 3ull * 4 * t.KL * t.NSEA < (1ull << 32)) {
-            // (32-bit buffer offsets over three frequency planes)
+        else if (t.cfg.refr_c && t.cfg.shallow && 6ull * 4 * t.KL * t.NSEA < (1ull << 32)) {
+            // (32-bit buffer offsets over three frequency planes, and land
+            // neighbours' offsets past them)
             const int c = ctx->counter;
             auto k = c == 1 ? propags_fused_sc_kernel<0>
                             : (c & (c - 1)) == 0 ? propags_fused_sc_kernel<1> : propags_fused_sc_kernel<2>;
@@ -2309,7 +2320,7 @@ double wam_gpu_get_scalar(wam_ctx *ctx, const char *name)
 
 int wam_gpu_eval_math(int32_t fn, const float *x, const float *y, float *out, int64_t n)
 {
-    if (fn < 0 || fn > 11 || !x || !out || n < 0 || ((fn == 8 || fn == 10 || fn == 11) && !y)) return WAM_E_ARG;
+    if (fn < 0 || fn > 12 || !x || !out || n < 0 || ((fn == 8 || fn == 10 || fn == 11) && !y)) return WAM_E_ARG;
     if (n == 0) return WAM_OK;
     float *d = nullptr;
     const size_t bytes = (size_t)n * sizeof(float);

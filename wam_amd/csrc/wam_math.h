@@ -69,6 +69,14 @@ WAM_HD int wam_isnan(float x) { return (wam_f2i(x) & 0x7fffffff) > 0x7f800000; }
 WAM_HD float wam_maxf(float a, float b) { return a > b ? a : b; }
 WAM_HD float wam_minf(float a, float b) { return a < b ? a : b; }
 
+/* x / 1.1f, correctly rounded, for every float x: the double product of x
+   and RN(1/1.1f) (1.1f = 9227469/2^23), rounded once to float.  The exact
+   quotient's fraction of an ulp is k/9227469, so it lies at least
+   2^-24.1 ulp (2^-48 relative) from a rounding boundary, while the double
+   product is within 2^-52 relative of it.  Compared with IEEE division over
+   all 2^32 floats by tests/math/div_sweep.c. */
+WAM_HD float wam_div_1p1(float x) { return (float)((double)x * 0x1.d1745c6e043b8p-1); }
+
 /* floor/ceil/round on floats without libm; exact. */
 WAM_HD float wam_truncf(float x)
 {

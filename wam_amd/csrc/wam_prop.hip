@@ -241,21 +241,27 @@ __global__ void prep_coef_kernel(DevTables t, float *__restrict__ coef)
     }
 }
 
-// Per-block min and max of one coefficient plane (order-free, exact).
+// min and max that return NaN when either argument is NaN (CHECK_CFL refuses
+// NaN weights), the reference's MIN/MAX otherwise
+__device__ __forceinline__ float nan_min(float a, float b) { return a != a ? a : (b != b ? b : wam_minf(a, b)); }
+__device__ __forceinline__ float nan_max(float a, float b) { return a != a ? a : (b != b ? b : wam_maxf(a, b)); }
+
+// Per-block min and max of one coefficient plane (order-free, exact; NaN if
+// the plane holds one).
 __global__ void minmax_kernel(const float *__restrict__ x, size_t n, float *__restrict__ out)
 {
     __shared__ float smin[256], smax[256];
     float lo = 3.4e38f, hi = -3.4e38f;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const float v = x[i];
-        lo = wam_minf(lo, v); hi = wam_maxf(hi, v);
+        lo = nan_min(lo, v); hi = nan_max(hi, v);
     }
     smin[threadIdx.x] = lo; smax[threadIdx.x] = hi;
     __syncthreads();
     for (int s = blockDim.x / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) {
-            smin[threadIdx.x] = wam_minf(smin[threadIdx.x], smin[threadIdx.x + s]);
-            smax[threadIdx.x] = wam_maxf(smax[threadIdx.x], smax[threadIdx.x + s]);
+            smin[threadIdx.x] = nan_min(smin[threadIdx.x], smin[threadIdx.x + s]);
+            smax[threadIdx.x] = nan_max(smax[threadIdx.x], smax[threadIdx.x + s]);
         }
         __syncthreads();
     }
@@ -448,7 +454,7 @@ __global__ void __launch_bounds__(256) cfl_minmax_kernel(DevTables t, float *__r
                 float c[P_NPLANES];
                 coef_bin<CURR>(t, ij, m, k, c);
 #pragma unroll
-                for (int p = 0; p < np; p++) { lo[p] = fminf(lo[p], c[p]); hi[p] = fmaxf(hi[p], c[p]); }
+                for (int p = 0; p < np; p++) { lo[p] = nan_min(lo[p], c[p]); hi[p] = nan_max(hi[p], c[p]); }
             }
     }
 #pragma unroll
@@ -457,8 +463,8 @@ __global__ void __launch_bounds__(256) cfl_minmax_kernel(DevTables t, float *__r
     for (int w = 128; w > 0; w >>= 1) {
         if (threadIdx.x < w)
             for (int p = 0; p < P_NPLANES; p++) {
-                red[2 * p][threadIdx.x] = fminf(red[2 * p][threadIdx.x], red[2 * p][threadIdx.x + w]);
-                red[2 * p + 1][threadIdx.x] = fmaxf(red[2 * p + 1][threadIdx.x], red[2 * p + 1][threadIdx.x + w]);
+                red[2 * p][threadIdx.x] = nan_min(red[2 * p][threadIdx.x], red[2 * p][threadIdx.x + w]);
+                red[2 * p + 1][threadIdx.x] = nan_max(red[2 * p + 1][threadIdx.x], red[2 * p + 1][threadIdx.x + w]);
             }
         __syncthreads();
     }
@@ -530,11 +536,16 @@ __global__ void __launch_bounds__(256) propags_fused_kernel(DevTables t, int cou
 // factors carried across the M loop, the per-direction refraction terms in a
 // window carried across the (cyclic) K loop, the per-direction constants from
 // KDIR (host-computed with the same operations).  Rows of F are addressed as
-// a uniform row pointer plus a per-lane 32-bit byte offset; a land neighbour
-// reads the point's own value and takes 0 instead (the reference's land slot).
-// RESCALE is CHECK_CFL's division of the weights by COUNTER: 0 none
-// (COUNTER = 1), 1 a power of two (the exact multiplication by 1/COUNTER),
-// 2 a true division.
+// a uniform row pointer plus a per-lane 32-bit byte offset; a land neighbour's
+// offset lies past the buffer's end, where the load returns +0 (the
+// reference's land slot).  MIN(0, x) is v_min_f32 (the same value as the
+// reference's for every x that is not NaN, signed zeros included); MAX(0, x)
+// in the running centre weight is v_max_f32, which differs only in the sign of
+// a zero result, and a zero of either sign leaves that sum unchanged (it
+// starts at 1 - MAX(0, DLE) and never becomes -0).  CHECK_CFL refuses NaN
+// weights, so no argument here is NaN.  RESCALE is CHECK_CFL's division of the
+// weights by COUNTER: 0 none (COUNTER = 1), 1 a power of two (the exact
+// multiplication by 1/COUNTER), 2 a true division.
 template <int RESCALE>
 __global__ void __launch_bounds__(256) propags_fused_sc_kernel(DevTables t, int counter,
                                                                 const float *__restrict__ fin,
@@ -555,8 +566,12 @@ __global__ void __launch_bounds__(256) propags_fused_sc_kernel(DevTables t, int 
     if (ij >= NS) return;
     const int ns = t.KLAT_S[ij], nn = t.KLAT_N[ij], nw = t.KLON_W[ij], ne = t.KLON_E[ij];
     const bool hs = ns >= 0, hn = nn >= 0, hw = nw >= 0, he = ne >= 0;
-    const unsigned o_c = 4u * ij, o_s = 4u * (hs ? ns : ij), o_n = 4u * (hn ? nn : ij);
-    const unsigned o_w = 4u * (hw ? nw : ij), o_e = 4u * (he ? ne : ij);
+    // the buffer spans three frequency planes (3*planeb bytes); a land
+    // neighbour's offset is that length, out of range whatever the row's
+    // scalar offset (the host keeps 6*planeb below 2**32, so no sum wraps)
+    const unsigned rowb = 4u * NS, planeb = rowb * KL, oob = 3u * planeb;
+    const unsigned o_c = 4u * ij, o_s = hs ? 4u * ns : oob, o_n = hn ? 4u * nn : oob;
+    const unsigned o_w = hw ? 4u * nw : oob, o_e = he ? 4u * ne : oob;
     // per point
     const float u = t.U[ij], v = t.V[ij];
     const float uw = hw ? t.U[nw] : 0.f, ue = he ? t.U[ne] : 0.f;
@@ -572,9 +587,7 @@ __global__ void __launch_bounds__(256) propags_fused_sc_kernel(DevTables t, int 
     const float rc = (float)counter, rinv = 1.0f / rc;
     // F through one buffer resource per frequency M, based at plane M-1
     // (clamped): rows (M', K) at scalar byte offsets (M' - M-1)*planeb +
-    // K*rowb, the sea point (or neighbour) as the lane's byte offset; the
-    // host keeps 3*planeb below 2**32
-    const unsigned rowb = 4u * NS, planeb = rowb * KL;
+    // K*rowb, the sea point (or neighbour) as the lane's byte offset
     auto rsrc = [&](const float *base, int m, unsigned bytes) {
         // the base as a wave-uniform value (kept in scalar registers)
         const uint64_t a = reinterpret_cast<uint64_t>(base + (size_t)m * KL * NS);
@@ -621,29 +634,33 @@ __global__ void __launch_bounds__(256) propags_fused_sc_kernel(DevTables t, int 
             const float w2 = (v + cd * cg_m) * t.DELPH0;
             const float dlw = (w1 + (uw + sd * cgw)) * della0;
             const float dle = (w1 + (ue + sd * cge)) * della0;
-            float dtc = (1.f - wam_maxf(0.f, dle)) + wam_minf(0.f, dlw);
+            const float mn_dlw = __builtin_fminf(0.f, dlw), mn_dle = __builtin_fminf(0.f, dle);
+            float dtc = (1.f - __builtin_fmaxf(0.f, dle)) + mn_dlw;
             const float dps = w2 + dpsn_s * ((vs + cd * cgs) * t.DELPH0);
             const float dpn = w2 + dpsn_n * ((vn + cd * cgn) * t.DELPH0);
-            dtc = (dtc - wam_maxf(0.f, dpn)) + wam_minf(0.f, dps);
+            const float mn_dps = __builtin_fminf(0.f, dps), mn_dpn = __builtin_fminf(0.f, dpn);
+            dtc = (dtc - __builtin_fmaxf(0.f, dpn)) + mn_dps;
             const float dtp = ((sp * drgm) * cg_m) + (th_k + th_kp) * t.DELTHR;
             const float dtm = ((sm * drgm) * cg_m) + (th_k + th_km) * t.DELTHR;
-            dtc = (dtc - wam_maxf(0.f, dtp)) + wam_minf(0.f, dtm);
+            const float mn_dtp = __builtin_fminf(0.f, dtp), mn_dtm = __builtin_fminf(0.f, dtm);
+            dtc = (dtc - __builtin_fmaxf(0.f, dtp)) + mn_dtm;
             // SIDC(K, M') = (S0(K)*CG(M') + TEMPD*TSIHKD(M'))*TFAK(M')
             const float ss = kd[4], sc = kd[5], cc = kd[6];
             const float s0 = (((-(sc * duphi)) - cc * dvphi) - ss * dulam) - sc * dvlam;
             const float sk = (s0 * cg_m + tempd * ts_m) * tf_m;
             const float dop = (sk + (s0 * cg_mp + tempd * ts_mp) * tf_mp) * dfp;
             const float dom = (sk + (s0 * cg_mm + tempd * ts_mm) * tf_mm) * dfm;
-            dtc = (dtc - wam_maxf(0.f, dop)) + wam_minf(0.f, dom);
+            const float mn_dop = __builtin_fminf(0.f, dop);
+            dtc = (dtc - __builtin_fmaxf(0.f, dop)) + __builtin_fminf(0.f, dom);
             float c[P_NPLANES];
             c[P_SUM] = dtc;
             c[P_LAT1] = wam_maxf(0.f, dps);
-            c[P_LAT2] = -wam_minf(0.f, dpn);
+            c[P_LAT2] = -mn_dpn;
             c[P_LON1] = wam_maxf(0.f, dlw);
-            c[P_LON2] = -wam_minf(0.f, dle);
-            c[P_TH1] = -wam_minf(0.f, dtp);
+            c[P_LON2] = -mn_dle;
+            c[P_TH1] = -mn_dtp;
             c[P_TH2] = wam_maxf(0.f, dtm);
-            c[P_SIG1] = -wam_minf(0.f, dop) / 1.1f;
+            c[P_SIG1] = wam_div_1p1(-mn_dop);
             c[P_SIG2] = wam_maxf(0.f, dom) * 1.1f;
             if (RESCALE == 1) {
                 c[0] = 1.f - (1.f - c[0]) * rinv;
@@ -655,10 +672,10 @@ __global__ void __launch_bounds__(256) propags_fused_sc_kernel(DevTables t, int 
                 for (int p = 1; p < P_NPLANES; p++) c[p] = c[p] / rc;
             }
             float sacc = c[P_SUM] * f_c;
-            sacc = sacc + c[P_LAT2] * (hn ? l_n : 0.f);
-            sacc = sacc + c[P_LAT1] * (hs ? l_s : 0.f);
-            sacc = sacc + c[P_LON2] * (he ? l_e : 0.f);
-            sacc = sacc + c[P_LON1] * (hw ? l_w : 0.f);
+            sacc = sacc + c[P_LAT2] * l_n;
+            sacc = sacc + c[P_LAT1] * l_s;
+            sacc = sacc + c[P_LON2] * l_e;
+            sacc = sacc + c[P_LON1] * l_w;
             sacc = sacc + c[P_TH1] * f_kp;
             sacc = sacc + c[P_TH2] * f_km;
             sacc = sacc + c[P_SIG1] * f_mp;
