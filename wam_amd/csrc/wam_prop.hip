@@ -620,16 +620,28 @@ __global__ void __launch_bounds__(256) propags_fused_sc_kernel(DevTables t, int 
             return temp * ts_m + thdc;
         };
         float th_km = thdd_k(KL - 1), th_k = thdd_k(0), th_kp = thdd_k(KL > 1 ? 1 : 0);
+        // the point's rows K-1, K, K+1 of plane M rotate through registers; the
+        // loads of the next direction are issued before this one's arithmetic
+        const int k1 = KL > 1 ? 1 : 0;
+        float r_km = ld(pf, o_c, sm0 + (KL - 1) * rowb), r_c = ld(pf, o_c, sm0), r_kp = ld(pf, o_c, sm0 + k1 * rowb);
+        float q_n = ld(pf, o_n, sm0), q_s = ld(pf, o_s, sm0), q_e = ld(pf, o_e, sm0), q_w = ld(pf, o_w, sm0);
+        float q_mp = ld(pf, o_c, sp0), q_mm = ld(pf, o_c, 0);
         for (int k = 0; k < KL; k++) {
             const int kp = k + 1 < KL ? k + 1 : 0, km = k > 0 ? k - 1 : KL - 1;
             const float *kd = kdir + 8 * k;
             const float sd = kd[0], cd = kd[1], sp = kd[2], sm = kd[3];
             // the nine spectral values of the stencil
-            const unsigned rk = sm0 + k * rowb;
-            const float f_c = ld(pf, o_c, rk);
-            const float l_n = ld(pf, o_n, rk), l_s = ld(pf, o_s, rk), l_e = ld(pf, o_e, rk), l_w = ld(pf, o_w, rk);
-            const float f_kp = ld(pf, o_c, sm0 + kp * rowb), f_km = ld(pf, o_c, sm0 + km * rowb);
-            const float f_mp = ld(pf, o_c, sp0 + k * rowb), f_mm = ld(pf, o_c, k * rowb);
+            const float f_c = r_c, f_kp = r_kp, f_km = r_km;
+            const float l_n = q_n, l_s = q_s, l_e = q_e, l_w = q_w, f_mp = q_mp, f_mm = q_mm;
+            {
+                // the next direction's loads (at the last direction, rows of direction 0 again: unused)
+                const int kn = kp, kn2 = kn + 1 < KL ? kn + 1 : 0;
+                const unsigned rn = sm0 + kn * rowb;
+                r_km = r_c; r_c = r_kp;
+                r_kp = ld(pf, o_c, sm0 + kn2 * rowb);
+                q_n = ld(pf, o_n, rn); q_s = ld(pf, o_s, rn); q_e = ld(pf, o_e, rn); q_w = ld(pf, o_w, rn);
+                q_mp = ld(pf, o_c, sp0 + kn * rowb); q_mm = ld(pf, o_c, kn * rowb);
+            }
             const float w1 = u + sd * cg_m;
             const float w2 = (v + cd * cg_m) * t.DELPH0;
             const float dlw = (w1 + (uw + sd * cgw)) * della0;
