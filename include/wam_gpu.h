@@ -33,7 +33,8 @@ enum {
     WAM_OK = 0,
     WAM_E_ARG = 1,        /* bad argument, size mismatch or unsupported option */
     WAM_E_HIP = 2,        /* HIP runtime error (allocation, launch) */
-    WAM_E_CFL = 3,        /* negative propagation weight (CHECK_CFL abort), or a CFL number not finite or above 2^20 */
+    WAM_E_CFL = 3,        /* negative propagation weight (CHECK_CFL abort), a NaN weight, or a CFL number not finite
+                             or above 2^20 */
     WAM_E_STATE = 4,      /* call order violated (e.g. IMPLSCH before PREPARE_PROPAGATION) */
     WAM_E_GRID = 5,       /* empty or invalid sea mask */
     WAM_E_IO = 6          /* restart or boundary file: cannot be opened, read or written, or inconsistent with the grid */
