@@ -6,7 +6,7 @@ Tolerances.  The oracle evaluates the Fortran intrinsics with wam_math.h,
 whose results are the correctly rounded ones (tests/test_math_cr.py); the
 reference used glibc's float functions behind flang, which differ from the
 correctly rounded value by an ulp for a few per cent of the arguments of
-tanhf, sinhf and atan2f and far fewer of the others.  Bars (north_star's):
+tanhf, sinhf and atan2f and far fewer of the others.  Bars:
   * spectrum after one step: |F - F_ref| <= 1e-5 * max(|F_ref|, 1e-6 max|F_ref|)
     per bin;
   * per-point outputs (USTAR, TAUW, Z0, PHIAW, energy fluxes, and the stress

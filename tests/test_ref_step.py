@@ -2,7 +2,7 @@
 step (wamodel.f90:332,354) for 6 to 24 hours, golden vectors written by
 tests/golden/make_ref_step.py from the reference built from its sources
 (oracle/ref_step_driver.f90).  The oracle (CPU) and the HIP path (through
-the C-ABI, spectra resident in HBM) are both held to north_star's bars:
+the C-ABI, spectra resident in HBM) are both held to these bars:
 
   * after the first step, the spectrum within 1e-5 per bin
     (|F - F_ref| <= 1e-5 max(|F_ref|, 1e-6 max|F_ref|)) and USTAR, TAUW, Z0
