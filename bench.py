@@ -712,9 +712,11 @@ def main():
     prop_bytes = n * kl * ml * 4 * 2 + n * 4 * 19
     prop_ms = ktime["propags"]
     prop_gbs = prop_bytes / (prop_ms * 1e-3) / 1e9
-    roofs = {"propags": dict(kernel="propags_fused_sc_kernel<1>", bound="hbm", achieved=round(prop_gbs, 1),
+    # KL a multiple of 3: the register-block kernel, else a direction at a time
+    prop_kernel = "propags_blk_kernel<1, 3>" if kl % 3 == 0 else "propags_fused_sc_kernel<1>"
+    roofs = {"propags": dict(kernel=prop_kernel, bound="hbm", achieved=round(prop_gbs, 1),
                              peak=HBM_PEAK_GBS, unit="GB/s", frac=round(prop_gbs / HBM_PEAK_GBS, 4),
-                             traffic=traffic.get("propags_fused_sc_kernel", {}).get("bytes_per_dispatch"),
+                             traffic=traffic.get(prop_kernel.split("<")[0], {}).get("bytes_per_dispatch"),
                              algorithmic_bytes_per_launch=prop_bytes, avg_launch_ms=round(prop_ms, 3),
                              note="one sub-step; weights computed in the stencil (no 9-plane weight stream)")}
     # IMPLSCH: compute bound (VALU FP32).  Algorithmic flops per sea point

@@ -41,6 +41,8 @@ CASES = {
     "c2cut_janssen_shallow_curr_kl12": lambda: W.c2_case(ny=30, nx=40, seed=6, kl=12, ml=25),
     # 30 directions (not a multiple of 4): IMPLSCH's single-kernel form and the generic PROP_CURR kernel
     "c2cut_janssen_shallow_curr_kl30": lambda: W.c2_case(ny=30, nx=40, seed=7, kl=30, ml=25),
+    # 16 directions (not a multiple of 3): PROPAGS a direction at a time instead of in blocks of three
+    "c2cut_janssen_shallow_curr_kl16": lambda: W.c2_case(ny=30, nx=40, seed=9, kl=16, ml=25),
     "deep_current": lambda: _with(W.c2_case(ny=30, nx=40, seed=4, kl=24, ml=25), shallow=0, deep_depth=True),
 }
 

@@ -33,6 +33,8 @@ template <bool CURR>
 __global__ void propags_fused_kernel(DevTables t, int counter, const float *fin, float *fout);
 template <int RESCALE>
 __global__ void propags_fused_sc_kernel(DevTables t, int counter, const float *fin, float *fout);
+template <int RESCALE, int KB>
+__global__ void propags_blk_kernel(DevTables t, int counter, const float *fin, float *fout);
 __global__ void wavnu2_kernel(DevTables t, float *wn, float *cgg);
 __global__ void sbottom_factor_kernel(DevTables t, float *sbot);
 hipError_t launch_implsch(const DevTables &t, const ImplArgs &a, int ncu, hipStream_t st);
@@ -748,9 +750,13 @@ static int propags_members(wam_ctx *ctx, int nmem, const float *d_in, float *d_o
         else if (t.cfg.refr_c && t.cfg.shallow && 6ull * 4 * t.KL * t.NSEA < (1ull << 32)) {
             // (32-bit buffer offsets over three frequency planes, and land
             // neighbours' offsets past them)
+            // in register blocks of three directions where KL allows, else a direction at a time
             const int c = ctx->counter;
-            auto k = c == 1 ? propags_fused_sc_kernel<0>
-                            : (c & (c - 1)) == 0 ? propags_fused_sc_kernel<1> : propags_fused_sc_kernel<2>;
+            const int rs = c == 1 ? 0 : (c & (c - 1)) == 0 ? 1 : 2;
+            auto k = t.KL % 3 == 0 ? (rs == 0 ? propags_blk_kernel<0, 3> : rs == 1 ? propags_blk_kernel<1, 3>
+                                                                              : propags_blk_kernel<2, 3>)
+                                   : (rs == 0 ? propags_fused_sc_kernel<0> : rs == 1 ? propags_fused_sc_kernel<1>
+                                                                                : propags_fused_sc_kernel<2>);
             hipLaunchKernelGGL(k, grid, block, 0, ctx->stream, ctx->dt, c, src, dst);
         }
         else if (t.cfg.refr_c)

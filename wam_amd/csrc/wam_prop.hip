@@ -702,6 +702,179 @@ __global__ void __launch_bounds__(256) propags_fused_sc_kernel(DevTables t, int 
     }
 }
 
+// PROP_CURR with the weights computed in place (as propags_fused_sc_kernel,
+// the same operations), in register blocks of KB directions with the
+// frequencies inside.  For its block K0 .. K0+KB-1 a thread keeps the point's
+// rows K0-1 .. K0+KB of planes M-1, M and M+1 in registers, so that each row
+// of F is read once per block as plane M+1; only the two rows beside the
+// block are read again by the neighbouring blocks.  What depends on the point
+// and K alone (THDD's temp and THDC, SIDC's S0, SINTH*DRGM) is formed once per
+// block, SIDC(K, M') is carried across M, and DELFR0/FR(M), DELFR0/FR(M-1)
+// come from a per-workgroup table.  Land neighbours, MIN/MAX and the division
+// by 1.1 as in propags_fused_sc_kernel.  KL must be a multiple of KB.
+template <int RESCALE, int KB>
+__global__ void __launch_bounds__(256) propags_blk_kernel(DevTables t, int counter, const float *__restrict__ fin,
+                                                          float *__restrict__ fout)
+{
+    fin += member_offset(t);
+    fout += member_offset(t);
+    __shared__ float4 kdir4[2 * 64];
+    __shared__ float2 dfr[64];
+    const int KL = t.KL, ML = t.ML, NS = t.NSEA, ND = t.NDEPTH;
+    for (int i = threadIdx.x; i < 2 * KL; i += blockDim.x) kdir4[i] = reinterpret_cast<const float4 *>(t.KDIR)[i];
+    for (int m = threadIdx.x; m < ML; m += blockDim.x)
+        dfr[m] = make_float2(t.DELFR0 / t.FR[m], t.DELFR0 / t.FR[m > 0 ? m - 1 : 0]);
+    __syncthreads();
+    const float *kdir = reinterpret_cast<const float *>(kdir4);
+    const int wg = blockDim.x, nch = (NS + wg - 1) / wg;
+    const int ch = xcd_chunk(nch);
+    if (ch >= nch) return;
+    const int ij = ch * wg + threadIdx.x;
+    if (ij >= NS) return;
+    const int ns = t.KLAT_S[ij], nn = t.KLAT_N[ij], nw = t.KLON_W[ij], ne = t.KLON_E[ij];
+    const bool hs = ns >= 0, hn = nn >= 0, hw = nw >= 0, he = ne >= 0;
+    // F through one buffer resource per frequency M, based at plane M-1
+    // (clamped), spanning three planes; a land neighbour's offset is the
+    // span's length, out of range, where the load returns +0
+    const unsigned rowb = 4u * NS, planeb = rowb * KL, oob = 3u * planeb;
+    const unsigned o_c = 4u * ij, o_s = hs ? 4u * ns : oob, o_n = hn ? 4u * nn : oob;
+    const unsigned o_w = hw ? 4u * nw : oob, o_e = he ? 4u * ne : oob;
+    const float u = t.U[ij], v = t.V[ij];
+    const float uw = hw ? t.U[nw] : 0.f, ue = he ? t.U[ne] : 0.f;
+    const float vs = hs ? t.V[ns] : 0.f, vn = hn ? t.V[nn] : 0.f;
+    const int i0 = t.INDEP[ij] - 1;
+    const int iw = hw ? t.INDEP[nw] - 1 : ND - 1, ie = he ? t.INDEP[ne] - 1 : ND - 1;
+    const int is = hs ? t.INDEP[ns] - 1 : ND - 1, in = hn ? t.INDEP[nn] - 1 : ND - 1;
+    const int ig = t.INDEP_G[ij] - 1;
+    const float ddphi = t.DDPHI[ij], ddlam = t.DDLAM[ij];
+    const float duphi = t.DUPHI[ij], dulam = t.DULAM[ij], dvphi = t.DVPHI[ij], dvlam = t.DVLAM[ij];
+    const float drgm = t.DRGM[ij], della0 = t.DELLA0[ij], dpsn_s = t.DPSN_S[ij], dpsn_n = t.DPSN_N[ij];
+    const float tempd = v * ddphi + u * ddlam;
+    const float rc = (float)counter, rinv = 1.0f / rc;
+    auto rsrc = [&](const float *base, int m, unsigned bytes) {
+        const uint64_t a = reinterpret_cast<uint64_t>(base + (size_t)m * KL * NS);
+        // the base as a wave-uniform value (kept in scalar registers)
+        const uint64_t uu = (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)a) |
+                            (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(a >> 32)) << 32;
+        return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void *>(uu), (short)0, (int)bytes, 0x00020000);
+    };
+    auto ld = [&](__amdgpu_buffer_rsrc_t r, unsigned off, unsigned soff) {
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
+    };
+    auto cgv = [&](int idx, int m) { return t.TCGOND[m * ND + idx]; };
+    for (int kb = 0; kb < KL; kb += KB) {
+        // THDD's direction terms of K0-1 .. K0+KB (cyclic)
+        int kk[KB + 2];
+        float tp[KB + 2], tc[KB + 2];
+#pragma unroll
+        for (int d = 0; d < KB + 2; d++) {
+            const int k = (kb - 1 + d + KL) % KL;
+            kk[d] = k;
+            const float *kd = kdir + 8 * k;
+            tp[d] = kd[0] * ddphi - kd[1] * ddlam;
+            tc[d] = ((kd[4] * duphi + kd[5] * dvphi) - kd[5] * dulam) - kd[6] * dvlam;
+        }
+        float s0[KB], spd[KB], smd[KB], sdk[KB], cdk[KB];
+#pragma unroll
+        for (int j = 0; j < KB; j++) {
+            const float *kd = kdir + 8 * (kb + j);
+            sdk[j] = kd[0]; cdk[j] = kd[1];
+            spd[j] = kd[2] * drgm; smd[j] = kd[3] * drgm;
+            s0[j] = (((-(kd[5] * duphi)) - kd[6] * dvphi) - kd[4] * dulam) - kd[5] * dvlam;
+        }
+        // the rows K0-1 .. K0+KB of planes M-1, M, M+1 (M-1 clamps to 1)
+        float P_mm[KB + 2], P_c[KB + 2], P_mp[KB + 2];
+        {
+            const __amdgpu_buffer_rsrc_t p0 = rsrc(fin, 0, planeb);
+#pragma unroll
+            for (int d = 0; d < KB + 2; d++) { P_c[d] = ld(p0, o_c, kk[d] * rowb); P_mm[d] = P_c[d]; }
+        }
+        float cg_m = cgv(i0, 0), ts_m = t.TSIHKD[ig], tf_m = t.TFAK[ig];
+        // SIDC(K, M) and SIDC(K, M-1) of the block's directions
+        float S_c[KB], S_mm[KB];
+#pragma unroll
+        for (int j = 0; j < KB; j++) { S_c[j] = (s0[j] * cg_m + tempd * ts_m) * tf_m; S_mm[j] = S_c[j]; }
+        for (int m = 0; m < ML; m++) {
+            const int mp = m < ML - 1 ? m + 1 : ML - 1, mm = m > 0 ? m - 1 : 0;
+            const float cg_mp = cgv(i0, mp), ts_mp = t.TSIHKD[mp * ND + ig], tf_mp = t.TFAK[mp * ND + ig];
+            const float cgw = cgv(iw, m), cge = cgv(ie, m), cgs = cgv(is, m), cgn = cgv(in, m);
+            const float2 df = dfr[m];
+            const __amdgpu_buffer_rsrc_t pf = rsrc(fin, mm, 3 * planeb), po = rsrc(fout, m, planeb);
+            const unsigned sm0 = (m - mm) * planeb, sp0 = (mp - mm) * planeb;
+            // plane M+1 (at M = ML the plane itself again, the clamp)
+#pragma unroll
+            for (int d = 0; d < KB + 2; d++) P_mp[d] = ld(pf, o_c, sp0 + kk[d] * rowb);
+            float th[KB + 2];
+#pragma unroll
+            for (int d = 0; d < KB + 2; d++) th[d] = tp[d] * ts_m + tc[d];
+#pragma unroll
+            for (int j = 0; j < KB; j++) {
+                const int k = kb + j;
+                const unsigned rk = sm0 + k * rowb;
+                const float l_n = ld(pf, o_n, rk), l_s = ld(pf, o_s, rk), l_e = ld(pf, o_e, rk), l_w = ld(pf, o_w, rk);
+                const float f_c = P_c[j + 1], f_kp = P_c[j + 2], f_km = P_c[j], f_mp = P_mp[j + 1], f_mm = P_mm[j + 1];
+                const float sd = sdk[j], cd = cdk[j];
+                const float w1 = u + sd * cg_m;
+                const float w2 = (v + cd * cg_m) * t.DELPH0;
+                const float dlw = (w1 + (uw + sd * cgw)) * della0;
+                const float dle = (w1 + (ue + sd * cge)) * della0;
+                const float mn_dlw = __builtin_fminf(0.f, dlw), mn_dle = __builtin_fminf(0.f, dle);
+                float dtc = (1.f - __builtin_fmaxf(0.f, dle)) + mn_dlw;
+                const float dps = w2 + dpsn_s * ((vs + cd * cgs) * t.DELPH0);
+                const float dpn = w2 + dpsn_n * ((vn + cd * cgn) * t.DELPH0);
+                const float mn_dps = __builtin_fminf(0.f, dps), mn_dpn = __builtin_fminf(0.f, dpn);
+                dtc = (dtc - __builtin_fmaxf(0.f, dpn)) + mn_dps;
+                const float dtp = (spd[j] * cg_m) + (th[j + 1] + th[j + 2]) * t.DELTHR;
+                const float dtm = (smd[j] * cg_m) + (th[j + 1] + th[j]) * t.DELTHR;
+                const float mn_dtp = __builtin_fminf(0.f, dtp), mn_dtm = __builtin_fminf(0.f, dtm);
+                dtc = (dtc - __builtin_fmaxf(0.f, dtp)) + mn_dtm;
+                const float s_mp = (s0[j] * cg_mp + tempd * ts_mp) * tf_mp;
+                const float dop = (S_c[j] + s_mp) * df.x;
+                const float dom = (S_c[j] + S_mm[j]) * df.y;
+                const float mn_dop = __builtin_fminf(0.f, dop);
+                dtc = (dtc - __builtin_fmaxf(0.f, dop)) + __builtin_fminf(0.f, dom);
+                float c[P_NPLANES];
+                c[P_SUM] = dtc;
+                c[P_LAT1] = wam_maxf(0.f, dps);
+                c[P_LAT2] = -mn_dpn;
+                c[P_LON1] = wam_maxf(0.f, dlw);
+                c[P_LON2] = -mn_dle;
+                c[P_TH1] = -mn_dtp;
+                c[P_TH2] = wam_maxf(0.f, dtm);
+                c[P_SIG1] = wam_div_1p1(-mn_dop);
+                c[P_SIG2] = wam_maxf(0.f, dom) * 1.1f;
+                if (RESCALE == 1) {
+                    c[0] = 1.f - (1.f - c[0]) * rinv;
+#pragma unroll
+                    for (int p = 1; p < P_NPLANES; p++) c[p] = c[p] * rinv;
+                } else if (RESCALE == 2) {
+                    c[0] = 1.f - (1.f - c[0]) / rc;
+#pragma unroll
+                    for (int p = 1; p < P_NPLANES; p++) c[p] = c[p] / rc;
+                }
+                float sacc = c[P_SUM] * f_c;
+                sacc = sacc + c[P_LAT2] * l_n;
+                sacc = sacc + c[P_LAT1] * l_s;
+                sacc = sacc + c[P_LON2] * l_e;
+                sacc = sacc + c[P_LON1] * l_w;
+                sacc = sacc + c[P_TH1] * f_kp;
+                sacc = sacc + c[P_TH2] * f_km;
+                sacc = sacc + c[P_SIG1] * f_mp;
+                sacc = sacc + c[P_SIG2] * f_mm;
+                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(sacc), po, o_c, k * rowb, 2);   // nt
+                S_mm[j] = S_c[j];
+                S_c[j] = s_mp;
+            }
+#pragma unroll
+            for (int d = 0; d < KB + 2; d++) { P_mm[d] = P_c[d]; P_c[d] = P_mp[d]; }
+            cg_m = cg_mp; ts_m = ts_mp; tf_m = tf_mp;
+        }
+    }
+}
+template __global__ void propags_blk_kernel<0, 3>(DevTables, int, const float *, float *);
+template __global__ void propags_blk_kernel<1, 3>(DevTables, int, const float *, float *);
+template __global__ void propags_blk_kernel<2, 3>(DevTables, int, const float *, float *);
+
 template __global__ void propags_fused_sc_kernel<0>(DevTables, int, const float *, float *);
 template __global__ void propags_fused_sc_kernel<1>(DevTables, int, const float *, float *);
 template __global__ void propags_fused_sc_kernel<2>(DevTables, int, const float *, float *);
